@@ -1,0 +1,51 @@
+"""Oracle: one compliance design iteration, filter -> SIMP -> assembly -> solve -> c -> dc/dx.
+
+Test infrastructure only.  Restates the module chain of
+``examples/topology_optimization/ex_compliance.py:84-110`` /
+``ex_compliance_multigrid.py:84-131`` with ``LinSolve`` + ``LDAWrapper`` semantics
+(``linalg.py:164-219``, ``solvers.py:140-306``) and the compliance adjoint
+(``generic.py:196-226``, ``assembly.py:298-315``, ``generic.py:101-140``, ``filter.py:214-220``).
+
+Pinned end-to-end against the reference's own outputs in ``tests/golden/*.npz``.
+"""
+import numpy as np
+
+from . import fe
+from .filter import ConvFilter, filter_weights
+from .solvers import CG, LDAS, DirectLU, build_mg
+
+
+def design_iteration(nel, x, bc, f, radius, xmin=1e-9, penal=3.0, plane="strain", solver="direct",
+                     mg_levels=4, omega=0.6, smooth_steps=2, tol=1e-8, filter_bcs=None, E=1.0, nu=0.3,
+                     x0=None, return_all=False):
+    nelx, nely, nelz = nel
+    dim = fe.grid_dim(nelx, nely, nelz)
+    w = filter_weights(radius, nel)
+    filt = ConvFilter(nel, w, filter_bcs or (("symmetric",) * 2,) * 3)
+    xf = filt.forward(x)
+    xs = fe.simp(xf, xmin, penal)
+    K0 = fe.element_stiffness(dim, E=E, nu=nu, plane=plane)
+    asm = fe.Assembly(nelx, nely, nelz, K0, bc=bc)
+    K = asm.matrix(xs)
+    if solver == "direct":
+        inner = DirectLU()
+        lda = LDAS(inner, tol=1e-7)
+    else:
+        mg = build_mg(nel, mg_levels, omega=omega, smooth_steps=smooth_steps)
+        inner = CG(preconditioner=mg, tol=tol)
+        lda = LDAS(inner, tol=5 * tol)
+    lda.update(K)
+    u = lda.solve(f, x0=x0)
+    c = float(f @ u)
+    lam = lda.solve(1.0 * f)  # LinSolve._sensitivity with dc = 1 (EinSum adjoint: dc * f)
+    dxs = asm.sens_contract([-lam], [u])
+    dxf = dxs * fe.dsimp(xf, xmin, penal)
+    dx = filt.adjoint(dxf)
+    out = dict(xf=xf, xsimp=xs, u=u, c=c, dcdxsimp=dxs, dcdxf=dxf, dcdx=dx)
+    if solver != "direct":
+        out["iterations"] = inner.iterations
+        out["history"] = np.array(inner.history)
+        out["adjoint_solved"] = bool(np.any(lda.did_solve))
+    if return_all:
+        out.update(K=K, K0=K0, asm=asm, filt=filt)
+    return out

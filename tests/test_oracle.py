@@ -1,0 +1,235 @@
+"""Pin the CPU oracle against the reference's golden vectors, its own known answers and closed forms."""
+import hashlib
+
+import numpy as np
+import numpy.testing as npt
+import pytest
+import scipy.sparse as sps
+from scipy.sparse.linalg import spsolve
+
+from conftest import golden
+from oracle import fe, problems
+from oracle.filter import ConvFilter, filter_weights
+from oracle.pipeline import design_iteration
+from oracle.solvers import CG, build_mg, interpolation
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+# ---------------------------------------------------------------- element stiffness
+def test_k0_matches_reference_golden():
+    g = golden("k0_cases.npz")
+    for i in range(int(g["ncases"])):
+        is3d, sx, sy, sz = g[f"spec_{i}"]
+        K0 = fe.element_stiffness(3 if is3d else 2, (sx, sy, sz), E=2.5, nu=0.27, plane=str(g[f"plane_{i}"]))
+        npt.assert_allclose(K0, g[f"K0_{i}"], rtol=1e-13, atol=1e-13 * np.abs(K0).max())
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_k0_rigid_body_nullspace_and_symmetry(dim):
+    K0 = fe.element_stiffness(dim, (0.7, 1.3, 0.9))
+    npt.assert_allclose(K0, K0.T, atol=1e-14)
+    ev = np.linalg.eigvalsh(K0)
+    nrbm = 3 if dim == 2 else 6
+    assert np.all(np.abs(ev[:nrbm]) < 1e-12 * ev[-1]) and ev[nrbm] > 1e-6 * ev[-1]
+
+
+def test_one_element_tensile_2d_closed_form():
+    """u = F L /(E A) and Poisson contraction (reference tests/test_assembly.py:101-153)."""
+    Lx, Ly, Lz, E, nu, F = 0.1, 0.2, 0.3, 210e9, 0.3, 1.5
+    K0 = fe.element_stiffness(2, (Lx, Ly, Lz), E=E, nu=nu, plane="stress")
+    asm = fe.Assembly(1, 1, 0, K0, bc=np.array([0, 4, 1]))
+    K = asm.matrix(np.ones(1)).toarray()
+    f = np.zeros(8)
+    f[[2, 6]] = F / 2
+    u = np.linalg.solve(K, f)
+    ux = F * Lx / (E * Ly * Lz)
+    npt.assert_allclose(u[[2, 6]], ux, rtol=1e-10)
+    npt.assert_allclose(u[7], -nu * ux / Lx * Ly, rtol=1e-10)
+    npt.assert_allclose(u[3], 0, atol=1e-10)
+
+
+def test_one_element_tensile_3d_closed_form():
+    """reference tests/test_assembly.py:156-207."""
+    Lx, Ly, Lz, E, nu, F = 0.1, 0.2, 0.3, 210e9, 0.3, 1.5e3
+    K0 = fe.element_stiffness(3, (Lx, Ly, Lz), E=E, nu=nu)
+    n = lambda i, j, k: (k * 2 + j) * 2 + i  # noqa: E731
+    left = [n(0, j, k) for j in range(2) for k in range(2)]
+    right = [n(1, j, k) for j in range(2) for k in range(2)]
+    bc = np.concatenate([np.array(left) * 3, np.array([n(0, 0, 0), n(0, 0, 1)]) * 3 + 1,
+                         np.array([n(0, 0, 0), n(0, 1, 0)]) * 3 + 2])
+    K = fe.Assembly(1, 1, 1, K0, bc=bc).matrix(np.ones(1)).toarray()
+    f = np.zeros(24)
+    f[np.array(right) * 3] = F / 4
+    u = np.linalg.solve(K, f)
+    ux = F * Lx / (E * Ly * Lz)
+    npt.assert_allclose(u[np.array(right) * 3], ux, rtol=1e-10)
+    npt.assert_allclose(u[n(1, 1, 0) * 3 + 1], -nu * ux / Lx * Ly, rtol=1e-10)
+    npt.assert_allclose(u[n(1, 0, 1) * 3 + 2], -nu * ux / Lx * Lz, rtol=1e-10)
+
+
+# ---------------------------------------------------------------- CSR pattern and values
+def test_csr_pattern_and_values_bit_exact_vs_reference():
+    g = golden("csr_cases.npz")
+    for i in range(int(g["ncases"])):
+        nel = tuple(int(v) for v in g[f"nel_{i}"])
+        dim = 3 if nel[2] > 0 else 2
+        K0 = fe.element_stiffness(dim, plane=str(g[f"plane_{i}"]))
+        asm = fe.Assembly(*nel, K0, bc=g[f"bc_{i}"])
+        npt.assert_array_equal(asm.indptr, g[f"indptr_{i}"].astype(np.int64))
+        npt.assert_array_equal(asm.indices, g[f"indices_{i}"].astype(np.int64))
+        data = asm.values(g[f"x_{i}"])
+        assert data.tobytes() == g[f"data_{i}"].tobytes()
+
+
+def test_csr_c1_hashes():
+    g = golden("c1_mbb.npz")
+    K0 = fe.element_stiffness(2, plane="stress")
+    npt.assert_array_equal(K0, g["K0"])
+    asm = fe.Assembly(120, 40, 0, K0, bc=g["bc"])
+    assert sha(asm.indptr) == str(g["sha_indptr"])
+    assert sha(asm.indices) == str(g["sha_indices"])
+    assert sha(asm.values(g["xsimp"])) == str(g["sha_data"])
+    assert str(g["index_dtype"]) == "int32"
+
+
+def test_csr_brute_force_dense():
+    nel = (3, 2, 2)
+    K0 = fe.element_stiffness(3)
+    x = np.random.default_rng(0).random(12)
+    asm = fe.Assembly(*nel, K0)
+    dc = fe.dofconn(*nel)
+    Kd = np.zeros((asm.n, asm.n))
+    for e in range(12):
+        Kd[np.ix_(dc[e], dc[e])] += x[e] * K0
+    npt.assert_allclose(asm.matrix(x).toarray(), Kd, rtol=1e-15, atol=1e-15)
+    coo = sps.coo_matrix((np.ones(dc.size * 24), (np.repeat(dc, 24, axis=1).ravel(), np.tile(dc, (1, 24)).ravel())),
+                         shape=(asm.n, asm.n)).tocsr()
+    coo.sum_duplicates()
+    coo.sort_indices()
+    npt.assert_array_equal(coo.indptr, asm.indptr)
+    npt.assert_array_equal(coo.indices, asm.indices)
+
+
+# ---------------------------------------------------------------- filter
+def _bcs_from(g, i):
+    raw = [str(s) for s in g[f"case{i}_bcs"]]
+    conv = [s if s in ("symmetric", "edge", "wrap") else float(s) for s in raw]
+    return ((conv[0], conv[1]), (conv[2], conv[3]), (conv[4], conv[5]))
+
+
+def test_filter_vs_reference_golden():
+    g = golden("filter_cases.npz")
+    for i in range(int(g["ncases"])):
+        nel = tuple(int(v) for v in g[f"case{i}_nel"])
+        r = float(g[f"case{i}_radius"])
+        w = g[f"case{i}_weights"]
+        if np.isfinite(r):
+            npt.assert_allclose(filter_weights(r, nel), w, rtol=1e-15)
+        flt = ConvFilter(nel, w, _bcs_from(g, i))
+        npt.assert_allclose(flt.forward(g[f"case{i}_x"]), g[f"case{i}_y"], rtol=1e-12, atol=1e-14)
+        npt.assert_allclose(flt.adjoint(g[f"case{i}_dy"]), g[f"case{i}_dx"], rtol=1e-12, atol=1e-14)
+
+
+def test_filter_known_answers_2d_interior_and_xmin_mirror():
+    """reference tests/test_filter.py:13-66."""
+    nel = (10, 12, 0)
+    w = filter_weights(2.0, nel)
+    e = lambda i, j: j * 10 + i  # noqa: E731
+    x = np.zeros(120)
+    x[e(5, 6)] = 1.0
+    y = ConvFilter(nel, w).forward(x)
+    npt.assert_allclose(y[e(5, 6)], w[1, 1, 0])
+    npt.assert_allclose(y[e(6, 7)], w[0, 0, 0])
+    x = np.zeros(120)
+    x[e(0, 6)] = 1.0
+    y = ConvFilter(nel, w).forward(x)
+    npt.assert_allclose(y[e(0, 6)], w[1, 1, 0] + w[2, 1, 0])
+    npt.assert_allclose(y[e(0, 5)], w[1, 2, 0] + w[2, 2, 0])
+    y = ConvFilter(nel, w, ((0.0, "symmetric"), ("symmetric",) * 2, ("symmetric",) * 2)).forward(x)
+    npt.assert_allclose(y[e(0, 6)], w[1, 1, 0])
+
+
+def test_filter_adjoint_identity_all_bcs():
+    rng = np.random.default_rng(3)
+    nel = (7, 6, 5)
+    w = rng.random((3, 5, 3))
+    flt = ConvFilter(nel, w, (("edge", 0.3), ("wrap", "wrap"), (1.0, "symmetric")))
+    # constant padding is affine: check adjointness of the linear part
+    x, y = rng.random(210), rng.random(210)
+    lin = flt.forward(x) - flt.forward(np.zeros(210))
+    npt.assert_allclose(lin @ y, x @ flt.adjoint(y), rtol=1e-12)
+
+
+# ---------------------------------------------------------------- multigrid transfers
+def test_restriction_sums_2d_3d():
+    """reference tests/test_solvers_multigrid.py:9-91."""
+    R = interpolation(10, 10, 0, 2)
+    uc = (R.T @ np.ones(R.shape[0])).reshape(6, 6, 2)
+    npt.assert_allclose(uc[1:-1, 1:-1], 4)
+    npt.assert_allclose(uc[0, 1:-1], 3)
+    npt.assert_allclose(uc[0, 0], 2.25)
+    npt.assert_allclose(R @ np.ones(R.shape[1]), 1.0)
+    R = interpolation(10, 10, 10, 3)
+    uc = (R.T @ np.ones(R.shape[0])).reshape(6, 6, 6, 3)
+    npt.assert_allclose(uc[1:-1, 1:-1, 1:-1], 8)
+    npt.assert_allclose(uc[0, 1:-1, 1:-1], 6)
+    npt.assert_allclose(uc[0, 0, 1:-1], 4.5)
+    npt.assert_allclose(uc[0, 0, 0], 3.375)
+    npt.assert_allclose(R @ np.ones(R.shape[1]), 1.0)
+
+
+def test_mgcg_matches_spsolve():
+    nel = (16, 8, 8)
+    bc, f = problems.cantilever3d(*nel)
+    x = np.random.default_rng(0).uniform(0.1, 1.0, 16 * 64)
+    asm = fe.Assembly(*nel, fe.element_stiffness(3), bc=bc)
+    K = asm.matrix(fe.simp(x))
+    cg = CG(preconditioner=build_mg(nel, 3, omega=0.6, smooth_steps=2), tol=1e-12)
+    cg.update(K)
+    u = cg.solve(f)
+    npt.assert_allclose(u, spsolve(K.tocsc(), f), rtol=1e-8, atol=1e-10 * np.abs(u).max())
+
+
+# ---------------------------------------------------------------- end-to-end vs the reference
+def test_pipeline_c1_mbb_vs_reference():
+    g = golden("c1_mbb.npz")
+    bc, f = problems.mbb2d(120, 40)
+    npt.assert_array_equal(bc, g["bc"])
+    npt.assert_array_equal(f, g["f"])
+    out = design_iteration((120, 40, 0), g["x"], bc, f, radius=2.0, plane="stress")
+    npt.assert_allclose(out["xf"], g["xf"], rtol=1e-13)
+    npt.assert_allclose(out["xsimp"], g["xsimp"], rtol=1e-13)
+    npt.assert_allclose(out["u"], g["u"], rtol=1e-9, atol=1e-12 * np.abs(g["u"]).max())
+    npt.assert_allclose(out["c"], float(g["c"]), rtol=1e-12)
+    npt.assert_allclose(out["dcdx"], g["dcdx"], rtol=1e-9, atol=1e-12 * np.abs(g["dcdx"]).max())
+
+
+@pytest.mark.parametrize("name", ["c2_direct_16x8x8.npz", "c2_direct_24x12x12.npz"])
+def test_pipeline_c2_direct_vs_reference(name):
+    g = golden(name)
+    nel = tuple(int(v) for v in g["nel"])
+    bc, f = problems.cantilever3d(*nel)
+    npt.assert_array_equal(bc, g["bc"])
+    npt.assert_array_equal(f, g["f"])
+    out = design_iteration(nel, g["x"], bc, f, radius=1.5)
+    npt.assert_allclose(out["xf"], g["xf"], rtol=1e-13)
+    npt.assert_allclose(out["c"], float(g["c"]), rtol=1e-12)
+    npt.assert_allclose(out["dcdx"], g["dcdx"], rtol=1e-9, atol=1e-12 * np.abs(g["dcdx"]).max())
+
+
+@pytest.mark.parametrize("levels", [3, 4])
+def test_pipeline_c2_mgcg_vs_reference(levels):
+    g = golden(f"c2_mgcg_32x16x16_L{levels}.npz")
+    nel = tuple(int(v) for v in g["nel"])
+    bc, f = problems.cantilever3d(*nel)
+    out = design_iteration(nel, g["x"], bc, f, radius=1.5, solver="mgcg", mg_levels=levels,
+                           omega=float(g["omega"]), smooth_steps=int(g["steps"]), tol=float(g["tol"]))
+    assert out["iterations"] == int(g["iterations"])
+    # the golden history was parsed from the reference's printed residuals (about 6 significant digits)
+    npt.assert_allclose(out["history"], g["history"], rtol=1e-3)
+    assert out["adjoint_solved"] == bool(g["adjoint_solved"]) is False
+    npt.assert_allclose(out["c"], float(g["c"]), rtol=1e-11)
+    npt.assert_allclose(out["dcdx"], g["dcdx"], rtol=1e-8, atol=1e-11 * np.abs(g["dcdx"]).max())
