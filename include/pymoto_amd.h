@@ -1,0 +1,151 @@
+/*
+ * C-ABI of the MI355X (gfx950) FE design-iteration hot path.
+ *
+ * Every entry point takes plain device pointers (allocated by the caller, e.g. torch tensors),
+ * sizes and a hipStream_t passed as void*; it returns PMT_OK (0) or a negative status and
+ * leaves a message for pmt_last_error().  All floating point is IEEE binary64 unless the name
+ * says otherwise.  Grid arguments are element counts (nelz = 0 for a 2D Q4 grid); numbering is
+ * pyMOTO's: element (k*nely+j)*nelx+i, node (k*(nely+1)+j)*(nelx+1)+i, dof node*D+d
+ * (pymoto/common/domain.py:200-251).  Boundary conditions are a per-node byte mask whose bit d
+ * marks dof d as Dirichlet (pymoto/modules/assembly.py:86-98).
+ *
+ * No entry point allocates device memory except pmt_mg_create/pmt_mg_update (setup).
+ * Calls on one pmt_mg handle are not thread-safe.
+ */
+#ifndef PYMOTO_AMD_H
+#define PYMOTO_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PMT_OK 0
+#define PMT_ERR_ARG (-1)
+#define PMT_ERR_HIP (-2)
+#define PMT_ERR_STATE (-3)
+
+/* Message of the last failing call on this thread. */
+const char* pmt_last_error(void);
+/* ABI revision of this header. */
+int pmt_abi_version(void);
+
+/* ---------------------------------------------------------------------------------------------
+ * FilterConv forward: y = W * xpad (scipy.signal.convolve 'valid' of the padded field).
+ * Replaces FilterConv.__call__ (pymoto/modules/filter.py:207-212) and get_padded_vector (183-187).
+ * w: kx*ky*kz weights in [x][y][z] order.  map/cval: per-axis padded-index maps concatenated
+ * x|y|z, lengths (nelx+kx-1), (nely+ky-1), (max(nelz,1)+kz-1); map<0 marks a constant-padded
+ * cell whose value is cval (z overrides y overrides x, filter.py:98-160).
+ */
+int pmt_filter_fwd(int nelx, int nely, int nelz, int kx, int ky, int kz, const double* w,
+                   const int32_t* map, const double* cval, const double* x, double* y, void* stream);
+/* FilterConv adjoint: correlate 'full', drop constant cells, fold back onto elements.
+ * Replaces FilterConv._sensitivity (filter.py:214-220).  inv_ptr/inv_idx: per-axis lists of the
+ * padded coordinates mapping to each element coordinate, concatenated x|y|z (inv_ptr holds
+ * nelx+1, nely+1, max(nelz,1)+1 entries, absolute offsets into inv_idx). */
+int pmt_filter_bwd(int nelx, int nely, int nelz, int kx, int ky, int kz, const double* w,
+                   const int32_t* inv_ptr, const int32_t* inv_idx, const double* dy, double* dx,
+                   void* stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * Power-law material interpolation y = a + b*x^p (SIMP, MathExpression "xmin + (1-xmin)*inp0^3",
+ * pymoto/modules/generic.py:94-99) and its adjoint dx (+)= dy*c*x^q (generic.py:101-140).
+ */
+int pmt_powlaw_fwd(long long n, double a, double b, double p, const double* x, double* y, void* stream);
+int pmt_powlaw_bwd(long long n, double c, double q, const double* x, const double* dy, double* dx,
+                   int accumulate, void* stream);
+
+/* Vector kernels.  y = a*x + b*y (x may be NULL: y = b*y). */
+int pmt_axpby(long long n, double a, const double* x, double b, double* y, void* stream);
+/* *out = x.y, deterministic two-stage reduction; work holds pmt_reduce_work_size() doubles.
+ * Replaces numpy einsum('i,i->') of EinSum (generic.py:193-194) and the CG dots (iterative.py:377-394). */
+int pmt_dot(long long n, const double* x, const double* y, double* out, double* work, void* stream);
+int pmt_reduce_work_size(void);
+/* Dirichlet split used by the linear-dependency-aware solve (pymoto/solvers/solvers.py:174-176):
+ * for bc dofs sol = rhs/bcdiag and rhs_loc = 0, elsewhere rhs_loc = rhs and sol = 0. */
+int pmt_bc_split(int nelx, int nely, int nelz, const uint8_t* bcmask, double bcdiag, const double* rhs,
+                 double* rhs_loc, double* sol, void* stream);
+/* y += a*x on free (non-bc) dofs only (solvers.py:233-236). */
+int pmt_axpy_free(int nelx, int nely, int nelz, const uint8_t* bcmask, double a, const double* x,
+                  double* y, void* stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * Matrix-free stiffness operator K(x) = sum_e x_e K0 with Dirichlet rows/cols zeroed and bcdiag on
+ * their diagonal.  Replaces the assembled CSR of AssembleGeneral.__call__ (assembly.py:245-296)
+ * and its SpMV `A @ p` (pymoto/solvers/iterative.py:374-395).  K0: (2^D*D)^2 element matrix.
+ */
+int pmt_stiffness_apply(int nelx, int nely, int nelz, const double* K0, const double* xe,
+                        const uint8_t* bcmask, double bcdiag, const double* u, double* y, void* stream);
+/* r = b - K u (LinearSolver.residual / CG restart, solvers.py:52-85, iterative.py:384) */
+int pmt_stiffness_residual(int nelx, int nely, int nelz, const double* K0, const double* xe,
+                           const uint8_t* bcmask, double bcdiag, const double* b, const double* u,
+                           double* r, void* stream);
+/* diag(K) (DampedJacobi.update, iterative.py:38-39) */
+int pmt_stiffness_diagonal(int nelx, int nely, int nelz, const double* K0, const double* xe,
+                           const uint8_t* bcmask, double bcdiag, double* diag, void* stream);
+
+/* Sensitivity dx_e (+)= alpha * u_e^T K0 v_e with Dirichlet entries of u, v zeroed.
+ * Replaces AssembleGeneral._sensitivity + DyadicMatrix.contract
+ * (assembly.py:298-315, pymoto/common/dyadcarrier.py:392-414). */
+int pmt_sens_contract(int nelx, int nely, int nelz, const double* K0, const uint8_t* bcmask,
+                      const double* u, const double* v, double alpha, double* dx, int accumulate,
+                      void* stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * Assembled CSR (the reference's output format).  Pattern: per row the sorted union of the dofs
+ * of every element touching the row's node, Dirichlet rows/cols kept as explicit zeros
+ * (assembly.py:100-230).  index_bytes = 4 (int32, what scipy stores) or 8 (int64).
+ */
+long long pmt_csr_nnz(int nelx, int nely, int nelz, int ndof);
+int pmt_csr_pattern(int nelx, int nely, int nelz, int ndof, int index_bytes, void* indptr, void* indices,
+                    void* stream);
+/* Values bit-identical to np.add.at (assembly.py:257-272): per slot, products x_e*K0_ij rounded
+ * and summed in ascending element order; bc slots 0 except bcdiag on the diagonal. */
+int pmt_csr_values(int nelx, int nely, int nelz, const double* K0, const double* xe,
+                   const uint8_t* bcmask, double bcdiag, int index_bytes, const void* indptr,
+                   double* data, void* stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * Geometric multigrid transfers with the trilinear (bilinear) weights 1, 1/2, 1/4, 1/8 of
+ * GeometricMultigrid.setup_interpolation (iterative.py:178-220); grids given by the FINE element
+ * counts (all even).  rc = R^T rf ; uf += R uc.
+ */
+int pmt_restrict(int nelx, int nely, int nelz, const double* rf, double* rc, void* stream);
+int pmt_prolong_add(int nelx, int nely, int nelz, const double* uc, double* uf, void* stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * MG-preconditioned CG runtime.  Replaces CG(preconditioner=GeometricMultigrid(...)) with nested
+ * GeometricMultigrid levels, DampedJacobi smoothers and a direct coarsest solve
+ * (iterative.py:21-47, 124-256, 295-403).  nlevels counts grids (fine grid included); level l
+ * (l < nlevels-1) smooths with omega[l], steps[l] sweeps; the coarsest grid is solved directly.
+ * Coarse operators are Galerkin R^T A R (iterative.py:173) built element by element.
+ */
+typedef struct pmt_mg pmt_mg;
+int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega, const int* steps,
+                  const double* K0_host, pmt_mg** out);
+int pmt_mg_destroy(pmt_mg* mg);
+/* Rebuild diagonals, Galerkin coarse operators and the coarsest inverse for new densities xe
+ * (GeometricMultigrid.update, iterative.py:168-176).  xe and bcmask must stay alive until the
+ * next update. */
+int pmt_mg_update(pmt_mg* mg, const double* xe, const uint8_t* bcmask, double bcdiag, void* stream);
+/* z = M r, one V-cycle (GeometricMultigrid.solve with x0=None, iterative.py:222-256). */
+int pmt_mg_vcycle(pmt_mg* mg, const double* r, double* z, void* stream);
+/* y = A_l u on level l (A_0 is the fine operator) - exposes the Galerkin operators. */
+int pmt_mg_level_apply(pmt_mg* mg, int level, const double* u, double* y, void* stream);
+/* Number of dofs of level l, or a negative status. */
+long long pmt_mg_level_ndof(pmt_mg* mg, int level);
+/* Bytes of device memory held by the handle. */
+long long pmt_mg_device_bytes(pmt_mg* mg);
+/* PCG with the V-cycle preconditioner, CG.solve semantics (iterative.py:340-403): x holds x0 on
+ * entry unless x0_is_zero; stops when ||r||/||b|| <= tol; explicit residual every `restart`
+ * iterations including the first.  iters/relres receive the iteration count and final relative
+ * residual; history (host, maxit+1 doubles, may be NULL) the relative residual per iteration. */
+int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double tol, int maxit,
+                  int restart, int* iters, double* relres, double* history, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYMOTO_AMD_H */

@@ -1,0 +1,123 @@
+"""ctypes binding of the C-ABI in ``include/pymoto_amd.h`` (library ``pymoto_amd/lib/libpymoto_amd.so``).
+
+The library is the only compute path of this package: if it is missing, or no GPU is visible,
+every device module raises - there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+import torch
+
+_LIBPATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libpymoto_amd.so")
+_lib = None
+
+c_int, c_ll, c_dbl, c_vp = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
+c_char_p = ctypes.c_char_p
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "pmt_last_error": (c_char_p, []),
+    "pmt_abi_version": (c_int, []),
+    "pmt_filter_fwd": (c_int, [c_int] * 6 + [c_vp] * 5 + [c_vp]),
+    "pmt_filter_bwd": (c_int, [c_int] * 6 + [c_vp] * 5 + [c_vp]),
+    "pmt_powlaw_fwd": (c_int, [c_ll, c_dbl, c_dbl, c_dbl, c_vp, c_vp, c_vp]),
+    "pmt_powlaw_bwd": (c_int, [c_ll, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_int, c_vp]),
+    "pmt_axpby": (c_int, [c_ll, c_dbl, c_vp, c_dbl, c_vp, c_vp]),
+    "pmt_dot": (c_int, [c_ll, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "pmt_reduce_work_size": (c_int, []),
+    "pmt_bc_split": (c_int, [c_int] * 3 + [c_vp, c_dbl, c_vp, c_vp, c_vp, c_vp]),
+    "pmt_axpy_free": (c_int, [c_int] * 3 + [c_vp, c_dbl, c_vp, c_vp, c_vp]),
+    "pmt_stiffness_apply": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp, c_dbl, c_vp, c_vp, c_vp]),
+    "pmt_stiffness_residual": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp, c_dbl, c_vp, c_vp, c_vp, c_vp]),
+    "pmt_stiffness_diagonal": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp, c_dbl, c_vp, c_vp]),
+    "pmt_sens_contract": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp, c_int, c_vp]),
+    "pmt_csr_nnz": (c_ll, [c_int] * 4),
+    "pmt_csr_pattern": (c_int, [c_int] * 5 + [c_vp, c_vp, c_vp]),
+    "pmt_csr_values": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp, c_dbl, c_int, c_vp, c_vp, c_vp]),
+    "pmt_restrict": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp]),
+    "pmt_prolong_add": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp]),
+    "pmt_mg_create": (c_int, [c_int] * 4 + [c_vp, c_vp, c_vp, c_vp]),
+    "pmt_mg_destroy": (c_int, [c_vp]),
+    "pmt_mg_update": (c_int, [c_vp, c_vp, c_vp, c_dbl, c_vp]),
+    "pmt_mg_vcycle": (c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "pmt_mg_level_apply": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp]),
+    "pmt_mg_level_ndof": (c_ll, [c_vp, c_int]),
+    "pmt_mg_device_bytes": (c_ll, [c_vp]),
+    "pmt_pcg_solve": (c_int, [c_vp, c_vp, c_vp, c_int, c_dbl, c_int, c_int, c_vp, c_vp, c_vp, c_vp]),
+}
+
+
+class HIPError(RuntimeError):
+    pass
+
+
+def load(path: str = _LIBPATH):
+    """Load the shared library and declare all signatures (no GPU needed)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise HIPError(f"HIP library not built: {path} is missing (run `python -m pymoto_amd.build`)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def lib():
+    """The library, after checking that a GPU is visible (device modules call this)."""
+    if not torch.cuda.is_available():
+        raise HIPError("pymoto_amd needs a ROCm GPU (gfx950); none is visible and there is no CPU fallback")
+    return load()
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        msg = load().pmt_last_error().decode(errors="replace")
+        raise HIPError(f"{what or 'HIP call'} failed with status {rc}: {msg}")
+
+
+def stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def device():
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def as_device(x, dtype=torch.float64):
+    """numpy / scalar / tensor -> contiguous device tensor of `dtype` (host data is copied once)."""
+    if isinstance(x, torch.Tensor):
+        t = x if x.is_cuda else x.to(device())
+        if t.dtype != dtype:
+            t = t.to(dtype)
+        return t.contiguous()
+    arr = np.ascontiguousarray(np.asarray(x), dtype=torch.empty(0, dtype=dtype).numpy().dtype)
+    return torch.from_numpy(arr).to(device())
+
+
+_work = {}
+
+
+def reduce_work():
+    dev = torch.cuda.current_device()
+    if dev not in _work:
+        _work[dev] = torch.empty(load().pmt_reduce_work_size(), dtype=torch.float64, device=device())
+    return _work[dev]

@@ -1,0 +1,402 @@
+// Coarse-grid operators of the geometric multigrid (pymoto/solvers/iterative.py:168-176).
+//
+// Galerkin R^T A R element by element: every fine element c lies in one coarse element E and
+// the trilinear prolongation restricted to c only reads E's nodes, u_c = P_c u_E with
+// P_c = p_c (x) I_D, p_c[a][A] = prod_dims w(f, A_bit), f = child_bit + a_bit in {0,1,2},
+// w(0,0) = w(2,1) = 1, w(1,*) = 1/2, else 0.  Hence R^T A R = assemble_E sum_c P_c^T A_c P_c.
+// On the fine level A_c = x_c M_c K0 M_c + D_c with M_c the Dirichlet mask and D_c the bcdiag
+// diagonal of Dirichlet dofs shared equally by the mult(node) elements around each node, so
+// that the element sum reproduces A = P K P + bcdiag I_bc of the reference assembly
+// (assembly.py:209-230, 267-272).  On coarser levels A_c is the stored element matrix of the
+// child.  The coarse element matrices are then summed per node into a 3^D block stencil.
+//
+// Coarsest level: dense matrix from the stencil, explicit inverse by blocked Gauss-Jordan
+// (no pivoting; the Galerkin operator is SPD), applied as a dense GEMV in every V-cycle -
+// the direct solve of auto_determine_solver (iterative.py:174-176).
+#include "launch.h"
+
+namespace pmt {
+
+__device__ inline double interp_w(int f, int Abit) {
+  return f == 1 ? 0.5 : ((f == 0) == (Abit == 0) ? 1.0 : 0.0);
+}
+
+template <int DIM>
+__device__ inline double pweight(int c, int a, int A) {
+  double w = 1.0;
+#pragma unroll
+  for (int b = 0; b < DIM; ++b) w *= interp_w(((c >> b) & 1) + ((a >> b) & 1), (A >> b) & 1);
+  return w;
+}
+
+// One block per coarse element; thread t = (A, B) node pair owns the D x D block (A, B) of K_E.
+// FINE: A_c built from x_c, K0, the Dirichlet mask and bcdiag/mult; else read from Kf.
+template <int DIM, bool FINE>
+__global__ void __launch_bounds__(64)
+galerkin_kernel(GridDims gf, GridDims gc, const double* __restrict__ K0, const double* __restrict__ xe,
+                const uint8_t* __restrict__ bcm, double bcdiag, const double* __restrict__ Kf,
+                double* __restrict__ Kc) {
+  constexpr int NN = 1 << DIM;
+  constexpr int NE = NN * DIM;
+  __shared__ double sA[NE * NE];
+  __shared__ double sT[NE * NE];
+  __shared__ double sK0[FINE ? NE * NE : 1];
+  const long long E = blockIdx.x;
+  const int tid = threadIdx.x;  // NN*NN threads
+  const int CI = (int)(E % gc.ex);
+  const long long tt = E / gc.ex;
+  const int CJ = (int)(tt % gc.ey);
+  const int CK = (int)(tt / gc.ey);
+  if (FINE) {
+    for (int q = tid; q < NE * NE; q += NN * NN) sK0[q] = K0[q];
+  }
+  const int tA = tid / NN, tB = tid % NN;
+  double acc[DIM][DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d)
+#pragma unroll
+    for (int d2 = 0; d2 < DIM; ++d2) acc[d][d2] = 0.0;
+
+  for (int c = 0; c < NN; ++c) {
+    const int fi = 2 * CI + (c & 1), fj = 2 * CJ + ((c >> 1) & 1), fk = DIM == 3 ? 2 * CK + ((c >> 2) & 1) : 0;
+    const long long fe = ((long long)fk * gf.ey + fj) * gf.ex + fi;
+    __syncthreads();  // previous child's sA/sT fully consumed (and sK0 loaded)
+    if (FINE) {
+      const double xc = xe[fe];
+      for (int q = tid; q < NE * NE; q += NN * NN) {
+        const int r = q / NE, s = q % NE;
+        const int a = r / DIM, d = r % DIM, b = s / DIM, d2 = s % DIM;
+        const int ai = fi + (a & 1), aj = fj + ((a >> 1) & 1), ak = DIM == 3 ? fk + ((a >> 2) & 1) : 0;
+        const int bi = fi + (b & 1), bj = fj + ((b >> 1) & 1), bk = DIM == 3 ? fk + ((b >> 2) & 1) : 0;
+        const long long na = ((long long)ak * gf.ny + aj) * gf.nx + ai;
+        const long long nb = ((long long)bk * gf.ny + bj) * gf.nx + bi;
+        const bool fa = bcm && ((bcm[na] >> d) & 1u);
+        const bool fb = bcm && ((bcm[nb] >> d2) & 1u);
+        double v = (fa || fb) ? 0.0 : xc * sK0[q];
+        if (r == s && fa) {
+          const int mult = ((ai > 0) + (ai < gf.nx - 1)) * ((aj > 0) + (aj < gf.ny - 1)) *
+                           (DIM == 3 ? ((ak > 0) + (ak < gf.nz - 1)) : 1);
+          v += bcdiag / (double)mult;
+        }
+        sA[q] = v;
+      }
+    } else {
+      for (int q = tid; q < NE * NE; q += NN * NN) sA[q] = Kf[fe * NE * NE + q];
+    }
+    __syncthreads();
+    // T = A_c P_c : thread (a = tA, B = tB) computes rows a*D+d, cols B*D+d2
+    {
+      const int a = tA, B = tB;
+      double t[DIM][DIM];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+#pragma unroll
+        for (int d2 = 0; d2 < DIM; ++d2) t[d][d2] = 0.0;
+      for (int b = 0; b < NN; ++b) {
+        const double w = pweight<DIM>(c, b, B);
+        if (w == 0.0) continue;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+#pragma unroll
+          for (int d2 = 0; d2 < DIM; ++d2) t[d][d2] = fma(w, sA[(a * DIM + d) * NE + b * DIM + d2], t[d][d2]);
+      }
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+#pragma unroll
+        for (int d2 = 0; d2 < DIM; ++d2) sT[(a * DIM + d) * NE + B * DIM + d2] = t[d][d2];
+    }
+    __syncthreads();
+    // K_E(A, B) += sum_a p_c[a][A] T(a, B)
+    for (int a = 0; a < NN; ++a) {
+      const double w = pweight<DIM>(c, a, tA);
+      if (w == 0.0) continue;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+#pragma unroll
+        for (int d2 = 0; d2 < DIM; ++d2) acc[d][d2] = fma(w, sT[(a * DIM + d) * NE + tB * DIM + d2], acc[d][d2]);
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; ++d)
+#pragma unroll
+    for (int d2 = 0; d2 < DIM; ++d2) Kc[E * NE * NE + (tA * DIM + d) * NE + tB * DIM + d2] = acc[d][d2];
+}
+
+// Per-node block stencil S[(o*D+d)*D+d2][n] = sum over elements E holding n and n+o of K_E(loc n, loc n+o)
+template <int DIM>
+__global__ void stencil_from_elements_kernel(GridDims g, const double* __restrict__ Ke, double* __restrict__ S) {
+  constexpr int NN = 1 << DIM;
+  constexpr int NE = NN * DIM;
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.nnodes) return;
+  const int i = (int)(n % g.nx);
+  const long long t = n / g.nx;
+  const int j = (int)(t % g.ny);
+  const int k = (int)(t / g.ny);
+  const long long nn = g.nnodes;
+  constexpr int ZLO = DIM == 3 ? -1 : 0, ZHI = DIM == 3 ? 1 : 0;
+  int o = 0;
+  for (int dz = ZLO; dz <= ZHI; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx, ++o) {
+        double blk[DIM][DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+#pragma unroll
+          for (int d2 = 0; d2 < DIM; ++d2) blk[d][d2] = 0.0;
+        for (int e = 0; e < NN; ++e) {
+          const int ex = e & 1, ey = (e >> 1) & 1, ez = DIM == 3 ? (e >> 2) & 1 : 1;
+          const int bx = dx + 1 - ex, by = dy + 1 - ey, bz = DIM == 3 ? dz + 1 - ez : 0;
+          if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
+          const int ei = i - 1 + ex, ej = j - 1 + ey, ek = DIM == 3 ? k - 1 + ez : 0;
+          if (ei < 0 || ei >= g.ex || ej < 0 || ej >= g.ey || ek < 0 || ek >= g.ez) continue;
+          const long long E = ((long long)ek * g.ey + ej) * g.ex + ei;
+          const int locn = (1 - ex) + 2 * (1 - ey) + (DIM == 3 ? 4 * (1 - ez) : 0);
+          const int locm = bx + 2 * by + 4 * bz;
+          const double* KE = Ke + E * NE * NE;
+#pragma unroll
+          for (int d = 0; d < DIM; ++d)
+#pragma unroll
+            for (int d2 = 0; d2 < DIM; ++d2) blk[d][d2] += KE[(locn * DIM + d) * NE + locm * DIM + d2];
+        }
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+#pragma unroll
+          for (int d2 = 0; d2 < DIM; ++d2) S[((long long)(o * DIM + d) * DIM + d2) * nn + n] = blk[d][d2];
+      }
+}
+
+// dense row-major A (n x n, n = nnodes*D) from the stencil; A must be zero-filled first
+template <int DIM>
+__global__ void dense_from_stencil_kernel(GridDims g, const double* __restrict__ S, double* __restrict__ A) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.nnodes) return;
+  const int i = (int)(n % g.nx);
+  const long long t = n / g.nx;
+  const int j = (int)(t % g.ny);
+  const int k = (int)(t / g.ny);
+  const long long N = g.nnodes * DIM;
+  constexpr int ZLO = DIM == 3 ? -1 : 0, ZHI = DIM == 3 ? 1 : 0;
+  int o = 0;
+  for (int dz = ZLO; dz <= ZHI; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx, ++o) {
+        const int ii = i + dx, jj = j + dy, kk = k + dz;
+        if (ii < 0 || ii >= g.nx || jj < 0 || jj >= g.ny || kk < 0 || kk >= g.nz) continue;
+        const long long m = ((long long)kk * g.ny + jj) * g.nx + ii;
+        for (int d = 0; d < DIM; ++d)
+          for (int d2 = 0; d2 < DIM; ++d2)
+            A[(n * DIM + d) * N + m * DIM + d2] = S[((long long)(o * DIM + d) * DIM + d2) * g.nnodes + n];
+      }
+}
+
+// ---------------- blocked Gauss-Jordan inversion (in place), block size GJB ----------------
+constexpr int GJB = 64;
+
+// invert the kk x kk diagonal block at (k0, k0) in LDS, write it to P (GJB x GJB, row-major)
+__global__ void __launch_bounds__(256) gj_diag_kernel(long long n, long long k0, int kk, const double* __restrict__ A,
+                                                       double* __restrict__ P) {
+  __shared__ double sM[GJB][GJB + 1];
+  const int tid = threadIdx.x;
+  for (int q = tid; q < kk * kk; q += blockDim.x) {
+    const int r = q / kk, c = q % kk;
+    sM[r][c] = A[(k0 + r) * n + k0 + c];
+  }
+  __syncthreads();
+  for (int p = 0; p < kk; ++p) {
+    const double piv = sM[p][p];
+    __syncthreads();
+    // scale pivot row (the pivot entry becomes 1/piv)
+    for (int c = tid; c < kk; c += blockDim.x) sM[p][c] = (c == p) ? 1.0 / piv : sM[p][c] / piv;
+    __syncthreads();
+    // eliminate column p from the other rows
+    for (int q = tid; q < kk * kk; q += blockDim.x) {
+      const int r = q / kk, c = q % kk;
+      if (r == p) continue;
+      const double f = sM[r][p];
+      if (c == p)
+        continue;
+      sM[r][c] -= f * sM[p][c];
+    }
+    __syncthreads();
+    for (int r = tid; r < kk; r += blockDim.x)
+      if (r != p) sM[r][p] = -sM[r][p] * sM[p][p];
+    __syncthreads();
+  }
+  for (int q = tid; q < GJB * GJB; q += blockDim.x) {  // zero outside kk x kk: consumers read full tiles
+    const int r = q / GJB, c = q % GJB;
+    P[q] = (r < kk && c < kk) ? sM[r][c] : 0.0;
+  }
+}
+
+// C[i][c] = A[i][k0+c] (old column panel), all rows i
+__global__ void gj_colcopy_kernel(long long n, long long k0, int kk, const double* __restrict__ A,
+                                  double* __restrict__ C) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n * kk) return;
+  const long long i = q / kk;
+  const int c = (int)(q % kk);
+  C[i * GJB + c] = A[i * n + k0 + c];
+}
+
+// row panel A[K][j] = P A[K][j] for j outside K (one thread per column)
+__global__ void __launch_bounds__(256) gj_rowpanel_kernel(long long n, long long k0, int kk, double* __restrict__ A,
+                                                           const double* __restrict__ P) {
+  __shared__ double sP[GJB * GJB];
+  for (int q = threadIdx.x; q < GJB * GJB; q += blockDim.x) sP[q] = P[q];
+  __syncthreads();
+  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n || (j >= k0 && j < k0 + kk)) return;
+  double col[GJB];
+#pragma unroll
+  for (int r = 0; r < GJB; ++r) col[r] = r < kk ? A[(k0 + r) * n + j] : 0.0;
+  for (int r = 0; r < kk; ++r) {
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < GJB; ++c) s = fma(sP[r * GJB + c], col[c], s);
+    A[(k0 + r) * n + j] = s;
+  }
+}
+
+// rank-kk update of all tiles outside the pivot row block:
+//   j outside K: A[i][j] -= C[i][:] A[K][j] ; j inside K: A[i][K] = -C[i][:] P ; pivot block = P
+__global__ void __launch_bounds__(256) gj_update_kernel(long long n, long long k0, int kk, double* __restrict__ A,
+                                                         const double* __restrict__ C, const double* __restrict__ P) {
+  __shared__ double sC[64][GJB + 1];
+  __shared__ double sR[GJB][64 + 1];
+  const long long i0 = (long long)blockIdx.y * 64, j0 = (long long)blockIdx.x * 64;
+  const int tid = threadIdx.x;
+  const bool rowK = (i0 == k0);  // tiles are aligned to GJB, so the pivot row block is one tile row
+  const bool colK = (j0 == k0);
+  if (rowK && !colK) return;  // already holds the new row panel
+  if (rowK && colK) {
+    for (int q = tid; q < kk * kk; q += blockDim.x) {
+      const int r = q / kk, c = q % kk;
+      A[(k0 + r) * n + k0 + c] = P[r * GJB + c];
+    }
+    return;
+  }
+  // stage C tile (64 x kk) and R tile (kk x 64): R = new row panel, or P for the pivot columns
+  for (int q = tid; q < 64 * GJB; q += blockDim.x) {
+    const int r = q / GJB, c = q % GJB;
+    const long long i = i0 + r;
+    sC[r][c] = (i < n && c < kk) ? C[i * GJB + c] : 0.0;
+  }
+  for (int q = tid; q < GJB * 64; q += blockDim.x) {
+    const int r = q / 64, c = q % 64;
+    const long long j = j0 + c;
+    double v = 0.0;
+    if (r < kk && j < n) v = colK ? (c < kk ? P[r * GJB + c] : 0.0) : A[(k0 + r) * n + j];
+    sR[r][c] = v;
+  }
+  __syncthreads();
+  const int tr = (tid / 16) * 4, tc = (tid % 16) * 4;
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+  for (int c = 0; c < kk; ++c) {
+    double cv[4], rv[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) cv[a] = sC[tr + a][c];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) rv[b] = sR[c][tc + b];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = fma(cv[a], rv[b], acc[a][b]);
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const long long i = i0 + tr + a;
+    if (i >= n) continue;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const long long j = j0 + tc + b;
+      if (j >= n) continue;
+      if (colK) {
+        if (j < k0 + kk) A[i * n + j] = -acc[a][b];
+      } else {
+        A[i * n + j] -= acc[a][b];
+      }
+    }
+  }
+}
+
+// y = A x, one 64-lane wave per row
+__global__ void __launch_bounds__(256) gemv_kernel(long long n, const double* __restrict__ A,
+                                                    const double* __restrict__ x, double* __restrict__ y) {
+  const long long row = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const double* a = A + row * n;
+  double s = 0.0;
+  for (long long c = lane; c < n; c += 64) s = fma(a[c], x[c], s);
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) y[row] = s;
+}
+
+int launch_galerkin_fine(const GridDims& gf, const double* K0, const double* xe, const uint8_t* bcm, double bcdiag,
+                         double* Kc, hipStream_t s) {
+  GridDims gc = make_grid(gf.ex / 2, gf.ey / 2, gf.dim == 3 ? gf.ez / 2 : 0);
+  if (gf.dim == 3)
+    galerkin_kernel<3, true><<<(unsigned)gc.nel, 64, 0, s>>>(gf, gc, K0, xe, bcm, bcdiag, nullptr, Kc);
+  else
+    galerkin_kernel<2, true><<<(unsigned)gc.nel, 16, 0, s>>>(gf, gc, K0, xe, bcm, bcdiag, nullptr, Kc);
+  return check_launch("galerkin_kernel<fine>");
+}
+
+int launch_galerkin_coarse(const GridDims& gf, const double* Kf, double* Kc, hipStream_t s) {
+  GridDims gc = make_grid(gf.ex / 2, gf.ey / 2, gf.dim == 3 ? gf.ez / 2 : 0);
+  if (gf.dim == 3)
+    galerkin_kernel<3, false><<<(unsigned)gc.nel, 64, 0, s>>>(gf, gc, nullptr, nullptr, nullptr, 0.0, Kf, Kc);
+  else
+    galerkin_kernel<2, false><<<(unsigned)gc.nel, 16, 0, s>>>(gf, gc, nullptr, nullptr, nullptr, 0.0, Kf, Kc);
+  return check_launch("galerkin_kernel<coarse>");
+}
+
+int launch_stencil_from_elements(const GridDims& g, const double* Ke, double* S, hipStream_t s) {
+  dim3 grid(blocks_for(g.nnodes, 128));
+  if (g.dim == 3)
+    stencil_from_elements_kernel<3><<<grid, 128, 0, s>>>(g, Ke, S);
+  else
+    stencil_from_elements_kernel<2><<<grid, 128, 0, s>>>(g, Ke, S);
+  return check_launch("stencil_from_elements_kernel");
+}
+
+int launch_dense_from_stencil(const GridDims& g, const double* S, double* A, hipStream_t s) {
+  const long long N = g.nnodes * g.dim;
+  PMT_TRY(launch_fill(N * N, 0.0, A, s));
+  dim3 grid(blocks_for(g.nnodes, 128));
+  if (g.dim == 3)
+    dense_from_stencil_kernel<3><<<grid, 128, 0, s>>>(g, S, A);
+  else
+    dense_from_stencil_kernel<2><<<grid, 128, 0, s>>>(g, S, A);
+  return check_launch("dense_from_stencil_kernel");
+}
+
+int gj_invert(long long n, double* A, double* wsP, double* wsC, hipStream_t s) {
+  const long long nb = (n + GJB - 1) / GJB;
+  for (long long kb = 0; kb < nb; ++kb) {
+    const long long k0 = kb * GJB;
+    const int kk = (int)((n - k0) < GJB ? (n - k0) : GJB);
+    gj_diag_kernel<<<1, 256, 0, s>>>(n, k0, kk, A, wsP);
+    PMT_TRY(check_launch("gj_diag_kernel"));
+    gj_colcopy_kernel<<<blocks_for(n * kk, 256), 256, 0, s>>>(n, k0, kk, A, wsC);
+    PMT_TRY(check_launch("gj_colcopy_kernel"));
+    gj_rowpanel_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, k0, kk, A, wsP);
+    PMT_TRY(check_launch("gj_rowpanel_kernel"));
+    dim3 grid((unsigned)nb, (unsigned)nb);
+    gj_update_kernel<<<grid, 256, 0, s>>>(n, k0, kk, A, wsC, wsP);
+    PMT_TRY(check_launch("gj_update_kernel"));
+  }
+  return PMT_OK;
+}
+
+int launch_gemv(long long n, const double* A, const double* x, double* y, hipStream_t s) {
+  gemv_kernel<<<blocks_for(n, 4), 256, 0, s>>>(n, A, x, y);
+  return check_launch("gemv_kernel");
+}
+
+}  // namespace pmt

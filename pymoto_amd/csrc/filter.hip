@@ -1,0 +1,132 @@
+// FilterConv forward and adjoint (pymoto/modules/filter.py:8-220).
+//
+// The reference pads an element-index array axis by axis with numpy padding modes and then
+// convolves.  Padding an index array along one axis applies the same 1-D index map to every
+// line, so the padded field is xpad[a,b,c] = x[mx[a], my[b], mz[c]] with one small map per axis
+// (built on the host at setup); constant-padded cells carry map = -1 and a value, with the later
+// axis winning at corners as the reference's override order does.  Forward: one thread per element
+// gathers the kx*ky*kz window ('valid' convolution).  Adjoint: one thread per element visits the
+// padded cells that map onto it (per-axis inverse lists; interior elements have exactly one) and
+// evaluates the 'full' correlation there - a gather, so no atomics and a deterministic sum.
+#include "common.h"
+
+namespace pmt {
+
+__global__ void __launch_bounds__(256)
+filter_fwd_kernel(int nx, int ny, int nz, int kx, int ky, int kz, const double* __restrict__ w,
+                  const int32_t* __restrict__ map, const double* __restrict__ cval, const double* __restrict__ x,
+                  double* __restrict__ y) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nel = (long long)nx * ny * nz;
+  if (e >= nel) return;
+  const int i = (int)(e % nx);
+  const long long t = e / nx;
+  const int j = (int)(t % ny);
+  const int k = (int)(t / ny);
+  const int Lx = nx + kx - 1, Ly = ny + ky - 1;
+  const int32_t* mx = map;
+  const int32_t* my = map + Lx;
+  const int32_t* mz = map + Lx + Ly;
+  const double* cx = cval;
+  const double* cy = cval + Lx;
+  const double* cz = cval + Lx + Ly;
+  double acc = 0.0;
+  for (int r = 0; r < kz; ++r) {
+    const int c = k + kz - 1 - r;
+    const int mc = mz[c];
+    for (int q = 0; q < ky; ++q) {
+      const int b = j + ky - 1 - q;
+      const int mb = my[b];
+      for (int p = 0; p < kx; ++p) {
+        const int a = i + kx - 1 - p;
+        const int ma = mx[a];
+        double v;
+        if (mc < 0)
+          v = cz[c];
+        else if (mb < 0)
+          v = cy[b];
+        else if (ma < 0)
+          v = cx[a];
+        else
+          v = x[((long long)mc * ny + mb) * nx + ma];
+        acc = fma(w[((long long)p * ky + q) * kz + r], v, acc);
+      }
+    }
+  }
+  y[e] = acc;
+}
+
+__global__ void __launch_bounds__(256)
+filter_bwd_kernel(int nx, int ny, int nz, int kx, int ky, int kz, const double* __restrict__ w,
+                  const int32_t* __restrict__ inv_ptr, const int32_t* __restrict__ inv_idx,
+                  const double* __restrict__ dy, double* __restrict__ dx) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nel = (long long)nx * ny * nz;
+  if (e >= nel) return;
+  const int i = (int)(e % nx);
+  const long long t = e / nx;
+  const int j = (int)(t % ny);
+  const int k = (int)(t / ny);
+  const int32_t* px = inv_ptr;
+  const int32_t* py = inv_ptr + (nx + 1);
+  const int32_t* pz = inv_ptr + (nx + 1) + (ny + 1);
+  double acc = 0.0;
+  for (int ic = pz[k]; ic < pz[k + 1]; ++ic) {
+    const int c = inv_idx[ic];
+    for (int ib = py[j]; ib < py[j + 1]; ++ib) {
+      const int b = inv_idx[ib];
+      for (int ia = px[i]; ia < px[i + 1]; ++ia) {
+        const int a = inv_idx[ia];
+        // correlate 'full' at padded cell (a, b, c)
+        for (int r = 0; r < kz; ++r) {
+          const int cc = c - (kz - 1) + r;
+          if (cc < 0 || cc >= nz) continue;
+          for (int q = 0; q < ky; ++q) {
+            const int bb = b - (ky - 1) + q;
+            if (bb < 0 || bb >= ny) continue;
+            for (int p = 0; p < kx; ++p) {
+              const int aa = a - (kx - 1) + p;
+              if (aa < 0 || aa >= nx) continue;
+              acc = fma(w[((long long)p * ky + q) * kz + r], dy[((long long)cc * ny + bb) * nx + aa], acc);
+            }
+          }
+        }
+      }
+    }
+  }
+  dx[e] = acc;
+}
+
+}  // namespace pmt
+
+using namespace pmt;
+
+extern "C" {
+
+int pmt_filter_fwd(int nelx, int nely, int nelz, int kx, int ky, int kz, const double* w, const int32_t* map,
+                   const double* cval, const double* x, double* y, void* stream) {
+  if (!grid_ok(nelx, nely, nelz) || kx < 1 || ky < 1 || kz < 1 || !(kx & 1) || !(ky & 1) || !(kz & 1) || !w ||
+      !map || !cval || !x || !y)
+    return arg_error("pmt_filter_fwd: bad argument");
+  const int nz = nelz > 0 ? nelz : 1;
+  if (nelz == 0 && kz != 1) return arg_error("pmt_filter_fwd: 2D grid needs kz == 1");
+  const long long nel = (long long)nelx * nely * nz;
+  filter_fwd_kernel<<<blocks_for(nel, 256), 256, 0, (hipStream_t)stream>>>(nelx, nely, nz, kx, ky, kz, w, map, cval,
+                                                                          x, y);
+  return check_launch("filter_fwd_kernel");
+}
+
+int pmt_filter_bwd(int nelx, int nely, int nelz, int kx, int ky, int kz, const double* w, const int32_t* inv_ptr,
+                   const int32_t* inv_idx, const double* dy, double* dx, void* stream) {
+  if (!grid_ok(nelx, nely, nelz) || kx < 1 || ky < 1 || kz < 1 || !(kx & 1) || !(ky & 1) || !(kz & 1) || !w ||
+      !inv_ptr || !inv_idx || !dy || !dx)
+    return arg_error("pmt_filter_bwd: bad argument");
+  const int nz = nelz > 0 ? nelz : 1;
+  if (nelz == 0 && kz != 1) return arg_error("pmt_filter_bwd: 2D grid needs kz == 1");
+  const long long nel = (long long)nelx * nely * nz;
+  filter_bwd_kernel<<<blocks_for(nel, 256), 256, 0, (hipStream_t)stream>>>(nelx, nely, nz, kx, ky, kz, w, inv_ptr,
+                                                                          inv_idx, dy, dx);
+  return check_launch("filter_bwd_kernel");
+}
+
+}  // extern "C"

@@ -1,0 +1,332 @@
+// Multigrid-preconditioned CG runtime (pymoto/solvers/iterative.py:124-403).
+//
+// A pmt_mg handle owns the grid hierarchy: per level the inverse diagonal, work vectors and,
+// below the fine grid, the Galerkin element matrices and the node stencil; the coarsest level
+// holds an explicit dense inverse.  update() rebuilds everything from the SIMP densities of the
+// fine grid; vcycle() and pcg_solve() only launch kernels on the caller's stream and never
+// allocate.  The CG loop follows CG.solve step by step (single right-hand side): alpha/beta are
+// read by the kernels from a small device array, and the only host synchronisation per
+// iteration is the 8-byte read of ||r||^2 for the stopping test.
+#include <cmath>
+#include <new>
+#include <vector>
+
+#include "launch.h"
+
+namespace pmt {
+
+constexpr int kMaxLevels = 12;
+
+struct Level {
+  GridDims g;
+  long long ndof = 0;
+  double omega = 0.5;
+  int steps = 1;
+  double* dinv = nullptr;
+  double* S = nullptr;   // coarse stencil (levels >= 1)
+  double* Ke = nullptr;  // coarse element matrices (levels >= 1)
+  double* u = nullptr;   // V-cycle correction (levels >= 1)
+  double* t = nullptr;   // ping-pong buffer
+  double* b = nullptr;   // restricted right-hand side (levels >= 1)
+  double* r = nullptr;   // residual
+};
+
+}  // namespace pmt
+
+struct pmt_mg {
+  int dim = 3;
+  int nlev = 0;
+  pmt::Level L[pmt::kMaxLevels];
+  int NE = 24;
+  double* K0 = nullptr;
+  const double* xe = nullptr;
+  const uint8_t* bcm = nullptr;
+  double bcdiag = 0.0;
+  bool ready = false;
+  long long nc = 0;
+  double* Ainv = nullptr;
+  double* gjP = nullptr;
+  double* gjC = nullptr;
+  double* p = nullptr;
+  double* q = nullptr;
+  double* z = nullptr;
+  double* r = nullptr;
+  double* sc = nullptr;    // device scalars
+  double* work = nullptr;  // reduction partials
+  double* host_sc = nullptr;
+  long long bytes = 0;
+  std::vector<void*> allocs;
+};
+
+namespace pmt {
+
+static int dalloc(pmt_mg* mg, double** p, long long n) {
+  void* ptr = nullptr;
+  PMT_HIP(hipMalloc(&ptr, (size_t)(n > 0 ? n : 1) * sizeof(double)));
+  mg->allocs.push_back(ptr);
+  mg->bytes += (n > 0 ? n : 1) * (long long)sizeof(double);
+  *p = (double*)ptr;
+  return PMT_OK;
+}
+
+static void release(pmt_mg* mg) {
+  for (void* p : mg->allocs) (void)hipFree(p);
+  mg->allocs.clear();
+  if (mg->host_sc) (void)hipHostFree(mg->host_sc);
+  mg->host_sc = nullptr;
+}
+
+static int level_apply(pmt_mg* mg, int l, int mode, const double* u, const double* b, double* y, hipStream_t s) {
+  Level& L = mg->L[l];
+  if (l == 0) return launch_fine(mode, L.g, mg->K0, mg->xe, mg->bcm, mg->bcdiag, u, b, L.dinv, L.omega, y, s);
+  return launch_stencil(mode, L.g, L.S, u, b, L.dinv, L.omega, y, s);
+}
+
+// out = V-cycle(b) on level l (GeometricMultigrid.solve with x0 = None, iterative.py:222-256)
+static int vcycle(pmt_mg* mg, int l, const double* b, double* out, hipStream_t s) {
+  if (l == mg->nlev - 1) return launch_gemv(mg->nc, mg->Ainv, b, out, s);
+  Level& L = mg->L[l];
+  Level& C = mg->L[l + 1];
+  // 2*steps-1 out-of-place sweeps follow the first one: start in the buffer that ends in `out`
+  double* cur = L.t;
+  double* oth = out;
+  PMT_TRY(launch_jacobi0(L.ndof, L.dinv, b, L.omega, cur, s));
+  for (int it = 1; it < L.steps; ++it) {
+    PMT_TRY(level_apply(mg, l, MODE_JACOBI, cur, b, oth, s));
+    double* tmp = cur;
+    cur = oth;
+    oth = tmp;
+  }
+  PMT_TRY(level_apply(mg, l, MODE_RESIDUAL, cur, b, L.r, s));
+  PMT_TRY(launch_restrict(L.g, C.g, L.r, C.b, s));
+  PMT_TRY(vcycle(mg, l + 1, C.b, C.u, s));
+  PMT_TRY(launch_prolong(L.g, C.g, C.u, cur, s));
+  for (int it = 0; it < L.steps; ++it) {
+    PMT_TRY(level_apply(mg, l, MODE_JACOBI, cur, b, oth, s));
+    double* tmp = cur;
+    cur = oth;
+    oth = tmp;
+  }
+  if (cur != out) return arg_error("vcycle: internal buffer parity error");
+  return PMT_OK;
+}
+
+static int read_scalar(pmt_mg* mg, int k, double* v, hipStream_t s) {
+  PMT_HIP(hipMemcpyAsync(mg->host_sc + k, mg->sc + k, sizeof(double), hipMemcpyDeviceToHost, s));
+  PMT_HIP(hipStreamSynchronize(s));
+  *v = mg->host_sc[k];
+  return PMT_OK;
+}
+
+}  // namespace pmt
+
+using namespace pmt;
+
+extern "C" {
+
+int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega, const int* steps,
+                  const double* K0_host, pmt_mg** out) {
+  if (!out || !K0_host || !grid_ok(nelx, nely, nelz) || nlevels < 2 || nlevels > kMaxLevels)
+    return arg_error("pmt_mg_create: bad argument (nlevels must be 2..12)");
+  *out = nullptr;
+  pmt_mg* mg = new (std::nothrow) pmt_mg();
+  if (!mg) return arg_error("pmt_mg_create: out of host memory");
+  mg->dim = nelz > 0 ? 3 : 2;
+  mg->nlev = nlevels;
+  mg->NE = (1 << mg->dim) * mg->dim;
+  int ex = nelx, ey = nely, ez = nelz;
+  for (int l = 0; l < nlevels; ++l) {
+    Level& L = mg->L[l];
+    L.g = make_grid(ex, ey, ez);
+    L.ndof = L.g.nnodes * mg->dim;
+    if (l < nlevels - 1) {
+      if (ex % 2 || ey % 2 || ez % 2 || ex < 2 || ey < 2) {
+        delete mg;
+        return arg_error("pmt_mg_create: every level above the coarsest needs even element counts");
+      }
+      L.omega = omega ? omega[l] : 0.5;
+      L.steps = steps ? steps[l] : 1;
+      if (L.steps < 1 || !(L.omega > 0.0 && L.omega <= 1.0)) {
+        delete mg;
+        return arg_error("pmt_mg_create: smoothing steps must be >= 1 and 0 < omega <= 1");
+      }
+      ex /= 2;
+      ey /= 2;
+      ez /= 2;
+    }
+  }
+  mg->nc = mg->L[nlevels - 1].ndof;
+  if (mg->nc > 20000) {
+    delete mg;
+    return arg_error("pmt_mg_create: coarsest grid too large for the dense direct solve (> 20000 dofs); add levels");
+  }
+  int rc = PMT_OK;
+#define ALLOC(ptr, n)                                  \
+  if (rc == PMT_OK) rc = dalloc(mg, &(ptr), (n));
+  const int NE = mg->NE;
+  ALLOC(mg->K0, (long long)NE * NE);
+  for (int l = 0; l < nlevels; ++l) {
+    Level& L = mg->L[l];
+    if (l == nlevels - 1) {
+      if (l > 0) {
+        ALLOC(L.Ke, L.g.nel * NE * NE);
+        ALLOC(L.S, L.g.nnodes * (mg->dim == 3 ? 243 : 36));
+        ALLOC(L.u, L.ndof);
+        ALLOC(L.b, L.ndof);
+      }
+      continue;
+    }
+    ALLOC(L.dinv, L.ndof);
+    ALLOC(L.t, L.ndof);
+    ALLOC(L.r, L.ndof);
+    if (l > 0) {
+      ALLOC(L.Ke, L.g.nel * NE * NE);
+      ALLOC(L.S, L.g.nnodes * (mg->dim == 3 ? 243 : 36));
+      ALLOC(L.u, L.ndof);
+      ALLOC(L.b, L.ndof);
+    }
+  }
+  ALLOC(mg->Ainv, mg->nc * mg->nc);
+  ALLOC(mg->gjP, 64 * 64);
+  ALLOC(mg->gjC, mg->nc * 64);
+  const long long n0 = mg->L[0].ndof;
+  ALLOC(mg->p, n0);
+  ALLOC(mg->q, n0);
+  ALLOC(mg->z, n0);
+  ALLOC(mg->r, n0);
+  ALLOC(mg->sc, 16);
+  ALLOC(mg->work, 2 * kReduceBlocks);
+#undef ALLOC
+  if (rc == PMT_OK) rc = hip_status(hipHostMalloc((void**)&mg->host_sc, 16 * sizeof(double), hipHostMallocDefault),
+                                    "hipHostMalloc");
+  if (rc == PMT_OK)
+    rc = hip_status(hipMemcpy(mg->K0, K0_host, (size_t)NE * NE * sizeof(double), hipMemcpyHostToDevice), "K0 upload");
+  if (rc != PMT_OK) {
+    release(mg);
+    delete mg;
+    return rc;
+  }
+  *out = mg;
+  return PMT_OK;
+}
+
+int pmt_mg_destroy(pmt_mg* mg) {
+  if (!mg) return PMT_OK;
+  (void)hipDeviceSynchronize();
+  release(mg);
+  delete mg;
+  return PMT_OK;
+}
+
+int pmt_mg_update(pmt_mg* mg, const double* xe, const uint8_t* bcmask, double bcdiag, void* stream) {
+  if (!mg || !xe) return arg_error("pmt_mg_update: bad argument");
+  hipStream_t s = (hipStream_t)stream;
+  mg->xe = xe;
+  mg->bcm = bcmask;
+  mg->bcdiag = bcdiag;
+  mg->ready = false;
+  Level& F = mg->L[0];
+  PMT_TRY(launch_fine_diag(F.g, mg->K0, xe, bcmask, bcdiag, F.dinv, 1, s));
+  for (int l = 1; l < mg->nlev; ++l) {
+    Level& L = mg->L[l];
+    Level& P = mg->L[l - 1];
+    if (l == 1)
+      PMT_TRY(launch_galerkin_fine(P.g, mg->K0, xe, bcmask, bcdiag, L.Ke, s));
+    else
+      PMT_TRY(launch_galerkin_coarse(P.g, P.Ke, L.Ke, s));
+    PMT_TRY(launch_stencil_from_elements(L.g, L.Ke, L.S, s));
+    if (l < mg->nlev - 1) PMT_TRY(launch_stencil_dinv(L.g, L.S, L.dinv, s));
+  }
+  Level& Cl = mg->L[mg->nlev - 1];
+  PMT_TRY(launch_dense_from_stencil(Cl.g, Cl.S, mg->Ainv, s));
+  PMT_TRY(gj_invert(mg->nc, mg->Ainv, mg->gjP, mg->gjC, s));
+  mg->ready = true;
+  return PMT_OK;
+}
+
+int pmt_mg_vcycle(pmt_mg* mg, const double* r, double* z, void* stream) {
+  if (!mg || !r || !z) return arg_error("pmt_mg_vcycle: bad argument");
+  if (!mg->ready) {
+    set_error("pmt_mg_vcycle: call pmt_mg_update first");
+    return PMT_ERR_STATE;
+  }
+  return vcycle(mg, 0, r, z, (hipStream_t)stream);
+}
+
+int pmt_mg_level_apply(pmt_mg* mg, int level, const double* u, double* y, void* stream) {
+  if (!mg || !u || !y || level < 0 || level >= mg->nlev) return arg_error("pmt_mg_level_apply: bad argument");
+  if (!mg->ready) {
+    set_error("pmt_mg_level_apply: call pmt_mg_update first");
+    return PMT_ERR_STATE;
+  }
+  return level_apply(mg, level, MODE_APPLY, u, nullptr, y, (hipStream_t)stream);
+}
+
+long long pmt_mg_level_ndof(pmt_mg* mg, int level) {
+  if (!mg || level < 0 || level >= mg->nlev) return PMT_ERR_ARG;
+  return mg->L[level].ndof;
+}
+
+long long pmt_mg_device_bytes(pmt_mg* mg) { return mg ? mg->bytes : PMT_ERR_ARG; }
+
+int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double tol, int maxit, int restart,
+                  int* iters, double* relres, double* history, void* stream) {
+  if (!mg || !b || !x || maxit < 0 || restart < 1) return arg_error("pmt_pcg_solve: bad argument");
+  if (!mg->ready) {
+    set_error("pmt_pcg_solve: call pmt_mg_update first");
+    return PMT_ERR_STATE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  Level& F = mg->L[0];
+  const long long n = F.ndof;
+  double* sc = mg->sc;
+  // sc: 0 p.q  1 p.r  2 r.r  3 q.z  4 b.b  5 z.z
+  if (x0_is_zero) {
+    PMT_TRY(launch_fill(n, 0.0, x, s));
+    PMT_TRY(launch_axpby(n, 1.0, b, 0.0, mg->r, s));
+  } else {
+    PMT_TRY(level_apply(mg, 0, MODE_RESIDUAL, x, b, mg->r, s));
+  }
+  PMT_TRY(launch_dot2(n, b, b, mg->r, mg->r, sc + 4, mg->work, s));  // sc[4] = b.b, sc[5] = r.r
+  PMT_HIP(hipMemcpyAsync(mg->host_sc + 4, sc + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  PMT_HIP(hipStreamSynchronize(s));
+  const double bn = std::sqrt(mg->host_sc[4]);
+  double tval = std::sqrt(mg->host_sc[5]) / bn;
+  if (history) history[0] = tval;
+  if (iters) *iters = 0;
+  if (relres) *relres = tval;
+  if (!(bn > 0.0)) {  // zero right-hand side: x = 0 is the solution
+    PMT_TRY(launch_fill(n, 0.0, x, s));
+    if (relres) *relres = 0.0;
+    return PMT_OK;
+  }
+  if (tval <= tol) return PMT_OK;
+  PMT_TRY(vcycle(mg, 0, mg->r, mg->z, s));
+  PMT_TRY(launch_dot(n, mg->z, mg->z, sc + 5, mg->work, s));
+  PMT_TRY(launch_scale_rsqrt(n, sc, 5, mg->z, mg->p, s));
+  int i = 0;
+  for (i = 0; i < maxit; ++i) {
+    PMT_TRY(level_apply(mg, 0, MODE_APPLY, mg->p, nullptr, mg->q, s));
+    PMT_TRY(launch_dot2(n, mg->p, mg->q, mg->p, mg->r, sc, mg->work, s));  // sc[0] = p.q, sc[1] = p.r
+    if (i % restart == 0) {
+      PMT_TRY(launch_cg_x_update(n, sc, mg->p, x, s));
+      PMT_TRY(level_apply(mg, 0, MODE_RESIDUAL, x, b, mg->r, s));
+      PMT_TRY(launch_dot(n, mg->r, mg->r, sc + 2, mg->work, s));
+    } else {
+      PMT_TRY(launch_cg_xr_update(n, sc, mg->p, mg->q, x, mg->r, mg->work, sc + 2, s));
+    }
+    double rr = 0.0;
+    PMT_TRY(read_scalar(mg, 2, &rr, s));
+    tval = std::sqrt(rr) / bn;
+    if (history) history[i + 1] = tval;
+    if (tval <= tol) break;
+    PMT_TRY(vcycle(mg, 0, mg->r, mg->z, s));
+    PMT_TRY(launch_dot(n, mg->q, mg->z, sc + 3, mg->work, s));
+    PMT_TRY(launch_cg_p_update(n, sc, mg->z, mg->p, s));
+  }
+  if (iters) *iters = i < maxit ? i + 1 : maxit;
+  if (relres) *relres = tval;
+  return PMT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,451 @@
+// Node-centric operator kernels for the structured Q4/hex8 grid.
+//
+// Fine level (matrix-free): every thread owns one node and gathers the rows of
+//   K(x) = sum_e x_e K0, with Dirichlet rows/cols zeroed and bcdiag on their diagonal,
+// over the 2^D elements around it.  Neighbour values are visited once (3^D of them) and the
+// partial products are kept per adjacent element, so a node costs exactly
+// 2^D * D * (2^D*D) multiply-adds (576 in 3D) - the flop count of an element-by-element
+// product - without any scatter, atomics or colouring.  The same kernel is fused with the
+// residual (r = b - K u) and the damped-Jacobi sweep (u' = u + w D^-1 (b - K u)) of
+// pymoto/solvers/iterative.py:21-47, 238-255.
+//
+// Coarse levels: a 3^D-point block stencil (D x D blocks) stored coefficient-major
+// S[c][node], c = (offset*D + d)*D + d2, which is the Galerkin operator R^T A R of
+// iterative.py:173 assembled per node.
+#include "launch.h"
+
+namespace pmt {
+
+template <int DIM>
+struct Cfg {
+  static constexpr int NN = 1 << DIM;     // nodes per element
+  static constexpr int NE = NN * DIM;     // dofs per element
+  static constexpr int NS = DIM == 3 ? 27 : 9;  // stencil points
+  static constexpr int NC = NS * DIM * DIM;     // stencil coefficients per node
+};
+
+template <int DIM, int MODE>
+__global__ void __launch_bounds__(256)
+fine_node_kernel(GridDims g, const double* __restrict__ K0, const double* __restrict__ xe,
+                 const uint8_t* __restrict__ bcm, double bcdiag, const double* __restrict__ u,
+                 const double* __restrict__ b, const double* __restrict__ dinv, double omega,
+                 double* __restrict__ y) {
+  constexpr int NN = Cfg<DIM>::NN;
+  constexpr int NE = Cfg<DIM>::NE;
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.nnodes) return;
+  const int i = (int)(n % g.nx);
+  const long long t = n / g.nx;
+  const int j = (int)(t % g.ny);
+  const int k = (int)(t / g.ny);
+  const long long sx = 1, sy = g.nx, sz = (long long)g.nx * g.ny;
+
+  double acc[NN][DIM];
+#pragma unroll
+  for (int e = 0; e < NN; ++e)
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) acc[e][d] = 0.0;
+
+  constexpr int ZLO = DIM == 3 ? -1 : 0, ZHI = DIM == 3 ? 1 : 0;
+#pragma unroll
+  for (int dz = ZLO; dz <= ZHI; ++dz) {
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy) {
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int ii = i + dx, jj = j + dy, kk = k + dz;
+        const bool valid = ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny && kk >= 0 && kk < g.nz;
+        double um[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) um[d] = 0.0;
+        if (valid) {
+          const long long m = n + dx * sx + dy * sy + dz * sz;
+          const unsigned mk = bcm ? bcm[m] : 0u;
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) um[d] = ((mk >> d) & 1u) ? 0.0 : u[m * DIM + d];
+        }
+        // elements (i-1+ex, j-1+ey, k-1+ez) containing both this node and the neighbour
+#pragma unroll
+        for (int e = 0; e < NN; ++e) {
+          const int ex = e & 1, ey = (e >> 1) & 1, ez = DIM == 3 ? (e >> 2) & 1 : 1;
+          const int bx = dx + 1 - ex, by = dy + 1 - ey, bz = DIM == 3 ? dz + 1 - ez : 0;
+          if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
+          const int locn = (1 - ex) + 2 * (1 - ey) + (DIM == 3 ? 4 * (1 - ez) : 0);
+          const int locm = bx + 2 * by + 4 * bz;
+#pragma unroll
+          for (int d = 0; d < DIM; ++d)
+#pragma unroll
+            for (int d2 = 0; d2 < DIM; ++d2)
+              acc[e][d] = fma(K0[(locn * DIM + d) * NE + locm * DIM + d2], um[d2], acc[e][d]);
+        }
+      }
+    }
+  }
+
+  double out[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) out[d] = 0.0;
+#pragma unroll
+  for (int e = 0; e < NN; ++e) {
+    const int ex = e & 1, ey = (e >> 1) & 1, ez = DIM == 3 ? (e >> 2) & 1 : 1;
+    const int ei = i - 1 + ex, ej = j - 1 + ey, ek = DIM == 3 ? k - 1 + ez : 0;
+    if (ei < 0 || ei >= g.ex || ej < 0 || ej >= g.ey || ek < 0 || ek >= g.ez) continue;
+    const double xv = xe[((long long)ek * g.ey + ej) * g.ex + ei];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) out[d] = fma(xv, acc[e][d], out[d]);
+  }
+
+  const unsigned mk = bcm ? bcm[n] : 0u;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const long long q = n * DIM + d;
+    const double ku = ((mk >> d) & 1u) ? bcdiag * u[q] : out[d];
+    if (MODE == MODE_APPLY) {
+      y[q] = ku;
+    } else if (MODE == MODE_RESIDUAL) {
+      y[q] = b[q] - ku;
+    } else {
+      y[q] = u[q] + omega * dinv[q] * (b[q] - ku);
+    }
+  }
+}
+
+template <int DIM>
+__global__ void fine_diag_kernel(GridDims g, const double* __restrict__ K0, const double* __restrict__ xe,
+                                 const uint8_t* __restrict__ bcm, double bcdiag, double* __restrict__ diag,
+                                 int invert) {
+  constexpr int NN = Cfg<DIM>::NN;
+  constexpr int NE = Cfg<DIM>::NE;
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.nnodes) return;
+  const int i = (int)(n % g.nx);
+  const long long t = n / g.nx;
+  const int j = (int)(t % g.ny);
+  const int k = (int)(t / g.ny);
+  double dg[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) dg[d] = 0.0;
+#pragma unroll
+  for (int e = 0; e < NN; ++e) {
+    const int ex = e & 1, ey = (e >> 1) & 1, ez = DIM == 3 ? (e >> 2) & 1 : 1;
+    const int ei = i - 1 + ex, ej = j - 1 + ey, ek = DIM == 3 ? k - 1 + ez : 0;
+    if (ei < 0 || ei >= g.ex || ej < 0 || ej >= g.ey || ek < 0 || ek >= g.ez) continue;
+    const double xv = xe[((long long)ek * g.ey + ej) * g.ex + ei];
+    const int loc = (1 - ex) + 2 * (1 - ey) + (DIM == 3 ? 4 * (1 - ez) : 0);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) dg[d] = fma(xv, K0[(loc * DIM + d) * NE + loc * DIM + d], dg[d]);
+  }
+  const unsigned mk = bcm ? bcm[n] : 0u;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const double v = ((mk >> d) & 1u) ? bcdiag : dg[d];
+    diag[n * DIM + d] = invert ? 1.0 / v : v;
+  }
+}
+
+template <int DIM, int MODE>
+__global__ void __launch_bounds__(256)
+stencil_node_kernel(GridDims g, const double* __restrict__ S, const double* __restrict__ u,
+                    const double* __restrict__ b, const double* __restrict__ dinv, double omega,
+                    double* __restrict__ y) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.nnodes) return;
+  const int i = (int)(n % g.nx);
+  const long long t = n / g.nx;
+  const int j = (int)(t % g.ny);
+  const int k = (int)(t / g.ny);
+  const long long nn = g.nnodes;
+  double out[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) out[d] = 0.0;
+  constexpr int ZLO = DIM == 3 ? -1 : 0, ZHI = DIM == 3 ? 1 : 0;
+  int o = 0;
+#pragma unroll
+  for (int dz = ZLO; dz <= ZHI; ++dz) {
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy) {
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx, ++o) {
+        const int ii = i + dx, jj = j + dy, kk = k + dz;
+        if (ii < 0 || ii >= g.nx || jj < 0 || jj >= g.ny || kk < 0 || kk >= g.nz) continue;
+        const long long m = n + dx + dy * (long long)g.nx + dz * (long long)g.nx * g.ny;
+        double um[DIM];
+#pragma unroll
+        for (int d2 = 0; d2 < DIM; ++d2) um[d2] = u[m * DIM + d2];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+#pragma unroll
+          for (int d2 = 0; d2 < DIM; ++d2)
+            out[d] = fma(S[((long long)(o * DIM + d) * DIM + d2) * nn + n], um[d2], out[d]);
+      }
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const long long q = n * DIM + d;
+    if (MODE == MODE_APPLY) {
+      y[q] = out[d];
+    } else if (MODE == MODE_RESIDUAL) {
+      y[q] = b[q] - out[d];
+    } else {
+      y[q] = u[q] + omega * dinv[q] * (b[q] - out[d]);
+    }
+  }
+}
+
+template <int DIM>
+__global__ void stencil_diag_kernel(GridDims g, const double* __restrict__ S, double* __restrict__ dinv) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.nnodes) return;
+  constexpr int center = Cfg<DIM>::NS / 2;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) dinv[n * DIM + d] = 1.0 / S[((long long)(center * DIM + d) * DIM + d) * g.nnodes + n];
+}
+
+// first damped-Jacobi sweep from zero: u = w D^-1 b (iterative.py:232-233)
+__global__ void jacobi0_kernel(long long n, const double* __restrict__ dinv, const double* __restrict__ b,
+                               double omega, double* __restrict__ u) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) u[q] = omega * dinv[q] * b[q];
+}
+
+// rc = R^T rf : coarse node gathers the 3^D fine nodes around its fine image (weights 1, 1/2, 1/4, 1/8)
+template <int DIM>
+__global__ void restrict_kernel(GridDims gf, GridDims gc, const double* __restrict__ rf, double* __restrict__ rc) {
+  const long long N = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (N >= gc.nnodes) return;
+  const int I = (int)(N % gc.nx);
+  const long long t = N / gc.nx;
+  const int J = (int)(t % gc.ny);
+  const int K = (int)(t / gc.ny);
+  double acc[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) acc[d] = 0.0;
+  constexpr int ZLO = DIM == 3 ? -1 : 0, ZHI = DIM == 3 ? 1 : 0;
+#pragma unroll
+  for (int dz = ZLO; dz <= ZHI; ++dz)
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int fi = 2 * I + dx, fj = 2 * J + dy, fk = 2 * K + dz;
+        if (fi < 0 || fi >= gf.nx || fj < 0 || fj >= gf.ny || fk < 0 || fk >= gf.nz) continue;
+        const double w = (dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0);
+        const long long m = ((long long)fk * gf.ny + fj) * gf.nx + fi;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) acc[d] = fma(w, rf[m * DIM + d], acc[d]);
+      }
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) rc[N * DIM + d] = acc[d];
+}
+
+// uf += R uc : fine node interpolates from the 1, 2, 4 or 8 coarse nodes around it
+template <int DIM>
+__global__ void prolong_kernel(GridDims gf, GridDims gc, const double* __restrict__ uc, double* __restrict__ uf) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= gf.nnodes) return;
+  const int i = (int)(n % gf.nx);
+  const long long t = n / gf.nx;
+  const int j = (int)(t % gf.ny);
+  const int k = (int)(t / gf.ny);
+  const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
+  const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (DIM == 3 && (k & 1)) ? 2 : 1;
+  const double w = ((i & 1) ? 0.5 : 1.0) * ((j & 1) ? 0.5 : 1.0) * ((DIM == 3 && (k & 1)) ? 0.5 : 1.0);
+  double acc[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) acc[d] = 0.0;
+  for (int c = 0; c < nk; ++c)
+    for (int bb = 0; bb < nj; ++bb)
+      for (int a = 0; a < ni; ++a) {
+        const long long N = ((long long)(k0 + c) * gc.ny + (j0 + bb)) * gc.nx + (i0 + a);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) acc[d] += uc[N * DIM + d];
+      }
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) uf[n * DIM + d] += w * acc[d];
+}
+
+// dx_e (+)= alpha * u_e^T K0 v_e with Dirichlet entries zeroed (element-centric, no scatter)
+template <int DIM>
+__global__ void __launch_bounds__(256)
+sens_kernel(GridDims g, const double* __restrict__ K0, const uint8_t* __restrict__ bcm,
+            const double* __restrict__ u, const double* __restrict__ v, double alpha,
+            double* __restrict__ dx, int accumulate) {
+  constexpr int NN = Cfg<DIM>::NN;
+  constexpr int NE = Cfg<DIM>::NE;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= g.nel) return;
+  const int ei = (int)(e % g.ex);
+  const long long t = e / g.ex;
+  const int ej = (int)(t % g.ey);
+  const int ek = (int)(t / g.ey);
+  double ue[NE], ve[NE];
+#pragma unroll
+  for (int a = 0; a < NN; ++a) {
+    const long long m = ((long long)(ek + ((a >> 2) & 1)) * g.ny + (ej + ((a >> 1) & 1))) * g.nx + (ei + (a & 1));
+    const unsigned mk = bcm ? bcm[m] : 0u;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const bool fixed = (mk >> d) & 1u;
+      ue[a * DIM + d] = fixed ? 0.0 : u[m * DIM + d];
+      ve[a * DIM + d] = fixed ? 0.0 : v[m * DIM + d];
+    }
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int r = 0; r < NE; ++r) {
+    double kv = 0.0;
+#pragma unroll
+    for (int c = 0; c < NE; ++c) kv = fma(K0[r * NE + c], ve[c], kv);
+    s = fma(ue[r], kv, s);
+  }
+  s *= alpha;
+  dx[e] = accumulate ? dx[e] + s : s;
+}
+
+// ---------------------------------------------------------------------------------------------
+// host launchers (shared with the multigrid runtime)
+// ---------------------------------------------------------------------------------------------
+int launch_fine(int mode, const GridDims& g, const double* K0, const double* xe, const uint8_t* bcm, double bcdiag,
+                const double* u, const double* b, const double* dinv, double omega, double* y, hipStream_t s) {
+  const int T = 256;
+  dim3 grid(blocks_for(g.nnodes, T));
+  if (g.dim == 3) {
+    if (mode == MODE_APPLY)
+      fine_node_kernel<3, MODE_APPLY><<<grid, T, 0, s>>>(g, K0, xe, bcm, bcdiag, u, b, dinv, omega, y);
+    else if (mode == MODE_RESIDUAL)
+      fine_node_kernel<3, MODE_RESIDUAL><<<grid, T, 0, s>>>(g, K0, xe, bcm, bcdiag, u, b, dinv, omega, y);
+    else
+      fine_node_kernel<3, MODE_JACOBI><<<grid, T, 0, s>>>(g, K0, xe, bcm, bcdiag, u, b, dinv, omega, y);
+  } else {
+    if (mode == MODE_APPLY)
+      fine_node_kernel<2, MODE_APPLY><<<grid, T, 0, s>>>(g, K0, xe, bcm, bcdiag, u, b, dinv, omega, y);
+    else if (mode == MODE_RESIDUAL)
+      fine_node_kernel<2, MODE_RESIDUAL><<<grid, T, 0, s>>>(g, K0, xe, bcm, bcdiag, u, b, dinv, omega, y);
+    else
+      fine_node_kernel<2, MODE_JACOBI><<<grid, T, 0, s>>>(g, K0, xe, bcm, bcdiag, u, b, dinv, omega, y);
+  }
+  return check_launch("fine_node_kernel");
+}
+
+int launch_fine_diag(const GridDims& g, const double* K0, const double* xe, const uint8_t* bcm, double bcdiag,
+                     double* diag, int invert, hipStream_t s) {
+  dim3 grid(blocks_for(g.nnodes, 256));
+  if (g.dim == 3)
+    fine_diag_kernel<3><<<grid, 256, 0, s>>>(g, K0, xe, bcm, bcdiag, diag, invert);
+  else
+    fine_diag_kernel<2><<<grid, 256, 0, s>>>(g, K0, xe, bcm, bcdiag, diag, invert);
+  return check_launch("fine_diag_kernel");
+}
+
+int launch_stencil(int mode, const GridDims& g, const double* S, const double* u, const double* b, const double* dinv,
+                   double omega, double* y, hipStream_t s) {
+  dim3 grid(blocks_for(g.nnodes, 256));
+  if (g.dim == 3) {
+    if (mode == MODE_APPLY)
+      stencil_node_kernel<3, MODE_APPLY><<<grid, 256, 0, s>>>(g, S, u, b, dinv, omega, y);
+    else if (mode == MODE_RESIDUAL)
+      stencil_node_kernel<3, MODE_RESIDUAL><<<grid, 256, 0, s>>>(g, S, u, b, dinv, omega, y);
+    else
+      stencil_node_kernel<3, MODE_JACOBI><<<grid, 256, 0, s>>>(g, S, u, b, dinv, omega, y);
+  } else {
+    if (mode == MODE_APPLY)
+      stencil_node_kernel<2, MODE_APPLY><<<grid, 256, 0, s>>>(g, S, u, b, dinv, omega, y);
+    else if (mode == MODE_RESIDUAL)
+      stencil_node_kernel<2, MODE_RESIDUAL><<<grid, 256, 0, s>>>(g, S, u, b, dinv, omega, y);
+    else
+      stencil_node_kernel<2, MODE_JACOBI><<<grid, 256, 0, s>>>(g, S, u, b, dinv, omega, y);
+  }
+  return check_launch("stencil_node_kernel");
+}
+
+int launch_stencil_dinv(const GridDims& g, const double* S, double* dinv, hipStream_t s) {
+  dim3 grid(blocks_for(g.nnodes, 256));
+  if (g.dim == 3)
+    stencil_diag_kernel<3><<<grid, 256, 0, s>>>(g, S, dinv);
+  else
+    stencil_diag_kernel<2><<<grid, 256, 0, s>>>(g, S, dinv);
+  return check_launch("stencil_diag_kernel");
+}
+
+int launch_jacobi0(long long n, const double* dinv, const double* b, double omega, double* u, hipStream_t s) {
+  jacobi0_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, dinv, b, omega, u);
+  return check_launch("jacobi0_kernel");
+}
+
+int launch_restrict(const GridDims& gf, const GridDims& gc, const double* rf, double* rc, hipStream_t s) {
+  dim3 grid(blocks_for(gc.nnodes, 256));
+  if (gf.dim == 3)
+    restrict_kernel<3><<<grid, 256, 0, s>>>(gf, gc, rf, rc);
+  else
+    restrict_kernel<2><<<grid, 256, 0, s>>>(gf, gc, rf, rc);
+  return check_launch("restrict_kernel");
+}
+
+int launch_prolong(const GridDims& gf, const GridDims& gc, const double* uc, double* uf, hipStream_t s) {
+  dim3 grid(blocks_for(gf.nnodes, 256));
+  if (gf.dim == 3)
+    prolong_kernel<3><<<grid, 256, 0, s>>>(gf, gc, uc, uf);
+  else
+    prolong_kernel<2><<<grid, 256, 0, s>>>(gf, gc, uc, uf);
+  return check_launch("prolong_kernel");
+}
+
+static bool even_grid(int nelx, int nely, int nelz) {
+  return nelx % 2 == 0 && nely % 2 == 0 && nelz % 2 == 0 && nelx >= 2 && nely >= 2;
+}
+
+}  // namespace pmt
+
+using namespace pmt;
+
+extern "C" {
+
+int pmt_stiffness_apply(int nelx, int nely, int nelz, const double* K0, const double* xe, const uint8_t* bcmask,
+                        double bcdiag, const double* u, double* y, void* stream) {
+  if (!grid_ok(nelx, nely, nelz) || !K0 || !xe || !u || !y) return arg_error("pmt_stiffness_apply: bad argument");
+  return launch_fine(MODE_APPLY, make_grid(nelx, nely, nelz), K0, xe, bcmask, bcdiag, u, nullptr, nullptr, 0.0, y,
+                     (hipStream_t)stream);
+}
+
+int pmt_stiffness_residual(int nelx, int nely, int nelz, const double* K0, const double* xe, const uint8_t* bcmask,
+                           double bcdiag, const double* b, const double* u, double* r, void* stream) {
+  if (!grid_ok(nelx, nely, nelz) || !K0 || !xe || !u || !b || !r)
+    return arg_error("pmt_stiffness_residual: bad argument");
+  return launch_fine(MODE_RESIDUAL, make_grid(nelx, nely, nelz), K0, xe, bcmask, bcdiag, u, b, nullptr, 0.0, r,
+                     (hipStream_t)stream);
+}
+
+int pmt_stiffness_diagonal(int nelx, int nely, int nelz, const double* K0, const double* xe, const uint8_t* bcmask,
+                           double bcdiag, double* diag, void* stream) {
+  if (!grid_ok(nelx, nely, nelz) || !K0 || !xe || !diag) return arg_error("pmt_stiffness_diagonal: bad argument");
+  return launch_fine_diag(make_grid(nelx, nely, nelz), K0, xe, bcmask, bcdiag, diag, 0, (hipStream_t)stream);
+}
+
+int pmt_sens_contract(int nelx, int nely, int nelz, const double* K0, const uint8_t* bcmask, const double* u,
+                      const double* v, double alpha, double* dx, int accumulate, void* stream) {
+  if (!grid_ok(nelx, nely, nelz) || !K0 || !u || !v || !dx) return arg_error("pmt_sens_contract: bad argument");
+  GridDims g = make_grid(nelx, nely, nelz);
+  dim3 grid(blocks_for(g.nel, 256));
+  if (g.dim == 3)
+    sens_kernel<3><<<grid, 256, 0, (hipStream_t)stream>>>(g, K0, bcmask, u, v, alpha, dx, accumulate);
+  else
+    sens_kernel<2><<<grid, 256, 0, (hipStream_t)stream>>>(g, K0, bcmask, u, v, alpha, dx, accumulate);
+  return check_launch("sens_kernel");
+}
+
+int pmt_restrict(int nelx, int nely, int nelz, const double* rf, double* rc, void* stream) {
+  if (!grid_ok(nelx, nely, nelz) || !even_grid(nelx, nely, nelz) || !rf || !rc)
+    return arg_error("pmt_restrict: grid must have even element counts");
+  return launch_restrict(make_grid(nelx, nely, nelz), make_grid(nelx / 2, nely / 2, nelz / 2), rf, rc,
+                         (hipStream_t)stream);
+}
+
+int pmt_prolong_add(int nelx, int nely, int nelz, const double* uc, double* uf, void* stream) {
+  if (!grid_ok(nelx, nely, nelz) || !even_grid(nelx, nely, nelz) || !uc || !uf)
+    return arg_error("pmt_prolong_add: grid must have even element counts");
+  return launch_prolong(make_grid(nelx, nely, nelz), make_grid(nelx / 2, nely / 2, nelz / 2), uc, uf,
+                        (hipStream_t)stream);
+}
+
+}  // extern "C"

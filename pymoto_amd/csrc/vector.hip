@@ -1,0 +1,261 @@
+// Element-wise and reduction kernels: SIMP power law, axpby, deterministic dot products,
+// Dirichlet split for the linear-dependency-aware solve, and the fused CG vector updates.
+//
+// Reductions use a fixed grid of kReduceBlocks blocks with grid-stride loops, per-block
+// partials in a caller workspace, and a single-block second stage summing the partials in a
+// fixed order, so every dot product is bit-reproducible run to run.
+#include <cstdio>
+#include <cstring>
+
+#include "launch.h"
+
+namespace pmt {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int hip_status(hipError_t err, const char* where) {
+  if (err == hipSuccess) return PMT_OK;
+  set_error(std::string(where) + ": " + hipGetErrorString(err));
+  return PMT_ERR_HIP;
+}
+
+int check_launch(const char* where) { return hip_status(hipGetLastError(), where); }
+
+__global__ void powlaw_fwd_kernel(long long n, double a, double b, double p, const double* __restrict__ x,
+                                  double* __restrict__ y) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a + b * pow(x[i], p);
+}
+
+__global__ void powlaw_bwd_kernel(long long n, double c, double q, const double* __restrict__ x,
+                                  const double* __restrict__ dy, double* __restrict__ dx, int accumulate) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const double v = dy[i] * (c * pow(x[i], q));
+    dx[i] = accumulate ? dx[i] + v : v;
+  }
+}
+
+__global__ void axpby_kernel(long long n, double a, const double* __restrict__ x, double b, double* __restrict__ y) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (x ? a * x[i] : 0.0) + (b == 0.0 ? 0.0 : b * y[i]);
+}
+
+__device__ inline double block_sum(double v) {
+  __shared__ double sh[kReduceThreads / 64];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < kReduceThreads / 64; ++w) r += sh[w];
+  }
+  __syncthreads();
+  return r;  // valid in thread 0
+}
+
+// stage 1: NOUT dot products sharing the grid-stride loop; partial[o*kReduceBlocks + block]
+template <int NOUT>
+__global__ void __launch_bounds__(kReduceThreads)
+dot_stage1(long long n, const double* __restrict__ a0, const double* __restrict__ b0, const double* __restrict__ a1,
+           const double* __restrict__ b1, double* __restrict__ partial) {
+  double s0 = 0.0, s1 = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    s0 = fma(a0[i], b0[i], s0);
+    if (NOUT > 1) s1 = fma(a1[i], b1[i], s1);
+  }
+  double r0 = block_sum(s0);
+  if (threadIdx.x == 0) partial[blockIdx.x] = r0;
+  if (NOUT > 1) {
+    double r1 = block_sum(s1);
+    if (threadIdx.x == 0) partial[kReduceBlocks + blockIdx.x] = r1;
+  }
+}
+
+// stage 2: out[o] = sum of partials (fixed order), one block
+template <int NOUT>
+__global__ void __launch_bounds__(kReduceThreads) dot_stage2(const double* __restrict__ partial, double* __restrict__ out) {
+  for (int o = 0; o < NOUT; ++o) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < kReduceBlocks; b += blockDim.x) s += partial[o * kReduceBlocks + b];
+    double r = block_sum(s);
+    if (threadIdx.x == 0) out[o] = r;
+  }
+}
+
+int launch_dot(long long n, const double* x, const double* y, double* out, double* work, hipStream_t s) {
+  dot_stage1<1><<<kReduceBlocks, kReduceThreads, 0, s>>>(n, x, y, nullptr, nullptr, work);
+  PMT_TRY(check_launch("dot_stage1"));
+  dot_stage2<1><<<1, kReduceThreads, 0, s>>>(work, out);
+  return check_launch("dot_stage2");
+}
+
+int launch_dot2(long long n, const double* a0, const double* b0, const double* a1, const double* b1, double* out,
+                double* work, hipStream_t s) {
+  dot_stage1<2><<<kReduceBlocks, kReduceThreads, 0, s>>>(n, a0, b0, a1, b1, work);
+  PMT_TRY(check_launch("dot_stage1<2>"));
+  dot_stage2<2><<<1, kReduceThreads, 0, s>>>(work, out);
+  return check_launch("dot_stage2<2>");
+}
+
+// CG: x += alpha p, r -= alpha q, partial ||r||^2; alpha = sc[1]/sc[0] read on device
+__global__ void __launch_bounds__(kReduceThreads)
+cg_xr_update(long long n, const double* __restrict__ sc, const double* __restrict__ p, const double* __restrict__ q,
+             double* __restrict__ x, double* __restrict__ r, double* __restrict__ partial) {
+  const double alpha = sc[1] / sc[0];
+  double s = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    x[i] = fma(alpha, p[i], x[i]);
+    const double rv = fma(-alpha, q[i], r[i]);
+    r[i] = rv;
+    s = fma(rv, rv, s);
+  }
+  double t = block_sum(s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// CG restart branch: x += alpha p only
+__global__ void cg_x_update(long long n, const double* __restrict__ sc, const double* __restrict__ p,
+                            double* __restrict__ x) {
+  const double alpha = sc[1] / sc[0];
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = fma(alpha, p[i], x[i]);
+}
+
+// CG: p = z + beta p with beta = -sc[3]/sc[0]   (iterative.py:394-395)
+__global__ void cg_p_update(long long n, const double* __restrict__ sc, const double* __restrict__ z,
+                            double* __restrict__ p) {
+  const double beta = -sc[3] / sc[0];
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = fma(beta, p[i], z[i]);
+}
+
+// p = z / sqrt(sc[k])  (orth(z, normalize=True) for one column, iterative.py:286-290)
+__global__ void scale_rsqrt(long long n, const double* __restrict__ sc, int k, const double* __restrict__ z,
+                            double* __restrict__ p) {
+  const double inv = 1.0 / sqrt(sc[k]);
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = z[i] * inv;
+}
+
+__global__ void bc_split_kernel(long long nnodes, int D, const uint8_t* __restrict__ bcm, double bcdiag,
+                                const double* __restrict__ rhs, double* __restrict__ rl, double* __restrict__ sol) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nnodes * D) return;
+  const long long n = q / D;
+  const int d = (int)(q - n * D);
+  const bool fixed = bcm && ((bcm[n] >> d) & 1u);
+  const double v = rhs[q];
+  rl[q] = fixed ? 0.0 : v;
+  sol[q] = fixed ? v / bcdiag : 0.0;
+}
+
+__global__ void axpy_free_kernel(long long nnodes, int D, const uint8_t* __restrict__ bcm, double a,
+                                 const double* __restrict__ x, double* __restrict__ y) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nnodes * D) return;
+  const long long n = q / D;
+  const int d = (int)(q - n * D);
+  const bool fixed = bcm && ((bcm[n] >> d) & 1u);
+  if (!fixed) y[q] = fma(a, x[q], y[q]);
+}
+
+__global__ void fill_kernel(long long n, double v, double* __restrict__ y) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = v;
+}
+
+int launch_axpby(long long n, double a, const double* x, double b, double* y, hipStream_t s) {
+  if (n <= 0) return PMT_OK;
+  axpby_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, a, x, b, y);
+  return check_launch("axpby_kernel");
+}
+
+int launch_cg_xr_update(long long n, const double* sc, const double* p, const double* q, double* x, double* r,
+                        double* work, double* out_rr, hipStream_t s) {
+  cg_xr_update<<<kReduceBlocks, kReduceThreads, 0, s>>>(n, sc, p, q, x, r, work);
+  PMT_TRY(check_launch("cg_xr_update"));
+  dot_stage2<1><<<1, kReduceThreads, 0, s>>>(work, out_rr);
+  return check_launch("dot_stage2");
+}
+
+int launch_cg_x_update(long long n, const double* sc, const double* p, double* x, hipStream_t s) {
+  cg_x_update<<<blocks_for(n, 256), 256, 0, s>>>(n, sc, p, x);
+  return check_launch("cg_x_update");
+}
+
+int launch_cg_p_update(long long n, const double* sc, const double* z, double* p, hipStream_t s) {
+  cg_p_update<<<blocks_for(n, 256), 256, 0, s>>>(n, sc, z, p);
+  return check_launch("cg_p_update");
+}
+
+int launch_scale_rsqrt(long long n, const double* sc, int k, const double* z, double* p, hipStream_t s) {
+  scale_rsqrt<<<blocks_for(n, 256), 256, 0, s>>>(n, sc, k, z, p);
+  return check_launch("scale_rsqrt");
+}
+
+int launch_fill(long long n, double v, double* y, hipStream_t s) {
+  if (n <= 0) return PMT_OK;
+  fill_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, v, y);
+  return check_launch("fill_kernel");
+}
+
+}  // namespace pmt
+
+using namespace pmt;
+
+extern "C" {
+
+const char* pmt_last_error(void) { return g_last_error.c_str(); }
+
+int pmt_abi_version(void) { return 1; }
+
+int pmt_reduce_work_size(void) { return 2 * kReduceBlocks; }
+
+int pmt_powlaw_fwd(long long n, double a, double b, double p, const double* x, double* y, void* stream) {
+  if (n < 0 || (n > 0 && (!x || !y))) return arg_error("pmt_powlaw_fwd: bad argument");
+  if (n == 0) return PMT_OK;
+  powlaw_fwd_kernel<<<blocks_for(n, 256), 256, 0, (hipStream_t)stream>>>(n, a, b, p, x, y);
+  return check_launch("powlaw_fwd_kernel");
+}
+
+int pmt_powlaw_bwd(long long n, double c, double q, const double* x, const double* dy, double* dx, int accumulate,
+                   void* stream) {
+  if (n < 0 || (n > 0 && (!x || !dy || !dx))) return arg_error("pmt_powlaw_bwd: bad argument");
+  if (n == 0) return PMT_OK;
+  powlaw_bwd_kernel<<<blocks_for(n, 256), 256, 0, (hipStream_t)stream>>>(n, c, q, x, dy, dx, accumulate);
+  return check_launch("powlaw_bwd_kernel");
+}
+
+int pmt_axpby(long long n, double a, const double* x, double b, double* y, void* stream) {
+  if (n < 0 || (n > 0 && !y)) return arg_error("pmt_axpby: bad argument");
+  return launch_axpby(n, a, x, b, y, (hipStream_t)stream);
+}
+
+int pmt_dot(long long n, const double* x, const double* y, double* out, double* work, void* stream) {
+  if (n < 0 || !x || !y || !out || !work) return arg_error("pmt_dot: bad argument");
+  return launch_dot(n, x, y, out, work, (hipStream_t)stream);
+}
+
+int pmt_bc_split(int nelx, int nely, int nelz, const uint8_t* bcmask, double bcdiag, const double* rhs,
+                 double* rhs_loc, double* sol, void* stream) {
+  if (!grid_ok(nelx, nely, nelz) || !rhs || !rhs_loc || !sol) return arg_error("pmt_bc_split: bad argument");
+  GridDims g = make_grid(nelx, nely, nelz);
+  bc_split_kernel<<<blocks_for(g.nnodes * g.dim, 256), 256, 0, (hipStream_t)stream>>>(g.nnodes, g.dim, bcmask, bcdiag,
+                                                                                      rhs, rhs_loc, sol);
+  return check_launch("bc_split_kernel");
+}
+
+int pmt_axpy_free(int nelx, int nely, int nelz, const uint8_t* bcmask, double a, const double* x, double* y,
+                  void* stream) {
+  if (!grid_ok(nelx, nely, nelz) || !x || !y) return arg_error("pmt_axpy_free: bad argument");
+  GridDims g = make_grid(nelx, nely, nelz);
+  axpy_free_kernel<<<blocks_for(g.nnodes * g.dim, 256), 256, 0, (hipStream_t)stream>>>(g.nnodes, g.dim, bcmask, a, x,
+                                                                                       y);
+  return check_launch("axpy_free_kernel");
+}
+
+}  // extern "C"

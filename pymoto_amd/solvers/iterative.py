@@ -1,0 +1,246 @@
+"""Multigrid-preconditioned CG on the device.
+
+Mirrors ``pymoto/solvers/iterative.py``:
+
+* ``DampedJacobi(A=None, w=1.0)`` - M = D/w (lines 21-47); on the device it is the smoother of
+  the multigrid levels (fused with the operator in one kernel).
+* ``GeometricMultigrid(domain, A=None, cycle='V', inner_level=None, smoother=None, smooth_steps=5)``
+  (lines 124-256): trilinear transfers, Galerkin coarse operators R^T A R, V-cycle whose first
+  pre-smoothing sweep starts from zero; nested instances form the hierarchy
+  (``ex_compliance_multigrid.py:108-121``); the innermost level is solved directly.
+* ``CG(A=None, preconditioner=..., tol=1e-7, maxit=10000, restart=50, verbosity=0)``
+  (lines 295-403) with an explicit residual every ``restart`` iterations.
+
+The hierarchy lives in one ``pmt_mg`` handle of the HIP runtime; ``CG.solve`` runs the whole
+iteration inside the library (``pmt_pcg_solve``).
+"""
+import ctypes
+import time
+import warnings
+
+import numpy as np
+import torch
+
+from .. import _lib
+from ..domain import VoxelDomain
+from .solvers import LinearSolver
+
+
+class Preconditioner(LinearSolver):
+    def update(self, A):
+        pass
+
+    def solve(self, rhs, x0=None, trans="N"):
+        return _lib.as_device(rhs).clone()
+
+
+class DampedJacobi(Preconditioner):
+    """M = D / w; used as the multigrid smoother (``iterative.py:21-47``)."""
+
+    def __init__(self, A=None, w=1.0):
+        assert 0 < w <= 1, "w must be between 0 and 1"
+        self.w = w
+        self.D = None
+        super().__init__(A)
+
+    def update(self, A):
+        self.D = A.diagonal()
+
+    def solve(self, rhs, x0=None, trans="N"):
+        raise NotImplementedError("on the device DampedJacobi is the smoother fused into the GeometricMultigrid "
+                                  "operator kernels; use it as GeometricMultigrid(smoother=DampedJacobi(w=...))")
+
+
+class GeometricMultigrid(Preconditioner):
+    """Geometric multigrid V-cycle preconditioner (``iterative.py:124-256``)."""
+
+    _available_cycles = ["v", "w"]
+
+    def __init__(self, domain: VoxelDomain, A=None, cycle: str = "V", inner_level=None, smoother=None,
+                 smooth_steps: int = 5):
+        assert domain.nelx % 2 == 0 and domain.nely % 2 == 0 and domain.nelz % 2 == 0, (
+            f"Domain sizes {domain.nelx, domain.nely, domain.nelz} must be divisible by 2")
+        assert cycle.lower() in self._available_cycles, f"Cycle ({cycle}) is not available"
+        if cycle.lower() != "v":
+            raise NotImplementedError("only the V-cycle is implemented (as in the reference)")
+        self.domain = domain
+        self.cycle = cycle
+        self.inner_level = inner_level
+        self.smoother = DampedJacobi(w=0.5) if smoother is None else smoother
+        self.smooth_steps = smooth_steps
+        self.sub_domain = domain.coarsen()
+        self.A = None
+        self._handle = None
+        self._key = None
+        super().__init__(A)
+
+    def levels(self):
+        """The chain of GeometricMultigrid objects from this one to the innermost."""
+        chain = [self]
+        while isinstance(chain[-1].inner_level, GeometricMultigrid):
+            nxt = chain[-1].inner_level
+            sd = chain[-1].sub_domain
+            if (nxt.domain.nelx, nxt.domain.nely, nxt.domain.nelz) != (sd.nelx, sd.nely, sd.nelz):
+                raise ValueError("inner multigrid level does not match the coarse grid of its parent")
+            chain.append(nxt)
+        return chain
+
+    @property
+    def nlevels(self):
+        return len(self.levels()) + 1
+
+    def _ensure_handle(self, A):
+        chain = self.levels()
+        omega = np.array([getattr(m.smoother, "w", 0.5) for m in chain], dtype=np.float64)
+        steps = np.array([m.smooth_steps for m in chain], dtype=np.int32)
+        d = self.domain
+        key = (d.nelx, d.nely, d.nelz, len(chain) + 1, tuple(omega), tuple(steps), A.grid.K0.tobytes())
+        if self._handle is not None and self._key == key:
+            return
+        self._release()
+        lib = _lib.lib()
+        h = ctypes.c_void_p()
+        K0 = np.ascontiguousarray(A.grid.K0, dtype=np.float64)
+        _lib.check(lib.pmt_mg_create(d.nelx, d.nely, d.nelz if d.dim == 3 else 0, len(chain) + 1,
+                                     omega.ctypes.data_as(ctypes.c_void_p), steps.ctypes.data_as(ctypes.c_void_p),
+                                     K0.ctypes.data_as(ctypes.c_void_p), ctypes.byref(h)), "multigrid setup")
+        self._handle, self._key = h, key
+
+    def _release(self):
+        if self._handle is not None:
+            try:
+                _lib.load().pmt_mg_destroy(self._handle)
+            except Exception:
+                pass
+            self._handle = None
+
+    def __del__(self):
+        self._release()
+
+    def update(self, A):
+        self.A = A
+        self._ensure_handle(A)
+        g = A.grid
+        _lib.check(_lib.lib().pmt_mg_update(self._handle, _lib.ptr(A.xe), _lib.ptr(g.bcmask_dev), g.bcdiag,
+                                            _lib.stream()), "multigrid update")
+
+    @property
+    def handle(self):
+        return self._handle
+
+    def device_bytes(self):
+        return _lib.load().pmt_mg_device_bytes(self._handle) if self._handle is not None else 0
+
+    def solve(self, rhs, x0=None, trans="N"):
+        if x0 is not None:
+            raise NotImplementedError("the device V-cycle starts from zero, as CG calls it")
+        rhs = _lib.as_device(rhs)
+        lib = _lib.lib()
+
+        def one(c):
+            z = torch.empty_like(c)
+            _lib.check(lib.pmt_mg_vcycle(self._handle, _lib.ptr(c), _lib.ptr(z), _lib.stream()), "V-cycle")
+            return z
+        if rhs.ndim == 1:
+            return one(rhs)
+        return torch.stack([one(rhs[:, k].contiguous()) for k in range(rhs.shape[1])], dim=1)
+
+    def level_apply(self, level, u):
+        """y = A_level u with the Galerkin operator of that level (level 0 = fine operator)."""
+        u = _lib.as_device(u)
+        y = torch.empty_like(u)
+        _lib.check(_lib.lib().pmt_mg_level_apply(self._handle, int(level), _lib.ptr(u), _lib.ptr(y), _lib.stream()),
+                   "level apply")
+        return y
+
+
+def auto_levels(domain: VoxelDomain, max_coarse_dofs: int = 6000) -> int:
+    """Grid count for a default hierarchy: halve while every extent is even and the coarse grid
+    still has more than ``max_coarse_dofs`` dofs (the reference example halves while the coarse
+    extents stay even and >= 8, ``ex_compliance_multigrid.py:108-121``)."""
+    nel = [domain.nelx, domain.nely] + ([domain.nelz] if domain.dim == 3 else [])
+    levels = 1
+    while all(n % 2 == 0 and n >= 2 for n in nel):
+        ndof = domain.dim * int(np.prod([n + 1 for n in nel]))
+        if ndof <= max_coarse_dofs:
+            break
+        nel = [n // 2 for n in nel]
+        levels += 1
+    return levels
+
+
+def build_hierarchy(domain: VoxelDomain, levels: int, omega=0.5, smooth_steps=5):
+    """Nested GeometricMultigrid objects with ``levels`` grids (fine grid included)."""
+    if levels < 2:
+        raise ValueError("a multigrid hierarchy needs at least two grids")
+    mgs = [GeometricMultigrid(domain, smoother=DampedJacobi(w=omega), smooth_steps=smooth_steps)]
+    for _ in range(levels - 2):
+        mgs.append(GeometricMultigrid(mgs[-1].sub_domain, smoother=DampedJacobi(w=omega), smooth_steps=smooth_steps))
+        mgs[-2].inner_level = mgs[-1]
+    return mgs[0]
+
+
+class CG(LinearSolver):
+    """Preconditioned conjugate gradients (``iterative.py:295-403``), run inside the HIP runtime."""
+
+    def __init__(self, A=None, preconditioner=None, tol: float = 1e-7, maxit: int = 10000, restart: int = 50,
+                 verbosity: int = 0):
+        self.preconditioner = preconditioner
+        self.A = A
+        self.tol = tol
+        self.maxit = maxit
+        self.restart = restart
+        self.verbosity = verbosity
+        self.iterations = 0
+        self.history = np.zeros(0)
+        self.relres = np.nan
+        super().__init__(A)
+
+    def update(self, A):
+        t0 = time.perf_counter()
+        self.A = A
+        if self.preconditioner is None:
+            self.preconditioner = build_hierarchy(A.grid.domain, max(2, auto_levels(A.grid.domain)))
+        if not isinstance(self.preconditioner, GeometricMultigrid):
+            raise NotImplementedError("the device CG is preconditioned by GeometricMultigrid")
+        self.preconditioner.update(A)
+        if self.verbosity >= 1:
+            torch.cuda.synchronize()
+            print(f"CG Preconditioner set up in {np.round(time.perf_counter() - t0, 3)}s")
+
+    def _solve_1rhs(self, b, x0):
+        lib = _lib.lib()
+        x = torch.zeros_like(b) if x0 is None else _lib.as_device(x0).clone()
+        it = ctypes.c_int(0)
+        rel = ctypes.c_double(0.0)
+        hist = np.zeros(self.maxit + 1)
+        _lib.check(lib.pmt_pcg_solve(self.preconditioner.handle, _lib.ptr(b), _lib.ptr(x), int(x0 is None),
+                                     float(self.tol), int(self.maxit), int(self.restart), ctypes.byref(it),
+                                     ctypes.byref(rel), hist.ctypes.data_as(ctypes.c_void_p), _lib.stream()),
+                   "PCG solve")
+        self.iterations, self.relres = it.value, rel.value
+        self.history = hist[: it.value + 1]
+        if rel.value > self.tol:
+            warnings.warn(f"CG Maximum iterations ({self.maxit}) reached, with final residuals {rel.value}")
+        elif self.verbosity >= 1:
+            print(f"CG Converged in {it.value} iterations, with final residual {rel.value}")
+        return x
+
+    def solve(self, rhs, x0=None, trans="N"):
+        rhs = _lib.as_device(rhs)
+        if rhs.ndim == 1:
+            return self._solve_1rhs(rhs, x0)
+        if rhs.ndim == 2 and rhs.shape[1] == 1:
+            return self._solve_1rhs(rhs[:, 0].contiguous(), None if x0 is None else _lib.as_device(x0).reshape(-1)
+                                    ).reshape(-1, 1)
+        cols = []
+        for k in range(rhs.shape[1]):
+            x0k = None if x0 is None else _lib.as_device(x0)[:, k].contiguous()
+            cols.append(self._solve_1rhs(rhs[:, k].contiguous(), x0k))
+        return torch.stack(cols, dim=1)
+
+
+def auto_determine_solver(A, tol=1e-12, **kwargs):
+    """Default solver for a device StiffnessMatrix: MG-preconditioned CG to a tight tolerance, in place of
+    the reference's sparse direct solve (``pymoto/solvers/auto_determine.py:109-122``)."""
+    return CG(preconditioner=build_hierarchy(A.grid.domain, max(2, auto_levels(A.grid.domain))), tol=tol)

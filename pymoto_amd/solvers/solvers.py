@@ -1,0 +1,140 @@
+"""Solver base class and the linear-dependency-aware wrapper on the device.
+
+Mirrors ``pymoto/solvers/solvers.py``: ``LinearSolver`` (``update``/``solve``/``residual``,
+lines 6-85) and ``LDAWrapper`` (lines 99-306).  LDAS keeps normalised pairs (x_k, b_k = A x_k) of
+earlier solutions; a new right-hand side is first projected onto them (modified Gram-Schmidt),
+the residual of that reconstruction is checked with one operator application, and the inner
+solver only runs when it exceeds ``tol`` - which is why the compliance adjoint costs no solve.
+Dirichlet dofs (rows with only a diagonal entry) are solved by division (lines 174-176).  All
+vector work runs in the HIP library; the scalars of the projections come back to the host.
+"""
+import warnings
+
+import numpy as np
+import torch
+
+from .. import _lib
+
+
+class LinearSolver:
+    defined = True
+
+    def __init__(self, A=None):
+        if A is not None:
+            self.update(A)
+
+    def update(self, A):
+        raise NotImplementedError("Solver not implemented")
+
+    def solve(self, rhs, x0=None, trans="N"):
+        raise NotImplementedError("Solver not implemented")
+
+    @staticmethod
+    def residual(A, x, b, trans="N"):
+        """||A x - b|| / ||b|| per column (1 for a zero b), ``solvers.py:52-85``."""
+        x, b = _lib.as_device(x), _lib.as_device(b)
+        cols = [(x, b)] if x.ndim == 1 else [(x[:, k].contiguous(), b[:, k].contiguous()) for k in range(x.shape[1])]
+        out = []
+        for xc, bc in cols:
+            r = A.residual(bc, xc)
+            bn = np.sqrt(_dot(bc, bc))
+            out.append(np.sqrt(_dot(r, r)) / (bn if bn > 0 else 1.0))
+        return out[0] if x.ndim == 1 else np.array(out)
+
+
+def _dot(a, b) -> float:
+    lib = _lib.lib()
+    out = torch.empty(1, dtype=torch.float64, device=a.device)
+    _lib.check(lib.pmt_dot(a.numel(), _lib.ptr(a), _lib.ptr(b), _lib.ptr(out), _lib.ptr(_lib.reduce_work()),
+                           _lib.stream()), "dot")
+    return float(out.item())
+
+
+def _axpy(a, x, y):
+    """y += a x (in place)"""
+    _lib.check(_lib.lib().pmt_axpby(y.numel(), float(a), _lib.ptr(x), 1.0, _lib.ptr(y), _lib.stream()), "axpby")
+
+
+def _scale(a, y):
+    _lib.check(_lib.lib().pmt_axpby(y.numel(), 0.0, None, float(a), _lib.ptr(y), _lib.stream()), "scale")
+
+
+class LDAWrapper(LinearSolver):
+    """Linear dependency aware solver (Koppen et al. 2022), ``solvers.py:99-306``, real symmetric case."""
+
+    def __init__(self, solver: LinearSolver, tol=1e-7, A=None, symmetric=None, hermitian=None):
+        self.solver = solver
+        self.tol = tol
+        self.x_stored, self.b_stored = [], []
+        self.A = None
+        self.symmetric = symmetric
+        self.hermitian = hermitian
+        self._did_solve = False
+        self._last_rtol = 0.0
+        super().__init__(A)
+
+    def update(self, A, skip_inner_update: bool = False):
+        self.A = A
+        self.x_stored.clear()
+        self.b_stored.clear()
+        if not skip_inner_update:
+            self.solver.update(A)
+
+    def _split(self, v):
+        """(v with Dirichlet entries zeroed, v[bc]/bcdiag on Dirichlet entries and 0 elsewhere)."""
+        g = self.A.grid
+        rl, sol = torch.empty_like(v), torch.empty_like(v)
+        _lib.check(_lib.lib().pmt_bc_split(*g.nel3, _lib.ptr(g.bcmask_dev), g.bcdiag if g.bcdiag else 1.0,
+                                           _lib.ptr(v), _lib.ptr(rl), _lib.ptr(sol), _lib.stream()), "bc split")
+        return rl, sol
+
+    def _solve_1rhs(self, rhs, x0):
+        A, g = self.A, self.A.grid
+        rl, sol = self._split(rhs)
+        for x, b in zip(self.x_stored, self.b_stored):
+            alpha = _dot(rl, b) / _dot(b, b)
+            _axpy(-alpha, b, rl)
+            _axpy(alpha, x, sol)
+        r = A.residual(rhs, sol)
+        bn = np.sqrt(_dot(rhs, rhs))
+        rtol = np.sqrt(_dot(r, r)) / (bn if bn > 0 else 1.0)
+        did = rtol > self.tol
+        if did:
+            x0l = None
+            if x0 is not None:
+                x0l, _ = self._split(_lib.as_device(x0).clone())
+                for x in self.x_stored:
+                    _axpy(-_dot(x0l, x) / _dot(x, x), x, x0l)
+            xnew = self.solver.solve(rl, x0=x0l)
+            _lib.check(_lib.lib().pmt_axpy_free(*g.nel3, _lib.ptr(g.bcmask_dev), 1.0, _lib.ptr(xnew), _lib.ptr(sol),
+                                                _lib.stream()), "axpy_free")
+            xadd, _ = self._split(xnew)
+            badd, _ = self._split(A @ xnew)
+            for x, b in zip(self.x_stored, self.b_stored):
+                beta = _dot(badd, b) / _dot(b, b)
+                _axpy(-beta, b, badd)
+                _axpy(-beta, x, xadd)
+            bnrm = np.sqrt(_dot(badd, badd))
+            if np.isfinite(bnrm) and bnrm > 0:
+                _scale(1.0 / bnrm, badd)
+                _scale(1.0 / bnrm, xadd)
+                self.x_stored.append(xadd)
+                self.b_stored.append(badd)
+        return sol, rtol, did
+
+    def solve(self, rhs, x0=None, trans="N"):
+        if trans.upper() not in ("N", "T", "H"):
+            raise TypeError("Only N, T, or H transposition is possible")
+        rhs = _lib.as_device(rhs)
+        if rhs.ndim == 1:
+            sol, self._last_rtol, self._did_solve = self._solve_1rhs(rhs, x0)
+            return sol
+        cols, rt, did = [], [], []
+        for k in range(rhs.shape[1]):
+            x0k = None if x0 is None else _lib.as_device(x0)[:, k].contiguous()
+            s, r, d = self._solve_1rhs(rhs[:, k].contiguous(), x0k)
+            cols.append(s)
+            rt.append(r)
+            did.append(d)
+        self._last_rtol, self._did_solve = np.array(rt), np.array(did)
+        return torch.stack(cols, dim=1)

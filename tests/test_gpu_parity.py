@@ -1,0 +1,285 @@
+"""GPU parity: every HIP kernel against the CPU oracle and the reference's golden vectors.
+
+Tolerances: integer/index work and the CSR data are bit-exact; FP64 element-wise and stencil
+work rtol 1e-12..1e-13 (different summation order only); solves are compared on compliance and
+dc/dx to <= 1e-9 relative (max-norm) as required for the FP64 form.
+"""
+import numpy as np
+import numpy.testing as npt
+import pytest
+import torch
+
+import pymoto_amd as pm
+from conftest import golden
+from oracle import fe, problems
+from oracle.filter import ConvFilter
+from oracle.pipeline import design_iteration
+from oracle.solvers import CG as OCG, GeometricMultigrid as OGMG, build_mg
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(x):
+    return torch.as_tensor(np.ascontiguousarray(x), dtype=torch.float64).cuda()
+
+
+def host(t):
+    return t.detach().cpu().numpy() if isinstance(t, torch.Tensor) else np.asarray(t)
+
+
+def relerr(a, b):
+    a, b = np.asarray(a, dtype=float), np.asarray(b, dtype=float)
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def _bcs(g, i):
+    raw = [str(s) for s in g[f"case{i}_bcs"]]
+    return [s if s in ("symmetric", "edge", "wrap") else float(s) for s in raw]
+
+
+@pytest.mark.parametrize("case", range(8))
+def test_filter_forward_adjoint_vs_reference(gpu_lib, case):
+    g = golden("filter_cases.npz")
+    nel = tuple(int(v) for v in g[f"case{case}_nel"])
+    b = _bcs(g, case)
+    m = pm.FilterConv(pm.VoxelDomain(*nel), weights=g[f"case{case}_weights"], xmin_bc=b[0], xmax_bc=b[1],
+                      ymin_bc=b[2], ymax_bc=b[3], zmin_bc=b[4], zmax_bc=b[5])
+    sx = pm.Signal("x", dev(g[f"case{case}_x"]))
+    sy = m(sx)
+    npt.assert_allclose(host(sy.state), g[f"case{case}_y"], rtol=1e-12, atol=1e-14)
+    sy.sensitivity = dev(g[f"case{case}_dy"])
+    m.sensitivity()
+    npt.assert_allclose(host(sx.sensitivity), g[f"case{case}_dx"], rtol=1e-12, atol=1e-14)
+
+
+def test_filter_radius_3d_interior_known_answer(gpu_lib):
+    """reference tests/test_filter.py:147-167: a unit impulse reproduces the kernel."""
+    d = pm.VoxelDomain(10, 11, 12)
+    x = np.zeros(d.nel)
+    x[d.get_elemnumber(5, 6, 7)] = 1.0
+    m = pm.FilterConv(d, radius=2)
+    y = host(m(pm.Signal("x", dev(x))).state)
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                npt.assert_allclose(y[d.get_elemnumber(4 + i, 5 + j, 6 + k)], m.weights[i, j, k], rtol=1e-14)
+
+
+def test_powlaw_simp_vs_oracle(gpu_lib):
+    x = np.random.default_rng(0).random(1001)
+    m = pm.MathExpression(f"{1e-9} + {1.0 - 1e-9}*inp0^3")
+    sx = pm.Signal("x", dev(x))
+    sy = m(sx)
+    npt.assert_allclose(host(sy.state), fe.simp(x), rtol=1e-15)
+    dy = np.random.default_rng(1).random(1001)
+    sy.sensitivity = dev(dy)
+    m.sensitivity()
+    npt.assert_allclose(host(sx.sensitivity), dy * fe.dsimp(x), rtol=1e-14)
+
+
+def test_csr_bit_exact_vs_reference(gpu_lib):
+    g = golden("csr_cases.npz")
+    for i in range(int(g["ncases"])):
+        nel = tuple(int(v) for v in g[f"nel_{i}"])
+        d = pm.VoxelDomain(*nel)
+        K = pm.AssembleStiffness(d, bc=g[f"bc_{i}"], plane=str(g[f"plane_{i}"]))(dev(g[f"x_{i}"]))
+        indptr, indices, data = K.tocsr()
+        npt.assert_array_equal(host(indptr), g[f"indptr_{i}"])
+        npt.assert_array_equal(host(indices), g[f"indices_{i}"])
+        assert host(data).tobytes() == g[f"data_{i}"].tobytes(), f"CSR data not bit-identical (case {i})"
+
+
+def test_csr_c1_bit_exact_hashes(gpu_lib):
+    import hashlib
+    g = golden("c1_mbb.npz")
+    d = pm.VoxelDomain(120, 40)
+    K = pm.AssembleStiffness(d, bc=g["bc"], plane="stress")(dev(g["xsimp"]))
+    indptr, indices, data = K.tocsr()
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()  # noqa: E731
+    assert sha(host(indptr).astype(np.int64)) == str(g["sha_indptr"])
+    assert sha(host(indices).astype(np.int64)) == str(g["sha_indices"])
+    assert sha(host(data)) == str(g["sha_data"])
+    i64 = K.tocsr(index_dtype=torch.int64)
+    npt.assert_array_equal(host(i64[1]), g["indices"])
+
+
+@pytest.mark.parametrize("nel", [(6, 4, 0), (7, 3, 0), (5, 4, 3), (8, 6, 4)])
+def test_stiffness_operator_vs_assembled_csr(gpu_lib, nel):
+    dim = 3 if nel[2] else 2
+    d = pm.VoxelDomain(*nel)
+    rng = np.random.default_rng(sum(nel))
+    x = rng.uniform(1e-3, 1.0, d.nel)
+    bc = problems.cantilever3d(*nel)[0] if dim == 3 else problems.mbb2d(nel[0], nel[1])[0]
+    K0 = fe.element_stiffness(dim)
+    Kref = fe.Assembly(*nel, K0, bc=bc).matrix(x)
+    K = pm.AssembleStiffness(d, bc=bc)(dev(x))
+    u = rng.standard_normal(Kref.shape[0])
+    npt.assert_allclose(host(K @ dev(u)), Kref @ u, rtol=1e-12, atol=1e-13 * np.abs(Kref @ u).max())
+    b = rng.standard_normal(Kref.shape[0])
+    npt.assert_allclose(host(K.residual(dev(b), dev(u))), b - Kref @ u, rtol=1e-12, atol=1e-12)
+    npt.assert_allclose(host(K.diagonal()), Kref.diagonal(), rtol=1e-14)
+
+
+def test_sensitivity_contraction_vs_oracle(gpu_lib):
+    for nel in [(6, 4, 0), (5, 4, 3)]:
+        dim = 3 if nel[2] else 2
+        d = pm.VoxelDomain(*nel)
+        rng = np.random.default_rng(3)
+        bc = problems.cantilever3d(*nel)[0] if dim == 3 else problems.mbb2d(nel[0], nel[1])[0]
+        asm = fe.Assembly(*nel, fe.element_stiffness(dim), bc=bc)
+        n = asm.n
+        u, v = rng.standard_normal(n), rng.standard_normal(n)
+        sx = pm.Signal("x", dev(rng.random(d.nel)))
+        mod = pm.AssembleStiffness(d, bc=bc)
+        sK = mod(sx)
+        sK.sensitivity = pm.DyadicMatrix(dev(u), dev(v), fac=-2.0)
+        mod.sensitivity()
+        npt.assert_allclose(host(sx.sensitivity), -2.0 * asm.sens_contract([u], [v]), rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("nel", [(10, 10, 0), (10, 10, 10), (8, 4, 6)])
+def test_transfers_vs_oracle_R(gpu_lib, nel):
+    from oracle.solvers import interpolation
+    from pymoto_amd import _lib
+    dim = 3 if nel[2] else 2
+    R = interpolation(*nel, dim)
+    rng = np.random.default_rng(5)
+    rf = rng.standard_normal(R.shape[0])
+    uc = rng.standard_normal(R.shape[1])
+    lib = _lib.lib()
+    rc = torch.empty(R.shape[1], dtype=torch.float64, device="cuda")
+    _lib.check(lib.pmt_restrict(*nel, _lib.ptr(dev(rf)), _lib.ptr(rc), _lib.stream()))
+    npt.assert_allclose(host(rc), R.T @ rf, rtol=1e-13, atol=1e-13)
+    uf = dev(rf)
+    _lib.check(lib.pmt_prolong_add(*nel, _lib.ptr(dev(uc)), _lib.ptr(uf), _lib.stream()))
+    npt.assert_allclose(host(uf), rf + R @ uc, rtol=1e-13, atol=1e-13)
+
+
+def _hierarchies(nel, levels, x, bc, omega=0.6, steps=2):
+    dim = 3 if nel[2] else 2
+    K0 = fe.element_stiffness(dim)
+    Kref = fe.Assembly(*nel, K0, bc=bc).matrix(x)
+    omg = build_mg(nel, levels, omega=omega, smooth_steps=steps)
+    omg.update(Kref)
+    d = pm.VoxelDomain(*nel)
+    K = pm.AssembleStiffness(d, bc=bc)(dev(x))
+    mg = pm.solvers.build_hierarchy(d, levels, omega=omega, smooth_steps=steps)
+    mg.update(K)
+    return Kref, omg, K, mg
+
+
+@pytest.mark.parametrize("nel,levels", [((16, 8, 0), 3), ((8, 4, 4), 3), ((16, 8, 8), 4)])
+def test_galerkin_levels_and_vcycle_vs_oracle(gpu_lib, nel, levels):
+    dim = 3 if nel[2] else 2
+    rng = np.random.default_rng(11)
+    ne = nel[0] * nel[1] * max(nel[2], 1)
+    x = fe.simp(rng.uniform(0.05, 1.0, ne))
+    bc = problems.cantilever3d(*nel)[0] if dim == 3 else problems.mbb2d(nel[0], nel[1])[0]
+    Kref, omg, K, mg = _hierarchies(nel, levels, x, bc)
+    # Galerkin operators level by level
+    ops, lvl = [Kref], omg
+    while isinstance(lvl, OGMG):
+        ops.append(lvl.Ac)
+        lvl = lvl.inner
+    for level, A in enumerate(ops):
+        v = rng.standard_normal(A.shape[0])
+        npt.assert_allclose(host(mg.level_apply(level, dev(v))), A @ v, rtol=1e-11, atol=1e-11 * np.abs(A @ v).max())
+    # one V-cycle
+    r = rng.standard_normal(Kref.shape[0])
+    z_ref = omg.solve(r)
+    z = host(mg.solve(dev(r)))
+    assert relerr(z, z_ref) < 1e-9
+
+
+@pytest.mark.parametrize("levels", [3, 4])
+def test_pcg_iterations_and_history_vs_reference(gpu_lib, levels):
+    g = golden(f"c2_mgcg_32x16x16_L{levels}.npz")
+    nel = tuple(int(v) for v in g["nel"])
+    ref = design_iteration(nel, g["x"], g["bc"], g["f"], radius=1.5, solver="mgcg", mg_levels=levels,
+                           omega=float(g["omega"]), smooth_steps=int(g["steps"]), tol=float(g["tol"]),
+                           return_all=True)
+    d = pm.VoxelDomain(*nel)
+    K = pm.AssembleStiffness(d, bc=g["bc"])(dev(ref["xsimp"]))
+    cg = pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(d, levels, omega=float(g["omega"]),
+                                                                 smooth_steps=int(g["steps"])), tol=float(g["tol"]))
+    cg.update(K)
+    u = host(cg.solve(dev(g["f"])))
+    assert cg.iterations == int(g["iterations"])
+    npt.assert_allclose(cg.history, g["history"], rtol=1e-3)
+    assert relerr(u, ref["u"]) < 1e-9
+
+
+def _device_chain(nel, x, bc, f, radius, plane="strain", solver=None):
+    d = pm.VoxelDomain(*nel)
+    sx = pm.Signal("x", dev(x))
+    with pm.Network() as fn:
+        sxf = pm.FilterConv(d, radius=radius)(sx)
+        ss = pm.MathExpression(f"{1e-9} + {1.0 - 1e-9}*inp0^3")(sxf)
+        sK = pm.AssembleStiffness(d, bc=bc, plane=plane)(ss)
+        lin = pm.LinSolve(hermitian=True, solver=solver)
+        su = lin(sK, dev(f))
+        sc = pm.EinSum("i,i->")(su, dev(f))
+    fn.response()
+    sc.sensitivity = 1.0
+    fn.sensitivity()
+    return dict(xf=host(sxf.state), xsimp=host(ss.state), u=host(su.state), c=float(sc.state),
+                dcdx=host(sx.sensitivity), lin=lin)
+
+
+def test_end_to_end_c1_mbb_vs_reference(gpu_lib):
+    g = golden("c1_mbb.npz")
+    out = _device_chain((120, 40, 0), g["x"], g["bc"], g["f"], 2.0, plane="stress")
+    npt.assert_allclose(out["xf"], g["xf"], rtol=1e-13)
+    npt.assert_allclose(out["xsimp"], g["xsimp"], rtol=1e-13)
+    assert abs(out["c"] - float(g["c"])) / abs(float(g["c"])) < 1e-9
+    assert relerr(out["dcdx"], g["dcdx"]) < 1e-9
+    assert relerr(out["u"], g["u"]) < 1e-9
+
+
+@pytest.mark.parametrize("name", ["c2_direct_16x8x8.npz", "c2_direct_24x12x12.npz"])
+def test_end_to_end_c2_vs_reference_direct(gpu_lib, name):
+    g = golden(name)
+    nel = tuple(int(v) for v in g["nel"])
+    out = _device_chain(nel, g["x"], g["bc"], g["f"], 1.5)
+    npt.assert_allclose(out["xf"], g["xf"], rtol=1e-13)
+    assert abs(out["c"] - float(g["c"])) / abs(float(g["c"])) < 1e-9
+    assert relerr(out["dcdx"], g["dcdx"]) < 1e-9
+
+
+@pytest.mark.parametrize("levels", [3, 4])
+def test_end_to_end_c2_mgcg_vs_reference(gpu_lib, levels):
+    g = golden(f"c2_mgcg_32x16x16_L{levels}.npz")
+    nel = tuple(int(v) for v in g["nel"])
+    d = pm.VoxelDomain(*nel)
+    solver = pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(d, levels, omega=float(g["omega"]),
+                                                                     smooth_steps=int(g["steps"])),
+                           tol=float(g["tol"]))
+    out = _device_chain(nel, g["x"], g["bc"], g["f"], 1.5, solver=solver)
+    # the network evaluates the response twice (construction + response()); the first CG call is the cold one
+    assert abs(out["c"] - float(g["c"])) / abs(float(g["c"])) < 1e-10
+    assert relerr(out["dcdx"], g["dcdx"]) < 1e-8
+
+
+def test_large_grid_properties(gpu_lib):
+    """Size-independent checks at a grid the oracle cannot factor quickly: symmetry, energy, residual."""
+    nel = (64, 32, 32)
+    d = pm.VoxelDomain(*nel)
+    bc, f = problems.cantilever3d(*nel)
+    rng = np.random.default_rng(9)
+    # filtered field as in the workload: on a raw uniform field damped Jacobi with w=0.6 is unstable
+    # (the reference algorithm, run by the oracle, stalls there too; w=0.5 converges)
+    xf = host(pm.FilterConv(d, radius=1.5)(dev(rng.uniform(0.0, 1.0, d.nel))))
+    x = fe.simp(xf)
+    K = pm.AssembleStiffness(d, bc=bc)(dev(x))
+    u, v = dev(rng.standard_normal(K.shape[0])), dev(rng.standard_normal(K.shape[0]))
+    Ku, Kv = K @ u, K @ v
+    uKv, vKu = float(torch.dot(u, Kv)), float(torch.dot(v, Ku))
+    assert abs(uKv - vKu) <= 1e-12 * (abs(uKv) + 1e-300) * 1e3
+    assert float(torch.dot(u, Ku)) > 0
+    cg = pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(d, 4, omega=0.6, smooth_steps=2), tol=1e-10)
+    cg.update(K)
+    sol = cg.solve(dev(f))
+    r = K.residual(dev(f), sol)
+    assert float(torch.linalg.norm(r)) / np.linalg.norm(f) <= 1e-10
+    c = float(torch.dot(sol, dev(f)))
+    assert abs(c - float(torch.dot(sol, K @ sol))) <= 1e-9 * abs(c)

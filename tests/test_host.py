@@ -1,0 +1,178 @@
+"""Host-side checks that need no GPU: C-ABI exports, numbering, setup-time maps, problem data."""
+import os
+import re
+
+import numpy as np
+import numpy.testing as npt
+import pytest
+
+import pymoto_amd as pm
+from conftest import ROOT, golden
+from oracle import fe, problems
+from oracle.filter import ConvFilter, filter_weights
+
+
+def test_library_exports_every_declared_symbol():
+    from pymoto_amd import _lib
+    lib = _lib.load()
+    header = open(os.path.join(ROOT, "include", "pymoto_amd.h")).read()
+    declared = set(re.findall(r"\b(pmt_[a-z0-9_]+)\s*\(", header))
+    assert declared, "no declarations found"
+    for name in sorted(declared):
+        assert hasattr(lib, name), f"{name} declared in include/pymoto_amd.h but not exported"
+    assert declared == set(_lib.exported_symbols())
+    assert lib.pmt_abi_version() == 1
+
+
+def test_host_helpers_of_the_library():
+    from pymoto_amd import _lib
+    lib = _lib.load()
+    assert lib.pmt_csr_nnz(120, 40, 0, 2) == 174724
+    g = golden("c2_direct_16x8x8.npz")
+    assert lib.pmt_csr_nnz(16, 8, 8, 3) == int(g["nnz"])
+    assert lib.pmt_reduce_work_size() >= 1024
+
+
+def test_errors_are_reported_without_gpu():
+    from pymoto_amd import _lib
+    lib = _lib.load()
+    rc = lib.pmt_restrict(3, 2, 2, None, None, None)
+    assert rc == -1
+    assert "even" in lib.pmt_last_error().decode()
+
+
+def test_domain_numbering_matches_oracle():
+    for nel in [(5, 4, 0), (3, 4, 5)]:
+        d = pm.VoxelDomain(*nel)
+        npt.assert_array_equal(d.conn, fe.elem_connectivity(*nel))
+        npt.assert_array_equal(d.get_dofconnectivity(d.dim), fe.dofconn(*nel))
+    d = pm.VoxelDomain(100, 142, 284, unitx=0.1, unity=0.2, unitz=0.3)
+    n = d.get_nodenumber(10, 20, 30)
+    assert tuple(d.get_node_indices(n)) == (10, 20, 30)
+    npt.assert_allclose(d.get_node_position(n), [1.0, 4.0, 9.0])
+
+
+def test_deprecated_alias():
+    with pytest.warns(DeprecationWarning):
+        assert pm.DomainDefinition is pm.VoxelDomain
+
+
+def test_element_stiffness_matches_golden():
+    from pymoto_amd.assembly import element_stiffness
+    g = golden("k0_cases.npz")
+    for i in range(int(g["ncases"])):
+        is3d, sx, sy, sz = g[f"spec_{i}"]
+        dom = pm.VoxelDomain(1, 1, 1 if is3d else 0, unitx=sx, unity=sy, unitz=sz)
+        K0 = element_stiffness(dom, 2.5, 0.27, str(g[f"plane_{i}"]))
+        npt.assert_allclose(K0, g[f"K0_{i}"], rtol=1e-13, atol=1e-13 * np.abs(K0).max())
+
+
+def test_problems_match_oracle():
+    from pymoto_amd import problems as pp
+    for nel in [(120, 40, 0), (16, 8, 8)]:
+        d = pm.VoxelDomain(*nel)
+        if d.dim == 2:
+            bc, f = pp.mbb2d(d)
+            obc, of = problems.mbb2d(nel[0], nel[1])
+        else:
+            bc, f = pp.cantilever3d(d)
+            obc, of = problems.cantilever3d(*nel)
+        npt.assert_array_equal(bc, obc)
+        npt.assert_array_equal(f, of)
+
+
+def test_bc_mask():
+    from pymoto_amd.assembly import bc_node_mask
+    d = pm.VoxelDomain(3, 3)
+    m = bc_node_mask(d, 2, [1, 2, 8, 15, 2])
+    assert m[0] == 2 and m[1] == 1 and m[4] == 1 and m[7] == 2 and m.sum() == 6
+
+
+@pytest.mark.parametrize("case", range(8))
+def test_filter_axis_maps_reproduce_reference_padding(case):
+    """The per-axis maps must rebuild exactly the reference's padded element-index array."""
+    from pymoto_amd.filter import FilterConv
+    g = golden("filter_cases.npz")
+    nel = tuple(int(v) for v in g[f"case{case}_nel"])
+    raw = [str(s) for s in g[f"case{case}_bcs"]]
+    bcs = [s if s in ("symmetric", "edge", "wrap") else float(s) for s in raw]
+    w = g[f"case{case}_weights"]
+    m = FilterConv(pm.VoxelDomain(*nel), weights=w, xmin_bc=bcs[0], xmax_bc=bcs[1], ymin_bc=bcs[2],
+                   ymax_bc=bcs[3], zmin_bc=bcs[4], zmax_bc=bcs[5])
+    ref = ConvFilter(nel, w, ((bcs[0], bcs[1]), (bcs[2], bcs[3]), (bcs[4], bcs[5])))
+    sizes = [nel[0], nel[1], max(nel[2], 1)]
+    L = [sizes[a] + w.shape[a] - 1 for a in range(3)]
+    mx, my, mz = np.split(m._map, [L[0], L[0] + L[1]])
+    cx, cy, cz = np.split(m._cval, [L[0], L[0] + L[1]])
+    # rebuild the padded field for a random x and compare with the reference's padded vector
+    x = np.random.default_rng(case).random(int(np.prod(sizes)))
+    xpad_ref = x[ref.el3d_pad]
+    for ix, val in ref.overrides:
+        xpad_ref[ix] = val
+    A, B, C = np.meshgrid(np.arange(L[0]), np.arange(L[1]), np.arange(L[2]), indexing="ij")
+    ma, mb, mc = mx[A], my[B], mz[C]
+    xv = x[((np.maximum(mc, 0) * sizes[1] + np.maximum(mb, 0)) * sizes[0] + np.maximum(ma, 0))]
+    xv = np.where(ma < 0, cx[A], xv)
+    xv = np.where(mb < 0, cy[B], xv)
+    xv = np.where(mc < 0, cz[C], xv)
+    npt.assert_array_equal(xv, xpad_ref)
+    if np.isfinite(float(g[f"case{case}_radius"])):
+        npt.assert_allclose(filter_weights(float(g[f"case{case}_radius"]), nel), w, rtol=1e-15)
+
+
+def test_powlaw_parse_matches_lambdify_constants():
+    from pymoto_amd.generic import _parse_powlaw
+    a, b, p, c, q = _parse_powlaw(f"{1e-9} + {1.0 - 1e-9}*inp0^3")
+    assert (a, b, p, c, q) == (1e-9, 0.999999999, 3.0, 2.999999997, 2.0)
+    with pytest.raises(NotImplementedError):
+        _parse_powlaw("sin(inp0)")
+
+
+def test_auto_levels():
+    from pymoto_amd.solvers import auto_levels
+    assert auto_levels(pm.VoxelDomain(120, 40)) == 2
+    assert auto_levels(pm.VoxelDomain(256, 128, 128)) == 5
+    assert auto_levels(pm.VoxelDomain(192, 96, 96)) == 5
+
+
+def test_device_modules_fail_loudly_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    d = pm.VoxelDomain(4, 2)
+    with pytest.raises(Exception):
+        pm.FilterConv(d, radius=1.5)(pm.Signal("x", np.ones(d.nel)))
+
+
+def _kernel_indptr(nelx, nely, nelz, D):
+    """Literal Python transcription of csr_pattern_kernel's closed-form row pointer (csrc/csr.hip)."""
+    nx, ny, nz = nelx + 1, nely + 1, (nelz + 1 if nelz > 0 else 1)
+
+    def cnt(i, n):
+        return 1 if n == 1 else 3 - (i == 0) - (i == n - 1)
+
+    def pre(i, n):
+        if i <= 0:
+            return 0
+        if n == 1:
+            return i
+        return 3 * i - 1 - (1 if i > n - 1 else 0)
+
+    Sx, Sy, Sz = pre(nx, nx), pre(ny, ny), pre(nz, nz)
+    out = []
+    for n in range(nx * ny * nz):
+        i, j, k = n % nx, (n // nx) % ny, n // (nx * ny)
+        before = pre(k, nz) * Sy * Sx + cnt(k, nz) * pre(j, ny) * Sx + cnt(k, nz) * cnt(j, ny) * pre(i, nx)
+        for d in range(D):
+            out.append(D * D * before + d * D * cnt(i, nx) * cnt(j, ny) * cnt(k, nz))
+    out.append(D * D * Sx * Sy * Sz)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("nel", [(6, 4, 0), (1, 1, 0), (3, 1, 0), (1, 5, 0), (4, 3, 2), (1, 1, 1), (2, 1, 3), (5, 4, 3)])
+def test_csr_closed_form_indptr_matches_oracle(nel):
+    dim = 3 if nel[2] > 0 else 2
+    asm = fe.Assembly(*nel, fe.element_stiffness(dim))
+    npt.assert_array_equal(_kernel_indptr(*nel, dim), asm.indptr)
+    from pymoto_amd import _lib
+    assert _lib.load().pmt_csr_nnz(*nel, dim) == asm.indices.size
