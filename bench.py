@@ -73,7 +73,8 @@ def kernel_roofline(pm, torch, net, reps=20):
     u = torch.randn(g.n, dtype=torch.float64, device="cuda")
     y = torch.empty_like(u)
     lib = _lib.lib()
-    args = (*g.nel3, _lib.ptr(g.K0_dev), _lib.ptr(K.xe), _lib.ptr(g.bcmask_dev), g.bcdiag, _lib.ptr(u), _lib.ptr(y))
+    args = (*g.nel3, _lib.ptr(g.K0_dev), g.k0_pattern, _lib.ptr(K.xe), _lib.ptr(g.bcmask_dev), g.bcdiag, _lib.ptr(u),
+            _lib.ptr(y))
     for _ in range(3):
         _lib.check(lib.pmt_stiffness_apply(*args, _lib.stream()))
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

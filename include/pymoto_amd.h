@@ -75,12 +75,16 @@ int pmt_axpy_free(int nelx, int nely, int nelz, const uint8_t* bcmask, double a,
  * their diagonal.  Replaces the assembled CSR of AssembleGeneral.__call__ (assembly.py:245-296)
  * and its SpMV `A @ p` (pymoto/solvers/iterative.py:374-395).  K0: (2^D*D)^2 element matrix.
  */
-int pmt_stiffness_apply(int nelx, int nely, int nelz, const double* K0, const double* xe,
+int pmt_stiffness_apply(int nelx, int nely, int nelz, const double* K0, int k0_pattern, const double* xe,
                         const uint8_t* bcmask, double bcdiag, const double* u, double* y, void* stream);
 /* r = b - K u (LinearSolver.residual / CG restart, solvers.py:52-85, iterative.py:384) */
-int pmt_stiffness_residual(int nelx, int nely, int nelz, const double* K0, const double* xe,
+int pmt_stiffness_residual(int nelx, int nely, int nelz, const double* K0, int k0_pattern, const double* xe,
                            const uint8_t* bcmask, double bcdiag, const double* b, const double* u,
                            double* r, void* stream);
+/* 1 if a host element matrix follows the isotropic box sign/class pattern (36 classes in 3D, 10 in
+ * 2D); then k0_pattern = 1 selects the kernel that keeps K0 in scalar registers.  Host only.
+ * (No reference counterpart: a property of K0 from AssembleStiffness.__init__, assembly.py:446-461.) */
+int pmt_k0_pattern(int dim, const double* K0_host);
 /* diag(K) (DampedJacobi.update, iterative.py:38-39) */
 int pmt_stiffness_diagonal(int nelx, int nely, int nelz, const double* K0, const double* xe,
                            const uint8_t* bcmask, double bcdiag, double* diag, void* stream);

@@ -28,8 +28,9 @@ _SIGS = {
     "pmt_reduce_work_size": (c_int, []),
     "pmt_bc_split": (c_int, [c_int] * 3 + [c_vp, c_dbl, c_vp, c_vp, c_vp, c_vp]),
     "pmt_axpy_free": (c_int, [c_int] * 3 + [c_vp, c_dbl, c_vp, c_vp, c_vp]),
-    "pmt_stiffness_apply": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp, c_dbl, c_vp, c_vp, c_vp]),
-    "pmt_stiffness_residual": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp, c_dbl, c_vp, c_vp, c_vp, c_vp]),
+    "pmt_stiffness_apply": (c_int, [c_int] * 3 + [c_vp, c_int, c_vp, c_vp, c_dbl, c_vp, c_vp, c_vp]),
+    "pmt_stiffness_residual": (c_int, [c_int] * 3 + [c_vp, c_int, c_vp, c_vp, c_dbl, c_vp, c_vp, c_vp, c_vp]),
+    "pmt_k0_pattern": (c_int, [c_int, c_vp]),
     "pmt_stiffness_diagonal": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp, c_dbl, c_vp, c_vp]),
     "pmt_sens_contract": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp, c_int, c_vp]),
     "pmt_csr_nnz": (c_ll, [c_int] * 4),

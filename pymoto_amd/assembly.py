@@ -119,7 +119,8 @@ class StiffnessMatrix:
     def _apply(self, u: torch.Tensor) -> torch.Tensor:
         g, lib = self.grid, _lib.lib()
         y = torch.empty_like(u)
-        _lib.check(lib.pmt_stiffness_apply(*g.nel3, _lib.ptr(g.K0_dev), _lib.ptr(self.xe), _lib.ptr(g.bcmask_dev),
+        _lib.check(lib.pmt_stiffness_apply(*g.nel3, _lib.ptr(g.K0_dev), g.k0_pattern, _lib.ptr(self.xe),
+                                           _lib.ptr(g.bcmask_dev),
                                            g.bcdiag, _lib.ptr(u), _lib.ptr(y), _lib.stream()), "stiffness apply")
         return y
 
@@ -135,7 +136,7 @@ class StiffnessMatrix:
         """b - K u in one kernel."""
         g, lib = self.grid, _lib.lib()
         r = torch.empty_like(u)
-        _lib.check(lib.pmt_stiffness_residual(*g.nel3, _lib.ptr(g.K0_dev), _lib.ptr(self.xe),
+        _lib.check(lib.pmt_stiffness_residual(*g.nel3, _lib.ptr(g.K0_dev), g.k0_pattern, _lib.ptr(self.xe),
                                               _lib.ptr(g.bcmask_dev), g.bcdiag, _lib.ptr(b), _lib.ptr(u),
                                               _lib.ptr(r), _lib.stream()), "stiffness residual")
         return r
@@ -184,6 +185,7 @@ class _GridData:
         self.bc = None if bc is None else np.asarray(bc).ravel()
         self.bcdiag = float(bcdiag) if bcdiag is not None else 0.0
         self.bcmask = bc_node_mask(domain, self.ndof, self.bc)
+        self.k0_pattern = int(_lib.load().pmt_k0_pattern(self.dim, self.K0.ctypes.data_as(_lib.c_vp)) == 1)
         self._dev = {}
 
     def _on_device(self, key, arr, dtype):

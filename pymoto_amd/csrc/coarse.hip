@@ -324,17 +324,24 @@ __global__ void __launch_bounds__(256) gj_update_kernel(long long n, long long k
   }
 }
 
-// y = A x, one 64-lane wave per row
-__global__ void __launch_bounds__(256) gemv_kernel(long long n, const double* __restrict__ A,
-                                                    const double* __restrict__ x, double* __restrict__ y) {
+// y = A x, one 64-lane wave per row (T = double for the FP64 V-cycle, float for FP32)
+template <typename T>
+__global__ void __launch_bounds__(256) gemv_kernel(long long n, const T* __restrict__ A, const T* __restrict__ x,
+                                                    T* __restrict__ y) {
   const long long row = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
-  const double* a = A + row * n;
-  double s = 0.0;
+  const T* a = A + row * n;
+  T s = 0;
   for (long long c = lane; c < n; c += 64) s = fma(a[c], x[c], s);
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   if (lane == 0) y[row] = s;
+}
+
+// y = (float) x
+__global__ void to_float_kernel(long long n, const double* __restrict__ x, float* __restrict__ y) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (float)x[i];
 }
 
 int launch_galerkin_fine(const GridDims& gf, const double* K0, const double* xe, const uint8_t* bcm, double bcdiag,
@@ -394,9 +401,18 @@ int gj_invert(long long n, double* A, double* wsP, double* wsC, hipStream_t s) {
   return PMT_OK;
 }
 
-int launch_gemv(long long n, const double* A, const double* x, double* y, hipStream_t s) {
-  gemv_kernel<<<blocks_for(n, 4), 256, 0, s>>>(n, A, x, y);
+template <typename T>
+int launch_gemv(long long n, const T* A, const T* x, T* y, hipStream_t s) {
+  gemv_kernel<T><<<blocks_for(n, 4), 256, 0, s>>>(n, A, x, y);
   return check_launch("gemv_kernel");
+}
+template int launch_gemv<double>(long long, const double*, const double*, double*, hipStream_t);
+template int launch_gemv<float>(long long, const float*, const float*, float*, hipStream_t);
+
+int launch_to_float(long long n, const double* x, float* y, hipStream_t s) {
+  if (n <= 0) return PMT_OK;
+  to_float_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, x, y);
+  return check_launch("to_float_kernel");
 }
 
 }  // namespace pmt

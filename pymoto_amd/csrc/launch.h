@@ -7,17 +7,24 @@ namespace pmt {
 
 enum { MODE_APPLY = 0, MODE_RESIDUAL = 1, MODE_JACOBI = 2 };
 
-// operator.hip
-int launch_fine(int mode, const GridDims& g, const double* K0, const double* xe, const uint8_t* bcm, double bcdiag,
-                const double* u, const double* b, const double* dinv, double omega, double* y, hipStream_t s);
+// operator.hip (level kernels exist for T = double and T = float)
+template <typename T>
+int launch_fine(int mode, const GridDims& g, const double* K0, int k0_pattern, const T* xe, const uint8_t* bcm,
+                double bcdiag, const T* u, const T* b, const T* dinv, double omega, T* y, hipStream_t s);
+int k0_matches_pattern(int dim, const double* K0_host);
 int launch_fine_diag(const GridDims& g, const double* K0, const double* xe, const uint8_t* bcm, double bcdiag,
                      double* diag, int invert, hipStream_t s);
-int launch_stencil(int mode, const GridDims& g, const double* S, const double* u, const double* b, const double* dinv,
-                   double omega, double* y, hipStream_t s);
-int launch_stencil_dinv(const GridDims& g, const double* S, double* dinv, hipStream_t s);
-int launch_jacobi0(long long n, const double* dinv, const double* b, double omega, double* u, hipStream_t s);
-int launch_restrict(const GridDims& gf, const GridDims& gc, const double* rf, double* rc, hipStream_t s);
-int launch_prolong(const GridDims& gf, const GridDims& gc, const double* uc, double* uf, hipStream_t s);
+template <typename T>
+int launch_stencil(int mode, const GridDims& g, const T* S, const T* u, const T* b, const T* dinv, double omega, T* y,
+                   hipStream_t s);
+template <typename T>
+int launch_stencil_dinv(const GridDims& g, const double* S, T* dinv, hipStream_t s);
+template <typename T>
+int launch_jacobi0(long long n, const T* dinv, const T* b, double omega, T* u, hipStream_t s);
+template <typename T>
+int launch_restrict(const GridDims& gf, const GridDims& gc, const T* rf, T* rc, hipStream_t s);
+template <typename T>
+int launch_prolong(const GridDims& gf, const GridDims& gc, const T* uc, T* uf, hipStream_t s);
 
 // vector.hip
 int launch_dot(long long n, const double* x, const double* y, double* out, double* work, hipStream_t s);
@@ -30,6 +37,7 @@ int launch_cg_x_update(long long n, const double* sc, const double* p, double* x
 int launch_cg_p_update(long long n, const double* sc, const double* z, double* p, hipStream_t s);
 int launch_scale_rsqrt(long long n, const double* sc, int k, const double* z, double* p, hipStream_t s);
 int launch_fill(long long n, double v, double* y, hipStream_t s);
+int launch_to_double(long long n, const float* x, double* y, hipStream_t s);
 
 // coarse.hip
 int launch_galerkin_fine(const GridDims& gf, const double* K0, const double* xe, const uint8_t* bcm, double bcdiag,
@@ -38,6 +46,8 @@ int launch_galerkin_coarse(const GridDims& gf, const double* Kf, double* Kc, hip
 int launch_stencil_from_elements(const GridDims& g, const double* Ke, double* S, hipStream_t s);
 int launch_dense_from_stencil(const GridDims& g, const double* S, double* A, hipStream_t s);
 int gj_invert(long long n, double* A, double* wsP, double* wsC, hipStream_t s);
-int launch_gemv(long long n, const double* A, const double* x, double* y, hipStream_t s);
+template <typename T>
+int launch_gemv(long long n, const T* A, const T* x, T* y, hipStream_t s);
+int launch_to_float(long long n, const double* x, float* y, hipStream_t s);
 
 }  // namespace pmt

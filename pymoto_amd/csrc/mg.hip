@@ -7,6 +7,12 @@
 // allocate.  The CG loop follows CG.solve step by step (single right-hand side): alpha/beta are
 // read by the kernels from a small device array, and the only host synchronisation per
 // iteration is the 8-byte read of ||r||^2 for the stopping test.
+//
+// Precision: the V-cycle runs in FP64 (the reference's arithmetic) or, after
+// pmt_mg_set_precision(mg, 32), in FP32 - operator, smoother, transfers, stencils and coarsest
+// inverse all in FP32, built from the FP64 Galerkin hierarchy - while the CG recurrences, the
+// outer operator, dots and the stopping test stay FP64.  CG reads the FP32 preconditioner output
+// through one conversion, so the solution still reaches an FP64 residual tolerance.
 #include <cmath>
 #include <new>
 #include <vector>
@@ -17,36 +23,57 @@ namespace pmt {
 
 constexpr int kMaxLevels = 12;
 
+template <typename T>
+struct LevelBuf {
+  T* dinv = nullptr;  // inverse diagonal (levels above the coarsest)
+  T* S = nullptr;     // coarse stencil (levels >= 1)
+  T* u = nullptr;     // V-cycle correction (levels >= 1)
+  T* t = nullptr;     // ping-pong buffer
+  T* b = nullptr;     // restricted right-hand side (levels >= 1; level 0 in FP32: converted input)
+  T* r = nullptr;     // residual
+};
+
 struct Level {
   GridDims g;
   long long ndof = 0;
   double omega = 0.5;
   int steps = 1;
-  double* dinv = nullptr;
-  double* S = nullptr;   // coarse stencil (levels >= 1)
-  double* Ke = nullptr;  // coarse element matrices (levels >= 1)
-  double* u = nullptr;   // V-cycle correction (levels >= 1)
-  double* t = nullptr;   // ping-pong buffer
-  double* b = nullptr;   // restricted right-hand side (levels >= 1)
-  double* r = nullptr;   // residual
+  double* Ke = nullptr;  // Galerkin element matrices (levels >= 1, FP64)
+  LevelBuf<double> d;
+  LevelBuf<float> f;
+  template <typename T>
+  LevelBuf<T>& buf();
 };
+template <>
+LevelBuf<double>& Level::buf<double>() {
+  return d;
+}
+template <>
+LevelBuf<float>& Level::buf<float>() {
+  return f;
+}
 
 }  // namespace pmt
 
 struct pmt_mg {
   int dim = 3;
   int nlev = 0;
+  int prec = 64;
   pmt::Level L[pmt::kMaxLevels];
   int NE = 24;
   double* K0 = nullptr;
+  int k0pat = 0;
   const double* xe = nullptr;
+  float* xe32 = nullptr;
   const uint8_t* bcm = nullptr;
   double bcdiag = 0.0;
   bool ready = false;
   long long nc = 0;
   double* Ainv = nullptr;
+  float* Ainv32 = nullptr;
   double* gjP = nullptr;
   double* gjC = nullptr;
+  float* z32 = nullptr;  // FP32 V-cycle output on the fine grid
   double* p = nullptr;
   double* q = nullptr;
   double* z = nullptr;
@@ -60,12 +87,14 @@ struct pmt_mg {
 
 namespace pmt {
 
-static int dalloc(pmt_mg* mg, double** p, long long n) {
+template <typename T>
+static int dalloc(pmt_mg* mg, T** p, long long n) {
   void* ptr = nullptr;
-  PMT_HIP(hipMalloc(&ptr, (size_t)(n > 0 ? n : 1) * sizeof(double)));
+  const long long nb = (n > 0 ? n : 1) * (long long)sizeof(T);
+  PMT_HIP(hipMalloc(&ptr, (size_t)nb));
   mg->allocs.push_back(ptr);
-  mg->bytes += (n > 0 ? n : 1) * (long long)sizeof(double);
-  *p = (double*)ptr;
+  mg->bytes += nb;
+  *p = (T*)ptr;
   return PMT_OK;
 }
 
@@ -76,34 +105,63 @@ static void release(pmt_mg* mg) {
   mg->host_sc = nullptr;
 }
 
-static int level_apply(pmt_mg* mg, int l, int mode, const double* u, const double* b, double* y, hipStream_t s) {
+template <typename T>
+static const T* fine_x(pmt_mg* mg);
+template <>
+const double* fine_x<double>(pmt_mg* mg) {
+  return mg->xe;
+}
+template <>
+const float* fine_x<float>(pmt_mg* mg) {
+  return mg->xe32;
+}
+
+template <typename T>
+static const T* coarse_inv(pmt_mg* mg);
+template <>
+const double* coarse_inv<double>(pmt_mg* mg) {
+  return mg->Ainv;
+}
+template <>
+const float* coarse_inv<float>(pmt_mg* mg) {
+  return mg->Ainv32;
+}
+
+template <typename T>
+static int level_apply(pmt_mg* mg, int l, int mode, const T* u, const T* b, T* y, hipStream_t s) {
   Level& L = mg->L[l];
-  if (l == 0) return launch_fine(mode, L.g, mg->K0, mg->xe, mg->bcm, mg->bcdiag, u, b, L.dinv, L.omega, y, s);
-  return launch_stencil(mode, L.g, L.S, u, b, L.dinv, L.omega, y, s);
+  LevelBuf<T>& B = L.buf<T>();
+  if (l == 0)
+    return launch_fine<T>(mode, L.g, mg->K0, mg->k0pat, fine_x<T>(mg), mg->bcm, mg->bcdiag, u, b, B.dinv, L.omega, y,
+                          s);
+  return launch_stencil<T>(mode, L.g, B.S, u, b, B.dinv, L.omega, y, s);
 }
 
 // out = V-cycle(b) on level l (GeometricMultigrid.solve with x0 = None, iterative.py:222-256)
-static int vcycle(pmt_mg* mg, int l, const double* b, double* out, hipStream_t s) {
-  if (l == mg->nlev - 1) return launch_gemv(mg->nc, mg->Ainv, b, out, s);
+template <typename T>
+static int vcycle(pmt_mg* mg, int l, const T* b, T* out, hipStream_t s) {
+  if (l == mg->nlev - 1) return launch_gemv<T>(mg->nc, coarse_inv<T>(mg), b, out, s);
   Level& L = mg->L[l];
   Level& C = mg->L[l + 1];
+  LevelBuf<T>& B = L.buf<T>();
+  LevelBuf<T>& CB = C.buf<T>();
   // 2*steps-1 out-of-place sweeps follow the first one: start in the buffer that ends in `out`
-  double* cur = L.t;
-  double* oth = out;
-  PMT_TRY(launch_jacobi0(L.ndof, L.dinv, b, L.omega, cur, s));
+  T* cur = B.t;
+  T* oth = out;
+  PMT_TRY(launch_jacobi0<T>(L.ndof, B.dinv, b, L.omega, cur, s));
   for (int it = 1; it < L.steps; ++it) {
-    PMT_TRY(level_apply(mg, l, MODE_JACOBI, cur, b, oth, s));
-    double* tmp = cur;
+    PMT_TRY(level_apply<T>(mg, l, MODE_JACOBI, cur, b, oth, s));
+    T* tmp = cur;
     cur = oth;
     oth = tmp;
   }
-  PMT_TRY(level_apply(mg, l, MODE_RESIDUAL, cur, b, L.r, s));
-  PMT_TRY(launch_restrict(L.g, C.g, L.r, C.b, s));
-  PMT_TRY(vcycle(mg, l + 1, C.b, C.u, s));
-  PMT_TRY(launch_prolong(L.g, C.g, C.u, cur, s));
+  PMT_TRY(level_apply<T>(mg, l, MODE_RESIDUAL, cur, b, B.r, s));
+  PMT_TRY(launch_restrict<T>(L.g, C.g, B.r, CB.b, s));
+  PMT_TRY(vcycle<T>(mg, l + 1, CB.b, CB.u, s));
+  PMT_TRY(launch_prolong<T>(L.g, C.g, CB.u, cur, s));
   for (int it = 0; it < L.steps; ++it) {
-    PMT_TRY(level_apply(mg, l, MODE_JACOBI, cur, b, oth, s));
-    double* tmp = cur;
+    PMT_TRY(level_apply<T>(mg, l, MODE_JACOBI, cur, b, oth, s));
+    T* tmp = cur;
     cur = oth;
     oth = tmp;
   }
@@ -111,11 +169,48 @@ static int vcycle(pmt_mg* mg, int l, const double* b, double* out, hipStream_t s
   return PMT_OK;
 }
 
+// the preconditioner as CG sees it: FP64 in, FP64 out
+static int precondition(pmt_mg* mg, const double* r, double* z, hipStream_t s) {
+  if (mg->prec == 64) return vcycle<double>(mg, 0, r, z, s);
+  Level& F = mg->L[0];
+  PMT_TRY(launch_to_float(F.ndof, r, F.f.b, s));
+  PMT_TRY(vcycle<float>(mg, 0, F.f.b, mg->z32, s));
+  return launch_to_double(F.ndof, mg->z32, z, s);
+}
+
 static int read_scalar(pmt_mg* mg, int k, double* v, hipStream_t s) {
   PMT_HIP(hipMemcpyAsync(mg->host_sc + k, mg->sc + k, sizeof(double), hipMemcpyDeviceToHost, s));
   PMT_HIP(hipStreamSynchronize(s));
   *v = mg->host_sc[k];
   return PMT_OK;
+}
+
+static int alloc_fp32(pmt_mg* mg) {
+  const int NS = mg->dim == 3 ? 243 : 36;
+  int rc = PMT_OK;
+  for (int l = 0; l < mg->nlev && rc == PMT_OK; ++l) {
+    Level& L = mg->L[l];
+    LevelBuf<float>& B = L.f;
+    if (l == mg->nlev - 1) {
+      if (l > 0) {
+        rc = dalloc(mg, &B.u, L.ndof);
+        if (rc == PMT_OK) rc = dalloc(mg, &B.b, L.ndof);
+      }
+      continue;
+    }
+    rc = dalloc(mg, &B.dinv, L.ndof);
+    if (rc == PMT_OK) rc = dalloc(mg, &B.t, L.ndof);
+    if (rc == PMT_OK) rc = dalloc(mg, &B.r, L.ndof);
+    if (rc == PMT_OK) rc = dalloc(mg, &B.b, L.ndof);
+    if (l > 0) {
+      if (rc == PMT_OK) rc = dalloc(mg, &B.S, L.g.nnodes * NS);
+      if (rc == PMT_OK) rc = dalloc(mg, &B.u, L.ndof);
+    }
+  }
+  if (rc == PMT_OK) rc = dalloc(mg, &mg->xe32, mg->L[0].g.nel);
+  if (rc == PMT_OK) rc = dalloc(mg, &mg->Ainv32, mg->nc * mg->nc);
+  if (rc == PMT_OK) rc = dalloc(mg, &mg->z32, mg->L[0].ndof);
+  return rc;
 }
 
 }  // namespace pmt
@@ -134,6 +229,7 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
   mg->dim = nelz > 0 ? 3 : 2;
   mg->nlev = nlevels;
   mg->NE = (1 << mg->dim) * mg->dim;
+  mg->k0pat = k0_matches_pattern(mg->dim, K0_host);
   int ex = nelx, ey = nely, ez = nelz;
   for (int l = 0; l < nlevels; ++l) {
     Level& L = mg->L[l];
@@ -161,29 +257,30 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
     return arg_error("pmt_mg_create: coarsest grid too large for the dense direct solve (> 20000 dofs); add levels");
   }
   int rc = PMT_OK;
-#define ALLOC(ptr, n)                                  \
+#define ALLOC(ptr, n) \
   if (rc == PMT_OK) rc = dalloc(mg, &(ptr), (n));
   const int NE = mg->NE;
+  const int NS = mg->dim == 3 ? 243 : 36;
   ALLOC(mg->K0, (long long)NE * NE);
   for (int l = 0; l < nlevels; ++l) {
     Level& L = mg->L[l];
     if (l == nlevels - 1) {
       if (l > 0) {
         ALLOC(L.Ke, L.g.nel * NE * NE);
-        ALLOC(L.S, L.g.nnodes * (mg->dim == 3 ? 243 : 36));
-        ALLOC(L.u, L.ndof);
-        ALLOC(L.b, L.ndof);
+        ALLOC(L.d.S, L.g.nnodes * NS);
+        ALLOC(L.d.u, L.ndof);
+        ALLOC(L.d.b, L.ndof);
       }
       continue;
     }
-    ALLOC(L.dinv, L.ndof);
-    ALLOC(L.t, L.ndof);
-    ALLOC(L.r, L.ndof);
+    ALLOC(L.d.dinv, L.ndof);
+    ALLOC(L.d.t, L.ndof);
+    ALLOC(L.d.r, L.ndof);
     if (l > 0) {
       ALLOC(L.Ke, L.g.nel * NE * NE);
-      ALLOC(L.S, L.g.nnodes * (mg->dim == 3 ? 243 : 36));
-      ALLOC(L.u, L.ndof);
-      ALLOC(L.b, L.ndof);
+      ALLOC(L.d.S, L.g.nnodes * NS);
+      ALLOC(L.d.u, L.ndof);
+      ALLOC(L.d.b, L.ndof);
     }
   }
   ALLOC(mg->Ainv, mg->nc * mg->nc);
@@ -218,15 +315,23 @@ int pmt_mg_destroy(pmt_mg* mg) {
   return PMT_OK;
 }
 
+int pmt_mg_set_precision(pmt_mg* mg, int bits) {
+  if (!mg || (bits != 64 && bits != 32)) return arg_error("pmt_mg_set_precision: bits must be 64 or 32");
+  if (bits == 32 && !mg->xe32) PMT_TRY(alloc_fp32(mg));
+  mg->prec = bits;
+  mg->ready = false;
+  return PMT_OK;
+}
+
 int pmt_mg_update(pmt_mg* mg, const double* xe, const uint8_t* bcmask, double bcdiag, void* stream) {
-  if (!mg || !xe) return arg_error("pmt_mg_update: bad argument");
+  if (!mg || !xe || !bcmask) return arg_error("pmt_mg_update: bad argument (the Dirichlet mask is mandatory)");
   hipStream_t s = (hipStream_t)stream;
   mg->xe = xe;
   mg->bcm = bcmask;
   mg->bcdiag = bcdiag;
   mg->ready = false;
   Level& F = mg->L[0];
-  PMT_TRY(launch_fine_diag(F.g, mg->K0, xe, bcmask, bcdiag, F.dinv, 1, s));
+  PMT_TRY(launch_fine_diag(F.g, mg->K0, xe, bcmask, bcdiag, F.d.dinv, 1, s));
   for (int l = 1; l < mg->nlev; ++l) {
     Level& L = mg->L[l];
     Level& P = mg->L[l - 1];
@@ -234,12 +339,23 @@ int pmt_mg_update(pmt_mg* mg, const double* xe, const uint8_t* bcmask, double bc
       PMT_TRY(launch_galerkin_fine(P.g, mg->K0, xe, bcmask, bcdiag, L.Ke, s));
     else
       PMT_TRY(launch_galerkin_coarse(P.g, P.Ke, L.Ke, s));
-    PMT_TRY(launch_stencil_from_elements(L.g, L.Ke, L.S, s));
-    if (l < mg->nlev - 1) PMT_TRY(launch_stencil_dinv(L.g, L.S, L.dinv, s));
+    PMT_TRY(launch_stencil_from_elements(L.g, L.Ke, L.d.S, s));
+    if (l < mg->nlev - 1) PMT_TRY(launch_stencil_dinv<double>(L.g, L.d.S, L.d.dinv, s));
   }
   Level& Cl = mg->L[mg->nlev - 1];
-  PMT_TRY(launch_dense_from_stencil(Cl.g, Cl.S, mg->Ainv, s));
+  PMT_TRY(launch_dense_from_stencil(Cl.g, Cl.d.S, mg->Ainv, s));
   PMT_TRY(gj_invert(mg->nc, mg->Ainv, mg->gjP, mg->gjC, s));
+  if (mg->prec == 32) {
+    PMT_TRY(launch_to_float(F.g.nel, xe, mg->xe32, s));
+    PMT_TRY(launch_to_float(F.ndof, F.d.dinv, F.f.dinv, s));
+    const int NS = mg->dim == 3 ? 243 : 36;
+    for (int l = 1; l < mg->nlev - 1; ++l) {
+      Level& L = mg->L[l];
+      PMT_TRY(launch_to_float(L.g.nnodes * NS, L.d.S, L.f.S, s));
+      PMT_TRY(launch_stencil_dinv<float>(L.g, L.d.S, L.f.dinv, s));
+    }
+    PMT_TRY(launch_to_float(mg->nc * mg->nc, mg->Ainv, mg->Ainv32, s));
+  }
   mg->ready = true;
   return PMT_OK;
 }
@@ -250,7 +366,7 @@ int pmt_mg_vcycle(pmt_mg* mg, const double* r, double* z, void* stream) {
     set_error("pmt_mg_vcycle: call pmt_mg_update first");
     return PMT_ERR_STATE;
   }
-  return vcycle(mg, 0, r, z, (hipStream_t)stream);
+  return precondition(mg, r, z, (hipStream_t)stream);
 }
 
 int pmt_mg_level_apply(pmt_mg* mg, int level, const double* u, double* y, void* stream) {
@@ -259,7 +375,7 @@ int pmt_mg_level_apply(pmt_mg* mg, int level, const double* u, double* y, void* 
     set_error("pmt_mg_level_apply: call pmt_mg_update first");
     return PMT_ERR_STATE;
   }
-  return level_apply(mg, level, MODE_APPLY, u, nullptr, y, (hipStream_t)stream);
+  return level_apply<double>(mg, level, MODE_APPLY, u, nullptr, y, (hipStream_t)stream);
 }
 
 long long pmt_mg_level_ndof(pmt_mg* mg, int level) {
@@ -285,7 +401,7 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
     PMT_TRY(launch_fill(n, 0.0, x, s));
     PMT_TRY(launch_axpby(n, 1.0, b, 0.0, mg->r, s));
   } else {
-    PMT_TRY(level_apply(mg, 0, MODE_RESIDUAL, x, b, mg->r, s));
+    PMT_TRY(level_apply<double>(mg, 0, MODE_RESIDUAL, x, b, mg->r, s));
   }
   PMT_TRY(launch_dot2(n, b, b, mg->r, mg->r, sc + 4, mg->work, s));  // sc[4] = b.b, sc[5] = r.r
   PMT_HIP(hipMemcpyAsync(mg->host_sc + 4, sc + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -301,16 +417,16 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
     return PMT_OK;
   }
   if (tval <= tol) return PMT_OK;
-  PMT_TRY(vcycle(mg, 0, mg->r, mg->z, s));
+  PMT_TRY(precondition(mg, mg->r, mg->z, s));
   PMT_TRY(launch_dot(n, mg->z, mg->z, sc + 5, mg->work, s));
   PMT_TRY(launch_scale_rsqrt(n, sc, 5, mg->z, mg->p, s));
   int i = 0;
   for (i = 0; i < maxit; ++i) {
-    PMT_TRY(level_apply(mg, 0, MODE_APPLY, mg->p, nullptr, mg->q, s));
+    PMT_TRY(level_apply<double>(mg, 0, MODE_APPLY, mg->p, nullptr, mg->q, s));
     PMT_TRY(launch_dot2(n, mg->p, mg->q, mg->p, mg->r, sc, mg->work, s));  // sc[0] = p.q, sc[1] = p.r
     if (i % restart == 0) {
       PMT_TRY(launch_cg_x_update(n, sc, mg->p, x, s));
-      PMT_TRY(level_apply(mg, 0, MODE_RESIDUAL, x, b, mg->r, s));
+      PMT_TRY(level_apply<double>(mg, 0, MODE_RESIDUAL, x, b, mg->r, s));
       PMT_TRY(launch_dot(n, mg->r, mg->r, sc + 2, mg->work, s));
     } else {
       PMT_TRY(launch_cg_xr_update(n, sc, mg->p, mg->q, x, mg->r, mg->work, sc + 2, s));
@@ -320,7 +436,7 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
     tval = std::sqrt(rr) / bn;
     if (history) history[i + 1] = tval;
     if (tval <= tol) break;
-    PMT_TRY(vcycle(mg, 0, mg->r, mg->z, s));
+    PMT_TRY(precondition(mg, mg->r, mg->z, s));
     PMT_TRY(launch_dot(n, mg->q, mg->z, sc + 3, mg->work, s));
     PMT_TRY(launch_cg_p_update(n, sc, mg->z, mg->p, s));
   }

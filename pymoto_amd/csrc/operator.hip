@@ -12,6 +12,7 @@
 // Coarse levels: a 3^D-point block stencil (D x D blocks) stored coefficient-major
 // S[c][node], c = (offset*D + d)*D + d2, which is the Galerkin operator R^T A R of
 // iterative.py:173 assembled per node.
+#include "k0_classes.h"
 #include "launch.h"
 
 namespace pmt {
@@ -24,23 +25,114 @@ struct Cfg {
   static constexpr int NC = NS * DIM * DIM;     // stencil coefficients per node
 };
 
-template <int DIM, int MODE>
+// Sign/class pattern of an isotropic box element matrix (k0_classes.h): K0[i] = sgn[i]*val[cls[i]].
+template <int DIM>
+struct K0Pattern;
+template <>
+struct K0Pattern<3> {
+  static constexpr int N = K0NCLS3;
+  __device__ static constexpr int cls(int i) { return K0CLS3[i]; }
+  __device__ static constexpr int sgn(int i) { return K0SGN3[i]; }
+  __device__ static constexpr int first(int c) { return K0FIRST3[c]; }
+};
+template <>
+struct K0Pattern<2> {
+  static constexpr int N = K0NCLS2;
+  __device__ static constexpr int cls(int i) { return K0CLS2[i]; }
+  __device__ static constexpr int sgn(int i) { return K0SGN2[i]; }
+  __device__ static constexpr int first(int c) { return K0FIRST2[c]; }
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so that each XCD walks one
+// contiguous range of nodes and the neighbour rows/planes of a stencil stay in its own L2.
+// The grid is padded to a multiple of 8 blocks.
+__device__ inline long long xcd_block() {
+  const unsigned per = gridDim.x >> 3;
+  return (long long)(blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+}
+
+inline unsigned xcd_blocks(long long n, int threads) {
+  unsigned b = blocks_for(n, threads);
+  return (b + 7u) & ~7u;
+}
+
+// Node tile of a block (TX*TY*TZ = 256 threads) and its one-node halo.
+template <int DIM>
+struct Tile;
+template <>
+struct Tile<3> {
+  static constexpr int X = 16, Y = 4, Z = 4;
+};
+template <>
+struct Tile<2> {
+  static constexpr int X = 16, Y = 16, Z = 1;
+};
+
+// One block = one node tile.  The block stages the Dirichlet-masked displacements of the tile plus
+// a one-node halo (structure of arrays, zero outside the grid) and the densities of the elements
+// touching the tile (zero outside the grid) in LDS; every thread then gathers its node's rows from
+// LDS with no bounds checks, so the 2^D*D*2^D*D multiply-adds run branch-free.
+template <int DIM, int MODE, bool CLS, typename T>
 __global__ void __launch_bounds__(256)
-fine_node_kernel(GridDims g, const double* __restrict__ K0, const double* __restrict__ xe,
-                 const uint8_t* __restrict__ bcm, double bcdiag, const double* __restrict__ u,
-                 const double* __restrict__ b, const double* __restrict__ dinv, double omega,
-                 double* __restrict__ y) {
+fine_node_kernel(GridDims g, int tiles_x, int tiles_y, int ntiles, const double* __restrict__ K0,
+                 const T* __restrict__ xe, const uint8_t* __restrict__ bcm, T bcdiag,
+                 const T* __restrict__ u, const T* __restrict__ b, const T* __restrict__ dinv,
+                 T omega, T* __restrict__ y) {
   constexpr int NN = Cfg<DIM>::NN;
   constexpr int NE = Cfg<DIM>::NE;
-  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= g.nnodes) return;
-  const int i = (int)(n % g.nx);
-  const long long t = n / g.nx;
-  const int j = (int)(t % g.ny);
-  const int k = (int)(t / g.ny);
-  const long long sx = 1, sy = g.nx, sz = (long long)g.nx * g.ny;
+  constexpr int TX = Tile<DIM>::X, TY = Tile<DIM>::Y, TZ = Tile<DIM>::Z;
+  constexpr int HX = TX + 2, HY = TY + 2, HZ = DIM == 3 ? TZ + 2 : 1;
+  constexpr int HN = HX * HY * HZ;
+  constexpr int EX = TX + 1, EY = TY + 1, EZ = DIM == 3 ? TZ + 1 : 1;
+  constexpr int EN = EX * EY * EZ;
+  __shared__ T su[DIM][HN];
+  __shared__ T sx[EN];
 
-  double acc[NN][DIM];
+  const int tile = (int)xcd_block();
+  if (tile >= ntiles) return;  // whole block leaves together (grid padded for the XCD remap)
+  const int tx0 = (tile % tiles_x) * TX;
+  const int trest = tile / tiles_x;
+  const int ty0 = (trest % tiles_y) * TY;
+  const int tz0 = DIM == 3 ? (trest / tiles_y) * TZ : 0;
+
+  // with the box pattern the whole element matrix is K0Pattern::N scalars (uniform, SGPR-resident)
+  T kv[CLS ? K0Pattern<DIM>::N : 1];
+  if (CLS) {
+#pragma unroll
+    for (int c = 0; c < K0Pattern<DIM>::N; ++c)
+      kv[c] = (T)(K0Pattern<DIM>::sgn(K0Pattern<DIM>::first(c)) * K0[K0Pattern<DIM>::first(c)]);
+  }
+
+  for (int h = threadIdx.x; h < HN; h += 256) {
+    const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
+    const int gi = tx0 - 1 + hx, gj = ty0 - 1 + hy, gk = DIM == 3 ? tz0 - 1 + hz : 0;
+    T v[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) v[d] = 0.0;
+    if (gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny && gk >= 0 && gk < g.nz) {
+      const long long m = ((long long)gk * g.ny + gj) * g.nx + gi;
+      const unsigned mk = bcm[m];  // the mask is mandatory: no branch around the compute below
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) v[d] = ((mk >> d) & 1u) ? 0.0 : u[m * DIM + d];
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) su[d][h] = v[d];
+  }
+  for (int q = threadIdx.x; q < EN; q += 256) {
+    const int ex = q % EX, ey = (q / EX) % EY, ez = q / (EX * EY);
+    const int gi = tx0 - 1 + ex, gj = ty0 - 1 + ey, gk = DIM == 3 ? tz0 - 1 + ez : 0;
+    T v = 0;
+    if (gi >= 0 && gi < g.ex && gj >= 0 && gj < g.ey && gk >= 0 && gk < g.ez)
+      v = xe[((long long)gk * g.ey + gj) * g.ex + gi];
+    sx[q] = v;
+  }
+  __syncthreads();
+
+  const int lx = threadIdx.x % TX, ly = (threadIdx.x / TX) % TY, lz = threadIdx.x / (TX * TY);
+  const int i = tx0 + lx, j = ty0 + ly, k = tz0 + lz;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+
+  T acc[NN][DIM];
 #pragma unroll
   for (int e = 0; e < NN; ++e)
 #pragma unroll
@@ -53,17 +145,10 @@ fine_node_kernel(GridDims g, const double* __restrict__ K0, const double* __rest
     for (int dy = -1; dy <= 1; ++dy) {
 #pragma unroll
       for (int dx = -1; dx <= 1; ++dx) {
-        const int ii = i + dx, jj = j + dy, kk = k + dz;
-        const bool valid = ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny && kk >= 0 && kk < g.nz;
-        double um[DIM];
+        const int h = ((DIM == 3 ? lz + 1 + dz : 0) * HY + (ly + 1 + dy)) * HX + (lx + 1 + dx);
+        T um[DIM];
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) um[d] = 0.0;
-        if (valid) {
-          const long long m = n + dx * sx + dy * sy + dz * sz;
-          const unsigned mk = bcm ? bcm[m] : 0u;
-#pragma unroll
-          for (int d = 0; d < DIM; ++d) um[d] = ((mk >> d) & 1u) ? 0.0 : u[m * DIM + d];
-        }
+        for (int d = 0; d < DIM; ++d) um[d] = su[d][h];
         // elements (i-1+ex, j-1+ey, k-1+ez) containing both this node and the neighbour
 #pragma unroll
         for (int e = 0; e < NN; ++e) {
@@ -75,31 +160,37 @@ fine_node_kernel(GridDims g, const double* __restrict__ K0, const double* __rest
 #pragma unroll
           for (int d = 0; d < DIM; ++d)
 #pragma unroll
-            for (int d2 = 0; d2 < DIM; ++d2)
-              acc[e][d] = fma(K0[(locn * DIM + d) * NE + locm * DIM + d2], um[d2], acc[e][d]);
+            for (int d2 = 0; d2 < DIM; ++d2) {
+              const int idx = (locn * DIM + d) * NE + locm * DIM + d2;
+              T kval;
+              if (CLS)
+                kval = K0Pattern<DIM>::sgn(idx) > 0 ? kv[K0Pattern<DIM>::cls(idx)] : -kv[K0Pattern<DIM>::cls(idx)];
+              else
+                kval = (T)K0[idx];
+              acc[e][d] = fma(kval, um[d2], acc[e][d]);
+            }
         }
       }
     }
   }
 
-  double out[DIM];
+  T out[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) out[d] = 0.0;
 #pragma unroll
   for (int e = 0; e < NN; ++e) {
-    const int ex = e & 1, ey = (e >> 1) & 1, ez = DIM == 3 ? (e >> 2) & 1 : 1;
-    const int ei = i - 1 + ex, ej = j - 1 + ey, ek = DIM == 3 ? k - 1 + ez : 0;
-    if (ei < 0 || ei >= g.ex || ej < 0 || ej >= g.ey || ek < 0 || ek >= g.ez) continue;
-    const double xv = xe[((long long)ek * g.ey + ej) * g.ex + ei];
+    const int ex = e & 1, ey = (e >> 1) & 1, ez = DIM == 3 ? (e >> 2) & 1 : 0;
+    const T xv = sx[((DIM == 3 ? lz + ez : 0) * EY + (ly + ey)) * EX + (lx + ex)];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) out[d] = fma(xv, acc[e][d], out[d]);
   }
 
-  const unsigned mk = bcm ? bcm[n] : 0u;
+  const long long n = ((long long)k * g.ny + j) * g.nx + i;
+  const unsigned mk = bcm[n];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
     const long long q = n * DIM + d;
-    const double ku = ((mk >> d) & 1u) ? bcdiag * u[q] : out[d];
+    const T ku = ((mk >> d) & 1u) ? bcdiag * u[q] : out[d];
     if (MODE == MODE_APPLY) {
       y[q] = ku;
     } else if (MODE == MODE_RESIDUAL) {
@@ -143,11 +234,11 @@ __global__ void fine_diag_kernel(GridDims g, const double* __restrict__ K0, cons
   }
 }
 
-template <int DIM, int MODE>
+template <int DIM, int MODE, typename T>
 __global__ void __launch_bounds__(256)
-stencil_node_kernel(GridDims g, const double* __restrict__ S, const double* __restrict__ u,
-                    const double* __restrict__ b, const double* __restrict__ dinv, double omega,
-                    double* __restrict__ y) {
+stencil_node_kernel(GridDims g, const T* __restrict__ S, const T* __restrict__ u,
+                    const T* __restrict__ b, const T* __restrict__ dinv, T omega,
+                    T* __restrict__ y) {
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= g.nnodes) return;
   const int i = (int)(n % g.nx);
@@ -155,7 +246,7 @@ stencil_node_kernel(GridDims g, const double* __restrict__ S, const double* __re
   const int j = (int)(t % g.ny);
   const int k = (int)(t / g.ny);
   const long long nn = g.nnodes;
-  double out[DIM];
+  T out[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) out[d] = 0.0;
   constexpr int ZLO = DIM == 3 ? -1 : 0, ZHI = DIM == 3 ? 1 : 0;
@@ -169,7 +260,7 @@ stencil_node_kernel(GridDims g, const double* __restrict__ S, const double* __re
         const int ii = i + dx, jj = j + dy, kk = k + dz;
         if (ii < 0 || ii >= g.nx || jj < 0 || jj >= g.ny || kk < 0 || kk >= g.nz) continue;
         const long long m = n + dx + dy * (long long)g.nx + dz * (long long)g.nx * g.ny;
-        double um[DIM];
+        T um[DIM];
 #pragma unroll
         for (int d2 = 0; d2 < DIM; ++d2) um[d2] = u[m * DIM + d2];
 #pragma unroll
@@ -193,34 +284,35 @@ stencil_node_kernel(GridDims g, const double* __restrict__ S, const double* __re
   }
 }
 
-template <int DIM>
-__global__ void stencil_diag_kernel(GridDims g, const double* __restrict__ S, double* __restrict__ dinv) {
+template <int DIM, typename T>
+__global__ void stencil_diag_kernel(GridDims g, const double* __restrict__ S, T* __restrict__ dinv) {
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= g.nnodes) return;
   constexpr int center = Cfg<DIM>::NS / 2;
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) dinv[n * DIM + d] = 1.0 / S[((long long)(center * DIM + d) * DIM + d) * g.nnodes + n];
+  for (int d = 0; d < DIM; ++d) dinv[n * DIM + d] = (T)(1.0 / S[((long long)(center * DIM + d) * DIM + d) * g.nnodes + n]);
 }
 
 // first damped-Jacobi sweep from zero: u = w D^-1 b (iterative.py:232-233)
-__global__ void jacobi0_kernel(long long n, const double* __restrict__ dinv, const double* __restrict__ b,
-                               double omega, double* __restrict__ u) {
+template <typename T>
+__global__ void jacobi0_kernel(long long n, const T* __restrict__ dinv, const T* __restrict__ b,
+                               T omega, T* __restrict__ u) {
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < n) u[q] = omega * dinv[q] * b[q];
 }
 
 // rc = R^T rf : coarse node gathers the 3^D fine nodes around its fine image (weights 1, 1/2, 1/4, 1/8)
-template <int DIM>
-__global__ void restrict_kernel(GridDims gf, GridDims gc, const double* __restrict__ rf, double* __restrict__ rc) {
+template <int DIM, typename T>
+__global__ void restrict_kernel(GridDims gf, GridDims gc, const T* __restrict__ rf, T* __restrict__ rc) {
   const long long N = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (N >= gc.nnodes) return;
   const int I = (int)(N % gc.nx);
   const long long t = N / gc.nx;
   const int J = (int)(t % gc.ny);
   const int K = (int)(t / gc.ny);
-  double acc[DIM];
+  T acc[DIM];
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) acc[d] = 0.0;
+  for (int d = 0; d < DIM; ++d) acc[d] = 0;
   constexpr int ZLO = DIM == 3 ? -1 : 0, ZHI = DIM == 3 ? 1 : 0;
 #pragma unroll
   for (int dz = ZLO; dz <= ZHI; ++dz)
@@ -230,7 +322,7 @@ __global__ void restrict_kernel(GridDims gf, GridDims gc, const double* __restri
       for (int dx = -1; dx <= 1; ++dx) {
         const int fi = 2 * I + dx, fj = 2 * J + dy, fk = 2 * K + dz;
         if (fi < 0 || fi >= gf.nx || fj < 0 || fj >= gf.ny || fk < 0 || fk >= gf.nz) continue;
-        const double w = (dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0);
+        const T w = (T)((dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0));
         const long long m = ((long long)fk * gf.ny + fj) * gf.nx + fi;
 #pragma unroll
         for (int d = 0; d < DIM; ++d) acc[d] = fma(w, rf[m * DIM + d], acc[d]);
@@ -240,8 +332,8 @@ __global__ void restrict_kernel(GridDims gf, GridDims gc, const double* __restri
 }
 
 // uf += R uc : fine node interpolates from the 1, 2, 4 or 8 coarse nodes around it
-template <int DIM>
-__global__ void prolong_kernel(GridDims gf, GridDims gc, const double* __restrict__ uc, double* __restrict__ uf) {
+template <int DIM, typename T>
+__global__ void prolong_kernel(GridDims gf, GridDims gc, const T* __restrict__ uc, T* __restrict__ uf) {
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= gf.nnodes) return;
   const int i = (int)(n % gf.nx);
@@ -250,10 +342,10 @@ __global__ void prolong_kernel(GridDims gf, GridDims gc, const double* __restric
   const int k = (int)(t / gf.ny);
   const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
   const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (DIM == 3 && (k & 1)) ? 2 : 1;
-  const double w = ((i & 1) ? 0.5 : 1.0) * ((j & 1) ? 0.5 : 1.0) * ((DIM == 3 && (k & 1)) ? 0.5 : 1.0);
-  double acc[DIM];
+  const T w = (T)(((i & 1) ? 0.5 : 1.0) * ((j & 1) ? 0.5 : 1.0) * ((DIM == 3 && (k & 1)) ? 0.5 : 1.0));
+  T acc[DIM];
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) acc[d] = 0.0;
+  for (int d = 0; d < DIM; ++d) acc[d] = 0;
   for (int c = 0; c < nk; ++c)
     for (int bb = 0; bb < nj; ++bb)
       for (int a = 0; a < ni; ++a) {
@@ -306,26 +398,57 @@ sens_kernel(GridDims g, const double* __restrict__ K0, const uint8_t* __restrict
 // ---------------------------------------------------------------------------------------------
 // host launchers (shared with the multigrid runtime)
 // ---------------------------------------------------------------------------------------------
-int launch_fine(int mode, const GridDims& g, const double* K0, const double* xe, const uint8_t* bcm, double bcdiag,
-                const double* u, const double* b, const double* dinv, double omega, double* y, hipStream_t s) {
-  const int T = 256;
-  dim3 grid(blocks_for(g.nnodes, T));
+template <int DIM, bool CLS, typename T>
+static void launch_fine_t(int mode, const GridDims& g, const double* K0, const T* xe, const uint8_t* bcm, T bcdiag,
+                          const T* u, const T* b, const T* dinv, T omega, T* y, hipStream_t s) {
+  const int tx = (g.nx + Tile<DIM>::X - 1) / Tile<DIM>::X;
+  const int ty = (g.ny + Tile<DIM>::Y - 1) / Tile<DIM>::Y;
+  const int tz = DIM == 3 ? (g.nz + Tile<DIM>::Z - 1) / Tile<DIM>::Z : 1;
+  const int nt = tx * ty * tz;
+  dim3 grid((unsigned)((nt + 7) & ~7));
+  if (mode == MODE_APPLY)
+    fine_node_kernel<DIM, MODE_APPLY, CLS, T><<<grid, 256, 0, s>>>(g, tx, ty, nt, K0, xe, bcm, bcdiag, u, b, dinv,
+                                                                   omega, y);
+  else if (mode == MODE_RESIDUAL)
+    fine_node_kernel<DIM, MODE_RESIDUAL, CLS, T><<<grid, 256, 0, s>>>(g, tx, ty, nt, K0, xe, bcm, bcdiag, u, b, dinv,
+                                                                      omega, y);
+  else
+    fine_node_kernel<DIM, MODE_JACOBI, CLS, T><<<grid, 256, 0, s>>>(g, tx, ty, nt, K0, xe, bcm, bcdiag, u, b, dinv,
+                                                                    omega, y);
+}
+
+template <typename T>
+int launch_fine(int mode, const GridDims& g, const double* K0, int k0_pattern, const T* xe, const uint8_t* bcm,
+                double bcdiag, const T* u, const T* b, const T* dinv, double omega, T* y, hipStream_t s) {
+  const T bd = (T)bcdiag, om = (T)omega;
   if (g.dim == 3) {
-    if (mode == MODE_APPLY)
-      fine_node_kernel<3, MODE_APPLY><<<grid, T, 0, s>>>(g, K0, xe, bcm, bcdiag, u, b, dinv, omega, y);
-    else if (mode == MODE_RESIDUAL)
-      fine_node_kernel<3, MODE_RESIDUAL><<<grid, T, 0, s>>>(g, K0, xe, bcm, bcdiag, u, b, dinv, omega, y);
+    if (k0_pattern)
+      launch_fine_t<3, true, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s);
     else
-      fine_node_kernel<3, MODE_JACOBI><<<grid, T, 0, s>>>(g, K0, xe, bcm, bcdiag, u, b, dinv, omega, y);
+      launch_fine_t<3, false, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s);
   } else {
-    if (mode == MODE_APPLY)
-      fine_node_kernel<2, MODE_APPLY><<<grid, T, 0, s>>>(g, K0, xe, bcm, bcdiag, u, b, dinv, omega, y);
-    else if (mode == MODE_RESIDUAL)
-      fine_node_kernel<2, MODE_RESIDUAL><<<grid, T, 0, s>>>(g, K0, xe, bcm, bcdiag, u, b, dinv, omega, y);
+    if (k0_pattern)
+      launch_fine_t<2, true, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s);
     else
-      fine_node_kernel<2, MODE_JACOBI><<<grid, T, 0, s>>>(g, K0, xe, bcm, bcdiag, u, b, dinv, omega, y);
+      launch_fine_t<2, false, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s);
   }
   return check_launch("fine_node_kernel");
+}
+
+// Does a host element matrix follow the isotropic box pattern (so the SGPR kernel applies)?
+int k0_matches_pattern(int dim, const double* K0) {
+  const int ne = (1 << dim) * dim;
+  const int8_t* cls = dim == 3 ? K0CLS3 : K0CLS2;
+  const int8_t* sgn = dim == 3 ? K0SGN3 : K0SGN2;
+  const int16_t* first = dim == 3 ? K0FIRST3 : K0FIRST2;
+  double scale = 0.0;
+  for (int i = 0; i < ne * ne; ++i) scale = fmax(scale, fabs(K0[i]));
+  if (!(scale > 0.0)) return 0;
+  for (int i = 0; i < ne * ne; ++i) {
+    const double v = sgn[first[cls[i]]] * K0[first[cls[i]]] * sgn[i];
+    if (fabs(v - K0[i]) > 1e-14 * scale) return 0;
+  }
+  return 1;
 }
 
 int launch_fine_diag(const GridDims& g, const double* K0, const double* xe, const uint8_t* bcm, double bcdiag,
@@ -338,58 +461,77 @@ int launch_fine_diag(const GridDims& g, const double* K0, const double* xe, cons
   return check_launch("fine_diag_kernel");
 }
 
-int launch_stencil(int mode, const GridDims& g, const double* S, const double* u, const double* b, const double* dinv,
-                   double omega, double* y, hipStream_t s) {
+template <typename T>
+int launch_stencil(int mode, const GridDims& g, const T* S, const T* u, const T* b, const T* dinv, double omega, T* y,
+                   hipStream_t s) {
   dim3 grid(blocks_for(g.nnodes, 256));
+  const T om = (T)omega;
   if (g.dim == 3) {
     if (mode == MODE_APPLY)
-      stencil_node_kernel<3, MODE_APPLY><<<grid, 256, 0, s>>>(g, S, u, b, dinv, omega, y);
+      stencil_node_kernel<3, MODE_APPLY, T><<<grid, 256, 0, s>>>(g, S, u, b, dinv, om, y);
     else if (mode == MODE_RESIDUAL)
-      stencil_node_kernel<3, MODE_RESIDUAL><<<grid, 256, 0, s>>>(g, S, u, b, dinv, omega, y);
+      stencil_node_kernel<3, MODE_RESIDUAL, T><<<grid, 256, 0, s>>>(g, S, u, b, dinv, om, y);
     else
-      stencil_node_kernel<3, MODE_JACOBI><<<grid, 256, 0, s>>>(g, S, u, b, dinv, omega, y);
+      stencil_node_kernel<3, MODE_JACOBI, T><<<grid, 256, 0, s>>>(g, S, u, b, dinv, om, y);
   } else {
     if (mode == MODE_APPLY)
-      stencil_node_kernel<2, MODE_APPLY><<<grid, 256, 0, s>>>(g, S, u, b, dinv, omega, y);
+      stencil_node_kernel<2, MODE_APPLY, T><<<grid, 256, 0, s>>>(g, S, u, b, dinv, om, y);
     else if (mode == MODE_RESIDUAL)
-      stencil_node_kernel<2, MODE_RESIDUAL><<<grid, 256, 0, s>>>(g, S, u, b, dinv, omega, y);
+      stencil_node_kernel<2, MODE_RESIDUAL, T><<<grid, 256, 0, s>>>(g, S, u, b, dinv, om, y);
     else
-      stencil_node_kernel<2, MODE_JACOBI><<<grid, 256, 0, s>>>(g, S, u, b, dinv, omega, y);
+      stencil_node_kernel<2, MODE_JACOBI, T><<<grid, 256, 0, s>>>(g, S, u, b, dinv, om, y);
   }
   return check_launch("stencil_node_kernel");
 }
 
-int launch_stencil_dinv(const GridDims& g, const double* S, double* dinv, hipStream_t s) {
+template <typename T>
+int launch_stencil_dinv(const GridDims& g, const double* S, T* dinv, hipStream_t s) {
   dim3 grid(blocks_for(g.nnodes, 256));
   if (g.dim == 3)
-    stencil_diag_kernel<3><<<grid, 256, 0, s>>>(g, S, dinv);
+    stencil_diag_kernel<3, T><<<grid, 256, 0, s>>>(g, S, dinv);
   else
-    stencil_diag_kernel<2><<<grid, 256, 0, s>>>(g, S, dinv);
+    stencil_diag_kernel<2, T><<<grid, 256, 0, s>>>(g, S, dinv);
   return check_launch("stencil_diag_kernel");
 }
 
-int launch_jacobi0(long long n, const double* dinv, const double* b, double omega, double* u, hipStream_t s) {
-  jacobi0_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, dinv, b, omega, u);
+template <typename T>
+int launch_jacobi0(long long n, const T* dinv, const T* b, double omega, T* u, hipStream_t s) {
+  jacobi0_kernel<T><<<blocks_for(n, 256), 256, 0, s>>>(n, dinv, b, (T)omega, u);
   return check_launch("jacobi0_kernel");
 }
 
-int launch_restrict(const GridDims& gf, const GridDims& gc, const double* rf, double* rc, hipStream_t s) {
+template <typename T>
+int launch_restrict(const GridDims& gf, const GridDims& gc, const T* rf, T* rc, hipStream_t s) {
   dim3 grid(blocks_for(gc.nnodes, 256));
   if (gf.dim == 3)
-    restrict_kernel<3><<<grid, 256, 0, s>>>(gf, gc, rf, rc);
+    restrict_kernel<3, T><<<grid, 256, 0, s>>>(gf, gc, rf, rc);
   else
-    restrict_kernel<2><<<grid, 256, 0, s>>>(gf, gc, rf, rc);
+    restrict_kernel<2, T><<<grid, 256, 0, s>>>(gf, gc, rf, rc);
   return check_launch("restrict_kernel");
 }
 
-int launch_prolong(const GridDims& gf, const GridDims& gc, const double* uc, double* uf, hipStream_t s) {
+template <typename T>
+int launch_prolong(const GridDims& gf, const GridDims& gc, const T* uc, T* uf, hipStream_t s) {
   dim3 grid(blocks_for(gf.nnodes, 256));
   if (gf.dim == 3)
-    prolong_kernel<3><<<grid, 256, 0, s>>>(gf, gc, uc, uf);
+    prolong_kernel<3, T><<<grid, 256, 0, s>>>(gf, gc, uc, uf);
   else
-    prolong_kernel<2><<<grid, 256, 0, s>>>(gf, gc, uc, uf);
+    prolong_kernel<2, T><<<grid, 256, 0, s>>>(gf, gc, uc, uf);
   return check_launch("prolong_kernel");
 }
+
+#define PMT_INSTANTIATE_LEVEL(T)                                                                                   \
+  template int launch_fine<T>(int, const GridDims&, const double*, int, const T*, const uint8_t*, double, const T*, \
+                              const T*, const T*, double, T*, hipStream_t);                                        \
+  template int launch_stencil<T>(int, const GridDims&, const T*, const T*, const T*, const T*, double, T*,         \
+                                 hipStream_t);                                                                     \
+  template int launch_stencil_dinv<T>(const GridDims&, const double*, T*, hipStream_t);                            \
+  template int launch_jacobi0<T>(long long, const T*, const T*, double, T*, hipStream_t);                          \
+  template int launch_restrict<T>(const GridDims&, const GridDims&, const T*, T*, hipStream_t);                    \
+  template int launch_prolong<T>(const GridDims&, const GridDims&, const T*, T*, hipStream_t);
+PMT_INSTANTIATE_LEVEL(double)
+PMT_INSTANTIATE_LEVEL(float)
+#undef PMT_INSTANTIATE_LEVEL
 
 static bool even_grid(int nelx, int nely, int nelz) {
   return nelx % 2 == 0 && nely % 2 == 0 && nelz % 2 == 0 && nelx >= 2 && nely >= 2;
@@ -401,19 +543,26 @@ using namespace pmt;
 
 extern "C" {
 
-int pmt_stiffness_apply(int nelx, int nely, int nelz, const double* K0, const double* xe, const uint8_t* bcmask,
-                        double bcdiag, const double* u, double* y, void* stream) {
-  if (!grid_ok(nelx, nely, nelz) || !K0 || !xe || !u || !y) return arg_error("pmt_stiffness_apply: bad argument");
-  return launch_fine(MODE_APPLY, make_grid(nelx, nely, nelz), K0, xe, bcmask, bcdiag, u, nullptr, nullptr, 0.0, y,
-                     (hipStream_t)stream);
+int pmt_k0_pattern(int dim, const double* K0_host) {
+  if ((dim != 2 && dim != 3) || !K0_host) return arg_error("pmt_k0_pattern: bad argument");
+  return k0_matches_pattern(dim, K0_host);
 }
 
-int pmt_stiffness_residual(int nelx, int nely, int nelz, const double* K0, const double* xe, const uint8_t* bcmask,
-                           double bcdiag, const double* b, const double* u, double* r, void* stream) {
-  if (!grid_ok(nelx, nely, nelz) || !K0 || !xe || !u || !b || !r)
-    return arg_error("pmt_stiffness_residual: bad argument");
-  return launch_fine(MODE_RESIDUAL, make_grid(nelx, nely, nelz), K0, xe, bcmask, bcdiag, u, b, nullptr, 0.0, r,
-                     (hipStream_t)stream);
+int pmt_stiffness_apply(int nelx, int nely, int nelz, const double* K0, int k0_pattern, const double* xe,
+                        const uint8_t* bcmask, double bcdiag, const double* u, double* y, void* stream) {
+  if (!grid_ok(nelx, nely, nelz) || !K0 || !xe || !bcmask || !u || !y)
+    return arg_error("pmt_stiffness_apply: bad argument (the Dirichlet mask is mandatory)");
+  return launch_fine<double>(MODE_APPLY, make_grid(nelx, nely, nelz), K0, k0_pattern, xe, bcmask, bcdiag, u, nullptr, nullptr,
+                     0.0, y, (hipStream_t)stream);
+}
+
+int pmt_stiffness_residual(int nelx, int nely, int nelz, const double* K0, int k0_pattern, const double* xe,
+                           const uint8_t* bcmask, double bcdiag, const double* b, const double* u, double* r,
+                           void* stream) {
+  if (!grid_ok(nelx, nely, nelz) || !K0 || !xe || !bcmask || !u || !b || !r)
+    return arg_error("pmt_stiffness_residual: bad argument (the Dirichlet mask is mandatory)");
+  return launch_fine<double>(MODE_RESIDUAL, make_grid(nelx, nely, nelz), K0, k0_pattern, xe, bcmask, bcdiag, u, b, nullptr,
+                     0.0, r, (hipStream_t)stream);
 }
 
 int pmt_stiffness_diagonal(int nelx, int nely, int nelz, const double* K0, const double* xe, const uint8_t* bcmask,

@@ -197,6 +197,17 @@ int launch_scale_rsqrt(long long n, const double* sc, int k, const double* z, do
   return check_launch("scale_rsqrt");
 }
 
+__global__ void to_double_kernel(long long n, const float* __restrict__ x, double* __restrict__ y) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (double)x[i];
+}
+
+int launch_to_double(long long n, const float* x, double* y, hipStream_t s) {
+  if (n <= 0) return PMT_OK;
+  to_double_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, x, y);
+  return check_launch("to_double_kernel");
+}
+
 int launch_fill(long long n, double v, double* y, hipStream_t s) {
   if (n <= 0) return PMT_OK;
   fill_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, v, y);

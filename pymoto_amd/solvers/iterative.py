@@ -154,17 +154,17 @@ class GeometricMultigrid(Preconditioner):
         return y
 
 
-def auto_levels(domain: VoxelDomain, max_coarse_dofs: int = 6000) -> int:
-    """Grid count for a default hierarchy: halve while every extent is even and the coarse grid
-    still has more than ``max_coarse_dofs`` dofs (the reference example halves while the coarse
-    extents stay even and >= 8, ``ex_compliance_multigrid.py:108-121``)."""
+def auto_levels(domain: VoxelDomain) -> int:
+    """Grid count of the reference example's hierarchy (``ex_compliance_multigrid.py:108-121``): one
+    multigrid level per halving, adding levels while the coarse grid's extents are all even and at
+    least 8; the last coarse grid is solved directly.  Returns the number of grids (fine included)."""
     nel = [domain.nelx, domain.nely] + ([domain.nelz] if domain.dim == 3 else [])
-    levels = 1
-    while all(n % 2 == 0 and n >= 2 for n in nel):
-        ndof = domain.dim * int(np.prod([n + 1 for n in nel]))
-        if ndof <= max_coarse_dofs:
-            break
-        nel = [n // 2 for n in nel]
+    if any(n % 2 for n in nel):
+        return 1
+    sub = [n // 2 for n in nel]
+    levels = 2  # fine grid + the first coarse grid
+    while all(n % 2 == 0 for n in sub) and all(n >= 8 for n in sub):
+        sub = [n // 2 for n in sub]
         levels += 1
     return levels
 

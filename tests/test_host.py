@@ -130,9 +130,11 @@ def test_powlaw_parse_matches_lambdify_constants():
 
 def test_auto_levels():
     from pymoto_amd.solvers import auto_levels
-    assert auto_levels(pm.VoxelDomain(120, 40)) == 2
-    assert auto_levels(pm.VoxelDomain(256, 128, 128)) == 5
+    # the reference example's rule (ex_compliance_multigrid.py:108-121): coarse extents even and >= 8
+    assert auto_levels(pm.VoxelDomain(120, 40)) == 4
+    assert auto_levels(pm.VoxelDomain(256, 128, 128)) == 6
     assert auto_levels(pm.VoxelDomain(192, 96, 96)) == 5
+    assert auto_levels(pm.VoxelDomain(16, 8, 8)) == 2
 
 
 def test_device_modules_fail_loudly_without_gpu():
