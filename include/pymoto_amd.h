@@ -129,6 +129,10 @@ typedef struct pmt_mg pmt_mg;
 int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega, const int* steps,
                   const double* K0_host, pmt_mg** out);
 int pmt_mg_destroy(pmt_mg* mg);
+/* Arithmetic of the V-cycle: 64 (FP64, the reference's) or 32 (FP32 operator, smoother, transfers,
+ * stencils and coarsest inverse built from the FP64 Galerkin hierarchy; CG stays FP64).  Takes
+ * effect at the next pmt_mg_update.  (No reference counterpart: the mixed-precision variant.) */
+int pmt_mg_set_precision(pmt_mg* mg, int bits);
 /* Rebuild diagonals, Galerkin coarse operators and the coarsest inverse for new densities xe
  * (GeometricMultigrid.update, iterative.py:168-176).  xe and bcmask must stay alive until the
  * next update. */

@@ -40,6 +40,7 @@ _SIGS = {
     "pmt_prolong_add": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp]),
     "pmt_mg_create": (c_int, [c_int] * 4 + [c_vp, c_vp, c_vp, c_vp]),
     "pmt_mg_destroy": (c_int, [c_vp]),
+    "pmt_mg_set_precision": (c_int, [c_vp, c_int]),
     "pmt_mg_update": (c_int, [c_vp, c_vp, c_vp, c_dbl, c_vp]),
     "pmt_mg_vcycle": (c_int, [c_vp, c_vp, c_vp, c_vp]),
     "pmt_mg_level_apply": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp]),

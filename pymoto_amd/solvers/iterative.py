@@ -57,7 +57,7 @@ class GeometricMultigrid(Preconditioner):
     _available_cycles = ["v", "w"]
 
     def __init__(self, domain: VoxelDomain, A=None, cycle: str = "V", inner_level=None, smoother=None,
-                 smooth_steps: int = 5):
+                 smooth_steps: int = 5, precision: str = "fp64"):
         assert domain.nelx % 2 == 0 and domain.nely % 2 == 0 and domain.nelz % 2 == 0, (
             f"Domain sizes {domain.nelx, domain.nely, domain.nelz} must be divisible by 2")
         assert cycle.lower() in self._available_cycles, f"Cycle ({cycle}) is not available"
@@ -68,6 +68,10 @@ class GeometricMultigrid(Preconditioner):
         self.inner_level = inner_level
         self.smoother = DampedJacobi(w=0.5) if smoother is None else smoother
         self.smooth_steps = smooth_steps
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be 'fp64' (the reference's arithmetic) or 'fp32'")
+        # arithmetic of the whole V-cycle (taken from the outermost level); not in the reference API
+        self.precision = precision
         self.sub_domain = domain.coarsen()
         self.A = None
         self._handle = None
@@ -94,7 +98,8 @@ class GeometricMultigrid(Preconditioner):
         omega = np.array([getattr(m.smoother, "w", 0.5) for m in chain], dtype=np.float64)
         steps = np.array([m.smooth_steps for m in chain], dtype=np.int32)
         d = self.domain
-        key = (d.nelx, d.nely, d.nelz, len(chain) + 1, tuple(omega), tuple(steps), A.grid.K0.tobytes())
+        key = (d.nelx, d.nely, d.nelz, len(chain) + 1, tuple(omega), tuple(steps), A.grid.K0.tobytes(),
+               self.precision)
         if self._handle is not None and self._key == key:
             return
         self._release()
@@ -105,6 +110,8 @@ class GeometricMultigrid(Preconditioner):
                                      omega.ctypes.data_as(ctypes.c_void_p), steps.ctypes.data_as(ctypes.c_void_p),
                                      K0.ctypes.data_as(ctypes.c_void_p), ctypes.byref(h)), "multigrid setup")
         self._handle, self._key = h, key
+        if self.precision == "fp32":
+            _lib.check(lib.pmt_mg_set_precision(h, 32), "multigrid precision")
 
     def _release(self):
         if self._handle is not None:
@@ -169,11 +176,11 @@ def auto_levels(domain: VoxelDomain) -> int:
     return levels
 
 
-def build_hierarchy(domain: VoxelDomain, levels: int, omega=0.5, smooth_steps=5):
+def build_hierarchy(domain: VoxelDomain, levels: int, omega=0.5, smooth_steps=5, precision="fp64"):
     """Nested GeometricMultigrid objects with ``levels`` grids (fine grid included)."""
     if levels < 2:
         raise ValueError("a multigrid hierarchy needs at least two grids")
-    mgs = [GeometricMultigrid(domain, smoother=DampedJacobi(w=omega), smooth_steps=smooth_steps)]
+    mgs = [GeometricMultigrid(domain, smoother=DampedJacobi(w=omega), smooth_steps=smooth_steps, precision=precision)]
     for _ in range(levels - 2):
         mgs.append(GeometricMultigrid(mgs[-1].sub_domain, smoother=DampedJacobi(w=omega), smooth_steps=smooth_steps))
         mgs[-2].inner_level = mgs[-1]

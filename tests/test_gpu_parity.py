@@ -283,3 +283,35 @@ def test_large_grid_properties(gpu_lib):
     assert float(torch.linalg.norm(r)) / np.linalg.norm(f) <= 1e-10
     c = float(torch.dot(sol, dev(f)))
     assert abs(c - float(torch.dot(sol, K @ sol))) <= 1e-9 * abs(c)
+
+
+# ---------------------------------------------------------------- mixed precision (FP32 V-cycle, FP64 CG)
+def test_fp32_vcycle_close_to_fp64(gpu_lib):
+    nel = (16, 8, 8)
+    rng = np.random.default_rng(21)
+    x = fe.simp(rng.uniform(0.05, 1.0, 16 * 64))
+    bc = problems.cantilever3d(*nel)[0]
+    d = pm.VoxelDomain(*nel)
+    K = pm.AssembleStiffness(d, bc=bc)(dev(x))
+    m64 = pm.solvers.build_hierarchy(d, 3, omega=0.6, smooth_steps=2)
+    m32 = pm.solvers.build_hierarchy(d, 3, omega=0.6, smooth_steps=2, precision="fp32")
+    m64.update(K)
+    m32.update(K)
+    r = dev(rng.standard_normal(K.shape[0]))
+    z64, z32 = host(m64.solve(r)), host(m32.solve(r))
+    assert relerr(z32, z64) < 1e-4  # single precision V-cycle
+    assert relerr(z32, z64) > 0.0   # and it really ran in FP32
+
+
+@pytest.mark.parametrize("levels", [3, 4])
+def test_fp32_vcycle_pcg_reaches_fp64_tolerance(gpu_lib, levels):
+    g = golden(f"c2_mgcg_32x16x16_L{levels}.npz")
+    nel = tuple(int(v) for v in g["nel"])
+    d = pm.VoxelDomain(*nel)
+    solver = pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(d, levels, omega=float(g["omega"]),
+                                                                     smooth_steps=int(g["steps"]), precision="fp32"),
+                           tol=float(g["tol"]))
+    out = _device_chain(nel, g["x"], g["bc"], g["f"], 1.5, solver=solver)
+    # FP64 residual tolerance reached by the outer CG; compliance within 1e-7 of the reference (FP64) path
+    assert abs(out["c"] - float(g["c"])) / abs(float(g["c"])) < 1e-7
+    assert relerr(out["dcdx"], g["dcdx"]) < 1e-6
