@@ -8,7 +8,14 @@ before the timed region).  Solver variants, timed one after the other on the sam
 
 * ``fp64``: the reference's arithmetic throughout (FP64 V-cycle and CG);
 * ``fp32``: FP32 V-cycle (operator, smoother, transfers, stencils, coarsest inverse) inside the
-  FP64 CG, whose residual still reaches 1e-8; compliance and dc/dx are compared with ``fp64``.
+  FP64 CG, whose residual still reaches 1e-8; compliance and dc/dx are compared with ``fp64``;
+* ``bf16``: as ``fp32``, with the fine-grid operator of the V-cycle on the matrix cores (BF16
+  element products, FP32 accumulation).
+
+A variant may name its own Jacobi damping as ``precision@omega``.  The reduced-precision variants
+use omega = 0.5 by default: with these SIMP fields lambda_max(D^-1 K) is 3.5-3.8, so omega = 0.6 lies
+above the Jacobi stability bound 2/lambda_max and the smoother amplifies the highest modes, which the
+FP64 V-cycle survives but which magnify the rounding of a reduced-precision one.
 
 ``value`` is the seconds per design iteration of the fastest variant whose compliance agrees with
 the FP64 variant to 1e-7; every variant's time, CG iterations and errors are in ``config``.
@@ -39,13 +46,21 @@ def parse():
     p.add_argument("--nel", type=int, nargs=3, default=[256, 128, 128])
     p.add_argument("--levels", type=int, default=0, help="grids in the MG hierarchy (0 = reference example rule)")
     p.add_argument("--tol", type=float, default=1e-8)
-    p.add_argument("--variants", default="fp64,fp32")
+    p.add_argument("--omega", type=float, default=0.6, help="Jacobi damping of variants without '@omega'")
+    p.add_argument("--variants", default="fp64,fp32@0.5,bf16@0.5",
+                   help="comma-separated precision[@omega] list; the first fp64 entry is the reference")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--cpu-nel", type=int, nargs=3, default=[48, 24, 24])
     return p.parse_args()
 
 
-def build_network(pm, torch, nel, levels, tol, precision):
+def parse_variant(v):
+    """'fp32' or 'fp32@0.5' -> (precision, omega or None)."""
+    prec, _, om = v.partition("@")
+    return prec.strip(), (float(om) if om else None)
+
+
+def build_network(pm, torch, nel, levels, tol, precision, omega=0.6, steps=2):
     from pymoto_amd import problems
     d = pm.VoxelDomain(*nel)
     bc, f = problems.cantilever3d(d)
@@ -53,7 +68,7 @@ def build_network(pm, torch, nel, levels, tol, precision):
     sx = pm.Signal("x", torch.full((d.nel,), 0.5, dtype=torch.float64, device="cuda"))
     if levels <= 0:
         levels = pm.solvers.auto_levels(d)
-    mg = pm.solvers.build_hierarchy(d, levels, omega=0.6, smooth_steps=2, precision=precision)
+    mg = pm.solvers.build_hierarchy(d, levels, omega=omega, smooth_steps=steps, precision=precision)
     solver = pm.solvers.CG(preconditioner=mg, tol=tol)
     with pm.Network() as fn:
         sxf = pm.FilterConv(d, radius=1.5)(sx)
@@ -120,8 +135,10 @@ def cpu_baseline(nel, steps=3):
     return float(np.mean(times[1:] if len(times) > 1 else times)), ndof, its
 
 
-def run_variant(pm, torch, args, precision, xs):
-    net = build_network(pm, torch, tuple(args.nel), args.levels, args.tol, precision)
+def run_variant(pm, torch, args, variant, xs):
+    precision, omega = parse_variant(variant)
+    net = build_network(pm, torch, tuple(args.nel), args.levels, args.tol, precision,
+                        omega=omega if omega is not None else args.omega)
     for k in range(args.warmup):
         design_iteration(net, xs[k])
     torch.cuda.synchronize()
@@ -158,7 +175,7 @@ def main():
         net, res = run_variant(pm, torch, args, v, xs)
         results[v] = res
         levels = net["levels"]
-        if v == "fp64":
+        if v == "fp64" and roof is None:
             roof = kernel_roofline(pm, torch, net)
         del net
         torch.cuda.empty_cache()
@@ -187,7 +204,8 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if best == "fp64" else "f64 CG / f32 V-cycle",
+        "dtype": {"fp64": "f64", "fp32": "f64 CG / f32 V-cycle",
+                  "bf16": "f64 CG / f32 V-cycle, bf16 MFMA fine smoother"}[parse_variant(best)[0]],
         "data": "synthetic: seeded uniform[0,1] densities per step, filtered r=1.5",
         "config": {"workload": f"3D cantilever {nel[0]}x{nel[1]}x{nel[2]} hex8, MG-PCG ({levels} grids, V(2,2) "
                                f"damped Jacobi w=0.6, Galerkin coarse, dense direct coarsest), tol {args.tol:g}, "

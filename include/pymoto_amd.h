@@ -81,10 +81,27 @@ int pmt_stiffness_apply(int nelx, int nely, int nelz, const double* K0, int k0_p
 int pmt_stiffness_residual(int nelx, int nely, int nelz, const double* K0, int k0_pattern, const double* xe,
                            const uint8_t* bcmask, double bcdiag, const double* b, const double* u,
                            double* r, void* stream);
+/* Reduced-precision forms of pmt_stiffness_apply for the mixed-precision V-cycle (xe, u, y in
+ * FP32): _f32 computes in FP32; _bf16 (hex8 only) contracts each element on the matrix cores with
+ * BF16 operands (u, K0) and FP32 accumulation, then scales by x_e in FP32.
+ * (No reference counterpart: the reduced-precision element products.) */
+int pmt_stiffness_apply_f32(int nelx, int nely, int nelz, const double* K0, int k0_pattern, const float* xe,
+                            const uint8_t* bcmask, double bcdiag, const float* u, float* y, void* stream);
+int pmt_stiffness_apply_bf16(int nelx, int nely, int nelz, const double* K0, const float* xe,
+                             const uint8_t* bcmask, double bcdiag, const float* u, float* y, void* stream);
 /* 1 if a host element matrix follows the isotropic box sign/class pattern (36 classes in 3D, 10 in
  * 2D); then k0_pattern = 1 selects the kernel that keeps K0 in scalar registers.  Host only.
  * (No reference counterpart: a property of K0 from AssembleStiffness.__init__, assembly.py:446-461.) */
 int pmt_k0_pattern(int dim, const double* K0_host);
+/* Launch-shape tuning of the fine-grid operator, process-wide, read at every launch.
+ * PMT_TUNE_FINE_KERNEL: 0 = 3D node tiles, 1 = xy node columns streamed along z (default; 3D only).
+ * PMT_TUNE_STREAM_PLANES: node planes per streamed column, 0 = chosen from the CU count (default).
+ * pmt_get_tuning returns the current value or a negative status.
+ * (No reference counterpart: the launch shapes picked by the autotuner.) */
+#define PMT_TUNE_FINE_KERNEL 1
+#define PMT_TUNE_STREAM_PLANES 2
+int pmt_set_tuning(int param, int value);
+int pmt_get_tuning(int param);
 /* diag(K) (DampedJacobi.update, iterative.py:38-39) */
 int pmt_stiffness_diagonal(int nelx, int nely, int nelz, const double* K0, const double* xe,
                            const uint8_t* bcmask, double bcdiag, double* diag, void* stream);
@@ -129,9 +146,11 @@ typedef struct pmt_mg pmt_mg;
 int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega, const int* steps,
                   const double* K0_host, pmt_mg** out);
 int pmt_mg_destroy(pmt_mg* mg);
-/* Arithmetic of the V-cycle: 64 (FP64, the reference's) or 32 (FP32 operator, smoother, transfers,
- * stencils and coarsest inverse built from the FP64 Galerkin hierarchy; CG stays FP64).  Takes
- * effect at the next pmt_mg_update.  (No reference counterpart: the mixed-precision variant.) */
+/* Arithmetic of the V-cycle: 64 (FP64, the reference's), 32 (FP32 operator, smoother, transfers and
+ * stencils built from the FP64 Galerkin hierarchy; the coarsest solve and CG stay FP64) or 16 (as 32,
+ * but the damped-Jacobi sweeps of the fine grid run on the matrix cores with BF16 operands and FP32
+ * accumulation; hex8 grids only).  Takes effect at the next pmt_mg_update.
+ * (No reference counterpart: the mixed-precision variants.) */
 int pmt_mg_set_precision(pmt_mg* mg, int bits);
 /* Rebuild diagonals, Galerkin coarse operators and the coarsest inverse for new densities xe
  * (GeometricMultigrid.update, iterative.py:168-176).  xe and bcmask must stay alive until the
@@ -151,6 +170,17 @@ long long pmt_mg_device_bytes(pmt_mg* mg);
  * residual; history (host, maxit+1 doubles, may be NULL) the relative residual per iteration. */
 int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double tol, int maxit,
                   int restart, int* iters, double* relres, double* history, void* stream);
+
+/* Block PCG for 1 <= k <= 8 right-hand sides sharing the operator: CG.solve with an (ndof, k)
+ * right-hand side (iterative.py:259-403), Gram-Schmidt orth with rank-deficient columns dropped.
+ * B and X hold k columns of pmt_mg_level_ndof(mg, 0) doubles each, column j at B + j*ndof (the
+ * transpose of the reference's (ndof, k) array).  Stops when max_j ||r_j||/||b_j|| <= tol; relres
+ * (host, k doubles) receives the final per-column values, history (host, maxit+1 doubles, may be
+ * NULL) the maximum per iteration.  use_graph = 1 replays each iteration from HIP graphs captured
+ * on the first call (re-captured after pmt_mg_update, pmt_mg_set_precision or pmt_set_tuning).
+ * Allocates its workspace on the first call with a larger k. */
+int pmt_block_pcg_solve(pmt_mg* mg, int k, const double* B, double* X, int x0_is_zero, double tol, int maxit,
+                        int restart, int use_graph, int* iters, double* relres, double* history, void* stream);
 
 #ifdef __cplusplus
 }

@@ -30,7 +30,11 @@ _SIGS = {
     "pmt_axpy_free": (c_int, [c_int] * 3 + [c_vp, c_dbl, c_vp, c_vp, c_vp]),
     "pmt_stiffness_apply": (c_int, [c_int] * 3 + [c_vp, c_int, c_vp, c_vp, c_dbl, c_vp, c_vp, c_vp]),
     "pmt_stiffness_residual": (c_int, [c_int] * 3 + [c_vp, c_int, c_vp, c_vp, c_dbl, c_vp, c_vp, c_vp, c_vp]),
+    "pmt_stiffness_apply_f32": (c_int, [c_int] * 3 + [c_vp, c_int, c_vp, c_vp, c_dbl, c_vp, c_vp, c_vp]),
+    "pmt_stiffness_apply_bf16": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp, c_dbl, c_vp, c_vp, c_vp]),
     "pmt_k0_pattern": (c_int, [c_int, c_vp]),
+    "pmt_set_tuning": (c_int, [c_int, c_int]),
+    "pmt_get_tuning": (c_int, [c_int]),
     "pmt_stiffness_diagonal": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp, c_dbl, c_vp, c_vp]),
     "pmt_sens_contract": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp, c_int, c_vp]),
     "pmt_csr_nnz": (c_ll, [c_int] * 4),
@@ -47,6 +51,8 @@ _SIGS = {
     "pmt_mg_level_ndof": (c_ll, [c_vp, c_int]),
     "pmt_mg_device_bytes": (c_ll, [c_vp]),
     "pmt_pcg_solve": (c_int, [c_vp, c_vp, c_vp, c_int, c_dbl, c_int, c_int, c_vp, c_vp, c_vp, c_vp]),
+    "pmt_block_pcg_solve": (c_int, [c_vp, c_int, c_vp, c_vp, c_int, c_dbl, c_int, c_int, c_int, c_vp, c_vp, c_vp,
+                                    c_vp]),
 }
 
 

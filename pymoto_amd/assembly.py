@@ -132,6 +132,26 @@ class StiffnessMatrix:
 
     dot = __matmul__
 
+    def apply_reduced(self, u, precision: str = "fp32") -> torch.Tensor:
+        """K u in reduced precision, as the mixed-precision V-cycle applies it on the fine grid:
+        ``fp32`` (FP32 arithmetic) or ``bf16`` (hex8: BF16 element products on the matrix cores,
+        FP32 accumulation).  Returns an FP32 tensor."""
+        g, lib = self.grid, _lib.lib()
+        u32 = _lib.as_device(u, dtype=torch.float32)
+        x32 = self.xe.to(torch.float32)
+        y = torch.empty_like(u32)
+        if precision == "fp32":
+            rc = lib.pmt_stiffness_apply_f32(*g.nel3, _lib.ptr(g.K0_dev), g.k0_pattern, _lib.ptr(x32),
+                                             _lib.ptr(g.bcmask_dev), g.bcdiag, _lib.ptr(u32), _lib.ptr(y),
+                                             _lib.stream())
+        elif precision == "bf16":
+            rc = lib.pmt_stiffness_apply_bf16(*g.nel3, _lib.ptr(g.K0_dev), _lib.ptr(x32), _lib.ptr(g.bcmask_dev),
+                                              g.bcdiag, _lib.ptr(u32), _lib.ptr(y), _lib.stream())
+        else:
+            raise ValueError("precision must be 'fp32' or 'bf16'")
+        _lib.check(rc, f"stiffness apply ({precision})")
+        return y
+
     def residual(self, b: torch.Tensor, u: torch.Tensor) -> torch.Tensor:
         """b - K u in one kernel."""
         g, lib = self.grid, _lib.lib()

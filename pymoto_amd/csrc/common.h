@@ -58,6 +58,14 @@ inline unsigned blocks_for(long long n, int threads) {
   return (unsigned)(b < 1 ? 1 : b);
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so that each XCD walks one
+// contiguous range of tiles and neighbouring tiles share its L2.  The grid is padded to a
+// multiple of 8 blocks.
+__device__ inline long long xcd_block() {
+  const unsigned per = gridDim.x >> 3;
+  return (long long)(blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+}
+
 constexpr int kReduceBlocks = 1024;  // fixed grid for deterministic two-stage reductions
 constexpr int kReduceThreads = 256;
 

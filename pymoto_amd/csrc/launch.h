@@ -12,6 +12,11 @@ template <typename T>
 int launch_fine(int mode, const GridDims& g, const double* K0, int k0_pattern, const T* xe, const uint8_t* bcm,
                 double bcdiag, const T* u, const T* b, const T* dinv, double omega, T* y, hipStream_t s);
 int k0_matches_pattern(int dim, const double* K0_host);
+int set_tuning(int param, int value);
+int get_tuning(int param);
+int tuned_stream_planes();  // 0 = automatic
+int tuning_epoch();         // changes whenever a launch shape changes
+int device_cu_count();      // of the current device
 int launch_fine_diag(const GridDims& g, const double* K0, const double* xe, const uint8_t* bcm, double bcdiag,
                      double* diag, int invert, hipStream_t s);
 template <typename T>
@@ -25,6 +30,11 @@ template <typename T>
 int launch_restrict(const GridDims& gf, const GridDims& gc, const T* rf, T* rc, hipStream_t s);
 template <typename T>
 int launch_prolong(const GridDims& gf, const GridDims& gc, const T* uc, T* uf, hipStream_t s);
+
+// mfma.hip (hex8 only; BF16 operands, FP32 accumulation and vectors)
+int launch_fine_mfma(int mode, const GridDims& g, const double* K0, const float* xe, const uint8_t* bcm,
+                     double bcdiag, const float* u, const float* b, const float* dinv, double omega, float* y,
+                     hipStream_t s);
 
 // vector.hip
 int launch_dot(long long n, const double* x, const double* y, double* out, double* work, hipStream_t s);

@@ -9,81 +9,19 @@
 // iteration is the 8-byte read of ||r||^2 for the stopping test.
 //
 // Precision: the V-cycle runs in FP64 (the reference's arithmetic) or, after
-// pmt_mg_set_precision(mg, 32), in FP32 - operator, smoother, transfers, stencils and coarsest
-// inverse all in FP32, built from the FP64 Galerkin hierarchy - while the CG recurrences, the
-// outer operator, dots and the stopping test stay FP64.  CG reads the FP32 preconditioner output
-// through one conversion, so the solution still reaches an FP64 residual tolerance.
+// pmt_mg_set_precision(mg, 32), in FP32 - operator, smoother, transfers and stencils in FP32, built
+// from the FP64 Galerkin hierarchy - while the coarsest solve, the CG recurrences, the outer
+// operator, dots and the stopping test stay FP64.  CG reads the FP32 preconditioner output through
+// one conversion, so the solution still reaches an FP64 residual tolerance.  With
+// pmt_mg_set_precision(mg, 16) the damped-Jacobi sweeps of the fine grid run on the matrix cores
+// with BF16 operands and FP32 accumulation (hex8 only); the fine residual that is restricted to the
+// coarse grid stays FP32, so the coarse correction sees no BF16 rounding.
 #include <cmath>
 #include <new>
+#include <type_traits>
 #include <vector>
 
-#include "launch.h"
-
-namespace pmt {
-
-constexpr int kMaxLevels = 12;
-
-template <typename T>
-struct LevelBuf {
-  T* dinv = nullptr;  // inverse diagonal (levels above the coarsest)
-  T* S = nullptr;     // coarse stencil (levels >= 1)
-  T* u = nullptr;     // V-cycle correction (levels >= 1)
-  T* t = nullptr;     // ping-pong buffer
-  T* b = nullptr;     // restricted right-hand side (levels >= 1; level 0 in FP32: converted input)
-  T* r = nullptr;     // residual
-};
-
-struct Level {
-  GridDims g;
-  long long ndof = 0;
-  double omega = 0.5;
-  int steps = 1;
-  double* Ke = nullptr;  // Galerkin element matrices (levels >= 1, FP64)
-  LevelBuf<double> d;
-  LevelBuf<float> f;
-  template <typename T>
-  LevelBuf<T>& buf();
-};
-template <>
-LevelBuf<double>& Level::buf<double>() {
-  return d;
-}
-template <>
-LevelBuf<float>& Level::buf<float>() {
-  return f;
-}
-
-}  // namespace pmt
-
-struct pmt_mg {
-  int dim = 3;
-  int nlev = 0;
-  int prec = 64;
-  pmt::Level L[pmt::kMaxLevels];
-  int NE = 24;
-  double* K0 = nullptr;
-  int k0pat = 0;
-  const double* xe = nullptr;
-  float* xe32 = nullptr;
-  const uint8_t* bcm = nullptr;
-  double bcdiag = 0.0;
-  bool ready = false;
-  long long nc = 0;
-  double* Ainv = nullptr;
-  float* Ainv32 = nullptr;
-  double* gjP = nullptr;
-  double* gjC = nullptr;
-  float* z32 = nullptr;  // FP32 V-cycle output on the fine grid
-  double* p = nullptr;
-  double* q = nullptr;
-  double* z = nullptr;
-  double* r = nullptr;
-  double* sc = nullptr;    // device scalars
-  double* work = nullptr;  // reduction partials
-  double* host_sc = nullptr;
-  long long bytes = 0;
-  std::vector<void*> allocs;
-};
+#include "mg_impl.h"
 
 namespace pmt {
 
@@ -99,6 +37,7 @@ static int dalloc(pmt_mg* mg, T** p, long long n) {
 }
 
 static void release(pmt_mg* mg) {
+  block_cg_release(mg);
   for (void* p : mg->allocs) (void)hipFree(p);
   mg->allocs.clear();
   if (mg->host_sc) (void)hipHostFree(mg->host_sc);
@@ -116,21 +55,31 @@ const float* fine_x<float>(pmt_mg* mg) {
   return mg->xe32;
 }
 
+// Coarsest grid: z = A_c^-1 b with the explicit FP64 inverse.  The reduced-precision V-cycles
+// convert in and out: the coarsest solve stays FP64 (an FP32 copy of the inverse of this badly
+// conditioned operator loses the smooth components the coarse correction exists for).
 template <typename T>
-static const T* coarse_inv(pmt_mg* mg);
+static int coarse_solve(pmt_mg* mg, const T* b, T* out, hipStream_t s);
 template <>
-const double* coarse_inv<double>(pmt_mg* mg) {
-  return mg->Ainv;
+int coarse_solve<double>(pmt_mg* mg, const double* b, double* out, hipStream_t s) {
+  return launch_gemv<double>(mg->nc, mg->Ainv, b, out, s);
 }
 template <>
-const float* coarse_inv<float>(pmt_mg* mg) {
-  return mg->Ainv32;
+int coarse_solve<float>(pmt_mg* mg, const float* b, float* out, hipStream_t s) {
+  Level& C = mg->L[mg->nlev - 1];
+  PMT_TRY(launch_to_double(mg->nc, b, C.d.b, s));
+  PMT_TRY(launch_gemv<double>(mg->nc, mg->Ainv, C.d.b, C.d.u, s));
+  return launch_to_float(mg->nc, C.d.u, out, s);
 }
 
 template <typename T>
 static int level_apply(pmt_mg* mg, int l, int mode, const T* u, const T* b, T* y, hipStream_t s) {
   Level& L = mg->L[l];
   LevelBuf<T>& B = L.buf<T>();
+  if constexpr (std::is_same<T, float>::value) {
+    if (l == 0 && mg->prec == 16 && mode == MODE_JACOBI)
+      return launch_fine_mfma(mode, L.g, mg->K0, fine_x<T>(mg), mg->bcm, mg->bcdiag, u, b, B.dinv, L.omega, y, s);
+  }
   if (l == 0)
     return launch_fine<T>(mode, L.g, mg->K0, mg->k0pat, fine_x<T>(mg), mg->bcm, mg->bcdiag, u, b, B.dinv, L.omega, y,
                           s);
@@ -140,7 +89,7 @@ static int level_apply(pmt_mg* mg, int l, int mode, const T* u, const T* b, T* y
 // out = V-cycle(b) on level l (GeometricMultigrid.solve with x0 = None, iterative.py:222-256)
 template <typename T>
 static int vcycle(pmt_mg* mg, int l, const T* b, T* out, hipStream_t s) {
-  if (l == mg->nlev - 1) return launch_gemv<T>(mg->nc, coarse_inv<T>(mg), b, out, s);
+  if (l == mg->nlev - 1) return coarse_solve<T>(mg, b, out, s);
   Level& L = mg->L[l];
   Level& C = mg->L[l + 1];
   LevelBuf<T>& B = L.buf<T>();
@@ -208,9 +157,21 @@ static int alloc_fp32(pmt_mg* mg) {
     }
   }
   if (rc == PMT_OK) rc = dalloc(mg, &mg->xe32, mg->L[0].g.nel);
-  if (rc == PMT_OK) rc = dalloc(mg, &mg->Ainv32, mg->nc * mg->nc);
   if (rc == PMT_OK) rc = dalloc(mg, &mg->z32, mg->L[0].ndof);
   return rc;
+}
+
+int mg_level_apply(pmt_mg* mg, int l, int mode, const double* u, const double* b, double* y, hipStream_t s) {
+  return level_apply<double>(mg, l, mode, u, b, y, s);
+}
+
+int mg_precondition(pmt_mg* mg, const double* r, double* z, hipStream_t s) { return precondition(mg, r, z, s); }
+
+int mg_alloc(pmt_mg* mg, void** p, long long bytes) {
+  char* q = nullptr;
+  PMT_TRY(dalloc(mg, &q, bytes));
+  *p = q;
+  return PMT_OK;
 }
 
 }  // namespace pmt
@@ -316,10 +277,14 @@ int pmt_mg_destroy(pmt_mg* mg) {
 }
 
 int pmt_mg_set_precision(pmt_mg* mg, int bits) {
-  if (!mg || (bits != 64 && bits != 32)) return arg_error("pmt_mg_set_precision: bits must be 64 or 32");
-  if (bits == 32 && !mg->xe32) PMT_TRY(alloc_fp32(mg));
+  if (!mg || (bits != 64 && bits != 32 && bits != 16))
+    return arg_error("pmt_mg_set_precision: bits must be 64, 32 or 16");
+  if (bits == 16 && mg->dim != 3)
+    return arg_error("pmt_mg_set_precision: the BF16 matrix-core operator is implemented for hex8 grids only");
+  if (bits != 64 && !mg->xe32) PMT_TRY(alloc_fp32(mg));
   mg->prec = bits;
   mg->ready = false;
+  block_cg_invalidate(mg);
   return PMT_OK;
 }
 
@@ -330,6 +295,7 @@ int pmt_mg_update(pmt_mg* mg, const double* xe, const uint8_t* bcmask, double bc
   mg->bcm = bcmask;
   mg->bcdiag = bcdiag;
   mg->ready = false;
+  block_cg_invalidate(mg);
   Level& F = mg->L[0];
   PMT_TRY(launch_fine_diag(F.g, mg->K0, xe, bcmask, bcdiag, F.d.dinv, 1, s));
   for (int l = 1; l < mg->nlev; ++l) {
@@ -345,7 +311,7 @@ int pmt_mg_update(pmt_mg* mg, const double* xe, const uint8_t* bcmask, double bc
   Level& Cl = mg->L[mg->nlev - 1];
   PMT_TRY(launch_dense_from_stencil(Cl.g, Cl.d.S, mg->Ainv, s));
   PMT_TRY(gj_invert(mg->nc, mg->Ainv, mg->gjP, mg->gjC, s));
-  if (mg->prec == 32) {
+  if (mg->prec != 64) {
     PMT_TRY(launch_to_float(F.g.nel, xe, mg->xe32, s));
     PMT_TRY(launch_to_float(F.ndof, F.d.dinv, F.f.dinv, s));
     const int NS = mg->dim == 3 ? 243 : 36;
@@ -354,7 +320,6 @@ int pmt_mg_update(pmt_mg* mg, const double* xe, const uint8_t* bcmask, double bc
       PMT_TRY(launch_to_float(L.g.nnodes * NS, L.d.S, L.f.S, s));
       PMT_TRY(launch_stencil_dinv<float>(L.g, L.d.S, L.f.dinv, s));
     }
-    PMT_TRY(launch_to_float(mg->nc * mg->nc, mg->Ainv, mg->Ainv32, s));
   }
   mg->ready = true;
   return PMT_OK;

@@ -1,0 +1,90 @@
+// Internal state of a pmt_mg handle (multigrid hierarchy + CG work vectors), shared by the
+// single- and multi-right-hand-side CG runtimes.
+#pragma once
+#include <vector>
+
+#include "launch.h"
+
+namespace pmt {
+
+constexpr int kMaxLevels = 12;
+struct BlockCG;
+
+template <typename T>
+struct LevelBuf {
+  T* dinv = nullptr;  // inverse diagonal (levels above the coarsest)
+  T* S = nullptr;     // coarse stencil (levels >= 1)
+  T* u = nullptr;     // V-cycle correction (levels >= 1)
+  T* t = nullptr;     // ping-pong buffer
+  T* b = nullptr;     // restricted right-hand side (levels >= 1; level 0 in FP32: converted input)
+  T* r = nullptr;     // residual
+};
+
+struct Level {
+  GridDims g;
+  long long ndof = 0;
+  double omega = 0.5;
+  int steps = 1;
+  double* Ke = nullptr;  // Galerkin element matrices (levels >= 1, FP64)
+  LevelBuf<double> d;
+  LevelBuf<float> f;
+  template <typename T>
+  LevelBuf<T>& buf();
+};
+template <>
+inline LevelBuf<double>& Level::buf<double>() {
+  return d;
+}
+template <>
+inline LevelBuf<float>& Level::buf<float>() {
+  return f;
+}
+
+}  // namespace pmt
+
+struct pmt_mg {
+  int dim = 3;
+  int nlev = 0;
+  int prec = 64;
+  pmt::Level L[pmt::kMaxLevels];
+  int NE = 24;
+  double* K0 = nullptr;
+  int k0pat = 0;
+  const double* xe = nullptr;
+  float* xe32 = nullptr;
+  const uint8_t* bcm = nullptr;
+  double bcdiag = 0.0;
+  bool ready = false;
+  long long nc = 0;
+  double* Ainv = nullptr;
+  double* gjP = nullptr;
+  double* gjC = nullptr;
+  float* z32 = nullptr;  // FP32 V-cycle output on the fine grid
+  double* p = nullptr;
+  double* q = nullptr;
+  double* z = nullptr;
+  double* r = nullptr;
+  double* sc = nullptr;    // device scalars
+  double* work = nullptr;  // reduction partials
+  double* host_sc = nullptr;
+  long long bytes = 0;
+  std::vector<void*> allocs;
+  // multi-right-hand-side CG (blockcg.hip): workspace for kcap columns and captured iterations
+  pmt::BlockCG* bcg = nullptr;
+};
+
+namespace pmt {
+
+// frees the block-CG workspace and graphs of a handle (blockcg.hip)
+void block_cg_release(pmt_mg* mg);
+// drops captured block-CG iterations (after an update or a change of precision or launch shapes)
+void block_cg_invalidate(pmt_mg* mg);
+
+// y = A_l u (mode apply), b - A_l u (residual) or a damped-Jacobi sweep on level l, FP64
+int mg_level_apply(pmt_mg* mg, int l, int mode, const double* u, const double* b, double* y, hipStream_t s);
+// z = M r: one V-cycle in the handle's precision, FP64 in and out
+int mg_precondition(pmt_mg* mg, const double* r, double* z, hipStream_t s);
+// device allocation owned (and freed) by the handle
+int mg_alloc(pmt_mg* mg, void** p, long long bytes);
+
+}  // namespace pmt

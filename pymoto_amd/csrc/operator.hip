@@ -43,14 +43,6 @@ struct K0Pattern<2> {
   __device__ static constexpr int first(int c) { return K0FIRST2[c]; }
 };
 
-// Workgroups are dispatched round-robin over the 8 XCDs; remap so that each XCD walks one
-// contiguous range of nodes and the neighbour rows/planes of a stencil stay in its own L2.
-// The grid is padded to a multiple of 8 blocks.
-__device__ inline long long xcd_block() {
-  const unsigned per = gridDim.x >> 3;
-  return (long long)(blockIdx.x & 7u) * per + (blockIdx.x >> 3);
-}
-
 inline unsigned xcd_blocks(long long n, int threads) {
   unsigned b = blocks_for(n, threads);
   return (b + 7u) & ~7u;
@@ -198,6 +190,206 @@ fine_node_kernel(GridDims g, int tiles_x, int tiles_y, int ntiles, const double*
     } else {
       y[q] = u[q] + omega * dinv[q] * (b[q] - ku);
     }
+  }
+}
+
+// 3D fine level, streamed along z: one block owns an SX x SY column of nodes and walks a chunk of
+// node planes.  LDS holds a ring of four Dirichlet-masked displacement planes (the three a node
+// plane reads plus the one being filled) and three density planes, each with its one-node
+// (one-element) halo.  While the block multiplies plane k, every thread already has the global
+// loads of displacement plane k+2 and density plane k+1 in flight in registers; they are written
+// to the free ring slots after the multiply, so one barrier per plane separates producer and
+// consumers and global latency hides behind the 576 multiply-adds per node.  The halo is loaded
+// once per plane instead of once per 3D tile.
+struct Stream {
+  static constexpr int SX = 32, SY = 8;                      // nodes per plane of a block (256 threads)
+  static constexpr int HX = SX + 2, HY = SY + 2, HP = HX * HY;  // displacement plane with halo
+  static constexpr int EX = SX + 1, EY = SY + 1, EP = EX * EY;  // density plane with halo
+  static constexpr int UL = (HP + 255) / 256, XL = (EP + 255) / 256;  // slots per thread
+};
+
+template <int MODE, bool CLS, typename T>
+__global__ void __launch_bounds__(256)
+fine_stream_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, const double* __restrict__ K0,
+                   const T* __restrict__ xe, const uint8_t* __restrict__ bcm, T bcdiag, const T* __restrict__ u,
+                   const T* __restrict__ b, const T* __restrict__ dinv, T omega, T* __restrict__ y) {
+  constexpr int HX = Stream::HX, HP = Stream::HP, EX = Stream::EX, EP = Stream::EP;
+  constexpr int UL = Stream::UL, XL = Stream::XL;
+  __shared__ T su[4][3][HP];
+  __shared__ T sx[3][EP];
+
+  const int work = (int)xcd_block();
+  if (work >= nwork) return;  // whole block leaves together
+  const int txy = work % ntiles_xy;
+  const int i0 = (txy % tiles_x) * Stream::SX, j0 = (txy / tiles_x) * Stream::SY;
+  const int kbeg = (work / ntiles_xy) * nzc;
+  const int kend = min(kbeg + nzc, g.nz);
+  const int tid = threadIdx.x;
+
+  T kv[CLS ? K0Pattern<3>::N : 1];
+  if (CLS) {
+#pragma unroll
+    for (int c = 0; c < K0Pattern<3>::N; ++c)
+      kv[c] = (T)(K0Pattern<3>::sgn(K0Pattern<3>::first(c)) * K0[K0Pattern<3>::first(c)]);
+  }
+
+  // per-thread halo slots of a displacement plane and of a density plane (plane-independent)
+  long long uoff[UL];
+  bool uin[UL];
+#pragma unroll
+  for (int l = 0; l < UL; ++l) {
+    const int h = tid + 256 * l;
+    const int gi = i0 - 1 + h % HX, gj = j0 - 1 + h / HX;
+    uin[l] = h < HP && gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny;
+    uoff[l] = (long long)gj * g.nx + gi;
+  }
+  long long xoff[XL];
+  bool xin[XL];
+#pragma unroll
+  for (int l = 0; l < XL; ++l) {
+    const int q = tid + 256 * l;
+    const int gi = i0 - 1 + q % EX, gj = j0 - 1 + q / EX;
+    xin[l] = q < EP && gi >= 0 && gi < g.ex && gj >= 0 && gj < g.ey;
+    xoff[l] = (long long)gj * g.ex + gi;
+  }
+  const long long uplane = (long long)g.nx * g.ny, xplane = (long long)g.ex * g.ey;
+
+  T ru[UL][3];
+  unsigned rm[UL];
+  T rx[XL];
+  // issue the loads of displacement plane p and density plane pe into registers (no waits)
+  auto fetch = [&](int p, int pe) {
+#pragma unroll
+    for (int l = 0; l < UL; ++l) {
+      rm[l] = 7u;
+      ru[l][0] = ru[l][1] = ru[l][2] = (T)0;
+      if (uin[l] && p >= 0 && p < g.nz) {
+        const long long m = (long long)p * uplane + uoff[l];
+        rm[l] = bcm[m];
+        ru[l][0] = u[3 * m];
+        ru[l][1] = u[3 * m + 1];
+        ru[l][2] = u[3 * m + 2];
+      }
+    }
+#pragma unroll
+    for (int l = 0; l < XL; ++l) {
+      rx[l] = (T)0;
+      if (xin[l] && pe >= 0 && pe < g.ez) rx[l] = xe[(long long)pe * xplane + xoff[l]];
+    }
+  };
+  auto store_u = [&](int p) {
+    T(*dst)[HP] = su[(p + 1) & 3];
+#pragma unroll
+    for (int l = 0; l < UL; ++l) {
+      const int h = tid + 256 * l;
+      if (h < HP) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) dst[d][h] = ((rm[l] >> d) & 1u) ? (T)0 : ru[l][d];
+      }
+    }
+  };
+  auto store_x = [&](int pe) {
+    T* dst = sx[(pe + 1) % 3];
+#pragma unroll
+    for (int l = 0; l < XL; ++l) {
+      const int q = tid + 256 * l;
+      if (q < EP) dst[q] = rx[l];
+    }
+  };
+
+  // prologue: displacement planes kbeg-1, kbeg, kbeg+1 and density planes kbeg-1, kbeg
+  fetch(kbeg - 1, kbeg - 1);
+  store_u(kbeg - 1);
+  store_x(kbeg - 1);
+  fetch(kbeg, kbeg);
+  store_u(kbeg);
+  store_x(kbeg);
+  fetch(kbeg + 1, -1);
+  store_u(kbeg + 1);
+  __syncthreads();
+
+  const int lx = tid % Stream::SX, ly = tid / Stream::SX;
+  const int i = i0 + lx, j = j0 + ly;
+  const bool own = i < g.nx && j < g.ny;
+
+  for (int k = kbeg; k < kend; ++k) {
+    fetch(k + 2, k + 1);  // in flight during the multiply
+    // own-node operands of the update (also in flight)
+    const long long n = ((long long)k * g.ny + j) * g.nx + i;
+    unsigned mk = 0;
+    T uo[3] = {0, 0, 0}, bo[3] = {0, 0, 0}, dv[3] = {0, 0, 0};
+    if (own) {
+      mk = bcm[n];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        uo[d] = u[3 * n + d];
+        if (MODE != MODE_APPLY) bo[d] = b[3 * n + d];
+        if (MODE == MODE_JACOBI) dv[d] = dinv[3 * n + d];
+      }
+    }
+
+    T acc[8][3];
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) acc[e][d] = 0.0;
+#pragma unroll
+    for (int dz = -1; dz <= 1; ++dz) {
+      T(*pl)[HP] = su[(k + dz + 1) & 3];
+#pragma unroll
+      for (int dy = -1; dy <= 1; ++dy) {
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int h = (ly + 1 + dy) * HX + (lx + 1 + dx);
+          T um[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) um[d] = pl[d][h];
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const int ex = e & 1, ey = (e >> 1) & 1, ez = (e >> 2) & 1;
+            const int bx = dx + 1 - ex, by = dy + 1 - ey, bz = dz + 1 - ez;
+            if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
+            const int locn = (1 - ex) + 2 * (1 - ey) + 4 * (1 - ez);
+            const int locm = bx + 2 * by + 4 * bz;
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+              for (int d2 = 0; d2 < 3; ++d2) {
+                const int idx = (locn * 3 + d) * 24 + locm * 3 + d2;
+                T kval;
+                if (CLS)
+                  kval = K0Pattern<3>::sgn(idx) > 0 ? kv[K0Pattern<3>::cls(idx)] : -kv[K0Pattern<3>::cls(idx)];
+                else
+                  kval = (T)K0[idx];
+                acc[e][d] = fma(kval, um[d2], acc[e][d]);
+              }
+          }
+        }
+      }
+    }
+    T out[3] = {0, 0, 0};
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int ex = e & 1, ey = (e >> 1) & 1, ez = (e >> 2) & 1;
+      const T xv = sx[(k - 1 + ez + 1) % 3][(ly + ey) * EX + (lx + ex)];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) out[d] = fma(xv, acc[e][d], out[d]);
+    }
+    if (own) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const T ku = ((mk >> d) & 1u) ? bcdiag * uo[d] : out[d];
+        if (MODE == MODE_APPLY)
+          y[3 * n + d] = ku;
+        else if (MODE == MODE_RESIDUAL)
+          y[3 * n + d] = bo[d] - ku;
+        else
+          y[3 * n + d] = uo[d] + omega * dv[d] * (bo[d] - ku);
+      }
+    }
+    store_u(k + 2);
+    store_x(k + 1);
+    __syncthreads();
   }
 }
 
@@ -417,11 +609,96 @@ static void launch_fine_t(int mode, const GridDims& g, const double* K0, const T
                                                                     omega, y);
 }
 
+// Launch-shape tuning of the fine operator (set by pmt_set_tuning; read at every launch).
+static int g_fine_kernel = 1;  // 0 = 3D tiles, 1 = z-streamed columns
+static int g_stream_nzc = 0;   // node planes per streamed block (0 = chosen from the CU count)
+
+static int g_tuning_epoch = 0;  // bumped by every change: captured launches must be re-captured
+
+int tuning_epoch() { return g_tuning_epoch; }
+
+int set_tuning(int param, int value) {
+  if (param == PMT_TUNE_FINE_KERNEL && (value == 0 || value == 1)) {
+    g_fine_kernel = value;
+    ++g_tuning_epoch;
+    return 0;
+  }
+  if (param == PMT_TUNE_STREAM_PLANES && value >= 0) {
+    g_stream_nzc = value;
+    ++g_tuning_epoch;
+    return 0;
+  }
+  return -1;
+}
+
+int tuned_stream_planes() { return g_stream_nzc; }
+
+int get_tuning(int param) {
+  if (param == PMT_TUNE_FINE_KERNEL) return g_fine_kernel;
+  if (param == PMT_TUNE_STREAM_PLANES) return g_stream_nzc;
+  return -1;
+}
+
+int device_cu_count() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cus[dev] == 0) {
+    int c = 0;
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
+    cus[dev] = c;
+  }
+  return cus[dev];
+}
+
+// Planes per streamed block: minimise (rounds of resident blocks) x (planes + 2 prologue planes),
+// assuming four resident 256-thread blocks per CU.
+static int stream_planes(int ntiles_xy, int nz) {
+  if (g_stream_nzc > 0) return g_stream_nzc < nz ? g_stream_nzc : nz;
+  const long long slots = 4LL * device_cu_count();
+  int best = nz;
+  long long best_cost = -1;
+  for (int c = 2; c <= nz; ++c) {
+    const long long blocks = (long long)ntiles_xy * ((nz + c - 1) / c);
+    const long long cost = ((blocks + slots - 1) / slots) * (c + 2);
+    if (best_cost < 0 || cost < best_cost || (cost == best_cost && c < best)) {
+      best_cost = cost;
+      best = c;
+    }
+  }
+  return best;
+}
+
+template <bool CLS, typename T>
+static void launch_stream_t(int mode, const GridDims& g, const double* K0, const T* xe, const uint8_t* bcm, T bcdiag,
+                            const T* u, const T* b, const T* dinv, T omega, T* y, hipStream_t s) {
+  const int tx = (g.nx + Stream::SX - 1) / Stream::SX;
+  const int ty = (g.ny + Stream::SY - 1) / Stream::SY;
+  const int nxy = tx * ty;
+  const int nzc = stream_planes(nxy, g.nz);
+  const int nwork = nxy * ((g.nz + nzc - 1) / nzc);
+  dim3 grid((unsigned)((nwork + 7) & ~7));
+  if (mode == MODE_APPLY)
+    fine_stream_kernel<MODE_APPLY, CLS, T><<<grid, 256, 0, s>>>(g, tx, nxy, nzc, nwork, K0, xe, bcm, bcdiag, u, b,
+                                                                dinv, omega, y);
+  else if (mode == MODE_RESIDUAL)
+    fine_stream_kernel<MODE_RESIDUAL, CLS, T><<<grid, 256, 0, s>>>(g, tx, nxy, nzc, nwork, K0, xe, bcm, bcdiag, u, b,
+                                                                   dinv, omega, y);
+  else
+    fine_stream_kernel<MODE_JACOBI, CLS, T><<<grid, 256, 0, s>>>(g, tx, nxy, nzc, nwork, K0, xe, bcm, bcdiag, u, b,
+                                                                 dinv, omega, y);
+}
+
 template <typename T>
 int launch_fine(int mode, const GridDims& g, const double* K0, int k0_pattern, const T* xe, const uint8_t* bcm,
                 double bcdiag, const T* u, const T* b, const T* dinv, double omega, T* y, hipStream_t s) {
   const T bd = (T)bcdiag, om = (T)omega;
-  if (g.dim == 3) {
+  if (g.dim == 3 && g_fine_kernel == 1) {
+    if (k0_pattern)
+      launch_stream_t<true, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s);
+    else
+      launch_stream_t<false, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s);
+  } else if (g.dim == 3) {
     if (k0_pattern)
       launch_fine_t<3, true, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s);
     else
@@ -546,6 +823,32 @@ extern "C" {
 int pmt_k0_pattern(int dim, const double* K0_host) {
   if ((dim != 2 && dim != 3) || !K0_host) return arg_error("pmt_k0_pattern: bad argument");
   return k0_matches_pattern(dim, K0_host);
+}
+
+int pmt_stiffness_apply_f32(int nelx, int nely, int nelz, const double* K0, int k0_pattern, const float* xe,
+                            const uint8_t* bcmask, double bcdiag, const float* u, float* y, void* stream) {
+  if (!grid_ok(nelx, nely, nelz) || !K0 || !xe || !bcmask || !u || !y)
+    return arg_error("pmt_stiffness_apply_f32: bad argument (the Dirichlet mask is mandatory)");
+  return launch_fine<float>(MODE_APPLY, make_grid(nelx, nely, nelz), K0, k0_pattern, xe, bcmask, bcdiag, u, nullptr,
+                            nullptr, 0.0, y, (hipStream_t)stream);
+}
+
+int pmt_stiffness_apply_bf16(int nelx, int nely, int nelz, const double* K0, const float* xe, const uint8_t* bcmask,
+                             double bcdiag, const float* u, float* y, void* stream) {
+  if (!grid_ok(nelx, nely, nelz) || nelz <= 0 || !K0 || !xe || !bcmask || !u || !y)
+    return arg_error("pmt_stiffness_apply_bf16: bad argument (hex8 grids only; the Dirichlet mask is mandatory)");
+  return launch_fine_mfma(MODE_APPLY, make_grid(nelx, nely, nelz), K0, xe, bcmask, bcdiag, u, nullptr, nullptr, 0.0, y,
+                          (hipStream_t)stream);
+}
+
+int pmt_set_tuning(int param, int value) {
+  if (set_tuning(param, value) != 0) return arg_error("pmt_set_tuning: unknown parameter or value out of range");
+  return PMT_OK;
+}
+
+int pmt_get_tuning(int param) {
+  const int v = get_tuning(param);
+  return v < 0 ? arg_error("pmt_get_tuning: unknown parameter") : v;
 }
 
 int pmt_stiffness_apply(int nelx, int nely, int nelz, const double* K0, int k0_pattern, const double* xe,

@@ -9,7 +9,9 @@ Mirrors ``pymoto/solvers/iterative.py``:
   pre-smoothing sweep starts from zero; nested instances form the hierarchy
   (``ex_compliance_multigrid.py:108-121``); the innermost level is solved directly.
 * ``CG(A=None, preconditioner=..., tol=1e-7, maxit=10000, restart=50, verbosity=0)``
-  (lines 295-403) with an explicit residual every ``restart`` iterations.
+  (lines 295-403) with an explicit residual every ``restart`` iterations; a 2D right-hand side is
+  solved by block PCG with the reference's Gram-Schmidt ``orth`` (lines 259-292), up to 8 columns
+  per block, each iteration replayed from captured HIP graphs.
 
 The hierarchy lives in one ``pmt_mg`` handle of the HIP runtime; ``CG.solve`` runs the whole
 iteration inside the library (``pmt_pcg_solve``).
@@ -55,9 +57,13 @@ class GeometricMultigrid(Preconditioner):
     """Geometric multigrid V-cycle preconditioner (``iterative.py:124-256``)."""
 
     _available_cycles = ["v", "w"]
+    # V-cycle arithmetic: fp32 = FP32 throughout the V-cycle; bf16 = as fp32 with the fine-grid
+    # operator on the matrix cores (BF16 operands, FP32 accumulation); CG stays FP64 in all cases
+    _precision_bits = {"fp64": 64, "fp32": 32, "bf16": 16}
 
     def __init__(self, domain: VoxelDomain, A=None, cycle: str = "V", inner_level=None, smoother=None,
                  smooth_steps: int = 5, precision: str = "fp64"):
+        self._handle = None
         assert domain.nelx % 2 == 0 and domain.nely % 2 == 0 and domain.nelz % 2 == 0, (
             f"Domain sizes {domain.nelx, domain.nely, domain.nelz} must be divisible by 2")
         assert cycle.lower() in self._available_cycles, f"Cycle ({cycle}) is not available"
@@ -68,8 +74,10 @@ class GeometricMultigrid(Preconditioner):
         self.inner_level = inner_level
         self.smoother = DampedJacobi(w=0.5) if smoother is None else smoother
         self.smooth_steps = smooth_steps
-        if precision not in ("fp64", "fp32"):
-            raise ValueError("precision must be 'fp64' (the reference's arithmetic) or 'fp32'")
+        if precision not in self._precision_bits:
+            raise ValueError("precision must be 'fp64' (the reference's arithmetic), 'fp32' or 'bf16'")
+        if precision == "bf16" and domain.dim != 3:
+            raise NotImplementedError("the BF16 matrix-core operator is implemented for hex8 grids only")
         # arithmetic of the whole V-cycle (taken from the outermost level); not in the reference API
         self.precision = precision
         self.sub_domain = domain.coarsen()
@@ -110,8 +118,8 @@ class GeometricMultigrid(Preconditioner):
                                      omega.ctypes.data_as(ctypes.c_void_p), steps.ctypes.data_as(ctypes.c_void_p),
                                      K0.ctypes.data_as(ctypes.c_void_p), ctypes.byref(h)), "multigrid setup")
         self._handle, self._key = h, key
-        if self.precision == "fp32":
-            _lib.check(lib.pmt_mg_set_precision(h, 32), "multigrid precision")
+        if self.precision != "fp64":
+            _lib.check(lib.pmt_mg_set_precision(h, self._precision_bits[self.precision]), "multigrid precision")
 
     def _release(self):
         if self._handle is not None:
@@ -190,9 +198,13 @@ def build_hierarchy(domain: VoxelDomain, levels: int, omega=0.5, smooth_steps=5,
 class CG(LinearSolver):
     """Preconditioned conjugate gradients (``iterative.py:295-403``), run inside the HIP runtime."""
 
+    max_block = 8  # right-hand sides per block solve (pmt_block_pcg_solve)
+
     def __init__(self, A=None, preconditioner=None, tol: float = 1e-7, maxit: int = 10000, restart: int = 50,
-                 verbosity: int = 0):
+                 verbosity: int = 0, graph: bool = True):
         self.preconditioner = preconditioner
+        # several right-hand sides: replay each block-CG iteration as captured HIP graphs
+        self.graph = graph
         self.A = A
         self.tol = tol
         self.maxit = maxit
@@ -233,6 +245,26 @@ class CG(LinearSolver):
             print(f"CG Converged in {it.value} iterations, with final residual {rel.value}")
         return x
 
+    def _solve_block(self, B, X0):
+        """Block PCG (``iterative.py:340-403`` with a 2D right-hand side) for 2..8 columns.  B, X0: (k, n)."""
+        lib = _lib.lib()
+        k = B.shape[0]
+        X = torch.zeros_like(B) if X0 is None else X0.clone()
+        it = ctypes.c_int(0)
+        rel = np.zeros(k)
+        hist = np.zeros(self.maxit + 1)
+        _lib.check(lib.pmt_block_pcg_solve(self.preconditioner.handle, k, _lib.ptr(B), _lib.ptr(X), int(X0 is None),
+                                           float(self.tol), int(self.maxit), int(self.restart), int(bool(self.graph)),
+                                           ctypes.byref(it), rel.ctypes.data_as(ctypes.c_void_p),
+                                           hist.ctypes.data_as(ctypes.c_void_p), _lib.stream()), "block PCG solve")
+        self.iterations, self.relres = it.value, rel
+        self.history = hist[: it.value + 1]
+        if rel.max() > self.tol:
+            warnings.warn(f"CG Maximum iterations ({self.maxit}) reached, with final residuals {rel}")
+        elif self.verbosity >= 1:
+            print(f"CG Converged in {it.value} iterations, with final (max) residual {rel.max()}")
+        return X
+
     def solve(self, rhs, x0=None, trans="N"):
         rhs = _lib.as_device(rhs)
         if rhs.ndim == 1:
@@ -240,11 +272,15 @@ class CG(LinearSolver):
         if rhs.ndim == 2 and rhs.shape[1] == 1:
             return self._solve_1rhs(rhs[:, 0].contiguous(), None if x0 is None else _lib.as_device(x0).reshape(-1)
                                     ).reshape(-1, 1)
-        cols = []
-        for k in range(rhs.shape[1]):
-            x0k = None if x0 is None else _lib.as_device(x0)[:, k].contiguous()
-            cols.append(self._solve_1rhs(rhs[:, k].contiguous(), x0k))
-        return torch.stack(cols, dim=1)
+        Bt = rhs.t().contiguous()  # column j contiguous
+        X0t = None if x0 is None else _lib.as_device(x0).t().contiguous()
+        out, its = [], []
+        for c0 in range(0, Bt.shape[0], self.max_block):
+            blk = slice(c0, c0 + self.max_block)
+            out.append(self._solve_block(Bt[blk].contiguous(), None if X0t is None else X0t[blk].contiguous()))
+            its.append(self.iterations)
+        self.iterations = max(its)
+        return torch.cat(out, dim=0).t().contiguous()
 
 
 def auto_determine_solver(A, tol=1e-12, **kwargs):

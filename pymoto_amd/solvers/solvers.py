@@ -88,53 +88,68 @@ class LDAWrapper(LinearSolver):
                                            _lib.ptr(v), _lib.ptr(rl), _lib.ptr(sol), _lib.stream()), "bc split")
         return rl, sol
 
-    def _solve_1rhs(self, rhs, x0):
+    def _solve_cols(self, cols, x0s):
+        """LDAS for a list of right-hand-side columns solved together (``solvers.py:160-248``):
+        reconstruct every column from the database, check each residual, solve the columns above
+        ``tol`` in one call of the inner solver (a block solve for several), add them to the database."""
         A, g = self.A, self.A.grid
-        rl, sol = self._split(rhs)
-        for x, b in zip(self.x_stored, self.b_stored):
-            alpha = _dot(rl, b) / _dot(b, b)
-            _axpy(-alpha, b, rl)
-            _axpy(alpha, x, sol)
-        r = A.residual(rhs, sol)
-        bn = np.sqrt(_dot(rhs, rhs))
-        rtol = np.sqrt(_dot(r, r)) / (bn if bn > 0 else 1.0)
-        did = rtol > self.tol
-        if did:
-            x0l = None
-            if x0 is not None:
-                x0l, _ = self._split(_lib.as_device(x0).clone())
-                for x in self.x_stored:
-                    _axpy(-_dot(x0l, x) / _dot(x, x), x, x0l)
-            xnew = self.solver.solve(rl, x0=x0l)
-            _lib.check(_lib.lib().pmt_axpy_free(*g.nel3, _lib.ptr(g.bcmask_dev), 1.0, _lib.ptr(xnew), _lib.ptr(sol),
-                                                _lib.stream()), "axpy_free")
-            xadd, _ = self._split(xnew)
-            badd, _ = self._split(A @ xnew)
+        sols, rls, rtols = [], [], []
+        for rhs in cols:
+            rl, sol = self._split(rhs)
             for x, b in zip(self.x_stored, self.b_stored):
-                beta = _dot(badd, b) / _dot(b, b)
-                _axpy(-beta, b, badd)
-                _axpy(-beta, x, xadd)
-            bnrm = np.sqrt(_dot(badd, badd))
-            if np.isfinite(bnrm) and bnrm > 0:
-                _scale(1.0 / bnrm, badd)
-                _scale(1.0 / bnrm, xadd)
-                self.x_stored.append(xadd)
-                self.b_stored.append(badd)
-        return sol, rtol, did
+                alpha = _dot(rl, b) / _dot(b, b)
+                _axpy(-alpha, b, rl)
+                _axpy(alpha, x, sol)
+            r = A.residual(rhs, sol)
+            bn = np.sqrt(_dot(rhs, rhs))
+            rtols.append(np.sqrt(_dot(r, r)) / (bn if bn > 0 else 1.0))
+            sols.append(sol)
+            rls.append(rl)
+        did = [rt > self.tol for rt in rtols]
+        todo = [k for k in range(len(cols)) if did[k]]
+        if todo:
+            x0l = None
+            if x0s is not None:
+                x0l = []
+                for k in todo:
+                    v, _ = self._split(_lib.as_device(x0s[k]).clone())
+                    for x in self.x_stored:
+                        _axpy(-_dot(v, x) / _dot(x, x), x, v)
+                    x0l.append(v)
+            if len(todo) == 1:
+                xnews = [self.solver.solve(rls[todo[0]], x0=None if x0l is None else x0l[0])]
+            else:
+                X = self.solver.solve(torch.stack([rls[k] for k in todo], dim=1),
+                                      x0=None if x0l is None else torch.stack(x0l, dim=1))
+                xnews = [X[:, i].contiguous() for i in range(len(todo))]
+            for k, xnew in zip(todo, xnews):
+                _lib.check(_lib.lib().pmt_axpy_free(*g.nel3, _lib.ptr(g.bcmask_dev), 1.0, _lib.ptr(xnew),
+                                                    _lib.ptr(sols[k]), _lib.stream()), "axpy_free")
+            for xnew in xnews:
+                xadd, _ = self._split(xnew)
+                badd, _ = self._split(A @ xnew)
+                for x, b in zip(self.x_stored, self.b_stored):
+                    beta = _dot(badd, b) / _dot(b, b)
+                    _axpy(-beta, b, badd)
+                    _axpy(-beta, x, xadd)
+                bnrm = np.sqrt(_dot(badd, badd))
+                if np.isfinite(bnrm) and bnrm > 0:
+                    _scale(1.0 / bnrm, badd)
+                    _scale(1.0 / bnrm, xadd)
+                    self.x_stored.append(xadd)
+                    self.b_stored.append(badd)
+        return sols, rtols, did
 
     def solve(self, rhs, x0=None, trans="N"):
         if trans.upper() not in ("N", "T", "H"):
             raise TypeError("Only N, T, or H transposition is possible")
         rhs = _lib.as_device(rhs)
         if rhs.ndim == 1:
-            sol, self._last_rtol, self._did_solve = self._solve_1rhs(rhs, x0)
-            return sol
-        cols, rt, did = [], [], []
-        for k in range(rhs.shape[1]):
-            x0k = None if x0 is None else _lib.as_device(x0)[:, k].contiguous()
-            s, r, d = self._solve_1rhs(rhs[:, k].contiguous(), x0k)
-            cols.append(s)
-            rt.append(r)
-            did.append(d)
+            sols, rt, did = self._solve_cols([rhs], None if x0 is None else [x0])
+            self._last_rtol, self._did_solve = rt[0], did[0]
+            return sols[0]
+        cols = [rhs[:, k].contiguous() for k in range(rhs.shape[1])]
+        x0s = None if x0 is None else [_lib.as_device(x0)[:, k].contiguous() for k in range(rhs.shape[1])]
+        sols, rt, did = self._solve_cols(cols, x0s)
         self._last_rtol, self._did_solve = np.array(rt), np.array(did)
-        return torch.stack(cols, dim=1)
+        return torch.stack(sols, dim=1)

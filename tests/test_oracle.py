@@ -193,6 +193,44 @@ def test_mgcg_matches_spsolve():
     npt.assert_allclose(u, spsolve(K.tocsc(), f), rtol=1e-8, atol=1e-10 * np.abs(u).max())
 
 
+def _block_case(k, seed=0):
+    nel = (16, 8, 8)
+    bc, _ = problems.cantilever3d(*nel)
+    x = np.random.default_rng(seed).uniform(0.1, 1.0, 16 * 64)
+    K = fe.Assembly(*nel, fe.element_stiffness(3), bc=bc).matrix(fe.simp(x))
+    B = np.random.default_rng(seed + 1).standard_normal((K.shape[0], k))
+    B[bc, :] = 0.0
+    return nel, K, B
+
+
+def test_block_mgcg_matches_spsolve():
+    """Block PCG (iterative.py:340-403 with a 2D rhs) converges every column to the direct solution
+    and needs no more iterations than the hardest single column."""
+    nel, K, B = _block_case(4)
+    cg = CG(preconditioner=build_mg(nel, 3, omega=0.6, smooth_steps=2), tol=1e-12)
+    cg.update(K)
+    X = cg.solve(B)
+    ref = spsolve(K.tocsc(), B)
+    npt.assert_allclose(X, ref, rtol=1e-8, atol=1e-10 * np.abs(ref).max())
+    its_block = cg.iterations
+    its_single = []
+    for j in range(B.shape[1]):
+        cg.solve(B[:, j])
+        its_single.append(cg.iterations)
+    assert its_block <= max(its_single)
+
+
+def test_block_mgcg_rank_deficient_columns():
+    """Linearly dependent columns are dropped by orth (iterative.py:283-285) without breakdown."""
+    nel, K, B = _block_case(2, seed=3)
+    B3 = np.stack([B[:, 0], 2.0 * B[:, 0], B[:, 1]], axis=1)
+    cg = CG(preconditioner=build_mg(nel, 3, omega=0.6, smooth_steps=2), tol=1e-12)
+    cg.update(K)
+    X = cg.solve(B3)
+    ref = spsolve(K.tocsc(), B3)
+    npt.assert_allclose(X, ref, rtol=1e-8, atol=1e-10 * np.abs(ref).max())
+
+
 # ---------------------------------------------------------------- end-to-end vs the reference
 def test_pipeline_c1_mbb_vs_reference():
     g = golden("c1_mbb.npz")
