@@ -12,10 +12,13 @@ before the timed region).  Solver variants, timed one after the other on the sam
 * ``bf16``: as ``fp32``, with the fine-grid operator of the V-cycle on the matrix cores (BF16
   element products, FP32 accumulation).
 
-A variant may name its own Jacobi damping as ``precision@omega``.  The reduced-precision variants
-use omega = 0.5 by default: with these SIMP fields lambda_max(D^-1 K) is 3.5-3.8, so omega = 0.6 lies
-above the Jacobi stability bound 2/lambda_max and the smoother amplifies the highest modes, which the
-FP64 V-cycle survives but which magnify the rounding of a reduced-precision one.
+A variant may name its own Jacobi damping as ``precision@omega``.  ``fp64`` is the reference
+configuration (omega = 0.6); the other default variants use omega = 0.5: with these SIMP fields
+lambda_max(D^-1 K) is 3.5-3.8, so omega = 0.6 lies above the Jacobi stability bound 2/lambda_max and
+the smoother amplifies the highest modes - the FP64 V-cycle still preconditions CG, but needs about
+four times the iterations, and reduced-precision V-cycles degrade further.  ``speedup_vs_fp64`` is
+against the reference configuration, ``speedup_vs_fp64_same_smoother`` against FP64 with the same
+omega as the fastest variant.
 
 ``value`` is the seconds per design iteration of the fastest variant whose compliance agrees with
 the FP64 variant to 1e-7; every variant's time, CG iterations and errors are in ``config``.
@@ -47,9 +50,10 @@ def parse():
     p.add_argument("--levels", type=int, default=0, help="grids in the MG hierarchy (0 = reference example rule)")
     p.add_argument("--tol", type=float, default=1e-8)
     p.add_argument("--omega", type=float, default=0.6, help="Jacobi damping of variants without '@omega'")
-    p.add_argument("--variants", default="fp64,fp32@0.5,bf16@0.5",
+    p.add_argument("--variants", default="fp64,fp64@0.5,fp32@0.5,bf16@0.5",
                    help="comma-separated precision[@omega] list; the first fp64 entry is the reference")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    p.add_argument("--no-batched", action="store_true", help="skip the batched multi-load workload")
     p.add_argument("--cpu-nel", type=int, nargs=3, default=[48, 24, 24])
     return p.parse_args()
 
@@ -113,6 +117,102 @@ def kernel_roofline(pm, torch, net, reps=20):
     bytes_ = 2.0 * ndof * 8 + nel * 8
     return dict(kernel="fine_node_kernel<3, apply, pattern, double> (K.u, FP64)", seconds=t, flops=flops,
                 bytes=bytes_, tflops=flops / t / 1e12, gbs=bytes_ / t / 1e9)
+
+
+def batched_workload(pm, torch, nel=(128, 64, 64), beta=8.0, etas=(0.3, 0.5, 0.7), nloads=4, nuneven=16, seed=7,
+                     precision="fp32", omega=0.5, tol=1e-8):
+    """Robust multi-load batch: 3 Heaviside designs (eta 0.3/0.5/0.7, beta 8) x 4 seeded point loads on
+    the x = max face, each design one 4-column block PCG (captured-graph iterations), timed against
+    the same 12 solves one right-hand side at a time; plus 16 independent problems of seeded sizes
+    from 64x32 Q4 to 96x48x48 hex8, solved concurrently on 8 streams and timed against one stream."""
+    import time as _t
+    from pymoto_amd import problems
+    from pymoto_amd.batch import solve_independent
+    rng = np.random.default_rng(seed)
+    d = pm.VoxelDomain(*nel)
+    bc, _ = problems.cantilever3d(d)
+    F = np.zeros((3 * d.nnodes, nloads))
+    for c in range(nloads):
+        j, k = int(rng.integers(0, nel[1] + 1)), int(rng.integers(0, nel[2] + 1))
+        F[3 * d.get_nodenumber(nel[0], j, k) + 1, c] = -1.0
+    Fd = torch.as_tensor(F, device="cuda")
+    x = torch.as_tensor(rng.uniform(0.0, 1.0, d.nel), device="cuda")
+    xf = pm.FilterConv(d, radius=1.5)(pm.Signal("x", x)).state
+    hv = "(tanh({b} * {e}) + tanh({b} * (inp0 - {e}))) / (tanh({b} * {e}) + tanh({b} * (1 - {e})))"
+    Ks = []
+    for eta in etas:
+        xp = pm.MathExpression(hv.format(b=beta, e=eta))(pm.Signal("xf", xf)).state
+        xs = pm.MathExpression(f"{1e-9} + {1.0 - 1e-9}*inp0^3")(pm.Signal("xp", xp)).state
+        Ks.append(pm.AssembleStiffness(d, bc=bc)(xs))
+    levels = pm.solvers.auto_levels(d)
+
+    def make_solver(graph=True):
+        return pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(d, levels, omega=omega, smooth_steps=2,
+                                                                       precision=precision), tol=tol, graph=graph)
+    solvers = [make_solver() for _ in etas]
+    for K, sv in zip(Ks, solvers):  # setup + graph capture, untimed
+        sv.update(K)
+        sv.solve(Fd)
+    torch.cuda.synchronize()
+    t0 = _t.perf_counter()
+    its_block = []
+    for K, sv in zip(Ks, solvers):
+        sv.update(K)
+        sv.solve(Fd)
+        its_block.append(int(sv.iterations))
+    torch.cuda.synchronize()
+    t_block = _t.perf_counter() - t0
+    t0 = _t.perf_counter()
+    its_single = []
+    for K, sv in zip(Ks, solvers):
+        sv.update(K)
+        its_single.append([])
+        for c in range(nloads):
+            sv.solve(Fd[:, c].contiguous())
+            its_single[-1].append(int(sv.iterations))
+    torch.cuda.synchronize()
+    t_single = _t.perf_counter() - t0
+
+    # uneven batch of independent problems
+    jobs = []
+    dofs = 0
+    for _ in range(nuneven):
+        if rng.random() < 0.5:
+            nx = int(rng.choice([64, 96, 128, 160, 192, 256]))
+            dd = pm.VoxelDomain(nx, nx // 2)
+            b2, f2 = problems.mbb2d(dd)
+        else:
+            nx = int(rng.choice([32, 48, 64, 80, 96]))
+            dd = pm.VoxelDomain(nx, nx // 2, nx // 2)
+            b2, f2 = problems.cantilever3d(dd)
+        xx = torch.as_tensor(rng.uniform(0.0, 1.0, dd.nel), device="cuda")
+        xxf = pm.FilterConv(dd, radius=1.5)(pm.Signal("x", xx)).state
+        xxs = pm.MathExpression(f"{1e-9} + {1.0 - 1e-9}*inp0^3")(pm.Signal("xf", xxf)).state
+        K2 = pm.AssembleStiffness(dd, bc=b2)(xxs)
+        prec = precision if (dd.dim == 3 or precision != "bf16") else "fp32"
+        sv = pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(dd, max(2, pm.solvers.auto_levels(dd)),
+                                                                     omega=omega, smooth_steps=2, precision=prec),
+                           tol=tol)
+        jobs.append((K2, torch.as_tensor(f2, device="cuda"), sv))
+        dofs += K2.shape[0]
+    solve_independent(jobs, workers=8)  # setup, untimed
+    t0 = _t.perf_counter()
+    _, its_uneven = solve_independent(jobs, workers=8)
+    t_conc = _t.perf_counter() - t0
+    t0 = _t.perf_counter()
+    solve_independent(jobs, workers=1)
+    t_seq = _t.perf_counter() - t0
+    ndof = 3 * d.nnodes
+    return {
+        "workload": f"robust {nel[0]}x{nel[1]}x{nel[2]} hex8, Heaviside beta {beta} eta {list(etas)}, {nloads} point "
+                    f"loads per design; {precision} V-cycle (omega {omega}), FP64 CG, tol {tol:g}",
+        "rhs_solves": len(etas) * nloads, "ndof": ndof,
+        "block_graph_s": t_block, "block_iterations": its_block,
+        "single_rhs_s": t_single, "single_rhs_iterations": its_single,
+        "dof_rhs_per_s_block": ndof * len(etas) * nloads / t_block,
+        "uneven": {"problems": nuneven, "total_dofs": dofs, "concurrent_8_streams_s": t_conc,
+                   "one_stream_s": t_seq, "iterations": its_uneven},
+    }
 
 
 def cpu_baseline(nel, steps=3):
@@ -212,8 +312,12 @@ def main():
                                f"warm start",
                    "ndof": ndof, "nel": d.nel, "variant": best, "dof_per_s": ndof / sec,
                    "variants": summary,
-                   "speedup_vs_fp64": (summary["fp64"]["s_per_iter"] / sec) if "fp64" in summary else None},
+                   "speedup_vs_fp64": (summary["fp64"]["s_per_iter"] / sec) if "fp64" in summary else None,
+                   "speedup_vs_fp64_same_smoother": (summary[f"fp64@{parse_variant(best)[1]}"]["s_per_iter"] / sec)
+                   if parse_variant(best)[1] is not None and f"fp64@{parse_variant(best)[1]}" in summary else None},
     }
+    if not args.no_batched:
+        line["config"]["batched"] = batched_workload(pm, torch)
     if roof is not None:
         line["roofline"] = {"bound": "mfma", "achieved": roof["tflops"], "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                             "frac": roof["tflops"] / PEAK_FP64_TFLOPS, "traffic": None, "kernel": roof["kernel"],

@@ -9,7 +9,8 @@
  * (pymoto/common/domain.py:200-251).  Boundary conditions are a per-node byte mask whose bit d
  * marks dof d as Dirichlet (pymoto/modules/assembly.py:86-98).
  *
- * No entry point allocates device memory except pmt_mg_create/pmt_mg_update (setup).
+ * No entry point allocates device memory except pmt_mg_create/pmt_mg_update (setup) and the first
+ * pmt_block_pcg_solve call on a handle with a given number of right-hand sides (its workspace).
  * Calls on one pmt_mg handle are not thread-safe.
  */
 #ifndef PYMOTO_AMD_H
@@ -56,6 +57,13 @@ int pmt_powlaw_fwd(long long n, double a, double b, double p, const double* x, d
 int pmt_powlaw_bwd(long long n, double c, double q, const double* x, const double* dy, double* dx,
                    int accumulate, void* stream);
 
+/* Heaviside projection y = (tanh(beta*eta) + tanh(beta*(x-eta))) / (tanh(beta*eta) + tanh(beta*(1-eta)))
+ * (the MathExpression of examples/topology_optimization/ex_compliance_robust.py:139) and its adjoint
+ * dx (+)= dy*dy/dx; dbeta (may be NULL) receives dy_i * dy_i/dbeta per entry. */
+int pmt_heaviside_fwd(long long n, double beta, double eta, const double* x, double* y, void* stream);
+int pmt_heaviside_bwd(long long n, double beta, double eta, const double* x, const double* dy, double* dx,
+                      int accumulate, double* dbeta, void* stream);
+
 /* Vector kernels.  y = a*x + b*y (x may be NULL: y = b*y). */
 int pmt_axpby(long long n, double a, const double* x, double b, double* y, void* stream);
 /* *out = x.y, deterministic two-stage reduction; work holds pmt_reduce_work_size() doubles.
@@ -96,10 +104,12 @@ int pmt_k0_pattern(int dim, const double* K0_host);
 /* Launch-shape tuning of the fine-grid operator, process-wide, read at every launch.
  * PMT_TUNE_FINE_KERNEL: 0 = 3D node tiles, 1 = xy node columns streamed along z (default; 3D only).
  * PMT_TUNE_STREAM_PLANES: node planes per streamed column, 0 = chosen from the CU count (default).
+ * PMT_TUNE_STREAM_TILE: streamed column shape, 0 = 32x8 (default), 1 = 64x4, 2 = 16x16 nodes.
  * pmt_get_tuning returns the current value or a negative status.
- * (No reference counterpart: the launch shapes picked by the autotuner.) */
+ * (No reference counterpart: the launch shapes picked by the autotuner, pymoto_amd/tune.py.) */
 #define PMT_TUNE_FINE_KERNEL 1
 #define PMT_TUNE_STREAM_PLANES 2
+#define PMT_TUNE_STREAM_TILE 3
 int pmt_set_tuning(int param, int value);
 int pmt_get_tuning(int param);
 /* diag(K) (DampedJacobi.update, iterative.py:38-39) */
@@ -181,6 +191,44 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
  * Allocates its workspace on the first call with a larger k. */
 int pmt_block_pcg_solve(pmt_mg* mg, int k, const double* B, double* X, int x0_is_zero, double tol, int maxit,
                         int restart, int use_graph, int* iters, double* relres, double* history, void* stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * Learned warm start (no reference counterpart): kernels of a 3D convolutional encoder-decoder on
+ * the element grid.  Activations are channels-last BF16 arrays X[voxel][c] (voxel in pyMOTO's
+ * element order, c contiguous), weights BF16 W[tap][cout][cin] with tap = (dz+1)*9 + (dy+1)*3 +
+ * (dx+1), 3x3x3 stencil with zero padding; products on the matrix cores with FP32 accumulation.
+ * BF16 buffers are passed as void* (2 bytes per value).  nx must be a multiple of 16.
+ */
+/* Y = act(bias + conv(X, W)); cin in {8,16,32,64}, cout in {16,32,64}; bias (FP32) may be NULL;
+ * relu = 1 applies max(0, .). */
+int pmt_conv3d_fwd(int nx, int ny, int nz, int cin, int cout, const void* X, const void* W, const float* bias,
+                   int relu, void* Y, void* stream);
+/* dX = conv^T(dY): the forward kernel on the flipped, transposed weights written to Wt (27*cin*cout
+ * BF16, caller-provided); cin in {16,32,64}. */
+int pmt_conv3d_bwd_data(int nx, int ny, int nz, int cin, int cout, const void* dY, const void* W, void* Wt, void* dX,
+                        void* stream);
+/* dW[tap][cout][cin] (FP32) = sum_v dY[v][co] X[v+off(tap)][ci]; dbias (FP32, may be NULL) = sum_v dY;
+ * work: pmt_conv3d_wgrad_work_size(cin, cout) floats.  Deterministic. */
+int pmt_conv3d_wgrad_work_size(int cin, int cout);
+int pmt_conv3d_bwd_weight(int nx, int ny, int nz, int cin, int cout, const void* X, const void* dY, float* dW,
+                          float* dbias, float* work, void* stream);
+/* G *= (Y > 0) elementwise (ReLU adjoint from the layer output). */
+int pmt_relu_mask_bf16(long long n, const void* Y, void* G, void* stream);
+/* 2x2x2 average pooling of a (nx, ny, nz, c) field and its adjoint (nx, ny, nz even). */
+int pmt_pool2_bf16(int nx, int ny, int nz, int c, const void* X, void* Y, void* stream);
+int pmt_pool2_bwd_bf16(int nx, int ny, int nz, int c, const void* dY, void* dX, void* stream);
+/* Y = nearest-upsample(X) + S on the fine (nx, ny, nz) grid; adjoint to X sums the 8 children. */
+int pmt_up2_add_bf16(int nx, int ny, int nz, int c, const void* X, const void* S, void* Y, void* stream);
+int pmt_up2_bwd_bf16(int nx, int ny, int nz, int c, const void* dY, void* dX, void* stream);
+/* Nodal displacement u (FP32, dof = node*3 + d) = mean of channels 0..2 over the elements around each
+ * node; and its adjoint into a (nel, c) BF16 gradient (channels >= 3 zeroed). */
+int pmt_elem_to_node(int nelx, int nely, int nelz, int c, const void* Xe, float* u, void* stream);
+int pmt_elem_to_node_bwd(int nelx, int nely, int nelz, int c, const float* du, void* dXe, void* stream);
+/* Adam on FP32 master weights (t = 1-based step), refreshing the BF16 copy wb (may be NULL). */
+int pmt_adam_step(long long n, float lr, float b1, float b2, float eps, int t, const float* grad, float* w, float* m,
+                  float* v, void* wb, void* stream);
+int pmt_f32_to_bf16(long long n, const float* x, void* y, void* stream);
+int pmt_bf16_to_f32(long long n, const void* x, float* y, void* stream);
 
 #ifdef __cplusplus
 }

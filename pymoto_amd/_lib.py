@@ -23,6 +23,8 @@ _SIGS = {
     "pmt_filter_bwd": (c_int, [c_int] * 6 + [c_vp] * 5 + [c_vp]),
     "pmt_powlaw_fwd": (c_int, [c_ll, c_dbl, c_dbl, c_dbl, c_vp, c_vp, c_vp]),
     "pmt_powlaw_bwd": (c_int, [c_ll, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_int, c_vp]),
+    "pmt_heaviside_fwd": (c_int, [c_ll, c_dbl, c_dbl, c_vp, c_vp, c_vp]),
+    "pmt_heaviside_bwd": (c_int, [c_ll, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_int, c_vp, c_vp]),
     "pmt_axpby": (c_int, [c_ll, c_dbl, c_vp, c_dbl, c_vp, c_vp]),
     "pmt_dot": (c_int, [c_ll, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "pmt_reduce_work_size": (c_int, []),
@@ -53,6 +55,22 @@ _SIGS = {
     "pmt_pcg_solve": (c_int, [c_vp, c_vp, c_vp, c_int, c_dbl, c_int, c_int, c_vp, c_vp, c_vp, c_vp]),
     "pmt_block_pcg_solve": (c_int, [c_vp, c_int, c_vp, c_vp, c_int, c_dbl, c_int, c_int, c_int, c_vp, c_vp, c_vp,
                                     c_vp]),
+    # learned warm start
+    "pmt_conv3d_fwd": (c_int, [c_int] * 5 + [c_vp, c_vp, c_vp, c_int, c_vp, c_vp]),
+    "pmt_conv3d_bwd_data": (c_int, [c_int] * 5 + [c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "pmt_conv3d_wgrad_work_size": (c_int, [c_int, c_int]),
+    "pmt_conv3d_bwd_weight": (c_int, [c_int] * 5 + [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "pmt_relu_mask_bf16": (c_int, [c_ll, c_vp, c_vp, c_vp]),
+    "pmt_pool2_bf16": (c_int, [c_int] * 4 + [c_vp, c_vp, c_vp]),
+    "pmt_pool2_bwd_bf16": (c_int, [c_int] * 4 + [c_vp, c_vp, c_vp]),
+    "pmt_up2_add_bf16": (c_int, [c_int] * 4 + [c_vp, c_vp, c_vp, c_vp]),
+    "pmt_up2_bwd_bf16": (c_int, [c_int] * 4 + [c_vp, c_vp, c_vp]),
+    "pmt_elem_to_node": (c_int, [c_int] * 4 + [c_vp, c_vp, c_vp]),
+    "pmt_elem_to_node_bwd": (c_int, [c_int] * 4 + [c_vp, c_vp, c_vp]),
+    "pmt_adam_step": (c_int, [c_ll, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, c_int, c_vp, c_vp,
+                              c_vp, c_vp, c_vp, c_vp]),
+    "pmt_f32_to_bf16": (c_int, [c_ll, c_vp, c_vp, c_vp]),
+    "pmt_bf16_to_f32": (c_int, [c_ll, c_vp, c_vp, c_vp]),
 }
 
 

@@ -201,14 +201,17 @@ fine_node_kernel(GridDims g, int tiles_x, int tiles_y, int ntiles, const double*
 // to the free ring slots after the multiply, so one barrier per plane separates producer and
 // consumers and global latency hides behind the 576 multiply-adds per node.  The halo is loaded
 // once per plane instead of once per 3D tile.
-struct Stream {
-  static constexpr int SX = 32, SY = 8;                      // nodes per plane of a block (256 threads)
+// Column shapes (SX x SY nodes per plane, 256 threads) selectable at run time (autotuner).
+template <int SX_, int SY_>
+struct StreamTile {
+  static constexpr int SX = SX_, SY = SY_;                      // nodes per plane of a block
   static constexpr int HX = SX + 2, HY = SY + 2, HP = HX * HY;  // displacement plane with halo
   static constexpr int EX = SX + 1, EY = SY + 1, EP = EX * EY;  // density plane with halo
   static constexpr int UL = (HP + 255) / 256, XL = (EP + 255) / 256;  // slots per thread
+  static_assert(SX * SY == 256, "one thread per node of a plane");
 };
 
-template <int MODE, bool CLS, typename T>
+template <class Stream, int MODE, bool CLS, typename T>
 __global__ void __launch_bounds__(256)
 fine_stream_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, const double* __restrict__ K0,
                    const T* __restrict__ xe, const uint8_t* __restrict__ bcm, T bcdiag, const T* __restrict__ u,
@@ -612,6 +615,7 @@ static void launch_fine_t(int mode, const GridDims& g, const double* K0, const T
 // Launch-shape tuning of the fine operator (set by pmt_set_tuning; read at every launch).
 static int g_fine_kernel = 1;  // 0 = 3D tiles, 1 = z-streamed columns
 static int g_stream_nzc = 0;   // node planes per streamed block (0 = chosen from the CU count)
+static int g_stream_tile = 0;  // streamed column shape: 0 = 32x8, 1 = 64x4, 2 = 16x16 nodes
 
 static int g_tuning_epoch = 0;  // bumped by every change: captured launches must be re-captured
 
@@ -628,6 +632,11 @@ int set_tuning(int param, int value) {
     ++g_tuning_epoch;
     return 0;
   }
+  if (param == PMT_TUNE_STREAM_TILE && value >= 0 && value <= 2) {
+    g_stream_tile = value;
+    ++g_tuning_epoch;
+    return 0;
+  }
   return -1;
 }
 
@@ -636,6 +645,7 @@ int tuned_stream_planes() { return g_stream_nzc; }
 int get_tuning(int param) {
   if (param == PMT_TUNE_FINE_KERNEL) return g_fine_kernel;
   if (param == PMT_TUNE_STREAM_PLANES) return g_stream_nzc;
+  if (param == PMT_TUNE_STREAM_TILE) return g_stream_tile;
   return -1;
 }
 
@@ -669,9 +679,9 @@ static int stream_planes(int ntiles_xy, int nz) {
   return best;
 }
 
-template <bool CLS, typename T>
-static void launch_stream_t(int mode, const GridDims& g, const double* K0, const T* xe, const uint8_t* bcm, T bcdiag,
-                            const T* u, const T* b, const T* dinv, T omega, T* y, hipStream_t s) {
+template <class Stream, bool CLS, typename T>
+static void launch_stream_shape(int mode, const GridDims& g, const double* K0, const T* xe, const uint8_t* bcm,
+                                T bcdiag, const T* u, const T* b, const T* dinv, T omega, T* y, hipStream_t s) {
   const int tx = (g.nx + Stream::SX - 1) / Stream::SX;
   const int ty = (g.ny + Stream::SY - 1) / Stream::SY;
   const int nxy = tx * ty;
@@ -679,14 +689,25 @@ static void launch_stream_t(int mode, const GridDims& g, const double* K0, const
   const int nwork = nxy * ((g.nz + nzc - 1) / nzc);
   dim3 grid((unsigned)((nwork + 7) & ~7));
   if (mode == MODE_APPLY)
-    fine_stream_kernel<MODE_APPLY, CLS, T><<<grid, 256, 0, s>>>(g, tx, nxy, nzc, nwork, K0, xe, bcm, bcdiag, u, b,
-                                                                dinv, omega, y);
+    fine_stream_kernel<Stream, MODE_APPLY, CLS, T><<<grid, 256, 0, s>>>(g, tx, nxy, nzc, nwork, K0, xe, bcm, bcdiag,
+                                                                        u, b, dinv, omega, y);
   else if (mode == MODE_RESIDUAL)
-    fine_stream_kernel<MODE_RESIDUAL, CLS, T><<<grid, 256, 0, s>>>(g, tx, nxy, nzc, nwork, K0, xe, bcm, bcdiag, u, b,
-                                                                   dinv, omega, y);
+    fine_stream_kernel<Stream, MODE_RESIDUAL, CLS, T><<<grid, 256, 0, s>>>(g, tx, nxy, nzc, nwork, K0, xe, bcm,
+                                                                           bcdiag, u, b, dinv, omega, y);
   else
-    fine_stream_kernel<MODE_JACOBI, CLS, T><<<grid, 256, 0, s>>>(g, tx, nxy, nzc, nwork, K0, xe, bcm, bcdiag, u, b,
-                                                                 dinv, omega, y);
+    fine_stream_kernel<Stream, MODE_JACOBI, CLS, T><<<grid, 256, 0, s>>>(g, tx, nxy, nzc, nwork, K0, xe, bcm, bcdiag,
+                                                                         u, b, dinv, omega, y);
+}
+
+template <bool CLS, typename T>
+static void launch_stream_t(int mode, const GridDims& g, const double* K0, const T* xe, const uint8_t* bcm, T bcdiag,
+                            const T* u, const T* b, const T* dinv, T omega, T* y, hipStream_t s) {
+  if (g_stream_tile == 1)
+    launch_stream_shape<StreamTile<64, 4>, CLS, T>(mode, g, K0, xe, bcm, bcdiag, u, b, dinv, omega, y, s);
+  else if (g_stream_tile == 2)
+    launch_stream_shape<StreamTile<16, 16>, CLS, T>(mode, g, K0, xe, bcm, bcdiag, u, b, dinv, omega, y, s);
+  else
+    launch_stream_shape<StreamTile<32, 8>, CLS, T>(mode, g, K0, xe, bcm, bcdiag, u, b, dinv, omega, y, s);
 }
 
 template <typename T>

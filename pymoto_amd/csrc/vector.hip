@@ -29,6 +29,34 @@ __global__ void powlaw_fwd_kernel(long long n, double a, double b, double p, con
   if (i < n) y[i] = a + b * pow(x[i], p);
 }
 
+// Heaviside projection (ex_compliance_robust.py:139, a MathExpression of the reference):
+//   y = (tanh(beta*eta) + tanh(beta*(x - eta))) / (tanh(beta*eta) + tanh(beta*(1 - eta)))
+// dx = dy * beta * (1 - tanh^2(beta*(x - eta))) / den; optional dbeta_i = dy_i * dy_i/dbeta.
+__global__ void heaviside_fwd_kernel(long long n, double beta, double eta, const double* __restrict__ x,
+                                     double* __restrict__ y) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double te = tanh(beta * eta);
+  const double den = te + tanh(beta * (1.0 - eta));
+  y[i] = (te + tanh(beta * (x[i] - eta))) / den;
+}
+
+__global__ void heaviside_bwd_kernel(long long n, double beta, double eta, const double* __restrict__ x,
+                                     const double* __restrict__ dy, double* __restrict__ dx, int accumulate,
+                                     double* __restrict__ dbeta) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double te = tanh(beta * eta), t1 = tanh(beta * (1.0 - eta)), tx = tanh(beta * (x[i] - eta));
+  const double den = te + t1, num = te + tx;
+  const double g = dy[i] * beta * (1.0 - tx * tx) / den;
+  dx[i] = accumulate ? dx[i] + g : g;
+  if (dbeta) {
+    const double dnum = eta * (1.0 - te * te) + (x[i] - eta) * (1.0 - tx * tx);
+    const double dden = eta * (1.0 - te * te) + (1.0 - eta) * (1.0 - t1 * t1);
+    dbeta[i] = dy[i] * (dnum / den - num * dden / (den * den));
+  }
+}
+
 __global__ void powlaw_bwd_kernel(long long n, double c, double q, const double* __restrict__ x,
                                   const double* __restrict__ dy, double* __restrict__ dx, int accumulate) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -239,6 +267,22 @@ int pmt_powlaw_bwd(long long n, double c, double q, const double* x, const doubl
   if (n == 0) return PMT_OK;
   powlaw_bwd_kernel<<<blocks_for(n, 256), 256, 0, (hipStream_t)stream>>>(n, c, q, x, dy, dx, accumulate);
   return check_launch("powlaw_bwd_kernel");
+}
+
+int pmt_heaviside_fwd(long long n, double beta, double eta, const double* x, double* y, void* stream) {
+  if (n < 0 || (n > 0 && (!x || !y))) return arg_error("pmt_heaviside_fwd: bad argument");
+  if (n == 0) return PMT_OK;
+  heaviside_fwd_kernel<<<blocks_for(n, 256), 256, 0, (hipStream_t)stream>>>(n, beta, eta, x, y);
+  return check_launch("heaviside_fwd_kernel");
+}
+
+int pmt_heaviside_bwd(long long n, double beta, double eta, const double* x, const double* dy, double* dx,
+                      int accumulate, double* dbeta, void* stream) {
+  if (n < 0 || (n > 0 && (!x || !dy || !dx))) return arg_error("pmt_heaviside_bwd: bad argument");
+  if (n == 0) return PMT_OK;
+  heaviside_bwd_kernel<<<blocks_for(n, 256), 256, 0, (hipStream_t)stream>>>(n, beta, eta, x, dy, dx, accumulate,
+                                                                           dbeta);
+  return check_launch("heaviside_bwd_kernel");
 }
 
 int pmt_axpby(long long n, double a, const double* x, double b, double* y, void* stream) {

@@ -3,7 +3,8 @@
 ``MathExpression`` (``pymoto/modules/generic.py:14-140``) parses the expression with sympy exactly
 as the reference does (``^`` -> ``**``, lower case, ``inp0`` for the first input) and
 differentiates it symbolically.  The device path covers the power-law family a + b*inp0**p that
-SIMP uses (``ex_compliance.py:89``: ``f"{xmin} + {1-xmin}*inp0^3"``); its value and derivative
+SIMP uses (``ex_compliance.py:89``: ``f"{xmin} + {1-xmin}*inp0^3"``) and the Heaviside projection
+of the robust formulation (``ex_compliance_robust.py:139``, beta constant or a second input); the
 constants are taken from the same printed form that ``sympy.lambdify`` hands to numpy, and one HIP
 kernel evaluates each.  Other expressions raise ``NotImplementedError``.
 
@@ -52,34 +53,115 @@ def _parse_powlaw(expression: str, names=()):
     return coeffs
 
 
+def _heaviside(x, beta, eta):
+    return (np.tanh(beta * eta) + np.tanh(beta * (x - eta))) / (np.tanh(beta * eta) + np.tanh(beta * (1 - eta)))
+
+
+def _parse_heaviside(expression: str, names=()):
+    """(beta or None, eta) if the expression is the Heaviside projection of the robust formulation
+    (ex_compliance_robust.py:139) in inp0 with a constant beta or beta = inp1."""
+    from sympy import Symbol, tanh
+    from sympy.parsing.sympy_parser import parse_expr
+    expr = parse_expr(expression.replace("^", "**").lower())
+    x, bsym = Symbol("inp0"), Symbol("inp1")
+    repl = {Symbol(n.lower()): s for n, s in zip(names, (x, bsym)) if n}
+    expr = expr.subs(repl)
+    if not expr.free_symbols or expr.free_symbols - {x, bsym}:
+        return None
+    args = [t.args[0] for t in expr.atoms(tanh)]
+    with_x = [a for a in args if x in a.free_symbols]
+    if not with_x:
+        return None
+    slope = with_x[0].diff(x)
+    beta = None if bsym in expr.free_symbols else slope
+    if beta is not None and not beta.is_number:
+        return None
+    b_eval = bsym if beta is None else beta
+    eta = -(with_x[0] - slope * x) / b_eval
+    eta = eta.simplify()
+    if not eta.is_number:
+        return None
+    beta_v = None if beta is None else _printed_float(beta)
+    eta_v = _printed_float(eta)
+    for xv in (0.05, 0.37, 0.5, 0.93):
+        for bv in ((1.5, 8.0) if beta is None else (beta_v,)):
+            ref = float(expr.subs({x: xv, bsym: bv}))
+            if not np.isclose(ref, _heaviside(xv, bv, eta_v), rtol=1e-12, atol=1e-14):
+                return None
+    return beta_v, eta_v
+
+
 class MathExpression(Module):
-    """y = f(inp0) for the power-law family a + b*inp0**p (SIMP); derivative by sympy."""
+    """y = f(inp0[, inp1]) for the families the density chain uses, derivative by sympy:
+
+    * power law a + b*inp0**p (SIMP, ``ex_compliance.py:89``);
+    * Heaviside projection (tanh(beta*eta) + tanh(beta*(inp0-eta))) / (tanh(beta*eta) + tanh(beta*(1-eta)))
+      with a constant beta or beta = inp1 (``ex_compliance_robust.py:139``).
+    """
 
     def __init__(self, expression: str):
         self.expression = expression
         self.coeffs = None
+        self.heaviside = None
+
+    def _parse(self, nargs):
+        names = [getattr(s, "tag", "") for s in (self.sig_in or [])]
+        hv = _parse_heaviside(self.expression, names)
+        if hv is not None:
+            if (hv[0] is None) != (nargs == 2):
+                raise NotImplementedError("Heaviside MathExpression: beta must be inp1 (2 inputs) or a constant")
+            self.heaviside = hv
+            return
+        if nargs != 1:
+            raise NotImplementedError("device MathExpression takes one input except for the Heaviside projection")
+        self.coeffs = _parse_powlaw(self.expression, names)
+
+    def _beta(self):
+        b = self.heaviside[0]
+        return float(np.asarray(self.sig_in[1].state if b is None else b, dtype=np.float64).reshape(()))
 
     def __call__(self, *args):
-        if len(args) != 1:
-            raise NotImplementedError("device MathExpression takes exactly one input")
-        if self.coeffs is None:
-            names = [getattr(s, "tag", "") for s in (self.sig_in or [])]
-            self.coeffs = _parse_powlaw(self.expression, names)
-        a, b, p, _, _ = self.coeffs
+        if len(args) not in (1, 2):
+            raise NotImplementedError("device MathExpression takes one or two inputs")
+        if self.coeffs is None and self.heaviside is None:
+            self._parse(len(args))
         x = _lib.as_device(args[0])
         y = torch.empty_like(x)
+        if self.heaviside is not None:
+            beta = float(np.asarray(args[1] if self.heaviside[0] is None else self.heaviside[0],
+                                    dtype=np.float64).reshape(()))
+            _lib.check(_lib.lib().pmt_heaviside_fwd(x.numel(), beta, self.heaviside[1], _lib.ptr(x), _lib.ptr(y),
+                                                    _lib.stream()), "heaviside")
+            return y
+        a, b, p, _, _ = self.coeffs
         _lib.check(_lib.lib().pmt_powlaw_fwd(x.numel(), a, b, p, _lib.ptr(x), _lib.ptr(y), _lib.stream()),
                    "power law")
         return y
 
     def _sensitivity(self, df_dy):
-        _, _, _, c, q = self.coeffs
         x = _lib.as_device(self.sig_in[0].state)
         dy = _lib.as_device(df_dy)
         dx = torch.empty_like(x)
+        as_np = isinstance(self.sig_in[0].state, np.ndarray)
+        if self.heaviside is not None:
+            beta, eta = self._beta(), self.heaviside[1]
+            two = self.heaviside[0] is None
+            db = torch.empty_like(x) if two else None
+            lib = _lib.lib()
+            _lib.check(lib.pmt_heaviside_bwd(x.numel(), beta, eta, _lib.ptr(x), _lib.ptr(dy), _lib.ptr(dx), 0,
+                                             _lib.ptr(db), _lib.stream()), "heaviside adjoint")
+            dxo = dx.cpu().numpy() if as_np else dx
+            if not two:
+                return dxo
+            ones = torch.ones_like(db)
+            out = torch.empty(1, dtype=torch.float64, device=db.device)
+            _lib.check(lib.pmt_dot(db.numel(), _lib.ptr(db), _lib.ptr(ones), _lib.ptr(out),
+                                   _lib.ptr(_lib.reduce_work()), _lib.stream()), "dot")
+            return [dxo, float(out.item())]
+        _, _, _, c, q = self.coeffs
         _lib.check(_lib.lib().pmt_powlaw_bwd(x.numel(), c, q, _lib.ptr(x), _lib.ptr(dy), _lib.ptr(dx), 0,
                                              _lib.stream()), "power law adjoint")
-        return dx.cpu().numpy() if isinstance(self.sig_in[0].state, np.ndarray) else dx
+        return dx.cpu().numpy() if as_np else dx
 
 
 class EinSum(Module):

@@ -480,3 +480,42 @@ def test_linsolve_multi_rhs_block_vs_direct(gpu_lib):
     from scipy.sparse.linalg import spsolve
     ref = spsolve(Kref.tocsc(), B)
     assert relerr(U, ref) < 1e-8
+
+
+def test_autotuner_picks_measured_shape_and_keeps_bits(gpu_lib):
+    from pymoto_amd import tune
+    nel = (40, 12, 10)
+    d = pm.VoxelDomain(*nel)
+    rng = np.random.default_rng(2)
+    K = pm.AssembleStiffness(d, bc=problems.cantilever3d(*nel)[0])(dev(rng.uniform(0.01, 1.0, d.nel)))
+    u = dev(rng.standard_normal(3 * d.nnodes))
+    before = host(K @ u)
+    try:
+        best, table = tune.autotune_fine_operator(K, reps=2)
+        assert tuple(sorted(best.items())) in table and len(table) > 3
+        assert tune.get_tuning() == best
+        assert host(K @ u).tobytes() == before.tobytes()  # every shape sums in the same order
+    finally:
+        tune.set_tuning(fine_kernel=1, stream_tile=0, stream_planes=0)
+
+
+@pytest.mark.parametrize("eta", [0.3, 0.5, 0.7])
+def test_heaviside_projection_vs_closed_form(gpu_lib, eta):
+    """Heaviside MathExpression (beta = second input) against its closed form; d/dx and d/dbeta against
+    central differences of the closed form."""
+    h = "(tanh(inp1 * {0}) + tanh(inp1 * (inp0 - {0}))) / (tanh(inp1 * {0}) + tanh(inp1 * (1 - {0})))"
+    rng = np.random.default_rng(int(eta * 10))
+    x = rng.uniform(0.0, 1.0, 997)
+    beta = 8.0
+    f = lambda xv, b: (np.tanh(b * eta) + np.tanh(b * (xv - eta))) / (np.tanh(b * eta) + np.tanh(b * (1 - eta)))  # noqa
+    sx, sb = pm.Signal("x", dev(x)), pm.Signal("beta", beta)
+    m = pm.MathExpression(h.format(eta))
+    sy = m(sx, sb)
+    npt.assert_allclose(host(sy.state), f(x, beta), rtol=1e-14, atol=1e-15)
+    dy = rng.standard_normal(997)
+    sy.sensitivity = dev(dy)
+    m.sensitivity()
+    e = 1e-6
+    npt.assert_allclose(host(sx.sensitivity), dy * (f(x + e, beta) - f(x - e, beta)) / (2 * e), rtol=1e-6, atol=1e-8)
+    dbeta = np.sum(dy * (f(x, beta + e) - f(x, beta - e)) / (2 * e))
+    npt.assert_allclose(float(sb.sensitivity), dbeta, rtol=1e-6)

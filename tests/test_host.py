@@ -178,3 +178,14 @@ def test_csr_closed_form_indptr_matches_oracle(nel):
     npt.assert_array_equal(_kernel_indptr(*nel, dim), asm.indptr)
     from pymoto_amd import _lib
     assert _lib.load().pmt_csr_nnz(*nel, dim) == asm.indices.size
+
+
+def test_heaviside_expression_parsing():
+    """The robust formulation's projection (ex_compliance_robust.py:139) is recognised with beta as a
+    second input or a constant; SIMP is not mistaken for it."""
+    from pymoto_amd.generic import _parse_heaviside
+    h = "(tanh(inp1 * {0}) + tanh(inp1 * (inp0 - {0}))) / (tanh(inp1 * {0}) + tanh(inp1 * (1 - {0})))"
+    assert _parse_heaviside(h.format(0.3)) == (None, 0.3)
+    assert _parse_heaviside(h.format(0.7).replace("inp1", "8")) == (8.0, 0.7)
+    assert _parse_heaviside(f"{1e-9} + {1 - 1e-9}*inp0^3") is None
+    assert _parse_heaviside("tanh(inp0)") is None
