@@ -64,6 +64,12 @@ int pmt_heaviside_fwd(long long n, double beta, double eta, const double* x, dou
 int pmt_heaviside_bwd(long long n, double beta, double eta, const double* x, const double* dy, double* dx,
                       int accumulate, double* dbeta, void* stream);
 
+/* One bisection step of the OC update (pymoto/common/optimizers.py:404-433): xnew = clip(x *
+ * sqrt(-min(dg,0)/lmid), max(xmin, x-move), min(xmax, x+move)) and *sum_out = sum(xnew) (device
+ * scalar, deterministic; work: pmt_reduce_work_size() doubles). */
+int pmt_oc_update(long long n, const double* x, const double* dg, double xmin, double xmax, double move,
+                  double lmid, double* xnew, double* sum_out, double* work, void* stream);
+
 /* Vector kernels.  y = a*x + b*y (x may be NULL: y = b*y). */
 int pmt_axpby(long long n, double a, const double* x, double b, double* y, void* stream);
 /* *out = x.y, deterministic two-stage reduction; work holds pmt_reduce_work_size() doubles.
@@ -197,7 +203,7 @@ int pmt_block_pcg_solve(pmt_mg* mg, int k, const double* B, double* X, int x0_is
  * the element grid.  Activations are channels-last BF16 arrays X[voxel][c] (voxel in pyMOTO's
  * element order, c contiguous), weights BF16 W[tap][cout][cin] with tap = (dz+1)*9 + (dy+1)*3 +
  * (dx+1), 3x3x3 stencil with zero padding; products on the matrix cores with FP32 accumulation.
- * BF16 buffers are passed as void* (2 bytes per value).  nx must be a multiple of 16.
+ * BF16 buffers are passed as void* (2 bytes per value).  The voxel count must be a multiple of 16.
  */
 /* Y = act(bias + conv(X, W)); cin in {8,16,32,64}, cout in {16,32,64}; bias (FP32) may be NULL;
  * relu = 1 applies max(0, .). */
@@ -227,6 +233,7 @@ int pmt_elem_to_node_bwd(int nelx, int nely, int nelz, int c, const float* du, v
 /* Adam on FP32 master weights (t = 1-based step), refreshing the BF16 copy wb (may be NULL). */
 int pmt_adam_step(long long n, float lr, float b1, float b2, float eps, int t, const float* grad, float* w, float* m,
                   float* v, void* wb, void* stream);
+int pmt_add_bf16(long long n, const void* a, const void* b, void* y, void* stream);
 int pmt_f32_to_bf16(long long n, const float* x, void* y, void* stream);
 int pmt_bf16_to_f32(long long n, const void* x, float* y, void* stream);
 

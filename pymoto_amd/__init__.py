@@ -16,6 +16,7 @@ from .assembly import AssembleGeneral, AssembleStiffness, StiffnessMatrix
 from .filter import FilterConv
 from .generic import EinSum, MathExpression
 from .linalg import LinSolve
+from .optim import OC
 
 __all__ = ["Signal", "SignalSlice", "Module", "Network", "make_signals", "VoxelDomain", "DyadicMatrix", "solvers",
            "AssembleGeneral", "AssembleStiffness", "StiffnessMatrix", "FilterConv", "MathExpression", "EinSum",

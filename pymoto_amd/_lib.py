@@ -25,6 +25,7 @@ _SIGS = {
     "pmt_powlaw_bwd": (c_int, [c_ll, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_int, c_vp]),
     "pmt_heaviside_fwd": (c_int, [c_ll, c_dbl, c_dbl, c_vp, c_vp, c_vp]),
     "pmt_heaviside_bwd": (c_int, [c_ll, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_int, c_vp, c_vp]),
+    "pmt_oc_update": (c_int, [c_ll, c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp]),
     "pmt_axpby": (c_int, [c_ll, c_dbl, c_vp, c_dbl, c_vp, c_vp]),
     "pmt_dot": (c_int, [c_ll, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "pmt_reduce_work_size": (c_int, []),
@@ -69,6 +70,7 @@ _SIGS = {
     "pmt_elem_to_node_bwd": (c_int, [c_int] * 4 + [c_vp, c_vp, c_vp]),
     "pmt_adam_step": (c_int, [c_ll, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, c_int, c_vp, c_vp,
                               c_vp, c_vp, c_vp, c_vp]),
+    "pmt_add_bf16": (c_int, [c_ll, c_vp, c_vp, c_vp, c_vp]),
     "pmt_f32_to_bf16": (c_int, [c_ll, c_vp, c_vp, c_vp]),
     "pmt_bf16_to_f32": (c_int, [c_ll, c_vp, c_vp, c_vp]),
 }

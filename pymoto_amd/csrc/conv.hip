@@ -7,7 +7,7 @@
 // tap = (dz+1)*9 + (dy+1)*3 + (dx+1); zero padding outside the grid.
 //
 // Convolution as an implicit GEMM on the matrix cores (v_mfma_f32_16x16x32_bf16): a wave owns 16
-// consecutive voxels of one x row (nx is a multiple of 16) and all output channels; the reduction
+// consecutive voxels (the voxel count is a multiple of 16) and all output channels; the reduction
 // runs over k = tap*cin + ci in chunks of 32, each lane loading 8 consecutive channels of one
 // neighbour voxel (16 bytes) for the A operand and 8 consecutive input channels of one output
 // channel for the B operand.  The backward pass to the input is the same kernel on the flipped,
@@ -52,10 +52,10 @@ __global__ void __launch_bounds__(256) conv3d_fwd_kernel(Vox g, const unsigned s
   const long long seg = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
   const long long v0 = seg * 16;
   if (v0 >= g.n) return;
-  const int x0 = (int)(v0 % g.nx);
-  const long long t = v0 / g.nx;
+  const long long va = v0 + (lane & 15);  // A-operand row (voxel) of this lane
+  const int xa = (int)(va % g.nx);
+  const long long t = va / g.nx;
   const int y = (int)(t % g.ny), z = (int)(t / g.ny);
-  const int xa = x0 + (lane & 15);  // A-operand row of this lane
   const int kq = lane >> 4;
   f4 acc[NT];
 #pragma unroll
@@ -318,6 +318,12 @@ __global__ void adam_kernel(long long n, float lr, float b1, float b2, float eps
   if (wb) wb[i] = f2bf(wi);
 }
 
+__global__ void add_bf16_kernel(long long n, const unsigned short* __restrict__ a, const unsigned short* __restrict__ b,
+                                unsigned short* __restrict__ y) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = f2bf(bf2f(a[i]) + bf2f(b[i]));
+}
+
 __global__ void f32_to_bf16_kernel(long long n, const float* __restrict__ x, unsigned short* __restrict__ y) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = f2bf(x[i]);
@@ -342,7 +348,7 @@ static Vox vox(int nx, int ny, int nz) {
 
 static bool conv_shape_ok(int nx, int ny, int nz, int cin, int cout) {
   const bool c_ok = (cin == 8 || cin == 16 || cin == 32 || cin == 64) && (cout == 16 || cout == 32 || cout == 64);
-  return c_ok && nx > 0 && ny > 0 && nz > 0 && nx % 16 == 0;
+  return c_ok && nx > 0 && ny > 0 && nz > 0 && ((long long)nx * ny * nz) % 16 == 0;
 }
 
 #define PMT_CONV_CASES(F) \
@@ -374,7 +380,7 @@ extern "C" {
 int pmt_conv3d_fwd(int nx, int ny, int nz, int cin, int cout, const void* X, const void* W, const float* bias,
                    int relu, void* Y, void* stream) {
   if (!conv_shape_ok(nx, ny, nz, cin, cout) || !X || !W || !Y)
-    return arg_error("pmt_conv3d_fwd: bad argument (cin in 8/16/32/64, cout in 16/32/64, nx % 16 == 0)");
+    return arg_error("pmt_conv3d_fwd: bad argument (cin in 8/16/32/64, cout in 16/32/64, voxels % 16 == 0)");
   return conv_fwd(vox(nx, ny, nz), cin, cout, (const unsigned short*)X, (const unsigned short*)W, bias, relu,
                   (unsigned short*)Y, (hipStream_t)stream);
 }
@@ -382,7 +388,7 @@ int pmt_conv3d_fwd(int nx, int ny, int nz, int cin, int cout, const void* X, con
 int pmt_conv3d_bwd_data(int nx, int ny, int nz, int cin, int cout, const void* dY, const void* W, void* Wt, void* dX,
                         void* stream) {
   if (!conv_shape_ok(nx, ny, nz, cout, cin) || cin < 16 || !dY || !W || !Wt || !dX)
-    return arg_error("pmt_conv3d_bwd_data: bad argument (cin in 16/32/64, cout in 16/32/64, nx % 16 == 0)");
+    return arg_error("pmt_conv3d_bwd_data: bad argument (cin in 16/32/64, cout in 16/32/64, voxels % 16 == 0)");
   hipStream_t s = (hipStream_t)stream;
   const int n = 27 * cin * cout;
   conv3d_flip_kernel<<<blocks_for(n, 256), 256, 0, s>>>(cin, cout, (const unsigned short*)W, (unsigned short*)Wt);
@@ -486,6 +492,14 @@ int pmt_adam_step(long long n, float lr, float b1, float b2, float eps, int t, c
   adam_kernel<<<blocks_for(n, 256), 256, 0, (hipStream_t)stream>>>(n, lr, b1, b2, eps, t, grad, w, m, v,
                                                                   (unsigned short*)wb);
   return check_launch("adam_kernel");
+}
+
+int pmt_add_bf16(long long n, const void* a, const void* b, void* y, void* stream) {
+  if (n < 0 || (n > 0 && (!a || !b || !y))) return arg_error("pmt_add_bf16: bad argument");
+  if (n == 0) return PMT_OK;
+  add_bf16_kernel<<<blocks_for(n, 256), 256, 0, (hipStream_t)stream>>>(n, (const unsigned short*)a,
+                                                                       (const unsigned short*)b, (unsigned short*)y);
+  return check_launch("add_bf16_kernel");
 }
 
 int pmt_f32_to_bf16(long long n, const float* x, void* y, void* stream) {

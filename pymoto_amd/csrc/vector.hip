@@ -57,6 +57,13 @@ __global__ void heaviside_bwd_kernel(long long n, double beta, double eta, const
   }
 }
 
+// OC update for one bisection value lmid (pymoto/common/optimizers.py:420-433):
+//   xnew = clip(x * sqrt(-min(dg, 0) / lmid), max(xmin, x - move), min(xmax, x + move))
+// and per-block partial sums of xnew (for the volume test).
+__global__ void oc_update_kernel(long long n, const double* __restrict__ x, const double* __restrict__ dg,
+                                 double xmin, double xmax, double move, double lmid, double* __restrict__ xnew,
+                                 double* __restrict__ partial);
+
 __global__ void powlaw_bwd_kernel(long long n, double c, double q, const double* __restrict__ x,
                                   const double* __restrict__ dy, double* __restrict__ dx, int accumulate) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -112,6 +119,21 @@ __global__ void __launch_bounds__(kReduceThreads) dot_stage2(const double* __res
     double r = block_sum(s);
     if (threadIdx.x == 0) out[o] = r;
   }
+}
+
+__global__ void oc_update_kernel(long long n, const double* __restrict__ x, const double* __restrict__ dg,
+                                 double xmin, double xmax, double move, double lmid, double* __restrict__ xnew,
+                                 double* __restrict__ partial) {
+  double s = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double xi = x[i];
+    const double lb = fmax(xmin, xi - move), ub = fmin(xmax, xi + move);
+    const double v = fmin(fmax(xi * sqrt(-fmin(dg[i], 0.0) / lmid), lb), ub);
+    xnew[i] = v;
+    s += v;
+  }
+  const double t = block_sum(s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
 int launch_dot(long long n, const double* x, const double* y, double* out, double* work, hipStream_t s) {
@@ -283,6 +305,17 @@ int pmt_heaviside_bwd(long long n, double beta, double eta, const double* x, con
   heaviside_bwd_kernel<<<blocks_for(n, 256), 256, 0, (hipStream_t)stream>>>(n, beta, eta, x, dy, dx, accumulate,
                                                                            dbeta);
   return check_launch("heaviside_bwd_kernel");
+}
+
+int pmt_oc_update(long long n, const double* x, const double* dg, double xmin, double xmax, double move,
+                  double lmid, double* xnew, double* sum_out, double* work, void* stream) {
+  if (n <= 0 || !x || !dg || !xnew || !sum_out || !work || !(lmid > 0.0))
+    return arg_error("pmt_oc_update: bad argument");
+  hipStream_t s = (hipStream_t)stream;
+  oc_update_kernel<<<kReduceBlocks, kReduceThreads, 0, s>>>(n, x, dg, xmin, xmax, move, lmid, xnew, work);
+  PMT_TRY(check_launch("oc_update_kernel"));
+  dot_stage2<1><<<1, kReduceThreads, 0, s>>>(work, sum_out);
+  return check_launch("dot_stage2");
 }
 
 int pmt_axpby(long long n, double a, const double* x, double b, double* y, void* stream) {

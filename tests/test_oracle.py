@@ -271,3 +271,17 @@ def test_pipeline_c2_mgcg_vs_reference(levels):
     assert out["adjoint_solved"] == bool(g["adjoint_solved"]) is False
     npt.assert_allclose(out["c"], float(g["c"]), rtol=1e-11)
     npt.assert_allclose(out["dcdx"], g["dcdx"], rtol=1e-8, atol=1e-11 * np.abs(g["dcdx"]).max())
+
+
+def test_oc_step_volume_move_limits_and_uniform_case():
+    from oracle.optim import oc_step
+    rng = np.random.default_rng(0)
+    x = rng.uniform(0.2, 0.8, 500)
+    dg = -rng.uniform(0.1, 2.0, 500)
+    xn = oc_step(x, dg, maxvol=0.5, move=0.2)
+    assert np.all(xn >= np.maximum(0.0, x - 0.2) - 1e-15) and np.all(xn <= np.minimum(1.0, x + 0.2) + 1e-15)
+    assert abs(np.mean(xn) - 0.5) < 1e-3
+    # uniform sensitivity and design: the multiplier makes x*sqrt(-dg/l) = maxvol exactly
+    xu = np.full(100, 0.4)
+    xn = oc_step(xu, np.full(100, -1.0), maxvol=0.45, move=0.2)
+    npt.assert_allclose(xn, 0.45, atol=1e-4)  # bisection tolerance 1e-4 on the multiplier
