@@ -17,12 +17,12 @@
 
 namespace pmt {
 
-__device__ inline double interp_w(int f, int Abit) {
+__host__ __device__ inline double interp_w(int f, int Abit) {
   return f == 1 ? 0.5 : ((f == 0) == (Abit == 0) ? 1.0 : 0.0);
 }
 
 template <int DIM>
-__device__ inline double pweight(int c, int a, int A) {
+__host__ __device__ inline double pweight(int c, int a, int A) {
   double w = 1.0;
 #pragma unroll
   for (int b = 0; b < DIM; ++b) w *= interp_w(((c >> b) & 1) + ((a >> b) & 1), (A >> b) & 1);
@@ -31,11 +31,61 @@ __device__ inline double pweight(int c, int a, int A) {
 
 // One block per coarse element; thread t = (A, B) node pair owns the D x D block (A, B) of K_E.
 // FINE: A_c built from x_c, K0, the Dirichlet mask and bcdiag/mult; else read from Kf.
+// Does coarse element (CI, CJ, CK) contain a Dirichlet node among its 3^D fine nodes?
+template <int DIM>
+__device__ inline bool touches_bc(const GridDims& gf, const uint8_t* __restrict__ bcm, int CI, int CJ, int CK) {
+  for (int q = 0; q < (DIM == 3 ? 27 : 9); ++q) {
+    const int i = 2 * CI + q % 3, j = 2 * CJ + (q / 3) % 3, k = DIM == 3 ? 2 * CK + q / 9 : 0;
+    if (bcm[((long long)k * gf.ny + j) * gf.nx + i]) return true;
+  }
+  return false;
+}
+
+// Fine -> first coarse level away from Dirichlet nodes: with M_c = I and no bcdiag terms,
+// P_c^T (x_c K0) P_c = x_c G_c with G_c = P_c^T K0 P_c fixed, so K_E = sum_c x_c G_c
+// (576*8 multiply-adds per coarse element instead of the triple products).  One thread per
+// (coarse element, D x D node block); elements that touch a Dirichlet node are left to
+// galerkin_kernel<DIM, true> with bc_only.
+template <int DIM>
+__global__ void __launch_bounds__(256)
+galerkin_fine_fast_kernel(GridDims gf, GridDims gc, const double* __restrict__ G, const double* __restrict__ xe,
+                          const uint8_t* __restrict__ bcm, double* __restrict__ Kc) {
+  constexpr int NN = 1 << DIM;
+  constexpr int NE = NN * DIM;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= gc.nel * NN * NN) return;
+  const long long E = t / (NN * NN);
+  const int blk = (int)(t % (NN * NN)), A = blk / NN, B = blk % NN;
+  const int CI = (int)(E % gc.ex);
+  const long long tt = E / gc.ex;
+  const int CJ = (int)(tt % gc.ey), CK = (int)(tt / gc.ey);
+  if (bcm && touches_bc<DIM>(gf, bcm, CI, CJ, CK)) return;
+  double acc[DIM][DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d)
+#pragma unroll
+    for (int d2 = 0; d2 < DIM; ++d2) acc[d][d2] = 0.0;
+#pragma unroll
+  for (int c = 0; c < NN; ++c) {
+    const int fi = 2 * CI + (c & 1), fj = 2 * CJ + ((c >> 1) & 1), fk = DIM == 3 ? 2 * CK + ((c >> 2) & 1) : 0;
+    const double xc = xe[((long long)fk * gf.ey + fj) * gf.ex + fi];
+    const double* g = G + c * NE * NE;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+#pragma unroll
+      for (int d2 = 0; d2 < DIM; ++d2) acc[d][d2] = fma(xc, g[(A * DIM + d) * NE + B * DIM + d2], acc[d][d2]);
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; ++d)
+#pragma unroll
+    for (int d2 = 0; d2 < DIM; ++d2) Kc[E * NE * NE + (A * DIM + d) * NE + B * DIM + d2] = acc[d][d2];
+}
+
 template <int DIM, bool FINE>
 __global__ void __launch_bounds__(64)
 galerkin_kernel(GridDims gf, GridDims gc, const double* __restrict__ K0, const double* __restrict__ xe,
                 const uint8_t* __restrict__ bcm, double bcdiag, const double* __restrict__ Kf,
-                double* __restrict__ Kc) {
+                double* __restrict__ Kc, int bc_only) {
   constexpr int NN = 1 << DIM;
   constexpr int NE = NN * DIM;
   __shared__ double sA[NE * NE];
@@ -47,6 +97,7 @@ galerkin_kernel(GridDims gf, GridDims gc, const double* __restrict__ K0, const d
   const long long tt = E / gc.ex;
   const int CJ = (int)(tt % gc.ey);
   const int CK = (int)(tt / gc.ey);
+  if (FINE && bc_only && !(bcm && touches_bc<DIM>(gf, bcm, CI, CJ, CK))) return;  // whole block leaves
   if (FINE) {
     for (int q = tid; q < NE * NE; q += NN * NN) sK0[q] = K0[q];
   }
@@ -344,22 +395,54 @@ __global__ void to_float_kernel(long long n, const double* __restrict__ x, float
   if (i < n) y[i] = (float)x[i];
 }
 
-int launch_galerkin_fine(const GridDims& gf, const double* K0, const double* xe, const uint8_t* bcm, double bcdiag,
-                         double* Kc, hipStream_t s) {
+void galerkin_child_matrices(int dim, const double* K0, double* G) {
+  const int NN = 1 << dim, NE = NN * dim;
+  for (int c = 0; c < NN; ++c) {
+    double* g = G + (long long)c * NE * NE;
+    for (int q = 0; q < NE * NE; ++q) g[q] = 0.0;
+    // G_c[A d][B d2] = sum_{a, b} p_c[a][A] K0[a d][b d2] p_c[b][B]
+    for (int A = 0; A < NN; ++A)
+      for (int B = 0; B < NN; ++B)
+        for (int a = 0; a < NN; ++a) {
+          const double wa = dim == 3 ? pweight<3>(c, a, A) : pweight<2>(c, a, A);
+          if (wa == 0.0) continue;
+          for (int b = 0; b < NN; ++b) {
+            const double wb = dim == 3 ? pweight<3>(c, b, B) : pweight<2>(c, b, B);
+            if (wb == 0.0) continue;
+            for (int d = 0; d < dim; ++d)
+              for (int d2 = 0; d2 < dim; ++d2)
+                g[(A * dim + d) * NE + B * dim + d2] += wa * wb * K0[(a * dim + d) * NE + b * dim + d2];
+          }
+        }
+  }
+}
+
+int launch_galerkin_fine(const GridDims& gf, const double* K0, const double* G, const double* xe, const uint8_t* bcm,
+                         double bcdiag, double* Kc, hipStream_t s) {
   GridDims gc = make_grid(gf.ex / 2, gf.ey / 2, gf.dim == 3 ? gf.ez / 2 : 0);
+  const int fast = G != nullptr;
+  if (fast) {
+    const int NN = 1 << gf.dim;
+    const long long nthreads = gc.nel * NN * NN;
+    if (gf.dim == 3)
+      galerkin_fine_fast_kernel<3><<<blocks_for(nthreads, 256), 256, 0, s>>>(gf, gc, G, xe, bcm, Kc);
+    else
+      galerkin_fine_fast_kernel<2><<<blocks_for(nthreads, 256), 256, 0, s>>>(gf, gc, G, xe, bcm, Kc);
+    PMT_TRY(check_launch("galerkin_fine_fast_kernel"));
+  }
   if (gf.dim == 3)
-    galerkin_kernel<3, true><<<(unsigned)gc.nel, 64, 0, s>>>(gf, gc, K0, xe, bcm, bcdiag, nullptr, Kc);
+    galerkin_kernel<3, true><<<(unsigned)gc.nel, 64, 0, s>>>(gf, gc, K0, xe, bcm, bcdiag, nullptr, Kc, fast);
   else
-    galerkin_kernel<2, true><<<(unsigned)gc.nel, 16, 0, s>>>(gf, gc, K0, xe, bcm, bcdiag, nullptr, Kc);
+    galerkin_kernel<2, true><<<(unsigned)gc.nel, 16, 0, s>>>(gf, gc, K0, xe, bcm, bcdiag, nullptr, Kc, fast);
   return check_launch("galerkin_kernel<fine>");
 }
 
 int launch_galerkin_coarse(const GridDims& gf, const double* Kf, double* Kc, hipStream_t s) {
   GridDims gc = make_grid(gf.ex / 2, gf.ey / 2, gf.dim == 3 ? gf.ez / 2 : 0);
   if (gf.dim == 3)
-    galerkin_kernel<3, false><<<(unsigned)gc.nel, 64, 0, s>>>(gf, gc, nullptr, nullptr, nullptr, 0.0, Kf, Kc);
+    galerkin_kernel<3, false><<<(unsigned)gc.nel, 64, 0, s>>>(gf, gc, nullptr, nullptr, nullptr, 0.0, Kf, Kc, 0);
   else
-    galerkin_kernel<2, false><<<(unsigned)gc.nel, 16, 0, s>>>(gf, gc, nullptr, nullptr, nullptr, 0.0, Kf, Kc);
+    galerkin_kernel<2, false><<<(unsigned)gc.nel, 16, 0, s>>>(gf, gc, nullptr, nullptr, nullptr, 0.0, Kf, Kc, 0);
   return check_launch("galerkin_kernel<coarse>");
 }
 

@@ -50,8 +50,11 @@ int launch_fill(long long n, double v, double* y, hipStream_t s);
 int launch_to_double(long long n, const float* x, double* y, hipStream_t s);
 
 // coarse.hip
-int launch_galerkin_fine(const GridDims& gf, const double* K0, const double* xe, const uint8_t* bcm, double bcdiag,
-                         double* Kc, hipStream_t s);
+// G (optional, 2^D fixed matrices P_c^T K0 P_c from galerkin_child_matrices): fast path away from
+// Dirichlet nodes
+void galerkin_child_matrices(int dim, const double* K0_host, double* G_host);
+int launch_galerkin_fine(const GridDims& gf, const double* K0, const double* G, const double* xe, const uint8_t* bcm,
+                         double bcdiag, double* Kc, hipStream_t s);
 int launch_galerkin_coarse(const GridDims& gf, const double* Kf, double* Kc, hipStream_t s);
 int launch_stencil_from_elements(const GridDims& g, const double* Ke, double* S, hipStream_t s);
 int launch_dense_from_stencil(const GridDims& g, const double* S, double* A, hipStream_t s);

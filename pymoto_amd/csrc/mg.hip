@@ -223,6 +223,7 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
   const int NE = mg->NE;
   const int NS = mg->dim == 3 ? 243 : 36;
   ALLOC(mg->K0, (long long)NE * NE);
+  ALLOC(mg->G, (long long)(1 << mg->dim) * NE * NE);
   for (int l = 0; l < nlevels; ++l) {
     Level& L = mg->L[l];
     if (l == nlevels - 1) {
@@ -259,6 +260,11 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
                                     "hipHostMalloc");
   if (rc == PMT_OK)
     rc = hip_status(hipMemcpy(mg->K0, K0_host, (size_t)NE * NE * sizeof(double), hipMemcpyHostToDevice), "K0 upload");
+  if (rc == PMT_OK) {
+    std::vector<double> G((size_t)(1 << mg->dim) * NE * NE);
+    galerkin_child_matrices(mg->dim, K0_host, G.data());
+    rc = hip_status(hipMemcpy(mg->G, G.data(), G.size() * sizeof(double), hipMemcpyHostToDevice), "G upload");
+  }
   if (rc != PMT_OK) {
     release(mg);
     delete mg;
@@ -302,7 +308,7 @@ int pmt_mg_update(pmt_mg* mg, const double* xe, const uint8_t* bcmask, double bc
     Level& L = mg->L[l];
     Level& P = mg->L[l - 1];
     if (l == 1)
-      PMT_TRY(launch_galerkin_fine(P.g, mg->K0, xe, bcmask, bcdiag, L.Ke, s));
+      PMT_TRY(launch_galerkin_fine(P.g, mg->K0, mg->G, xe, bcmask, bcdiag, L.Ke, s));
     else
       PMT_TRY(launch_galerkin_coarse(P.g, P.Ke, L.Ke, s));
     PMT_TRY(launch_stencil_from_elements(L.g, L.Ke, L.d.S, s));

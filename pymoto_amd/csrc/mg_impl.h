@@ -49,6 +49,7 @@ struct pmt_mg {
   pmt::Level L[pmt::kMaxLevels];
   int NE = 24;
   double* K0 = nullptr;
+  double* G = nullptr;  // P_c^T K0 P_c per child position (fast fine -> level 1 Galerkin)
   int k0pat = 0;
   const double* xe = nullptr;
   float* xe32 = nullptr;
