@@ -54,17 +54,19 @@ def parse():
                    help="comma-separated precision[@omega] list; the first fp64 entry is the reference")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--no-batched", action="store_true", help="skip the batched multi-load workload")
+    p.add_argument("--learned-samples", type=int, default=24, help="training pairs for '+learned' variants")
+    p.add_argument("--learned-epochs", type=int, default=4)
     p.add_argument("--cpu-nel", type=int, nargs=3, default=[48, 24, 24])
     return p.parse_args()
 
 
 def parse_variant(v):
-    """'fp32' or 'fp32@0.5' -> (precision, omega or None)."""
-    prec, _, om = v.partition("@")
+    """'fp32', 'fp32@0.5' or 'fp32@0.5+learned' -> (precision, omega or None)."""
+    prec, _, om = v.split("+")[0].partition("@")
     return prec.strip(), (float(om) if om else None)
 
 
-def build_network(pm, torch, nel, levels, tol, precision, omega=0.6, steps=2):
+def build_network(pm, torch, nel, levels, tol, precision, omega=0.6, steps=2, warm_net=None):
     from pymoto_amd import problems
     d = pm.VoxelDomain(*nel)
     bc, f = problems.cantilever3d(d)
@@ -78,10 +80,36 @@ def build_network(pm, torch, nel, levels, tol, precision, omega=0.6, steps=2):
         sxf = pm.FilterConv(d, radius=1.5)(sx)
         ss = pm.MathExpression(f"{1e-9} + {1.0 - 1e-9}*inp0^3")(sxf)
         sK = pm.AssembleStiffness(d, bc=bc)(ss)
-        lin = pm.LinSolve(hermitian=True, solver=solver)
+        inner = solver
+        if warm_net is not None:
+            from pymoto_amd.learned import WarmStartCG
+            inner = WarmStartCG(solver, warm_net, d, sxf)
+        lin = pm.LinSolve(hermitian=True, solver=inner)
         su = lin(sK, fd)
         sc = pm.EinSum("i,i->")(su, fd)
-    return dict(domain=d, sx=sx, sc=sc, su=su, sK=sK, fn=fn, solver=solver, mg=mg, levels=levels, f=fd, lin=lin)
+    return dict(domain=d, sx=sx, sc=sc, su=su, sK=sK, sxf=sxf, fn=fn, solver=solver, inner=inner, mg=mg,
+                levels=levels, f=fd, lin=lin)
+
+
+def train_warm_start(pm, torch, args, precision, omega, nsamples, epochs):
+    """Train the learned warm start on solver-generated pairs: seeded designs from the benchmark's
+    distribution (filtered uniform densities, seeds disjoint from the timed ones), solved with the
+    given variant; returns (network, seconds spent on data and training, loss history)."""
+    from pymoto_amd import learned
+    t0 = time.perf_counter()
+    net = build_network(pm, torch, tuple(args.nel), args.levels, args.tol, precision, omega=omega)
+    nel = tuple(args.nel)
+    samples = []
+    for k in range(nsamples):
+        x = torch.as_tensor(np.random.default_rng(5000 + k).uniform(0.0, 1.0, nel[0] * nel[1] * nel[2]),
+                            device="cuda")
+        design_iteration(net, x)
+        samples.append((learned.make_input(net["domain"], net["sxf"].state, net["f"]),
+                        learned.normalized_target(net["su"].state)))
+    model = learned.WarmStartNet(nel, seed=0)
+    hist = learned.train(model, samples, epochs=epochs, lr=2e-3)
+    torch.cuda.synchronize()
+    return model, time.perf_counter() - t0, hist
 
 
 def design_iteration(net, x):
@@ -235,10 +263,10 @@ def cpu_baseline(nel, steps=3):
     return float(np.mean(times[1:] if len(times) > 1 else times)), ndof, its
 
 
-def run_variant(pm, torch, args, variant, xs):
+def run_variant(pm, torch, args, variant, xs, warm_net=None):
     precision, omega = parse_variant(variant)
     net = build_network(pm, torch, tuple(args.nel), args.levels, args.tol, precision,
-                        omega=omega if omega is not None else args.omega)
+                        omega=omega if omega is not None else args.omega, warm_net=warm_net)
     for k in range(args.warmup):
         design_iteration(net, xs[k])
     torch.cuda.synchronize()
@@ -246,7 +274,7 @@ def run_variant(pm, torch, args, variant, xs):
     t0 = time.perf_counter()
     for k in range(args.warmup, args.warmup + args.steps):
         comp.append(float(design_iteration(net, xs[k])))
-        its.append(net["solver"].iterations)
+        its.append(int(net["inner"].iterations))
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     dcdx = net["sx"].sensitivity.clone()
@@ -271,8 +299,20 @@ def main():
     torch.cuda.synchronize()
     variants = [v.strip() for v in args.variants.split(",") if v.strip()]
     results, roof, levels = {}, None, None
+    learned_info = None
     for v in variants:
-        net, res = run_variant(pm, torch, args, v, xs)
+        warm = None
+        if v.endswith("+learned"):
+            base = v[: -len("+learned")]
+            prec, om = parse_variant(base)
+            warm, t_train, hist = train_warm_start(pm, torch, args, prec, om if om is not None else args.omega,
+                                                   args.learned_samples, args.learned_epochs)
+            learned_info = dict(samples=args.learned_samples, epochs=args.learned_epochs, train_s=t_train,
+                                loss_history=[float(h) for h in hist], params=int(warm.nparams()))
+            v_run = base
+        else:
+            v_run = v
+        net, res = run_variant(pm, torch, args, v_run, xs, warm_net=warm)
         results[v] = res
         levels = net["levels"]
         if v == "fp64" and roof is None:
@@ -316,6 +356,8 @@ def main():
                    "speedup_vs_fp64_same_smoother": (summary[f"fp64@{parse_variant(best)[1]}"]["s_per_iter"] / sec)
                    if parse_variant(best)[1] is not None and f"fp64@{parse_variant(best)[1]}" in summary else None},
     }
+    if learned_info is not None:
+        line["config"]["learned"] = learned_info
     if not args.no_batched:
         line["config"]["batched"] = batched_workload(pm, torch)
     if roof is not None:

@@ -49,6 +49,13 @@ int pmt_filter_bwd(int nelx, int nely, int nelz, int kx, int ky, int kz, const d
                    const int32_t* inv_ptr, const int32_t* inv_idx, const double* dy, double* dx,
                    void* stream);
 
+/* DensityFilter (pymoto/modules/filter.py:223-378): y_i = so_i * sum_j H_ij * si_j * x_j with
+ * H_ij = max(0, radius - |i - j|) over the window |j - i|_inf <= int(radius) clipped to the grid
+ * (element units).  si / so may be NULL (= 1): forward si = NULL, so = 1/s; adjoint si = 1/s,
+ * so = NULL; the row sums s = H 1 with both NULL and x = 1. */
+int pmt_density_filter(int nelx, int nely, int nelz, double radius, const double* x, const double* scale_in,
+                       const double* scale_out, double* y, void* stream);
+
 /* ---------------------------------------------------------------------------------------------
  * Power-law material interpolation y = a + b*x^p (SIMP, MathExpression "xmin + (1-xmin)*inp0^3",
  * pymoto/modules/generic.py:94-99) and its adjoint dx (+)= dy*c*x^q (generic.py:101-140).

@@ -84,3 +84,43 @@ class ConvFilter:
         dx = np.zeros_like(dy)
         np.add.at(dx, self.el3d_pad, g)
         return dx
+
+
+def density_filter_matrix(nel, radius=2.0):
+    """H of DensityFilter._calculate_h (pymoto/modules/filter.py:310-378): coo entries
+    max(0, r - |i - j|) (element units) over the window |j - i|_inf <= int(r) clipped to the grid;
+    nel = (nelx, nely, nelz or 0)."""
+    import scipy.sparse as sps
+    nx, ny, nz = nel[0], nel[1], max(nel[2], 1)
+    d = int(radius)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    rows, cols, vals = [], [], []
+    for dk in range(-d, d + 1):
+        for dj in range(-d, d + 1):
+            for di in range(-d, d + 1):
+                ii, jj, kk = i + di, j + dj, k + dk
+                ok = (ii >= 0) & (ii < nx) & (jj >= 0) & (jj < ny) & (kk >= 0) & (kk < nz)
+                w = max(0.0, radius - np.sqrt(di * di + dj * dj + dk * dk))
+                rows.append(((k * ny + j) * nx + i)[ok])
+                cols.append(((kk * ny + jj) * nx + ii)[ok])
+                vals.append(np.full(ok.sum(), w))
+    n = nx * ny * nz
+    return sps.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n)).tocsc()
+
+
+class DensityFilterOracle:
+    """y = H x / s, adjoint H (dy / s) (filter.py:242-267)."""
+
+    def __init__(self, nel, radius=2.0, nonpadding=None):
+        self.H = density_filter_matrix(nel, radius)
+        self.Hs = np.asarray(self.H.sum(1)).ravel()
+        if nonpadding is not None:
+            inds = ~np.isin(np.arange(len(self.Hs)), nonpadding)
+            self.Hs[inds] = np.max(self.Hs)
+
+    def forward(self, x):
+        return self.H @ x / self.Hs
+
+    def adjoint(self, dy):
+        return self.H @ (dy / self.Hs)

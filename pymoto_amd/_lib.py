@@ -21,6 +21,7 @@ _SIGS = {
     "pmt_abi_version": (c_int, []),
     "pmt_filter_fwd": (c_int, [c_int] * 6 + [c_vp] * 5 + [c_vp]),
     "pmt_filter_bwd": (c_int, [c_int] * 6 + [c_vp] * 5 + [c_vp]),
+    "pmt_density_filter": (c_int, [c_int] * 3 + [c_dbl, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "pmt_powlaw_fwd": (c_int, [c_ll, c_dbl, c_dbl, c_dbl, c_vp, c_vp, c_vp]),
     "pmt_powlaw_bwd": (c_int, [c_ll, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_int, c_vp]),
     "pmt_heaviside_fwd": (c_int, [c_ll, c_dbl, c_dbl, c_vp, c_vp, c_vp]),

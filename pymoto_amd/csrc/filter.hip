@@ -97,6 +97,28 @@ filter_bwd_kernel(int nx, int ny, int nz, int kx, int ky, int kz, const double* 
   dx[e] = acc;
 }
 
+// DensityFilter (pymoto/modules/filter.py:223-378): y_i = so_i * sum_j H_ij (si_j * x_j) over the
+// clipped window |j - i|_inf <= int(radius) with H_ij = max(0, radius - |i - j|) in element units.
+// Forward: si = 1, so = 1/s; adjoint: si = 1/s, so = 1 (H symmetric); s = H 1 with si = so = 1.
+__global__ void density_filter_kernel(int nx, int ny, int nz, double radius, int dxy, int dz,
+                                      const double* __restrict__ x, const double* __restrict__ si,
+                                      const double* __restrict__ so, double* __restrict__ y) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)nx * ny * nz) return;
+  const int i = (int)(e % nx), j = (int)((e / nx) % ny), k = (int)(e / ((long long)nx * ny));
+  double acc = 0.0;
+  for (int kk = max(k - dz, 0); kk <= min(k + dz, nz - 1); ++kk)
+    for (int jj = max(j - dxy, 0); jj <= min(j + dxy, ny - 1); ++jj)
+      for (int ii = max(i - dxy, 0); ii <= min(i + dxy, nx - 1); ++ii) {
+        const double di = ii - i, dj = jj - j, dk = kk - k;
+        const double w = radius - sqrt(di * di + dj * dj + dk * dk);
+        if (w <= 0.0) continue;
+        const long long m = ((long long)kk * ny + jj) * nx + ii;
+        acc = fma(w, si ? si[m] * x[m] : x[m], acc);
+      }
+  y[e] = so ? acc * so[e] : acc;
+}
+
 }  // namespace pmt
 
 using namespace pmt;
@@ -114,6 +136,18 @@ int pmt_filter_fwd(int nelx, int nely, int nelz, int kx, int ky, int kz, const d
   filter_fwd_kernel<<<blocks_for(nel, 256), 256, 0, (hipStream_t)stream>>>(nelx, nely, nz, kx, ky, kz, w, map, cval,
                                                                           x, y);
   return check_launch("filter_fwd_kernel");
+}
+
+int pmt_density_filter(int nelx, int nely, int nelz, double radius, const double* x, const double* scale_in,
+                       const double* scale_out, double* y, void* stream) {
+  if (!grid_ok(nelx, nely, nelz) || !(radius > 0.0) || radius > 15.0 || !x || !y)
+    return arg_error("pmt_density_filter: bad argument (0 < radius <= 15)");
+  const int nz = nelz > 0 ? nelz : 1;
+  const long long nel = (long long)nelx * nely * nz;
+  density_filter_kernel<<<blocks_for(nel, 256), 256, 0, (hipStream_t)stream>>>(nelx, nely, nz, radius, (int)radius,
+                                                                              nelz > 0 ? (int)radius : 0, x,
+                                                                              scale_in, scale_out, y);
+  return check_launch("density_filter_kernel");
 }
 
 int pmt_filter_bwd(int nelx, int nely, int nelz, int kx, int ky, int kz, const double* w, const int32_t* inv_ptr,

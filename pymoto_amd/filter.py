@@ -143,3 +143,45 @@ class FilterConv(Module):
                                       _lib.ptr(dy), _lib.ptr(dx), _lib.stream()), "filter adjoint")
         xs = self.sig_in[0].state
         return dx.cpu().numpy() if isinstance(xs, np.ndarray) else dx
+
+
+class DensityFilter(Module):
+    """Row-normalised density filter y = H x / s (``filter.py:223-378``), H_ij = max(0, r - |i - j|)
+    in element units over the window |j - i|_inf <= int(r) clipped to the domain, s = H 1; elements
+    not listed in ``nonpadding`` get s = max(s) (mimics zero padding outside the domain).  The
+    adjoint is H (dy / s) (H is symmetric)."""
+
+    def __init__(self, domain: VoxelDomain, radius: float = 2.0, nonpadding=None):
+        self.domain = domain
+        self.radius = float(radius)
+        self.nonpadding = nonpadding
+        self._s_inv = None
+
+    def _grid(self):
+        d = self.domain
+        return d.nelx, d.nely, d.nelz if d.dim == 3 else 0
+
+    def _apply(self, x, si=None, so=None):
+        y = torch.empty_like(x)
+        _lib.check(_lib.lib().pmt_density_filter(*self._grid(), self.radius, _lib.ptr(x), _lib.ptr(si), _lib.ptr(so),
+                                                 _lib.ptr(y), _lib.stream()), "density filter")
+        return y
+
+    def _scale(self, like):
+        if self._s_inv is None:
+            s = self._apply(torch.ones_like(like))
+            if self.nonpadding is not None:
+                keep = torch.zeros(s.numel(), dtype=torch.bool, device=s.device)
+                keep[torch.as_tensor(np.asarray(self.nonpadding), device=s.device, dtype=torch.long)] = True
+                s = torch.where(keep, s, s.max())
+            self._s_inv = (1.0 / s).contiguous()
+        return self._s_inv
+
+    def __call__(self, x):
+        xd = _lib.as_device(x)
+        return self._apply(xd, so=self._scale(xd))
+
+    def _sensitivity(self, dfdy):
+        dy = _lib.as_device(dfdy)
+        dx = self._apply(dy, si=self._scale(dy))
+        return dx.cpu().numpy() if isinstance(self.sig_in[0].state, np.ndarray) else dx

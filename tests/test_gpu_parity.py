@@ -519,3 +519,22 @@ def test_heaviside_projection_vs_closed_form(gpu_lib, eta):
     npt.assert_allclose(host(sx.sensitivity), dy * (f(x + e, beta) - f(x - e, beta)) / (2 * e), rtol=1e-6, atol=1e-8)
     dbeta = np.sum(dy * (f(x, beta + e) - f(x, beta - e)) / (2 * e))
     npt.assert_allclose(float(sb.sensitivity), dbeta, rtol=1e-6)
+
+
+@pytest.mark.parametrize("nel,radius,nonpad", [((30, 20, 0), 2.0, False), ((12, 10, 8), 1.5, False),
+                                               ((12, 10, 8), 3.2, True)])
+def test_density_filter_vs_oracle(gpu_lib, nel, radius, nonpad):
+    from oracle.filter import DensityFilterOracle
+    d = pm.VoxelDomain(*nel)
+    rng = np.random.default_rng(int(radius * 10))
+    nonpadding = np.flatnonzero(rng.random(d.nel) < 0.7) if nonpad else None
+    ref = DensityFilterOracle(nel, radius=radius, nonpadding=nonpadding)
+    x = rng.random(d.nel)
+    m = pm.DensityFilter(d, radius=radius, nonpadding=nonpadding)
+    sx = pm.Signal("x", dev(x))
+    sy = m(sx)
+    npt.assert_allclose(host(sy.state), ref.forward(x), rtol=1e-12, atol=1e-14)
+    dy = rng.standard_normal(d.nel)
+    sy.sensitivity = dev(dy)
+    m.sensitivity()
+    npt.assert_allclose(host(sx.sensitivity), ref.adjoint(dy), rtol=1e-12, atol=1e-13)

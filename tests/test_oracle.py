@@ -285,3 +285,23 @@ def test_oc_step_volume_move_limits_and_uniform_case():
     xu = np.full(100, 0.4)
     xn = oc_step(xu, np.full(100, -1.0), maxvol=0.45, move=0.2)
     npt.assert_allclose(xn, 0.45, atol=1e-4)  # bisection tolerance 1e-4 on the multiplier
+
+
+def test_density_filter_oracle_interior_matches_convolution_and_is_self_adjoint():
+    """DensityFilter (filter.py:223-378): away from the boundary it is the normalised convolution of
+    FilterConv (same weights max(0, r - d)); H is symmetric so <F x, y> = <x, H (y / s)>."""
+    from oracle.filter import DensityFilterOracle
+    nel = (12, 10, 8)
+    rng = np.random.default_rng(4)
+    x = rng.random(12 * 10 * 8)
+    for r in (1.5, 2.0):
+        F = DensityFilterOracle(nel, radius=r)
+        y = F.forward(x)
+        yc = ConvFilter(nel, filter_weights(r, nel)).forward(x)
+        d = int(r)
+        k, j, i = np.meshgrid(np.arange(8), np.arange(10), np.arange(12), indexing="ij")
+        inner = ((i >= d) & (i < 12 - d) & (j >= d) & (j < 10 - d) & (k >= d) & (k < 8 - d)).ravel()
+        npt.assert_allclose(y[inner], yc[inner], rtol=1e-13)
+        assert abs(F.H - F.H.T).max() == 0.0
+        dy = rng.random(x.size)
+        npt.assert_allclose(np.dot(F.forward(x), dy), np.dot(x, F.adjoint(dy)), rtol=1e-12)
