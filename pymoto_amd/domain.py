@@ -156,6 +156,11 @@ class VoxelDomain:
         rbm[1::self.dim, -1] = c[0]
         return rbm
 
+    def write_to_vti(self, vectors: dict, filename="out.vti", scale=1.0):
+        """Write cell/point vectors (numpy or device tensors) to a Paraview VTI file (``domain.py:452-604``)."""
+        from .io import write_to_vti
+        write_to_vti(self, vectors, filename, scale)
+
     def coarsen(self):
         """The 2x coarser grid used by geometric multigrid (``iterative.py:163-165``)."""
         return VoxelDomain(self.nelx // 2, self.nely // 2, self.nelz // 2,

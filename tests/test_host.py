@@ -189,3 +189,30 @@ def test_heaviside_expression_parsing():
     assert _parse_heaviside(h.format(0.7).replace("inp1", "8")) == (8.0, 0.7)
     assert _parse_heaviside(f"{1e-9} + {1 - 1e-9}*inp0^3") is None
     assert _parse_heaviside("tanh(inp0)") is None
+
+
+def test_vti_writer_matches_oracle_bytes_and_decodes(tmp_path):
+    """write_to_vti (domain.py:452-604): same bytes as the oracle restatement for 2D/3D point and cell
+    data (2-component padding, (n, k) blocks), and the arrays decode back to the inputs."""
+    from oracle.io import read_vti_arrays, write_vti
+    rng = np.random.default_rng(0)
+    for nel in [(7, 5, 0), (4, 3, 2)]:
+        d = pm.VoxelDomain(*nel)
+        vecs = {"x": rng.random(d.nel), "u": rng.standard_normal(d.dim * d.nnodes),
+                "U": rng.standard_normal((d.dim * d.nnodes, 3)), "S": rng.random((2, d.nel))}
+        a, b = tmp_path / f"a{nel[2]}.vti", tmp_path / f"b{nel[2]}.vti"
+        import pymoto_amd.io as pio
+        chunk = pio._CHUNK
+        pio._CHUNK = 3 * 8  # stream in many small chunks
+        try:
+            d.write_to_vti(vecs, str(a), scale=2.0)
+        finally:
+            pio._CHUNK = chunk
+        write_vti(d, vecs, str(b), scale=2.0)
+        assert a.read_bytes() == b.read_bytes()
+        arrays = read_vti_arrays(str(a))
+        npt.assert_allclose(arrays["x"][1], vecs["x"].astype(np.float32))
+        u = arrays["u"][1].reshape(-1, 3)
+        npt.assert_allclose(u[:, : d.dim].ravel(), vecs["u"].astype(np.float32))
+        npt.assert_allclose(arrays["U(1)"][1].reshape(-1, 3)[:, : d.dim].ravel(), vecs["U"][:, 1].astype(np.float32))
+        npt.assert_allclose(arrays["S(1)"][1], vecs["S"][1].astype(np.float32))
