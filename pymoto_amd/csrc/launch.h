@@ -24,6 +24,13 @@ int launch_stencil(int mode, const GridDims& g, const T* S, const T* u, const T*
                    hipStream_t s);
 template <typename T>
 int launch_stencil_dinv(const GridDims& g, const double* S, T* dinv, hipStream_t s);
+// symmetric half stencil (centre + forward blocks) built from the full one, and its operator
+int half_stencil_coefficients(int dim);
+template <typename T>
+int launch_stencil_compress(const GridDims& g, const double* S, T* H, hipStream_t s);
+template <typename T>
+int launch_stencil_half(int mode, const GridDims& g, const T* H, const T* u, const T* b, const T* dinv, double omega,
+                        T* y, hipStream_t s);
 template <typename T>
 int launch_jacobi0(long long n, const T* dinv, const T* b, double omega, T* u, hipStream_t s);
 template <typename T>

@@ -83,7 +83,13 @@ static int level_apply(pmt_mg* mg, int l, int mode, const T* u, const T* b, T* y
   if (l == 0)
     return launch_fine<T>(mode, L.g, mg->K0, mg->k0pat, fine_x<T>(mg), mg->bcm, mg->bcdiag, u, b, B.dinv, L.omega, y,
                           s);
-  return launch_stencil<T>(mode, L.g, B.S, u, b, B.dinv, L.omega, y, s);
+  if (l == mg->nlev - 1) {  // coarsest: only its full FP64 stencil exists (it is solved directly)
+    if constexpr (std::is_same<T, double>::value)
+      return launch_stencil<double>(mode, L.g, L.d.S, u, b, B.dinv, L.omega, y, s);
+    else
+      return arg_error("level_apply: the coarsest level has no reduced-precision operator");
+  }
+  return launch_stencil_half<T>(mode, L.g, B.H, u, b, B.dinv, L.omega, y, s);
 }
 
 // out = V-cycle(b) on level l (GeometricMultigrid.solve with x0 = None, iterative.py:222-256)
@@ -135,7 +141,6 @@ static int read_scalar(pmt_mg* mg, int k, double* v, hipStream_t s) {
 }
 
 static int alloc_fp32(pmt_mg* mg) {
-  const int NS = mg->dim == 3 ? 243 : 36;
   int rc = PMT_OK;
   for (int l = 0; l < mg->nlev && rc == PMT_OK; ++l) {
     Level& L = mg->L[l];
@@ -152,7 +157,7 @@ static int alloc_fp32(pmt_mg* mg) {
     if (rc == PMT_OK) rc = dalloc(mg, &B.r, L.ndof);
     if (rc == PMT_OK) rc = dalloc(mg, &B.b, L.ndof);
     if (l > 0) {
-      if (rc == PMT_OK) rc = dalloc(mg, &B.S, L.g.nnodes * NS);
+      if (rc == PMT_OK) rc = dalloc(mg, &B.H, L.g.nnodes * half_stencil_coefficients(mg->dim));
       if (rc == PMT_OK) rc = dalloc(mg, &B.u, L.ndof);
     }
   }
@@ -241,6 +246,7 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
     if (l > 0) {
       ALLOC(L.Ke, L.g.nel * NE * NE);
       ALLOC(L.d.S, L.g.nnodes * NS);
+      ALLOC(L.d.H, L.g.nnodes * half_stencil_coefficients(mg->dim));
       ALLOC(L.d.u, L.ndof);
       ALLOC(L.d.b, L.ndof);
     }
@@ -312,7 +318,10 @@ int pmt_mg_update(pmt_mg* mg, const double* xe, const uint8_t* bcmask, double bc
     else
       PMT_TRY(launch_galerkin_coarse(P.g, P.Ke, L.Ke, s));
     PMT_TRY(launch_stencil_from_elements(L.g, L.Ke, L.d.S, s));
-    if (l < mg->nlev - 1) PMT_TRY(launch_stencil_dinv<double>(L.g, L.d.S, L.d.dinv, s));
+    if (l < mg->nlev - 1) {
+      PMT_TRY(launch_stencil_dinv<double>(L.g, L.d.S, L.d.dinv, s));
+      PMT_TRY(launch_stencil_compress<double>(L.g, L.d.S, L.d.H, s));
+    }
   }
   Level& Cl = mg->L[mg->nlev - 1];
   PMT_TRY(launch_dense_from_stencil(Cl.g, Cl.d.S, mg->Ainv, s));
@@ -320,10 +329,9 @@ int pmt_mg_update(pmt_mg* mg, const double* xe, const uint8_t* bcmask, double bc
   if (mg->prec != 64) {
     PMT_TRY(launch_to_float(F.g.nel, xe, mg->xe32, s));
     PMT_TRY(launch_to_float(F.ndof, F.d.dinv, F.f.dinv, s));
-    const int NS = mg->dim == 3 ? 243 : 36;
     for (int l = 1; l < mg->nlev - 1; ++l) {
       Level& L = mg->L[l];
-      PMT_TRY(launch_to_float(L.g.nnodes * NS, L.d.S, L.f.S, s));
+      PMT_TRY(launch_stencil_compress<float>(L.g, L.d.S, L.f.H, s));
       PMT_TRY(launch_stencil_dinv<float>(L.g, L.d.S, L.f.dinv, s));
     }
   }

@@ -13,7 +13,8 @@ struct BlockCG;
 template <typename T>
 struct LevelBuf {
   T* dinv = nullptr;  // inverse diagonal (levels above the coarsest)
-  T* S = nullptr;     // coarse stencil (levels >= 1)
+  T* S = nullptr;     // coarse stencil, full (levels >= 1, FP64 only: setup, diagonal, coarsest)
+  T* H = nullptr;     // coarse stencil, symmetric half (levels >= 1 above the coarsest: the operator)
   T* u = nullptr;     // V-cycle correction (levels >= 1)
   T* t = nullptr;     // ping-pong buffer
   T* b = nullptr;     // restricted right-hand side (levels >= 1; level 0 in FP32: converted input)

@@ -479,6 +479,88 @@ stencil_node_kernel(GridDims g, const T* __restrict__ S, const T* __restrict__ u
   }
 }
 
+// Half stencil: the Galerkin operators are symmetric, A(n, n+o) = A(n+o, n)^T, so only the blocks of
+// the centre and the forward offsets o > centre are stored per node, H[c][node] with
+// c = ((o - centre)*D + d)*D + d2 (126 coefficients per node in 3D instead of 243, 20 in 2D).  The
+// block towards a backward neighbour m = n - off(o) is the transpose of m's forward block o; for a
+// wave of 64 consecutive nodes those reads are 64 consecutive entries of the same coefficient row
+// (coalesced) that the neighbour rows of the same sweep have just brought into L2 / the last-level
+// cache, so HBM carries each stored coefficient about once.
+template <int DIM>
+struct Half {
+  static constexpr int C = (Cfg<DIM>::NS - 1) / 2;   // centre offset index
+  static constexpr int NB = Cfg<DIM>::NS - C;        // stored blocks
+  static constexpr int NC = NB * DIM * DIM;          // stored coefficients per node
+};
+
+template <int DIM, typename T>
+__global__ void stencil_compress_kernel(GridDims g, const double* __restrict__ S, T* __restrict__ H) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.nnodes) return;
+  constexpr int C = Half<DIM>::C;
+#pragma unroll
+  for (int c = 0; c < Half<DIM>::NC; ++c)
+    H[(long long)c * g.nnodes + n] = (T)S[((long long)C * DIM * DIM + c) * g.nnodes + n];
+}
+
+template <int DIM, int MODE, typename T>
+__global__ void __launch_bounds__(256)
+stencil_half_kernel(GridDims g, const T* __restrict__ H, const T* __restrict__ u, const T* __restrict__ b,
+                    const T* __restrict__ dinv, T omega, T* __restrict__ y) {
+  const long long n = (long long)xcd_block() * blockDim.x + threadIdx.x;
+  if (n >= g.nnodes) return;
+  const int i = (int)(n % g.nx);
+  const long long t = n / g.nx;
+  const int j = (int)(t % g.ny);
+  const int k = (int)(t / g.ny);
+  const long long nn = g.nnodes;
+  constexpr int C = Half<DIM>::C;
+  T out[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) out[d] = 0.0;
+  constexpr int ZLO = DIM == 3 ? -1 : 0, ZHI = DIM == 3 ? 1 : 0;
+  int o = 0;
+#pragma unroll
+  for (int dz = ZLO; dz <= ZHI; ++dz) {
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy) {
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx, ++o) {
+        const int ii = i + dx, jj = j + dy, kk = k + dz;
+        if (ii < 0 || ii >= g.nx || jj < 0 || jj >= g.ny || kk < 0 || kk >= g.nz) continue;
+        const long long m = n + dx + dy * (long long)g.nx + dz * (long long)g.nx * g.ny;
+        T um[DIM];
+#pragma unroll
+        for (int d2 = 0; d2 < DIM; ++d2) um[d2] = u[m * DIM + d2];
+        if (o >= C) {  // stored at n: block (n, n+o)
+          const int ob = o - C;
+#pragma unroll
+          for (int d = 0; d < DIM; ++d)
+#pragma unroll
+            for (int d2 = 0; d2 < DIM; ++d2) out[d] = fma(H[((long long)(ob * DIM + d) * DIM + d2) * nn + n], um[d2], out[d]);
+        } else {  // backward: (n, m) = (m, m + o')^T with o' the mirrored offset, stored at m
+          const int ob = (2 * C - o) - C;
+#pragma unroll
+          for (int d = 0; d < DIM; ++d)
+#pragma unroll
+            for (int d2 = 0; d2 < DIM; ++d2) out[d] = fma(H[((long long)(ob * DIM + d2) * DIM + d) * nn + m], um[d2], out[d]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const long long q = n * DIM + d;
+    if (MODE == MODE_APPLY) {
+      y[q] = out[d];
+    } else if (MODE == MODE_RESIDUAL) {
+      y[q] = b[q] - out[d];
+    } else {
+      y[q] = u[q] + omega * dinv[q] * (b[q] - out[d]);
+    }
+  }
+}
+
 template <int DIM, typename T>
 __global__ void stencil_diag_kernel(GridDims g, const double* __restrict__ S, T* __restrict__ dinv) {
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -759,6 +841,41 @@ int launch_fine_diag(const GridDims& g, const double* K0, const double* xe, cons
   return check_launch("fine_diag_kernel");
 }
 
+int half_stencil_coefficients(int dim) { return dim == 3 ? Half<3>::NC : Half<2>::NC; }
+
+template <typename T>
+int launch_stencil_compress(const GridDims& g, const double* S, T* H, hipStream_t s) {
+  dim3 grid(blocks_for(g.nnodes, 256));
+  if (g.dim == 3)
+    stencil_compress_kernel<3, T><<<grid, 256, 0, s>>>(g, S, H);
+  else
+    stencil_compress_kernel<2, T><<<grid, 256, 0, s>>>(g, S, H);
+  return check_launch("stencil_compress_kernel");
+}
+
+template <typename T>
+int launch_stencil_half(int mode, const GridDims& g, const T* H, const T* u, const T* b, const T* dinv, double omega,
+                        T* y, hipStream_t s) {
+  dim3 grid(xcd_blocks(g.nnodes, 256));
+  const T om = (T)omega;
+  if (g.dim == 3) {
+    if (mode == MODE_APPLY)
+      stencil_half_kernel<3, MODE_APPLY, T><<<grid, 256, 0, s>>>(g, H, u, b, dinv, om, y);
+    else if (mode == MODE_RESIDUAL)
+      stencil_half_kernel<3, MODE_RESIDUAL, T><<<grid, 256, 0, s>>>(g, H, u, b, dinv, om, y);
+    else
+      stencil_half_kernel<3, MODE_JACOBI, T><<<grid, 256, 0, s>>>(g, H, u, b, dinv, om, y);
+  } else {
+    if (mode == MODE_APPLY)
+      stencil_half_kernel<2, MODE_APPLY, T><<<grid, 256, 0, s>>>(g, H, u, b, dinv, om, y);
+    else if (mode == MODE_RESIDUAL)
+      stencil_half_kernel<2, MODE_RESIDUAL, T><<<grid, 256, 0, s>>>(g, H, u, b, dinv, om, y);
+    else
+      stencil_half_kernel<2, MODE_JACOBI, T><<<grid, 256, 0, s>>>(g, H, u, b, dinv, om, y);
+  }
+  return check_launch("stencil_half_kernel");
+}
+
 template <typename T>
 int launch_stencil(int mode, const GridDims& g, const T* S, const T* u, const T* b, const T* dinv, double omega, T* y,
                    hipStream_t s) {
@@ -824,6 +941,9 @@ int launch_prolong(const GridDims& gf, const GridDims& gc, const T* uc, T* uf, h
   template int launch_stencil<T>(int, const GridDims&, const T*, const T*, const T*, const T*, double, T*,         \
                                  hipStream_t);                                                                     \
   template int launch_stencil_dinv<T>(const GridDims&, const double*, T*, hipStream_t);                            \
+  template int launch_stencil_compress<T>(const GridDims&, const double*, T*, hipStream_t);                        \
+  template int launch_stencil_half<T>(int, const GridDims&, const T*, const T*, const T*, const T*, double, T*,    \
+                                      hipStream_t);                                                                \
   template int launch_jacobi0<T>(long long, const T*, const T*, double, T*, hipStream_t);                          \
   template int launch_restrict<T>(const GridDims&, const GridDims&, const T*, T*, hipStream_t);                    \
   template int launch_prolong<T>(const GridDims&, const GridDims&, const T*, T*, hipStream_t);
