@@ -41,15 +41,16 @@ __device__ inline bool touches_bc(const GridDims& gf, const uint8_t* __restrict_
   return false;
 }
 
-// Fine -> first coarse level away from Dirichlet nodes: with M_c = I and no bcdiag terms,
-// P_c^T (x_c K0) P_c = x_c G_c with G_c = P_c^T K0 P_c fixed, so K_E = sum_c x_c G_c
-// (576*8 multiply-adds per coarse element instead of the triple products).  One thread per
-// (coarse element, D x D node block); elements that touch a Dirichlet node are left to
-// galerkin_kernel<DIM, true> with bc_only.
+// Fine -> first coarse level: for a child without Dirichlet nodes (M_c = I, no bcdiag terms)
+// P_c^T (x_c K0) P_c = x_c G_c with G_c = P_c^T K0 P_c fixed, so K_E = sum_c x_c G_c (576*8
+// multiply-adds per coarse element instead of the triple products); the few children with Dirichlet
+// nodes get the masked triple product of their block directly.  One thread per (coarse element,
+// D x D node block), one coarse element per wave (the branch is uniform across the wave).
 template <int DIM>
 __global__ void __launch_bounds__(256)
-galerkin_fine_fast_kernel(GridDims gf, GridDims gc, const double* __restrict__ G, const double* __restrict__ xe,
-                          const uint8_t* __restrict__ bcm, double* __restrict__ Kc) {
+galerkin_fine_fast_kernel(GridDims gf, GridDims gc, const double* __restrict__ G, const double* __restrict__ K0,
+                          const double* __restrict__ xe, const uint8_t* __restrict__ bcm, double bcdiag,
+                          double* __restrict__ Kc) {
   constexpr int NN = 1 << DIM;
   constexpr int NE = NN * DIM;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -59,21 +60,52 @@ galerkin_fine_fast_kernel(GridDims gf, GridDims gc, const double* __restrict__ G
   const int CI = (int)(E % gc.ex);
   const long long tt = E / gc.ex;
   const int CJ = (int)(tt % gc.ey), CK = (int)(tt / gc.ey);
-  if (bcm && touches_bc<DIM>(gf, bcm, CI, CJ, CK)) return;
   double acc[DIM][DIM];
 #pragma unroll
   for (int d = 0; d < DIM; ++d)
 #pragma unroll
     for (int d2 = 0; d2 < DIM; ++d2) acc[d][d2] = 0.0;
-#pragma unroll
   for (int c = 0; c < NN; ++c) {
     const int fi = 2 * CI + (c & 1), fj = 2 * CJ + ((c >> 1) & 1), fk = DIM == 3 ? 2 * CK + ((c >> 2) & 1) : 0;
     const double xc = xe[((long long)fk * gf.ey + fj) * gf.ex + fi];
-    const double* g = G + c * NE * NE;
+    // Dirichlet masks of the child's nodes (uniform across the wave: one coarse element per wave)
+    unsigned mk[NN];
+    bool any = false;
 #pragma unroll
-    for (int d = 0; d < DIM; ++d)
+    for (int a = 0; a < NN; ++a) {
+      const int ai = fi + (a & 1), aj = fj + ((a >> 1) & 1), ak = DIM == 3 ? fk + ((a >> 2) & 1) : 0;
+      mk[a] = bcm ? bcm[((long long)ak * gf.ny + aj) * gf.nx + ai] : 0u;
+      any = any || mk[a] != 0u;
+    }
+    if (!any) {
+      const double* g = G + c * NE * NE;
 #pragma unroll
-      for (int d2 = 0; d2 < DIM; ++d2) acc[d][d2] = fma(xc, g[(A * DIM + d) * NE + B * DIM + d2], acc[d][d2]);
+      for (int d = 0; d < DIM; ++d)
+#pragma unroll
+        for (int d2 = 0; d2 < DIM; ++d2) acc[d][d2] = fma(xc, g[(A * DIM + d) * NE + B * DIM + d2], acc[d][d2]);
+      continue;
+    }
+    // child with Dirichlet nodes: block (A, B) of P_c^T (x_c M K0 M + D_c) P_c directly
+    for (int a = 0; a < NN; ++a) {
+      const double wa = pweight<DIM>(c, a, A);
+      if (wa == 0.0) continue;
+      for (int b = 0; b < NN; ++b) {
+        const double wb = pweight<DIM>(c, b, B);
+        if (wb == 0.0) continue;
+        for (int d = 0; d < DIM; ++d)
+          for (int d2 = 0; d2 < DIM; ++d2) {
+            const bool fa = (mk[a] >> d) & 1u, fb = (mk[b] >> d2) & 1u;
+            double v = (fa || fb) ? 0.0 : xc * K0[(a * DIM + d) * NE + b * DIM + d2];
+            if (a == b && d == d2 && fa) {
+              const int ai = fi + (a & 1), aj = fj + ((a >> 1) & 1), ak = DIM == 3 ? fk + ((a >> 2) & 1) : 0;
+              const int mult = ((ai > 0) + (ai < gf.nx - 1)) * ((aj > 0) + (aj < gf.ny - 1)) *
+                               (DIM == 3 ? ((ak > 0) + (ak < gf.nz - 1)) : 1);
+              v += bcdiag / (double)mult;
+            }
+            acc[d][d2] = fma(wa * wb, v, acc[d][d2]);
+          }
+      }
+    }
   }
 #pragma unroll
   for (int d = 0; d < DIM; ++d)
@@ -244,37 +276,48 @@ __global__ void dense_from_stencil_kernel(GridDims g, const double* __restrict__
 // ---------------- blocked Gauss-Jordan inversion (in place), block size GJB ----------------
 constexpr int GJB = 64;
 
-// invert the kk x kk diagonal block at (k0, k0) in LDS, write it to P (GJB x GJB, row-major)
+// invert the kk x kk diagonal block at (k0, k0) in LDS, write it to P (GJB x GJB, row-major).
+// 16 x 16 threads, each owning a 4 x 4 set of entries (rows tr + 16 i, columns tc + 16 j); the block
+// is padded with the identity beyond kk, so every pivot step is the same 16 multiply-adds per thread
+// between two barriers (in-place Gauss-Jordan, no pivoting: the Galerkin operator is SPD).
 __global__ void __launch_bounds__(256) gj_diag_kernel(long long n, long long k0, int kk, const double* __restrict__ A,
                                                        double* __restrict__ P) {
   __shared__ double sM[GJB][GJB + 1];
-  const int tid = threadIdx.x;
-  for (int q = tid; q < kk * kk; q += blockDim.x) {
-    const int r = q / kk, c = q % kk;
-    sM[r][c] = A[(k0 + r) * n + k0 + c];
-  }
+  const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = tr + 16 * i, c = tc + 16 * j;
+      sM[r][c] = (r < kk && c < kk) ? A[(k0 + r) * n + k0 + c] : (r == c ? 1.0 : 0.0);
+    }
   __syncthreads();
   for (int p = 0; p < kk; ++p) {
-    const double piv = sM[p][p];
+    const double inv = 1.0 / sM[p][p];
+    double mrp[4], mpc[4], v[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) mrp[i] = sM[tr + 16 * i][p];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) mpc[j] = sM[p][tc + 16 * j];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r = tr + 16 * i, c = tc + 16 * j;
+        const double m = sM[r][c];
+        if (r == p)
+          v[i][j] = c == p ? inv : m * inv;
+        else
+          v[i][j] = c == p ? -mrp[i] * inv : m - mrp[i] * (mpc[j] * inv);
+      }
     __syncthreads();
-    // scale pivot row (the pivot entry becomes 1/piv)
-    for (int c = tid; c < kk; c += blockDim.x) sM[p][c] = (c == p) ? 1.0 / piv : sM[p][c] / piv;
-    __syncthreads();
-    // eliminate column p from the other rows
-    for (int q = tid; q < kk * kk; q += blockDim.x) {
-      const int r = q / kk, c = q % kk;
-      if (r == p) continue;
-      const double f = sM[r][p];
-      if (c == p)
-        continue;
-      sM[r][c] -= f * sM[p][c];
-    }
-    __syncthreads();
-    for (int r = tid; r < kk; r += blockDim.x)
-      if (r != p) sM[r][p] = -sM[r][p] * sM[p][p];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) sM[tr + 16 * i][tc + 16 * j] = v[i][j];
     __syncthreads();
   }
-  for (int q = tid; q < GJB * GJB; q += blockDim.x) {  // zero outside kk x kk: consumers read full tiles
+  for (int q = threadIdx.x; q < GJB * GJB; q += blockDim.x) {  // zero outside kk x kk: consumers read full tiles
     const int r = q / GJB, c = q % GJB;
     P[q] = (r < kk && c < kk) ? sM[r][c] : 0.0;
   }
@@ -420,20 +463,15 @@ void galerkin_child_matrices(int dim, const double* K0, double* G) {
 int launch_galerkin_fine(const GridDims& gf, const double* K0, const double* G, const double* xe, const uint8_t* bcm,
                          double bcdiag, double* Kc, hipStream_t s) {
   GridDims gc = make_grid(gf.ex / 2, gf.ey / 2, gf.dim == 3 ? gf.ez / 2 : 0);
-  const int fast = G != nullptr;
-  if (fast) {
-    const int NN = 1 << gf.dim;
-    const long long nthreads = gc.nel * NN * NN;
-    if (gf.dim == 3)
-      galerkin_fine_fast_kernel<3><<<blocks_for(nthreads, 256), 256, 0, s>>>(gf, gc, G, xe, bcm, Kc);
-    else
-      galerkin_fine_fast_kernel<2><<<blocks_for(nthreads, 256), 256, 0, s>>>(gf, gc, G, xe, bcm, Kc);
-    PMT_TRY(check_launch("galerkin_fine_fast_kernel"));
+  if (G != nullptr && gf.dim == 3) {
+    const long long nthreads = gc.nel * 64;
+    galerkin_fine_fast_kernel<3><<<blocks_for(nthreads, 256), 256, 0, s>>>(gf, gc, G, K0, xe, bcm, bcdiag, Kc);
+    return check_launch("galerkin_fine_fast_kernel");
   }
   if (gf.dim == 3)
-    galerkin_kernel<3, true><<<(unsigned)gc.nel, 64, 0, s>>>(gf, gc, K0, xe, bcm, bcdiag, nullptr, Kc, fast);
+    galerkin_kernel<3, true><<<(unsigned)gc.nel, 64, 0, s>>>(gf, gc, K0, xe, bcm, bcdiag, nullptr, Kc, 0);
   else
-    galerkin_kernel<2, true><<<(unsigned)gc.nel, 16, 0, s>>>(gf, gc, K0, xe, bcm, bcdiag, nullptr, Kc, fast);
+    galerkin_kernel<2, true><<<(unsigned)gc.nel, 16, 0, s>>>(gf, gc, K0, xe, bcm, bcdiag, nullptr, Kc, 0);
   return check_launch("galerkin_kernel<fine>");
 }
 
