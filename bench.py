@@ -61,8 +61,10 @@ def parse():
 
 
 def parse_variant(v):
-    """'fp32', 'fp32@0.5' or 'fp32@0.5+learned' -> (precision, omega or None)."""
+    """'fp32', 'fp32@0.5', 'fp32@cheb' or 'fp32@0.5+learned' -> (precision, omega, "chebyshev" or None)."""
     prec, _, om = v.split("+")[0].partition("@")
+    if om == "cheb":
+        return prec.strip(), "chebyshev"
     return prec.strip(), (float(om) if om else None)
 
 
@@ -164,7 +166,9 @@ def batched_workload(pm, torch, nel=(128, 64, 64), beta=8.0, etas=(0.3, 0.5, 0.7
         j, k = int(rng.integers(0, nel[1] + 1)), int(rng.integers(0, nel[2] + 1))
         F[3 * d.get_nodenumber(nel[0], j, k) + 1, c] = -1.0
     Fd = torch.as_tensor(F, device="cuda")
-    x = torch.as_tensor(rng.uniform(0.0, 1.0, d.nel), device="cuda")
+    # densities in [0.35, 1]: after the r = 1.5 filter all three projections (eta 0.3/0.5/0.7) keep a
+    # connected structure (from uniform[0, 1] the eroded design is almost void and singular)
+    x = torch.as_tensor(rng.uniform(0.35, 1.0, d.nel), device="cuda")
     xf = pm.FilterConv(d, radius=1.5)(pm.Signal("x", x)).state
     hv = "(tanh({b} * {e}) + tanh({b} * (inp0 - {e}))) / (tanh({b} * {e}) + tanh({b} * (1 - {e})))"
     Ks = []
@@ -176,7 +180,8 @@ def batched_workload(pm, torch, nel=(128, 64, 64), beta=8.0, etas=(0.3, 0.5, 0.7
 
     def make_solver(graph=True):
         return pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(d, levels, omega=omega, smooth_steps=2,
-                                                                       precision=precision), tol=tol, graph=graph)
+                                                                       precision=precision), tol=tol, graph=graph,
+                             maxit=2000)
     solvers = [make_solver() for _ in etas]
     for K, sv in zip(Ks, solvers):  # setup + graph capture, untimed
         sv.update(K)
@@ -220,7 +225,7 @@ def batched_workload(pm, torch, nel=(128, 64, 64), beta=8.0, etas=(0.3, 0.5, 0.7
         prec = precision if (dd.dim == 3 or precision != "bf16") else "fp32"
         sv = pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(dd, max(2, pm.solvers.auto_levels(dd)),
                                                                      omega=omega, smooth_steps=2, precision=prec),
-                           tol=tol)
+                           tol=tol, maxit=2000)
         jobs.append((K2, torch.as_tensor(f2, device="cuda"), sv))
         dofs += K2.shape[0]
     solve_independent(jobs, workers=8)  # setup, untimed
@@ -330,6 +335,8 @@ def main():
         summary[v] = item
     ok = [v for v in summary if v == "fp64" or ref is None or summary[v].get("compliance_rel_err_vs_fp64", 1.0) <= 1e-7]
     best = min(ok, key=lambda v: summary[v]["s_per_iter"])
+    best_base = best.split("+")[0]
+    same = "fp64" + (best_base[best_base.index("@"):] if "@" in best_base else "")
     sec = summary[best]["s_per_iter"]
     d = pm.VoxelDomain(*nel)
     ndof = 3 * d.nnodes
@@ -353,8 +360,7 @@ def main():
                    "ndof": ndof, "nel": d.nel, "variant": best, "dof_per_s": ndof / sec,
                    "variants": summary,
                    "speedup_vs_fp64": (summary["fp64"]["s_per_iter"] / sec) if "fp64" in summary else None,
-                   "speedup_vs_fp64_same_smoother": (summary[f"fp64@{parse_variant(best)[1]}"]["s_per_iter"] / sec)
-                   if parse_variant(best)[1] is not None and f"fp64@{parse_variant(best)[1]}" in summary else None},
+                   "speedup_vs_fp64_same_smoother": (summary[same]["s_per_iter"] / sec) if same in summary else None},
     }
     if learned_info is not None:
         line["config"]["learned"] = learned_info

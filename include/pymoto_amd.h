@@ -175,6 +175,14 @@ int pmt_mg_destroy(pmt_mg* mg);
  * accumulation; hex8 grids only).  Takes effect at the next pmt_mg_update.
  * (No reference counterpart: the mixed-precision variants.) */
 int pmt_mg_set_precision(pmt_mg* mg, int bits);
+/* Smoother of every level: type 0 = damped Jacobi with the per-level omega (the reference's), type 1 =
+ * Chebyshev polynomial in D^-1 A of degree `steps` (same number of operator sweeps) on
+ * [lower*hi, hi], hi = lambda_max(diag(K0)^-1 K0) on the fine grid (an upper bound of
+ * lambda_max(D^-1 K) for any positive densities) and upper * (power-iteration estimate) on the
+ * Galerkin levels.  Takes effect at the next pmt_mg_update.  pmt_mg_smoother_interval returns the
+ * interval in use on a level.  (No reference counterpart.) */
+int pmt_mg_set_smoother(pmt_mg* mg, int type, double lower, double upper);
+int pmt_mg_smoother_interval(pmt_mg* mg, int level, double* lo, double* hi);
 /* Rebuild diagonals, Galerkin coarse operators and the coarsest inverse for new densities xe
  * (GeometricMultigrid.update, iterative.py:168-176).  xe and bcmask must stay alive until the
  * next update. */

@@ -49,6 +49,8 @@ _SIGS = {
     "pmt_mg_create": (c_int, [c_int] * 4 + [c_vp, c_vp, c_vp, c_vp]),
     "pmt_mg_destroy": (c_int, [c_vp]),
     "pmt_mg_set_precision": (c_int, [c_vp, c_int]),
+    "pmt_mg_set_smoother": (c_int, [c_vp, c_int, c_dbl, c_dbl]),
+    "pmt_mg_smoother_interval": (c_int, [c_vp, c_int, c_vp, c_vp]),
     "pmt_mg_update": (c_int, [c_vp, c_vp, c_vp, c_dbl, c_vp]),
     "pmt_mg_vcycle": (c_int, [c_vp, c_vp, c_vp, c_vp]),
     "pmt_mg_level_apply": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp]),

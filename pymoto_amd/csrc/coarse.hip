@@ -41,76 +41,91 @@ __device__ inline bool touches_bc(const GridDims& gf, const uint8_t* __restrict_
   return false;
 }
 
-// Fine -> first coarse level: for a child without Dirichlet nodes (M_c = I, no bcdiag terms)
+// Fine -> first coarse level (hex8): for a child without Dirichlet nodes (M_c = I, no bcdiag terms)
 // P_c^T (x_c K0) P_c = x_c G_c with G_c = P_c^T K0 P_c fixed, so K_E = sum_c x_c G_c (576*8
 // multiply-adds per coarse element instead of the triple products); the few children with Dirichlet
-// nodes get the masked triple product of their block directly.  One thread per (coarse element,
-// D x D node block), one coarse element per wave (the branch is uniform across the wave).
-template <int DIM>
+// nodes get the masked triple product of their block directly.  A block stages the 8 G_c (36.9 KB)
+// in LDS once and walks many coarse elements, one per wave; thread = (A, B) node block of K_E, and
+// the 27 Dirichlet bytes of the element's fine nodes come through LDS (the branch is wave-uniform).
 __global__ void __launch_bounds__(256)
 galerkin_fine_fast_kernel(GridDims gf, GridDims gc, const double* __restrict__ G, const double* __restrict__ K0,
                           const double* __restrict__ xe, const uint8_t* __restrict__ bcm, double bcdiag,
                           double* __restrict__ Kc) {
-  constexpr int NN = 1 << DIM;
-  constexpr int NE = NN * DIM;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= gc.nel * NN * NN) return;
-  const long long E = t / (NN * NN);
-  const int blk = (int)(t % (NN * NN)), A = blk / NN, B = blk % NN;
-  const int CI = (int)(E % gc.ex);
-  const long long tt = E / gc.ex;
-  const int CJ = (int)(tt % gc.ey), CK = (int)(tt / gc.ey);
-  double acc[DIM][DIM];
-#pragma unroll
-  for (int d = 0; d < DIM; ++d)
-#pragma unroll
-    for (int d2 = 0; d2 < DIM; ++d2) acc[d][d2] = 0.0;
-  for (int c = 0; c < NN; ++c) {
-    const int fi = 2 * CI + (c & 1), fj = 2 * CJ + ((c >> 1) & 1), fk = DIM == 3 ? 2 * CK + ((c >> 2) & 1) : 0;
-    const double xc = xe[((long long)fk * gf.ey + fj) * gf.ex + fi];
-    // Dirichlet masks of the child's nodes (uniform across the wave: one coarse element per wave)
-    unsigned mk[NN];
-    bool any = false;
-#pragma unroll
-    for (int a = 0; a < NN; ++a) {
-      const int ai = fi + (a & 1), aj = fj + ((a >> 1) & 1), ak = DIM == 3 ? fk + ((a >> 2) & 1) : 0;
-      mk[a] = bcm ? bcm[((long long)ak * gf.ny + aj) * gf.nx + ai] : 0u;
-      any = any || mk[a] != 0u;
-    }
-    if (!any) {
-      const double* g = G + c * NE * NE;
-#pragma unroll
-      for (int d = 0; d < DIM; ++d)
-#pragma unroll
-        for (int d2 = 0; d2 < DIM; ++d2) acc[d][d2] = fma(xc, g[(A * DIM + d) * NE + B * DIM + d2], acc[d][d2]);
-      continue;
-    }
-    // child with Dirichlet nodes: block (A, B) of P_c^T (x_c M K0 M + D_c) P_c directly
-    for (int a = 0; a < NN; ++a) {
-      const double wa = pweight<DIM>(c, a, A);
-      if (wa == 0.0) continue;
-      for (int b = 0; b < NN; ++b) {
-        const double wb = pweight<DIM>(c, b, B);
-        if (wb == 0.0) continue;
-        for (int d = 0; d < DIM; ++d)
-          for (int d2 = 0; d2 < DIM; ++d2) {
-            const bool fa = (mk[a] >> d) & 1u, fb = (mk[b] >> d2) & 1u;
-            double v = (fa || fb) ? 0.0 : xc * K0[(a * DIM + d) * NE + b * DIM + d2];
-            if (a == b && d == d2 && fa) {
-              const int ai = fi + (a & 1), aj = fj + ((a >> 1) & 1), ak = DIM == 3 ? fk + ((a >> 2) & 1) : 0;
-              const int mult = ((ai > 0) + (ai < gf.nx - 1)) * ((aj > 0) + (aj < gf.ny - 1)) *
-                               (DIM == 3 ? ((ak > 0) + (ak < gf.nz - 1)) : 1);
-              v += bcdiag / (double)mult;
-            }
-            acc[d][d2] = fma(wa * wb, v, acc[d][d2]);
-          }
+  constexpr int NN = 8, NE = 24;
+  __shared__ double sG[NN * NE * NE];
+  __shared__ unsigned smask[4][27];
+  for (int q = threadIdx.x; q < NN * NE * NE; q += 256) sG[q] = G[q];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int A = lane >> 3, B = lane & 7;
+  for (long long E0 = (long long)blockIdx.x * 4; E0 < gc.nel; E0 += (long long)gridDim.x * 4) {
+    const long long E = E0 + wave;
+    const bool valid = E < gc.nel;
+    int CI = 0, CJ = 0, CK = 0;
+    if (valid) {
+      CI = (int)(E % gc.ex);
+      const long long tt = E / gc.ex;
+      CJ = (int)(tt % gc.ey);
+      CK = (int)(tt / gc.ey);
+      if (lane < 27) {
+        const int i = 2 * CI + lane % 3, j = 2 * CJ + (lane / 3) % 3, k = 2 * CK + lane / 9;
+        smask[wave][lane] = bcm ? bcm[((long long)k * gf.ny + j) * gf.nx + i] : 0u;
       }
     }
+    __syncthreads();  // sG (first pass) and this wave's masks visible
+    if (valid) {
+      double acc[3][3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int d2 = 0; d2 < 3; ++d2) acc[d][d2] = 0.0;
+      for (int c = 0; c < NN; ++c) {
+        const int ox = c & 1, oy = (c >> 1) & 1, oz = (c >> 2) & 1;
+        const int fi = 2 * CI + ox, fj = 2 * CJ + oy, fk = 2 * CK + oz;
+        const double xc = xe[((long long)fk * gf.ey + fj) * gf.ex + fi];
+        unsigned mk[NN];
+        unsigned any = 0u;
+#pragma unroll
+        for (int a = 0; a < NN; ++a) {
+          mk[a] = smask[wave][(oz + ((a >> 2) & 1)) * 9 + (oy + ((a >> 1) & 1)) * 3 + ox + (a & 1)];
+          any |= mk[a];
+        }
+        if (!any) {
+          const double* g = sG + c * NE * NE;
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int d2 = 0; d2 < 3; ++d2) acc[d][d2] = fma(xc, g[(A * 3 + d) * NE + B * 3 + d2], acc[d][d2]);
+          continue;
+        }
+        // child with Dirichlet nodes: block (A, B) of P_c^T (x_c M K0 M + D_c) P_c directly
+        for (int a = 0; a < NN; ++a) {
+          const double wa = pweight<3>(c, a, A);
+          if (wa == 0.0) continue;
+          for (int b = 0; b < NN; ++b) {
+            const double wb = pweight<3>(c, b, B);
+            if (wb == 0.0) continue;
+            for (int d = 0; d < 3; ++d)
+              for (int d2 = 0; d2 < 3; ++d2) {
+                const bool fa = (mk[a] >> d) & 1u, fb = (mk[b] >> d2) & 1u;
+                double v = (fa || fb) ? 0.0 : xc * K0[(a * 3 + d) * NE + b * 3 + d2];
+                if (a == b && d == d2 && fa) {
+                  const int ai = fi + (a & 1), aj = fj + ((a >> 1) & 1), ak = fk + ((a >> 2) & 1);
+                  const int mult = ((ai > 0) + (ai < gf.nx - 1)) * ((aj > 0) + (aj < gf.ny - 1)) *
+                                   ((ak > 0) + (ak < gf.nz - 1));
+                  v += bcdiag / (double)mult;
+                }
+                acc[d][d2] = fma(wa * wb, v, acc[d][d2]);
+              }
+          }
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int d2 = 0; d2 < 3; ++d2) Kc[E * NE * NE + (A * 3 + d) * NE + B * 3 + d2] = acc[d][d2];
+    }
+    __syncthreads();  // masks consumed before the next element overwrites them
   }
-#pragma unroll
-  for (int d = 0; d < DIM; ++d)
-#pragma unroll
-    for (int d2 = 0; d2 < DIM; ++d2) Kc[E * NE * NE + (A * DIM + d) * NE + B * DIM + d2] = acc[d][d2];
 }
 
 template <int DIM, bool FINE>
@@ -464,8 +479,9 @@ int launch_galerkin_fine(const GridDims& gf, const double* K0, const double* G, 
                          double bcdiag, double* Kc, hipStream_t s) {
   GridDims gc = make_grid(gf.ex / 2, gf.ey / 2, gf.dim == 3 ? gf.ez / 2 : 0);
   if (G != nullptr && gf.dim == 3) {
-    const long long nthreads = gc.nel * 64;
-    galerkin_fine_fast_kernel<3><<<blocks_for(nthreads, 256), 256, 0, s>>>(gf, gc, G, K0, xe, bcm, bcdiag, Kc);
+    long long blocks = (gc.nel + 3) / 4;
+    if (blocks > 4096) blocks = 4096;  // each block stages G once and walks its share of coarse elements
+    galerkin_fine_fast_kernel<<<(unsigned)blocks, 256, 0, s>>>(gf, gc, G, K0, xe, bcm, bcdiag, Kc);
     return check_launch("galerkin_fine_fast_kernel");
   }
   if (gf.dim == 3)

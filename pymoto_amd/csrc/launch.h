@@ -5,12 +5,15 @@
 
 namespace pmt {
 
-enum { MODE_APPLY = 0, MODE_RESIDUAL = 1, MODE_JACOBI = 2 };
+// MODE_CHEB: y = u + omega D^-1 (b - A u) + beta (u - up), the second-order (Chebyshev) sweep;
+// up may be NULL (= 0) and may alias y (each entry reads its own up before writing it).
+enum { MODE_APPLY = 0, MODE_RESIDUAL = 1, MODE_JACOBI = 2, MODE_CHEB = 3 };
 
 // operator.hip (level kernels exist for T = double and T = float)
 template <typename T>
 int launch_fine(int mode, const GridDims& g, const double* K0, int k0_pattern, const T* xe, const uint8_t* bcm,
-                double bcdiag, const T* u, const T* b, const T* dinv, double omega, T* y, hipStream_t s);
+                double bcdiag, const T* u, const T* b, const T* dinv, double omega, T* y, hipStream_t s,
+                const T* up = nullptr, double beta = 0.0);
 int k0_matches_pattern(int dim, const double* K0_host);
 int set_tuning(int param, int value);
 int get_tuning(int param);
@@ -21,7 +24,7 @@ int launch_fine_diag(const GridDims& g, const double* K0, const double* xe, cons
                      double* diag, int invert, hipStream_t s);
 template <typename T>
 int launch_stencil(int mode, const GridDims& g, const T* S, const T* u, const T* b, const T* dinv, double omega, T* y,
-                   hipStream_t s);
+                   hipStream_t s);  // MODE_APPLY / RESIDUAL / JACOBI
 template <typename T>
 int launch_stencil_dinv(const GridDims& g, const double* S, T* dinv, hipStream_t s);
 // symmetric half stencil (centre + forward blocks) built from the full one, and its operator
@@ -30,7 +33,7 @@ template <typename T>
 int launch_stencil_compress(const GridDims& g, const double* S, T* H, hipStream_t s);
 template <typename T>
 int launch_stencil_half(int mode, const GridDims& g, const T* H, const T* u, const T* b, const T* dinv, double omega,
-                        T* y, hipStream_t s);
+                        T* y, hipStream_t s, const T* up = nullptr, double beta = 0.0);
 template <typename T>
 int launch_jacobi0(long long n, const T* dinv, const T* b, double omega, T* u, hipStream_t s);
 template <typename T>
@@ -41,7 +44,7 @@ int launch_prolong(const GridDims& gf, const GridDims& gc, const T* uc, T* uf, h
 // mfma.hip (hex8 only; BF16 operands, FP32 accumulation and vectors)
 int launch_fine_mfma(int mode, const GridDims& g, const double* K0, const float* xe, const uint8_t* bcm,
                      double bcdiag, const float* u, const float* b, const float* dinv, double omega, float* y,
-                     hipStream_t s);
+                     hipStream_t s, const float* up = nullptr, double beta = 0.0);
 
 // vector.hip
 int launch_dot(long long n, const double* x, const double* y, double* out, double* work, hipStream_t s);

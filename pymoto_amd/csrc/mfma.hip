@@ -46,7 +46,7 @@ __global__ void __launch_bounds__(256)
 fine_mfma_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, const double* __restrict__ K0,
                  const float* __restrict__ xe, const uint8_t* __restrict__ bcm, float bcdiag,
                  const float* __restrict__ u, const float* __restrict__ b, const float* __restrict__ dinv,
-                 float omega, float* __restrict__ y) {
+                 float omega, float* y, const float* up, float beta) {
   constexpr int HX = MTile::HX, HP = MTile::HP, EX = MTile::EX, EP = MTile::EP, ES = MTile::ES;
   constexpr int UL = MTile::UL, XL = MTile::XL;
   __shared__ uint2 su[3][HP];    // BF16 (x, y, z, 0) per node, ring of 3 planes
@@ -173,14 +173,15 @@ fine_mfma_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
     const bool out_plane = k >= kbeg;
     const long long n = ((long long)k * g.ny + j) * g.nx + i;
     unsigned mk = 0;
-    float uo[3] = {0, 0, 0}, bo[3] = {0, 0, 0}, dv[3] = {0, 0, 0};
+    float uo[3] = {0, 0, 0}, bo[3] = {0, 0, 0}, dv[3] = {0, 0, 0}, po[3] = {0, 0, 0};
     if (own && out_plane) {
       mk = bcm[n];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         uo[d] = u[3 * n + d];
         if (MODE != MODE_APPLY) bo[d] = b[3 * n + d];
-        if (MODE == MODE_JACOBI) dv[d] = dinv[3 * n + d];
+        if (MODE == MODE_JACOBI || MODE == MODE_CHEB) dv[d] = dinv[3 * n + d];
+        if (MODE == MODE_CHEB && up) po[d] = up[3 * n + d];
       }
     }
 
@@ -244,8 +245,10 @@ fine_mfma_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
           y[3 * n + d] = ku;
         else if (MODE == MODE_RESIDUAL)
           y[3 * n + d] = bo[d] - ku;
-        else
+        else if (MODE == MODE_JACOBI)
           y[3 * n + d] = uo[d] + omega * dv[d] * (bo[d] - ku);
+        else
+          y[3 * n + d] = uo[d] + omega * dv[d] * (bo[d] - ku) + beta * (uo[d] - po[d]);
       }
     }
     store_u(k + 2);
@@ -273,7 +276,7 @@ static int mfma_planes(int ntiles_xy, int nz, int cus) {
 
 int launch_fine_mfma(int mode, const GridDims& g, const double* K0, const float* xe, const uint8_t* bcm,
                      double bcdiag, const float* u, const float* b, const float* dinv, double omega, float* y,
-                     hipStream_t s) {
+                     hipStream_t s, const float* up, double beta) {
   if (g.dim != 3) {
     set_error("the BF16 matrix-core operator is implemented for hex8 grids only");
     return PMT_ERR_ARG;
@@ -285,13 +288,18 @@ int launch_fine_mfma(int mode, const GridDims& g, const double* K0, const float*
   const int nzc = planes > 0 ? (planes < g.nz ? planes : g.nz) : mfma_planes(nxy, g.nz, device_cu_count());
   const int nwork = nxy * ((g.nz + nzc - 1) / nzc);
   dim3 grid((unsigned)((nwork + 7) & ~7));
-  const float bd = (float)bcdiag, om = (float)omega;
+  const float bd = (float)bcdiag, om = (float)omega, be = (float)beta;
+#define PMT_MFMA(M) \
+  fine_mfma_kernel<M><<<grid, 256, 0, s>>>(g, tx, nxy, nzc, nwork, K0, xe, bcm, bd, u, b, dinv, om, y, up, be)
   if (mode == MODE_APPLY)
-    fine_mfma_kernel<MODE_APPLY><<<grid, 256, 0, s>>>(g, tx, nxy, nzc, nwork, K0, xe, bcm, bd, u, b, dinv, om, y);
+    PMT_MFMA(MODE_APPLY);
   else if (mode == MODE_RESIDUAL)
-    fine_mfma_kernel<MODE_RESIDUAL><<<grid, 256, 0, s>>>(g, tx, nxy, nzc, nwork, K0, xe, bcm, bd, u, b, dinv, om, y);
+    PMT_MFMA(MODE_RESIDUAL);
+  else if (mode == MODE_JACOBI)
+    PMT_MFMA(MODE_JACOBI);
   else
-    fine_mfma_kernel<MODE_JACOBI><<<grid, 256, 0, s>>>(g, tx, nxy, nzc, nwork, K0, xe, bcm, bd, u, b, dinv, om, y);
+    PMT_MFMA(MODE_CHEB);
+#undef PMT_MFMA
   return check_launch("fine_mfma_kernel");
 }
 

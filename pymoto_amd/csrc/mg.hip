@@ -72,24 +72,79 @@ int coarse_solve<float>(pmt_mg* mg, const float* b, float* out, hipStream_t s) {
   return launch_to_float(mg->nc, C.d.u, out, s);
 }
 
+// level operator in `mode`; Jacobi / Chebyshev sweeps use `omega` (and `up`, `beta`)
 template <typename T>
-static int level_apply(pmt_mg* mg, int l, int mode, const T* u, const T* b, T* y, hipStream_t s) {
+static int level_apply(pmt_mg* mg, int l, int mode, const T* u, const T* b, T* y, hipStream_t s, double omega,
+                       const T* up = nullptr, double beta = 0.0) {
   Level& L = mg->L[l];
   LevelBuf<T>& B = L.buf<T>();
   if constexpr (std::is_same<T, float>::value) {
-    if (l == 0 && mg->prec == 16 && mode == MODE_JACOBI)
-      return launch_fine_mfma(mode, L.g, mg->K0, fine_x<T>(mg), mg->bcm, mg->bcdiag, u, b, B.dinv, L.omega, y, s);
+    if (l == 0 && mg->prec == 16 && (mode == MODE_JACOBI || mode == MODE_CHEB))
+      return launch_fine_mfma(mode, L.g, mg->K0, fine_x<T>(mg), mg->bcm, mg->bcdiag, u, b, B.dinv, omega, y, s, up,
+                              beta);
   }
   if (l == 0)
-    return launch_fine<T>(mode, L.g, mg->K0, mg->k0pat, fine_x<T>(mg), mg->bcm, mg->bcdiag, u, b, B.dinv, L.omega, y,
-                          s);
+    return launch_fine<T>(mode, L.g, mg->K0, mg->k0pat, fine_x<T>(mg), mg->bcm, mg->bcdiag, u, b, B.dinv, omega, y,
+                          s, up, beta);
   if (l == mg->nlev - 1) {  // coarsest: only its full FP64 stencil exists (it is solved directly)
     if constexpr (std::is_same<T, double>::value)
-      return launch_stencil<double>(mode, L.g, L.d.S, u, b, B.dinv, L.omega, y, s);
-    else
-      return arg_error("level_apply: the coarsest level has no reduced-precision operator");
+      if (mode != MODE_CHEB) return launch_stencil<double>(mode, L.g, L.d.S, u, b, B.dinv, omega, y, s);
+    return arg_error("level_apply: the coarsest level is solved directly");
   }
-  return launch_stencil_half<T>(mode, L.g, B.H, u, b, B.dinv, L.omega, y, s);
+  return launch_stencil_half<T>(mode, L.g, B.H, u, b, B.dinv, omega, y, s, up, beta);
+}
+
+template <typename T>
+static int level_apply(pmt_mg* mg, int l, int mode, const T* u, const T* b, T* y, hipStream_t s) {
+  return level_apply<T>(mg, l, mode, u, b, y, s, mg->L[l].omega);
+}
+
+// `steps` smoothing sweeps on level l, from zero or from cur.  Damped Jacobi: u += omega D^-1 (b - A u)
+// (iterative.py:21-47, 232-255).  Chebyshev (no reference counterpart): the same sweep count as a
+// degree-`steps` Chebyshev polynomial in D^-1 A on [lo, hi], by the three-term recurrence
+// u_{k+1} = u_k + alpha_k D^-1 (b - A u_k) + beta_k (u_k - u_{k-1}), u_1 = u_0 + D^-1 (b - A u_0) / theta,
+// theta = (hi+lo)/2, delta = (hi-lo)/2, sigma = theta/delta, rho_0 = 1/sigma,
+// rho_k = 1/(2 sigma - rho_{k-1}), alpha_k = 2 rho_k/delta, beta_k = rho_k rho_{k-1}.
+// The ping-pong buffers cur/oth swap after every sweep; a Chebyshev sweep overwrites u_{k-1} in place.
+template <typename T>
+static int smooth(pmt_mg* mg, int l, const T* b, T*& cur, T*& oth, bool from_zero, hipStream_t s) {
+  Level& L = mg->L[l];
+  LevelBuf<T>& B = L.buf<T>();
+  if (mg->smoother == 0) {
+    int it0 = 0;
+    if (from_zero) {
+      PMT_TRY(launch_jacobi0<T>(L.ndof, B.dinv, b, L.omega, cur, s));
+      it0 = 1;
+    }
+    for (int it = it0; it < L.steps; ++it) {
+      PMT_TRY(level_apply<T>(mg, l, MODE_JACOBI, cur, b, oth, s, L.omega));
+      T* tmp = cur;
+      cur = oth;
+      oth = tmp;
+    }
+    return PMT_OK;
+  }
+  const double theta = 0.5 * (L.cheb_hi + L.cheb_lo), delta = 0.5 * (L.cheb_hi - L.cheb_lo);
+  const double sigma = theta / delta;
+  double rho_prev = 1.0 / sigma;
+  if (from_zero) {
+    PMT_TRY(launch_jacobi0<T>(L.ndof, B.dinv, b, 1.0 / theta, cur, s));
+  } else {
+    PMT_TRY(level_apply<T>(mg, l, MODE_JACOBI, cur, b, oth, s, 1.0 / theta));
+    T* tmp = cur;
+    cur = oth;
+    oth = tmp;
+  }
+  for (int it = 1; it < L.steps; ++it) {
+    const double rho = 1.0 / (2.0 * sigma - rho_prev);
+    const T* up = (from_zero && it == 1) ? nullptr : oth;  // u_{k-1} (zero before the first sweep)
+    PMT_TRY(level_apply<T>(mg, l, MODE_CHEB, cur, b, oth, s, 2.0 * rho / delta, up, rho * rho_prev));
+    rho_prev = rho;
+    T* tmp = cur;
+    cur = oth;
+    oth = tmp;
+  }
+  return PMT_OK;
 }
 
 // out = V-cycle(b) on level l (GeometricMultigrid.solve with x0 = None, iterative.py:222-256)
@@ -103,23 +158,12 @@ static int vcycle(pmt_mg* mg, int l, const T* b, T* out, hipStream_t s) {
   // 2*steps-1 out-of-place sweeps follow the first one: start in the buffer that ends in `out`
   T* cur = B.t;
   T* oth = out;
-  PMT_TRY(launch_jacobi0<T>(L.ndof, B.dinv, b, L.omega, cur, s));
-  for (int it = 1; it < L.steps; ++it) {
-    PMT_TRY(level_apply<T>(mg, l, MODE_JACOBI, cur, b, oth, s));
-    T* tmp = cur;
-    cur = oth;
-    oth = tmp;
-  }
+  PMT_TRY(smooth<T>(mg, l, b, cur, oth, true, s));
   PMT_TRY(level_apply<T>(mg, l, MODE_RESIDUAL, cur, b, B.r, s));
   PMT_TRY(launch_restrict<T>(L.g, C.g, B.r, CB.b, s));
   PMT_TRY(vcycle<T>(mg, l + 1, CB.b, CB.u, s));
   PMT_TRY(launch_prolong<T>(L.g, C.g, CB.u, cur, s));
-  for (int it = 0; it < L.steps; ++it) {
-    PMT_TRY(level_apply<T>(mg, l, MODE_JACOBI, cur, b, oth, s));
-    T* tmp = cur;
-    cur = oth;
-    oth = tmp;
-  }
+  PMT_TRY(smooth<T>(mg, l, b, cur, oth, false, s));
   if (cur != out) return arg_error("vcycle: internal buffer parity error");
   return PMT_OK;
 }
@@ -166,6 +210,80 @@ static int alloc_fp32(pmt_mg* mg) {
   return rc;
 }
 
+__global__ void hash_fill_kernel(long long n, double* __restrict__ v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    unsigned h = (unsigned)(i * 2654435761ull) ^ 0x9e3779b9u;
+    h ^= h >> 15;
+    h *= 0x2c1b3c6du;
+    h ^= h >> 12;
+    v[i] = (double)(h & 0xffffffu) / 16777216.0 - 0.5;
+  }
+}
+
+// y = a * x (mode 0) or y = x / a (mode 1), elementwise
+__global__ void ew_kernel(long long n, int mode, const double* __restrict__ a, const double* __restrict__ x,
+                          double* __restrict__ y) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = mode ? x[i] / a[i] : a[i] * x[i];
+}
+
+// lambda_max(D^-1 A_l) by power iteration with the Rayleigh quotient (v.Av)/(v.Dv); levels >= 1
+static int estimate_lmax(pmt_mg* mg, int l, int iters, hipStream_t s, double* out) {
+  Level& L = mg->L[l];
+  LevelBuf<double>& B = L.d;
+  const long long n = L.ndof;
+  double* v = B.t;
+  double* w = B.r;
+  double* z = B.b;
+  hash_fill_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, v);
+  PMT_TRY(check_launch("hash_fill_kernel"));
+  double lam = 0.0;
+  for (int it = 0; it < iters; ++it) {
+    PMT_TRY(level_apply<double>(mg, l, MODE_APPLY, v, nullptr, w, s));
+    ew_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, 1, B.dinv, v, z);  // z = D v
+    PMT_TRY(check_launch("ew_kernel"));
+    PMT_TRY(launch_dot(n, v, w, mg->sc + 6, mg->work, s));
+    PMT_TRY(launch_dot(n, v, z, mg->sc + 7, mg->work, s));
+    ew_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, 0, B.dinv, w, z);  // z = D^-1 A v
+    PMT_TRY(check_launch("ew_kernel"));
+    PMT_TRY(launch_dot(n, z, z, mg->sc + 8, mg->work, s));
+    PMT_HIP(hipMemcpyAsync(mg->host_sc + 6, mg->sc + 6, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+    PMT_HIP(hipStreamSynchronize(s));
+    lam = mg->host_sc[6] / mg->host_sc[7];
+    const double nz = std::sqrt(mg->host_sc[8]);
+    if (!(nz > 0.0)) break;
+    PMT_TRY(launch_axpby(n, 1.0 / nz, z, 0.0, v, s));
+  }
+  *out = lam;
+  return PMT_OK;
+}
+
+// lambda_max(diag(K0)^-1 K0) on the host (power iteration on the symmetric scaling)
+static double element_lmax(int ne, const double* K0) {
+  std::vector<double> d(ne), v(ne, 1.0), w(ne);
+  for (int i = 0; i < ne; ++i) d[i] = 1.0 / std::sqrt(K0[i * ne + i]);
+  for (int i = 0; i < ne; ++i) v[i] = 1.0 + 0.01 * i;
+  double lam = 0.0;
+  for (int it = 0; it < 2000; ++it) {
+    double nn = 0.0, vw = 0.0, vv = 0.0;
+    for (int i = 0; i < ne; ++i) {
+      double acc = 0.0;
+      for (int j = 0; j < ne; ++j) acc += d[i] * K0[i * ne + j] * d[j] * v[j];
+      w[i] = acc;
+    }
+    for (int i = 0; i < ne; ++i) {
+      vw += v[i] * w[i];
+      vv += v[i] * v[i];
+      nn += w[i] * w[i];
+    }
+    lam = vw / vv;
+    nn = std::sqrt(nn);
+    for (int i = 0; i < ne; ++i) v[i] = w[i] / nn;
+  }
+  return lam;
+}
+
 int mg_level_apply(pmt_mg* mg, int l, int mode, const double* u, const double* b, double* y, hipStream_t s) {
   return level_apply<double>(mg, l, mode, u, b, y, s);
 }
@@ -196,6 +314,7 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
   mg->nlev = nlevels;
   mg->NE = (1 << mg->dim) * mg->dim;
   mg->k0pat = k0_matches_pattern(mg->dim, K0_host);
+  mg->lmax_elem = element_lmax(mg->NE, K0_host);
   int ex = nelx, ey = nely, ez = nelz;
   for (int l = 0; l < nlevels; ++l) {
     Level& L = mg->L[l];
@@ -300,6 +419,24 @@ int pmt_mg_set_precision(pmt_mg* mg, int bits) {
   return PMT_OK;
 }
 
+int pmt_mg_set_smoother(pmt_mg* mg, int type, double lower, double upper) {
+  if (!mg || (type != 0 && type != 1) || !(lower > 0.0 && lower < 1.0) || !(upper >= 1.0))
+    return arg_error("pmt_mg_set_smoother: type 0 (damped Jacobi) or 1 (Chebyshev), 0 < lower < 1 <= upper");
+  mg->smoother = type;
+  mg->cheb_lower = lower;
+  mg->cheb_upper = upper;
+  mg->ready = false;
+  block_cg_invalidate(mg);
+  return PMT_OK;
+}
+
+int pmt_mg_smoother_interval(pmt_mg* mg, int level, double* lo, double* hi) {
+  if (!mg || level < 0 || level >= mg->nlev - 1 || !lo || !hi) return arg_error("pmt_mg_smoother_interval: bad argument");
+  *lo = mg->L[level].cheb_lo;
+  *hi = mg->L[level].cheb_hi;
+  return PMT_OK;
+}
+
 int pmt_mg_update(pmt_mg* mg, const double* xe, const uint8_t* bcmask, double bcdiag, void* stream) {
   if (!mg || !xe || !bcmask) return arg_error("pmt_mg_update: bad argument (the Dirichlet mask is mandatory)");
   hipStream_t s = (hipStream_t)stream;
@@ -326,6 +463,15 @@ int pmt_mg_update(pmt_mg* mg, const double* xe, const uint8_t* bcmask, double bc
   Level& Cl = mg->L[mg->nlev - 1];
   PMT_TRY(launch_dense_from_stencil(Cl.g, Cl.d.S, mg->Ainv, s));
   PMT_TRY(gj_invert(mg->nc, mg->Ainv, mg->gjP, mg->gjC, s));
+  if (mg->smoother == 1) {
+    for (int l = 0; l < mg->nlev - 1; ++l) {
+      double lam = mg->lmax_elem;  // the fine grid: lambda_max(D^-1 K) <= lambda_max(D0^-1 K0)
+      if (l > 0) PMT_TRY(estimate_lmax(mg, l, 8, s, &lam));
+      Level& L = mg->L[l];
+      L.cheb_hi = (l > 0 ? mg->cheb_upper : 1.0) * lam;
+      L.cheb_lo = mg->cheb_lower * L.cheb_hi;
+    }
+  }
   if (mg->prec != 64) {
     PMT_TRY(launch_to_float(F.g.nel, xe, mg->xe32, s));
     PMT_TRY(launch_to_float(F.ndof, F.d.dinv, F.f.dinv, s));

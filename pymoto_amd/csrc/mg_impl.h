@@ -26,6 +26,7 @@ struct Level {
   long long ndof = 0;
   double omega = 0.5;
   int steps = 1;
+  double cheb_lo = 0.0, cheb_hi = 0.0;  // Chebyshev interval of D^-1 A (set at update)
   double* Ke = nullptr;  // Galerkin element matrices (levels >= 1, FP64)
   LevelBuf<double> d;
   LevelBuf<float> f;
@@ -47,6 +48,10 @@ struct pmt_mg {
   int dim = 3;
   int nlev = 0;
   int prec = 64;
+  int smoother = 0;            // 0 damped Jacobi, 1 Chebyshev
+  double cheb_lower = 0.25;    // lo = cheb_lower * hi
+  double cheb_upper = 1.1;     // hi = cheb_upper * lambda_max estimate
+  double lmax_elem = 0.0;      // lambda_max(diag(K0)^-1 K0): bounds lambda_max(D^-1 K) of the fine grid
   pmt::Level L[pmt::kMaxLevels];
   int NE = 24;
   double* K0 = nullptr;

@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(256)
 fine_node_kernel(GridDims g, int tiles_x, int tiles_y, int ntiles, const double* __restrict__ K0,
                  const T* __restrict__ xe, const uint8_t* __restrict__ bcm, T bcdiag,
                  const T* __restrict__ u, const T* __restrict__ b, const T* __restrict__ dinv,
-                 T omega, T* __restrict__ y) {
+                 T omega, T* y, const T* up, T beta) {
   constexpr int NN = Cfg<DIM>::NN;
   constexpr int NE = Cfg<DIM>::NE;
   constexpr int TX = Tile<DIM>::X, TY = Tile<DIM>::Y, TZ = Tile<DIM>::Z;
@@ -187,8 +187,11 @@ fine_node_kernel(GridDims g, int tiles_x, int tiles_y, int ntiles, const double*
       y[q] = ku;
     } else if (MODE == MODE_RESIDUAL) {
       y[q] = b[q] - ku;
-    } else {
+    } else if (MODE == MODE_JACOBI) {
       y[q] = u[q] + omega * dinv[q] * (b[q] - ku);
+    } else {
+      const T uq = u[q], pq = up ? up[q] : (T)0;
+      y[q] = uq + omega * dinv[q] * (b[q] - ku) + beta * (uq - pq);
     }
   }
 }
@@ -215,7 +218,7 @@ template <class Stream, int MODE, bool CLS, typename T>
 __global__ void __launch_bounds__(256)
 fine_stream_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, const double* __restrict__ K0,
                    const T* __restrict__ xe, const uint8_t* __restrict__ bcm, T bcdiag, const T* __restrict__ u,
-                   const T* __restrict__ b, const T* __restrict__ dinv, T omega, T* __restrict__ y) {
+                   const T* __restrict__ b, const T* __restrict__ dinv, T omega, T* y, const T* up, T beta) {
   constexpr int HX = Stream::HX, HP = Stream::HP, EX = Stream::EX, EP = Stream::EP;
   constexpr int UL = Stream::UL, XL = Stream::XL;
   __shared__ T su[4][3][HP];
@@ -320,14 +323,15 @@ fine_stream_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, c
     // own-node operands of the update (also in flight)
     const long long n = ((long long)k * g.ny + j) * g.nx + i;
     unsigned mk = 0;
-    T uo[3] = {0, 0, 0}, bo[3] = {0, 0, 0}, dv[3] = {0, 0, 0};
+    T uo[3] = {0, 0, 0}, bo[3] = {0, 0, 0}, dv[3] = {0, 0, 0}, po[3] = {0, 0, 0};
     if (own) {
       mk = bcm[n];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         uo[d] = u[3 * n + d];
         if (MODE != MODE_APPLY) bo[d] = b[3 * n + d];
-        if (MODE == MODE_JACOBI) dv[d] = dinv[3 * n + d];
+        if (MODE == MODE_JACOBI || MODE == MODE_CHEB) dv[d] = dinv[3 * n + d];
+        if (MODE == MODE_CHEB && up) po[d] = up[3 * n + d];
       }
     }
 
@@ -386,8 +390,10 @@ fine_stream_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, c
           y[3 * n + d] = ku;
         else if (MODE == MODE_RESIDUAL)
           y[3 * n + d] = bo[d] - ku;
-        else
+        else if (MODE == MODE_JACOBI)
           y[3 * n + d] = uo[d] + omega * dv[d] * (bo[d] - ku);
+        else
+          y[3 * n + d] = uo[d] + omega * dv[d] * (bo[d] - ku) + beta * (uo[d] - po[d]);
       }
     }
     store_u(k + 2);
@@ -506,7 +512,7 @@ __global__ void stencil_compress_kernel(GridDims g, const double* __restrict__ S
 template <int DIM, int MODE, typename T>
 __global__ void __launch_bounds__(256)
 stencil_half_kernel(GridDims g, const T* __restrict__ H, const T* __restrict__ u, const T* __restrict__ b,
-                    const T* __restrict__ dinv, T omega, T* __restrict__ y) {
+                    const T* __restrict__ dinv, T omega, T* y, const T* up, T beta) {
   const long long n = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (n >= g.nnodes) return;
   const int i = (int)(n % g.nx);
@@ -555,8 +561,11 @@ stencil_half_kernel(GridDims g, const T* __restrict__ H, const T* __restrict__ u
       y[q] = out[d];
     } else if (MODE == MODE_RESIDUAL) {
       y[q] = b[q] - out[d];
-    } else {
+    } else if (MODE == MODE_JACOBI) {
       y[q] = u[q] + omega * dinv[q] * (b[q] - out[d]);
+    } else {
+      const T uq = u[q], pq = up ? up[q] : (T)0;
+      y[q] = uq + omega * dinv[q] * (b[q] - out[d]) + beta * (uq - pq);
     }
   }
 }
@@ -677,21 +686,23 @@ sens_kernel(GridDims g, const double* __restrict__ K0, const uint8_t* __restrict
 // ---------------------------------------------------------------------------------------------
 template <int DIM, bool CLS, typename T>
 static void launch_fine_t(int mode, const GridDims& g, const double* K0, const T* xe, const uint8_t* bcm, T bcdiag,
-                          const T* u, const T* b, const T* dinv, T omega, T* y, hipStream_t s) {
+                          const T* u, const T* b, const T* dinv, T omega, T* y, hipStream_t s, const T* up, T beta) {
   const int tx = (g.nx + Tile<DIM>::X - 1) / Tile<DIM>::X;
   const int ty = (g.ny + Tile<DIM>::Y - 1) / Tile<DIM>::Y;
   const int tz = DIM == 3 ? (g.nz + Tile<DIM>::Z - 1) / Tile<DIM>::Z : 1;
   const int nt = tx * ty * tz;
   dim3 grid((unsigned)((nt + 7) & ~7));
+#define PMT_FINE_T(M) \
+  fine_node_kernel<DIM, M, CLS, T><<<grid, 256, 0, s>>>(g, tx, ty, nt, K0, xe, bcm, bcdiag, u, b, dinv, omega, y, up, beta)
   if (mode == MODE_APPLY)
-    fine_node_kernel<DIM, MODE_APPLY, CLS, T><<<grid, 256, 0, s>>>(g, tx, ty, nt, K0, xe, bcm, bcdiag, u, b, dinv,
-                                                                   omega, y);
+    PMT_FINE_T(MODE_APPLY);
   else if (mode == MODE_RESIDUAL)
-    fine_node_kernel<DIM, MODE_RESIDUAL, CLS, T><<<grid, 256, 0, s>>>(g, tx, ty, nt, K0, xe, bcm, bcdiag, u, b, dinv,
-                                                                      omega, y);
+    PMT_FINE_T(MODE_RESIDUAL);
+  else if (mode == MODE_JACOBI)
+    PMT_FINE_T(MODE_JACOBI);
   else
-    fine_node_kernel<DIM, MODE_JACOBI, CLS, T><<<grid, 256, 0, s>>>(g, tx, ty, nt, K0, xe, bcm, bcdiag, u, b, dinv,
-                                                                    omega, y);
+    PMT_FINE_T(MODE_CHEB);
+#undef PMT_FINE_T
 }
 
 // Launch-shape tuning of the fine operator (set by pmt_set_tuning; read at every launch).
@@ -763,54 +774,59 @@ static int stream_planes(int ntiles_xy, int nz) {
 
 template <class Stream, bool CLS, typename T>
 static void launch_stream_shape(int mode, const GridDims& g, const double* K0, const T* xe, const uint8_t* bcm,
-                                T bcdiag, const T* u, const T* b, const T* dinv, T omega, T* y, hipStream_t s) {
+                                T bcdiag, const T* u, const T* b, const T* dinv, T omega, T* y, hipStream_t s,
+                                const T* up, T beta) {
   const int tx = (g.nx + Stream::SX - 1) / Stream::SX;
   const int ty = (g.ny + Stream::SY - 1) / Stream::SY;
   const int nxy = tx * ty;
   const int nzc = stream_planes(nxy, g.nz);
   const int nwork = nxy * ((g.nz + nzc - 1) / nzc);
   dim3 grid((unsigned)((nwork + 7) & ~7));
+#define PMT_STREAM(M)                                                                                             \
+  fine_stream_kernel<Stream, M, CLS, T><<<grid, 256, 0, s>>>(g, tx, nxy, nzc, nwork, K0, xe, bcm, bcdiag, u, b, \
+                                                             dinv, omega, y, up, beta)
   if (mode == MODE_APPLY)
-    fine_stream_kernel<Stream, MODE_APPLY, CLS, T><<<grid, 256, 0, s>>>(g, tx, nxy, nzc, nwork, K0, xe, bcm, bcdiag,
-                                                                        u, b, dinv, omega, y);
+    PMT_STREAM(MODE_APPLY);
   else if (mode == MODE_RESIDUAL)
-    fine_stream_kernel<Stream, MODE_RESIDUAL, CLS, T><<<grid, 256, 0, s>>>(g, tx, nxy, nzc, nwork, K0, xe, bcm,
-                                                                           bcdiag, u, b, dinv, omega, y);
+    PMT_STREAM(MODE_RESIDUAL);
+  else if (mode == MODE_JACOBI)
+    PMT_STREAM(MODE_JACOBI);
   else
-    fine_stream_kernel<Stream, MODE_JACOBI, CLS, T><<<grid, 256, 0, s>>>(g, tx, nxy, nzc, nwork, K0, xe, bcm, bcdiag,
-                                                                         u, b, dinv, omega, y);
+    PMT_STREAM(MODE_CHEB);
+#undef PMT_STREAM
 }
 
 template <bool CLS, typename T>
 static void launch_stream_t(int mode, const GridDims& g, const double* K0, const T* xe, const uint8_t* bcm, T bcdiag,
-                            const T* u, const T* b, const T* dinv, T omega, T* y, hipStream_t s) {
+                            const T* u, const T* b, const T* dinv, T omega, T* y, hipStream_t s, const T* up, T beta) {
   if (g_stream_tile == 1)
-    launch_stream_shape<StreamTile<64, 4>, CLS, T>(mode, g, K0, xe, bcm, bcdiag, u, b, dinv, omega, y, s);
+    launch_stream_shape<StreamTile<64, 4>, CLS, T>(mode, g, K0, xe, bcm, bcdiag, u, b, dinv, omega, y, s, up, beta);
   else if (g_stream_tile == 2)
-    launch_stream_shape<StreamTile<16, 16>, CLS, T>(mode, g, K0, xe, bcm, bcdiag, u, b, dinv, omega, y, s);
+    launch_stream_shape<StreamTile<16, 16>, CLS, T>(mode, g, K0, xe, bcm, bcdiag, u, b, dinv, omega, y, s, up, beta);
   else
-    launch_stream_shape<StreamTile<32, 8>, CLS, T>(mode, g, K0, xe, bcm, bcdiag, u, b, dinv, omega, y, s);
+    launch_stream_shape<StreamTile<32, 8>, CLS, T>(mode, g, K0, xe, bcm, bcdiag, u, b, dinv, omega, y, s, up, beta);
 }
 
 template <typename T>
 int launch_fine(int mode, const GridDims& g, const double* K0, int k0_pattern, const T* xe, const uint8_t* bcm,
-                double bcdiag, const T* u, const T* b, const T* dinv, double omega, T* y, hipStream_t s) {
-  const T bd = (T)bcdiag, om = (T)omega;
+                double bcdiag, const T* u, const T* b, const T* dinv, double omega, T* y, hipStream_t s, const T* up,
+                double beta) {
+  const T bd = (T)bcdiag, om = (T)omega, be = (T)beta;
   if (g.dim == 3 && g_fine_kernel == 1) {
     if (k0_pattern)
-      launch_stream_t<true, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s);
+      launch_stream_t<true, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
     else
-      launch_stream_t<false, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s);
+      launch_stream_t<false, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
   } else if (g.dim == 3) {
     if (k0_pattern)
-      launch_fine_t<3, true, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s);
+      launch_fine_t<3, true, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
     else
-      launch_fine_t<3, false, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s);
+      launch_fine_t<3, false, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
   } else {
     if (k0_pattern)
-      launch_fine_t<2, true, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s);
+      launch_fine_t<2, true, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
     else
-      launch_fine_t<2, false, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s);
+      launch_fine_t<2, false, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
   }
   return check_launch("fine_node_kernel");
 }
@@ -855,24 +871,30 @@ int launch_stencil_compress(const GridDims& g, const double* S, T* H, hipStream_
 
 template <typename T>
 int launch_stencil_half(int mode, const GridDims& g, const T* H, const T* u, const T* b, const T* dinv, double omega,
-                        T* y, hipStream_t s) {
+                        T* y, hipStream_t s, const T* up, double beta) {
   dim3 grid(xcd_blocks(g.nnodes, 256));
-  const T om = (T)omega;
+  const T om = (T)omega, be = (T)beta;
+#define PMT_HALF(D, M) stencil_half_kernel<D, M, T><<<grid, 256, 0, s>>>(g, H, u, b, dinv, om, y, up, be)
   if (g.dim == 3) {
     if (mode == MODE_APPLY)
-      stencil_half_kernel<3, MODE_APPLY, T><<<grid, 256, 0, s>>>(g, H, u, b, dinv, om, y);
+      PMT_HALF(3, MODE_APPLY);
     else if (mode == MODE_RESIDUAL)
-      stencil_half_kernel<3, MODE_RESIDUAL, T><<<grid, 256, 0, s>>>(g, H, u, b, dinv, om, y);
+      PMT_HALF(3, MODE_RESIDUAL);
+    else if (mode == MODE_JACOBI)
+      PMT_HALF(3, MODE_JACOBI);
     else
-      stencil_half_kernel<3, MODE_JACOBI, T><<<grid, 256, 0, s>>>(g, H, u, b, dinv, om, y);
+      PMT_HALF(3, MODE_CHEB);
   } else {
     if (mode == MODE_APPLY)
-      stencil_half_kernel<2, MODE_APPLY, T><<<grid, 256, 0, s>>>(g, H, u, b, dinv, om, y);
+      PMT_HALF(2, MODE_APPLY);
     else if (mode == MODE_RESIDUAL)
-      stencil_half_kernel<2, MODE_RESIDUAL, T><<<grid, 256, 0, s>>>(g, H, u, b, dinv, om, y);
+      PMT_HALF(2, MODE_RESIDUAL);
+    else if (mode == MODE_JACOBI)
+      PMT_HALF(2, MODE_JACOBI);
     else
-      stencil_half_kernel<2, MODE_JACOBI, T><<<grid, 256, 0, s>>>(g, H, u, b, dinv, om, y);
+      PMT_HALF(2, MODE_CHEB);
   }
+#undef PMT_HALF
   return check_launch("stencil_half_kernel");
 }
 
@@ -937,13 +959,13 @@ int launch_prolong(const GridDims& gf, const GridDims& gc, const T* uc, T* uf, h
 
 #define PMT_INSTANTIATE_LEVEL(T)                                                                                   \
   template int launch_fine<T>(int, const GridDims&, const double*, int, const T*, const uint8_t*, double, const T*, \
-                              const T*, const T*, double, T*, hipStream_t);                                        \
+                              const T*, const T*, double, T*, hipStream_t, const T*, double);                      \
   template int launch_stencil<T>(int, const GridDims&, const T*, const T*, const T*, const T*, double, T*,         \
                                  hipStream_t);                                                                     \
   template int launch_stencil_dinv<T>(const GridDims&, const double*, T*, hipStream_t);                            \
   template int launch_stencil_compress<T>(const GridDims&, const double*, T*, hipStream_t);                        \
   template int launch_stencil_half<T>(int, const GridDims&, const T*, const T*, const T*, const T*, double, T*,    \
-                                      hipStream_t);                                                                \
+                                      hipStream_t, const T*, double);                                              \
   template int launch_jacobi0<T>(long long, const T*, const T*, double, T*, hipStream_t);                          \
   template int launch_restrict<T>(const GridDims&, const GridDims&, const T*, T*, hipStream_t);                    \
   template int launch_prolong<T>(const GridDims&, const GridDims&, const T*, T*, hipStream_t);

@@ -53,6 +53,22 @@ class DampedJacobi(Preconditioner):
                                   "operator kernels; use it as GeometricMultigrid(smoother=DampedJacobi(w=...))")
 
 
+class ChebyshevSmoother:
+    """Chebyshev polynomial smoother in D^-1 A for the device multigrid (no reference counterpart).
+
+    Each level applies a polynomial of degree ``smooth_steps`` (as many operator sweeps as the damped
+    Jacobi smoother it replaces) on the interval [lower*hi, hi]: on the fine grid hi is
+    lambda_max(diag(K0)^-1 K0), an upper bound of lambda_max(D^-1 K) for any positive densities; on the
+    Galerkin levels hi = upper * (power-iteration estimate).  Unlike a fixed omega it cannot exceed the
+    Jacobi stability bound of a heterogeneous SIMP operator."""
+
+    def __init__(self, lower: float = 0.25, upper: float = 1.1):
+        self.lower, self.upper = float(lower), float(upper)
+
+    def update(self, A):
+        pass
+
+
 class GeometricMultigrid(Preconditioner):
     """Geometric multigrid V-cycle preconditioner (``iterative.py:124-256``)."""
 
@@ -104,10 +120,11 @@ class GeometricMultigrid(Preconditioner):
     def _ensure_handle(self, A):
         chain = self.levels()
         omega = np.array([getattr(m.smoother, "w", 0.5) for m in chain], dtype=np.float64)
+        cheb = isinstance(self.smoother, ChebyshevSmoother)
         steps = np.array([m.smooth_steps for m in chain], dtype=np.int32)
         d = self.domain
         key = (d.nelx, d.nely, d.nelz, len(chain) + 1, tuple(omega), tuple(steps), A.grid.K0.tobytes(),
-               self.precision)
+               self.precision, (self.smoother.lower, self.smoother.upper) if cheb else None)
         if self._handle is not None and self._key == key:
             return
         self._release()
@@ -120,6 +137,8 @@ class GeometricMultigrid(Preconditioner):
         self._handle, self._key = h, key
         if self.precision != "fp64":
             _lib.check(lib.pmt_mg_set_precision(h, self._precision_bits[self.precision]), "multigrid precision")
+        if cheb:
+            _lib.check(lib.pmt_mg_set_smoother(h, 1, self.smoother.lower, self.smoother.upper), "multigrid smoother")
 
     def _release(self):
         if self._handle is not None:
@@ -160,6 +179,13 @@ class GeometricMultigrid(Preconditioner):
             return one(rhs)
         return torch.stack([one(rhs[:, k].contiguous()) for k in range(rhs.shape[1])], dim=1)
 
+    def smoother_interval(self, level):
+        """(lo, hi) of the Chebyshev smoother on a level (after update)."""
+        lo, hi = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        _lib.check(_lib.load().pmt_mg_smoother_interval(self._handle, int(level), ctypes.byref(lo), ctypes.byref(hi)),
+                   "smoother interval")
+        return lo.value, hi.value
+
     def level_apply(self, level, u):
         """y = A_level u with the Galerkin operator of that level (level 0 = fine operator)."""
         u = _lib.as_device(u)
@@ -185,12 +211,16 @@ def auto_levels(domain: VoxelDomain) -> int:
 
 
 def build_hierarchy(domain: VoxelDomain, levels: int, omega=0.5, smooth_steps=5, precision="fp64"):
-    """Nested GeometricMultigrid objects with ``levels`` grids (fine grid included)."""
+    """Nested GeometricMultigrid objects with ``levels`` grids (fine grid included); omega is the damped
+    Jacobi factor, or "chebyshev" for the Chebyshev smoother on every level."""
     if levels < 2:
         raise ValueError("a multigrid hierarchy needs at least two grids")
-    mgs = [GeometricMultigrid(domain, smoother=DampedJacobi(w=omega), smooth_steps=smooth_steps, precision=precision)]
+
+    def smoother():
+        return ChebyshevSmoother() if omega == "chebyshev" else DampedJacobi(w=omega)
+    mgs = [GeometricMultigrid(domain, smoother=smoother(), smooth_steps=smooth_steps, precision=precision)]
     for _ in range(levels - 2):
-        mgs.append(GeometricMultigrid(mgs[-1].sub_domain, smoother=DampedJacobi(w=omega), smooth_steps=smooth_steps))
+        mgs.append(GeometricMultigrid(mgs[-1].sub_domain, smoother=smoother(), smooth_steps=smooth_steps))
         mgs[-2].inner_level = mgs[-1]
     return mgs[0]
 

@@ -428,10 +428,11 @@ def test_block_pcg_vs_oracle(gpu_lib, graph):
                        graph=graph)
     cg.update(K)
     X = host(cg.solve(dev(B)))
-    # Block CG's iteration count depends on rounding far more than single-vector CG's: perturbing the
-    # oracle's orth() inputs by 1e-15 (relative) moves it between 55 and 59 on this problem (53
-    # unperturbed), so the count is compared to 10%; the solution and residual are compared tightly.
-    assert abs(cg.iterations - ocg.iterations) <= max(2, ocg.iterations // 10)
+    # Block CG's iteration count on this problem is chaotic in the rounding, in the reference's own
+    # algorithm: perturbing the oracle's K by 1e-15 (relative, symmetric) gives 53, 121, 54, 53, 57,
+    # 56, 55, 55 iterations for 8 seeds.  So the count is only bounded; the solution and residual are
+    # compared tightly.
+    assert cg.iterations <= 3 * ocg.iterations
     assert relerr(X, Xref) < 1e-8
     R = B - Kref @ X
     assert (np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)).max() <= 1e-10
@@ -538,3 +539,49 @@ def test_density_filter_vs_oracle(gpu_lib, nel, radius, nonpad):
     sy.sensitivity = dev(dy)
     m.sensitivity()
     npt.assert_allclose(host(sx.sensitivity), ref.adjoint(dy), rtol=1e-12, atol=1e-13)
+
+
+# ---------------------------------------------------------------- Chebyshev smoother
+@pytest.mark.parametrize("levels", [3, 4])
+def test_chebyshev_vcycle_pcg_vs_reference(gpu_lib, levels):
+    """Chebyshev-smoothed V-cycle (no reference counterpart): the FP64 PCG reaches the reference's
+    tolerance on the golden 32x16x16 problem with compliance and dc/dx of the reference's run, and
+    needs fewer iterations than the reference's damped Jacobi (omega 0.6)."""
+    g = golden(f"c2_mgcg_32x16x16_L{levels}.npz")
+    nel = tuple(int(v) for v in g["nel"])
+    d = pm.VoxelDomain(*nel)
+    for precision in ("fp64", "fp32"):
+        solver = pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(d, levels, omega="chebyshev",
+                                                                         smooth_steps=int(g["steps"]),
+                                                                         precision=precision),
+                               tol=float(g["tol"]))
+        out = _device_chain(nel, g["x"], g["bc"], g["f"], 1.5, solver=solver)
+        assert abs(out["c"] - float(g["c"])) / abs(float(g["c"])) < 1e-7
+        assert relerr(out["dcdx"], g["dcdx"]) < 1e-6
+        assert solver.iterations < int(g["iterations"])
+
+
+def test_chebyshev_intervals_cover_the_spectrum(gpu_lib):
+    """Fine grid: hi = lambda_max(diag(K0)^-1 K0) bounds lambda_max(D^-1 K) (checked with eigsh on the
+    oracle's K); level 1: the power-iteration estimate is within 10% below the true lambda_max of the
+    oracle's Galerkin operator, and hi = 1.1 x estimate covers it."""
+    from scipy.sparse.linalg import eigsh
+    import scipy.sparse as sps
+    nel = (16, 8, 8)
+    rng = np.random.default_rng(8)
+    x = fe.simp(rng.uniform(0.05, 1.0, 16 * 64))
+    bc = problems.cantilever3d(*nel)[0]
+    Kref, omg, K, _ = _hierarchies(nel, 3, x, bc)
+    d = pm.VoxelDomain(*nel)
+    mg = pm.solvers.build_hierarchy(d, 3, omega="chebyshev", smooth_steps=2)
+    mg.update(K)
+
+    def lmax(A):
+        A = sps.csr_matrix(A)
+        s = 1.0 / np.sqrt(A.diagonal())
+        return eigsh(sps.diags(s) @ A @ sps.diags(s), k=1, which="LA")[0][0]
+    lo0, hi0 = mg.smoother_interval(0)
+    assert hi0 >= lmax(Kref) and abs(lo0 - 0.25 * hi0) < 1e-12 * hi0
+    lo1, hi1 = mg.smoother_interval(1)
+    l1 = lmax(omg.Ac)
+    assert 0.9 * l1 <= hi1 / 1.1 <= l1 * (1 + 1e-9) and hi1 >= l1
