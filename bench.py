@@ -123,30 +123,58 @@ def design_iteration(net, x):
     return net["sc"].state
 
 
-def kernel_roofline(pm, torch, net, reps=20):
-    """Average duration of the fine-grid K.u kernel (HIP events on its launch stream) and its roofline."""
-    from pymoto_amd import _lib
-    K = net["sK"].state
+def kernel_roofline(pm, torch, nel, reps=20):
+    """Average duration of the fine-grid K.u kernel (FP64, and the FP32 product of the V-cycle) from HIP
+    events on the stream it is launched on, and the HBM roofline of its compulsory bytes: u read and
+    y written (3 dofs per node) plus one density per element (the Dirichlet mask byte per node not
+    counted).  The element product runs in the reflection basis (~0.24 kflop per element), so the
+    kernel's roof is HBM, not the FP64 FMA rate."""
+    import json
+    from pymoto_amd import _lib, problems
+    d = pm.VoxelDomain(*nel)
+    bc, _ = problems.cantilever3d(d)
+    x = torch.as_tensor(np.random.default_rng(3).uniform(1e-3, 1.0, d.nel), device="cuda")
+    K = pm.AssembleStiffness(d, bc=bc)(x)
     g = K.grid
     u = torch.randn(g.n, dtype=torch.float64, device="cuda")
     y = torch.empty_like(u)
     lib = _lib.lib()
+    stream = torch.cuda.current_stream()
     args = (*g.nel3, _lib.ptr(g.K0_dev), g.k0_pattern, _lib.ptr(K.xe), _lib.ptr(g.bcmask_dev), g.bcdiag, _lib.ptr(u),
-            _lib.ptr(y))
-    for _ in range(3):
-        _lib.check(lib.pmt_stiffness_apply(*args, _lib.stream()))
-    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    s.record()
-    for _ in range(reps):
-        _lib.check(lib.pmt_stiffness_apply(*args, _lib.stream()))
-    e.record()
-    torch.cuda.synchronize()
-    t = s.elapsed_time(e) / reps * 1e-3
-    nel, ndof = net["domain"].nel, g.n
-    flops = 1152.0 * nel
-    bytes_ = 2.0 * ndof * 8 + nel * 8
-    return dict(kernel="fine_node_kernel<3, apply, pattern, double> (K.u, FP64)", seconds=t, flops=flops,
-                bytes=bytes_, tflops=flops / t / 1e12, gbs=bytes_ / t / 1e9)
+            _lib.ptr(y), _lib.stream())
+    x32, u32 = K.xe.float(), u.float()
+    y32 = torch.empty_like(u32)
+    args32 = (*g.nel3, _lib.ptr(g.K0_dev), g.k0_pattern, _lib.ptr(x32), _lib.ptr(g.bcmask_dev), g.bcdiag,
+              _lib.ptr(u32), _lib.ptr(y32), _lib.stream())
+
+    def timed(fn, a):
+        for _ in range(3):
+            _lib.check(fn(*a))
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record(stream)
+        for _ in range(reps):
+            _lib.check(fn(*a))
+        e.record(stream)
+        torch.cuda.synchronize()
+        return s.elapsed_time(e) / reps * 1e-3
+
+    t64 = timed(lib.pmt_stiffness_apply, args)
+    t32 = timed(lib.pmt_stiffness_apply_f32, args32)
+    from pymoto_amd import tune
+    tuning = {k: int(v) for k, v in tune.get_tuning().items()}
+    b64 = 2.0 * g.n * 8 + d.nel * 8
+    b32 = 2.0 * g.n * 4 + d.nel * 4
+    out = dict(kernel="fine_refl_kernel<16, apply, double> (K.u, FP64, reflection basis)", seconds=t64, bytes=b64,
+               gbs=b64 / t64 / 1e9, fp32=dict(kernel="fine_refl_kernel<16, apply, float>", us=t32 * 1e6,
+                                                 bytes=b32, gbs=b32 / t32 / 1e9, frac=b32 / t32 / 1e9 / PEAK_HBM_GBS),
+               tuning=tuning, traffic=None)
+    pmc = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "fine_refl_fp64_traffic.json")
+    if os.path.exists(pmc):
+        with open(pmc) as fh:
+            rec = json.load(fh)
+        if tuple(rec.get("nel", ())) == tuple(nel):
+            out["traffic"] = rec["hbm_bytes_per_launch"]
+    return out
 
 
 def batched_workload(pm, torch, nel=(128, 64, 64), beta=8.0, etas=(0.3, 0.5, 0.7), nloads=4, nuneven=16, seed=7,
@@ -320,10 +348,9 @@ def main():
         net, res = run_variant(pm, torch, args, v_run, xs, warm_net=warm)
         results[v] = res
         levels = net["levels"]
-        if v == "fp64" and roof is None:
-            roof = kernel_roofline(pm, torch, net)
         del net
         torch.cuda.empty_cache()
+    roof = kernel_roofline(pm, torch, nel)
     ref = results.get("fp64")
     summary = {}
     for v, r in results.items():
@@ -367,12 +394,11 @@ def main():
     if not args.no_batched:
         line["config"]["batched"] = batched_workload(pm, torch)
     if roof is not None:
-        line["roofline"] = {"bound": "mfma", "achieved": roof["tflops"], "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                            "frac": roof["tflops"] / PEAK_FP64_TFLOPS, "traffic": None, "kernel": roof["kernel"],
+        line["roofline"] = {"bound": "hbm", "achieved": roof["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                            "frac": roof["gbs"] / PEAK_HBM_GBS, "traffic": roof["traffic"], "kernel": roof["kernel"],
                             "kernel_us": roof["seconds"] * 1e6, "algorithmic_bytes": roof["bytes"],
-                            "hbm_gbs_equiv": roof["gbs"],
-                            "note": "FP64 vector-FMA kernel, 1152 flop/element; the FP64 peak is the same for vector "
-                                    "and matrix units"}
+                            "fp32_product": roof["fp32"], "tuning": roof["tuning"],
+                            "note": "compulsory bytes 2*ndof*8 + nel*8 per K.u; HIP events on the launch stream"}
     if not args.no_cpu:
         t_cpu, ndof_cpu, its = cpu_baseline(tuple(args.cpu_nel))
         line["cpu_baseline"] = {

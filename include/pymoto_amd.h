@@ -114,8 +114,25 @@ int pmt_stiffness_apply_bf16(int nelx, int nely, int nelz, const double* K0, con
  * 2D); then k0_pattern = 1 selects the kernel that keeps K0 in scalar registers.  Host only.
  * (No reference counterpart: a property of K0 from AssembleStiffness.__init__, assembly.py:446-461.) */
 int pmt_k0_pattern(int dim, const double* K0_host);
+/* k0_pattern is a bit set.  PMT_K0_CLASS_PATTERN: K0 follows the sign/class pattern above.
+ * PMT_K0_REFLECTION (hex8 only): the device K0 buffer is the PMT_K0_EXT_LEN-double extended form
+ * written by pmt_k0_extend - K0 itself, then its 8 diagonal 3x3 blocks in the basis of the three
+ * mid-plane reflections (FP64, then FP32) - and the fine operator multiplies each element in that
+ * basis (2*72 add/sub + 72 multiply-adds instead of 576 multiply-adds).  Kernels that only need K0
+ * read the first (2^D*D)^2 entries of the same buffer.
+ * pmt_k0_extend fills ext_host[PMT_K0_EXT_LEN] from a host K0 and returns the k0_pattern bits it
+ * supports (PMT_K0_REFLECTION when K0 commutes with the reflections to 1e-13 of its largest entry).
+ * Host only.  (No reference counterpart: a property of K0 from AssembleStiffness.__init__,
+ * assembly.py:446-461.) */
+#define PMT_K0_CLASS_PATTERN 1
+#define PMT_K0_REFLECTION 2
+#define PMT_K0_EXT_LEN 684
+int pmt_k0_extend(int dim, const double* K0_host, double* ext_host);
 /* Launch-shape tuning of the fine-grid operator, process-wide, read at every launch.
- * PMT_TUNE_FINE_KERNEL: 0 = 3D node tiles, 1 = xy node columns streamed along z (default; 3D only).
+ * PMT_TUNE_FINE_KERNEL: 0 = 3D node tiles, 1 = xy node columns streamed along z, 2 = streamed
+ * columns multiplying in the reflection basis (default; used when k0_pattern has PMT_K0_REFLECTION,
+ * else 1; 3D only).
+ * PMT_TUNE_REFL_ROWS: node rows per block of the reflection-basis kernel, 16 (default) or 32.
  * PMT_TUNE_STREAM_PLANES: node planes per streamed column, 0 = chosen from the CU count (default).
  * PMT_TUNE_STREAM_TILE: streamed column shape, 0 = 32x8 (default), 1 = 64x4, 2 = 16x16 nodes.
  * pmt_get_tuning returns the current value or a negative status.
@@ -123,6 +140,7 @@ int pmt_k0_pattern(int dim, const double* K0_host);
 #define PMT_TUNE_FINE_KERNEL 1
 #define PMT_TUNE_STREAM_PLANES 2
 #define PMT_TUNE_STREAM_TILE 3
+#define PMT_TUNE_REFL_ROWS 4
 int pmt_set_tuning(int param, int value);
 int pmt_get_tuning(int param);
 /* diag(K) (DampedJacobi.update, iterative.py:38-39) */

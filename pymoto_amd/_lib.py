@@ -9,6 +9,7 @@ import os
 import numpy as np
 import torch
 
+K0_EXT_LEN = 684  # PMT_K0_EXT_LEN: K0 plus its reflection-basis blocks (FP64, FP32)
 _LIBPATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libpymoto_amd.so")
 _lib = None
 
@@ -37,6 +38,7 @@ _SIGS = {
     "pmt_stiffness_apply_f32": (c_int, [c_int] * 3 + [c_vp, c_int, c_vp, c_vp, c_dbl, c_vp, c_vp, c_vp]),
     "pmt_stiffness_apply_bf16": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp, c_dbl, c_vp, c_vp, c_vp]),
     "pmt_k0_pattern": (c_int, [c_int, c_vp]),
+    "pmt_k0_extend": (c_int, [c_int, c_vp, c_vp]),
     "pmt_set_tuning": (c_int, [c_int, c_int]),
     "pmt_get_tuning": (c_int, [c_int]),
     "pmt_stiffness_diagonal": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp, c_dbl, c_vp, c_vp]),

@@ -205,7 +205,12 @@ class _GridData:
         self.bc = None if bc is None else np.asarray(bc).ravel()
         self.bcdiag = float(bcdiag) if bcdiag is not None else 0.0
         self.bcmask = bc_node_mask(domain, self.ndof, self.bc)
-        self.k0_pattern = int(_lib.load().pmt_k0_pattern(self.dim, self.K0.ctypes.data_as(_lib.c_vp)) == 1)
+        # device copy in the extended layout: K0, then its reflection-basis blocks (hex8)
+        self.K0_ext = np.zeros(_lib.K0_EXT_LEN, dtype=np.float64)
+        self.k0_pattern = int(_lib.load().pmt_k0_extend(self.dim, self.K0.ctypes.data_as(_lib.c_vp),
+                                                        self.K0_ext.ctypes.data_as(_lib.c_vp)))
+        if self.k0_pattern < 0:
+            raise ValueError("element matrix rejected by pmt_k0_extend")
         self._dev = {}
 
     def _on_device(self, key, arr, dtype):
@@ -216,7 +221,7 @@ class _GridData:
 
     @property
     def K0_dev(self):
-        return self._on_device("K0", self.K0, torch.float64)
+        return self._on_device("K0", self.K0_ext, torch.float64)
 
     @property
     def bcmask_dev(self):

@@ -44,20 +44,20 @@ __device__ inline bool touches_bc(const GridDims& gf, const uint8_t* __restrict_
 // Fine -> first coarse level (hex8): for a child without Dirichlet nodes (M_c = I, no bcdiag terms)
 // P_c^T (x_c K0) P_c = x_c G_c with G_c = P_c^T K0 P_c fixed, so K_E = sum_c x_c G_c (576*8
 // multiply-adds per coarse element instead of the triple products); the few children with Dirichlet
-// nodes get the masked triple product of their block directly.  A block stages the 8 G_c (36.9 KB)
-// in LDS once and walks many coarse elements, one per wave; thread = (A, B) node block of K_E, and
-// the 27 Dirichlet bytes of the element's fine nodes come through LDS (the branch is wave-uniform).
+// nodes get the masked triple product of their block directly.  One coarse element per wave, thread
+// = (A, B) node block of K_E; the 27 Dirichlet bytes of the element's fine nodes come through LDS and
+// one ballot marks the children that touch them (the branch is wave-uniform).
 __global__ void __launch_bounds__(256)
 galerkin_fine_fast_kernel(GridDims gf, GridDims gc, const double* __restrict__ G, const double* __restrict__ K0,
                           const double* __restrict__ xe, const uint8_t* __restrict__ bcm, double bcdiag,
                           double* __restrict__ Kc) {
   constexpr int NN = 8, NE = 24;
-  __shared__ double sG[NN * NE * NE];
   __shared__ unsigned smask[4][27];
-  for (int q = threadIdx.x; q < NN * NE * NE; q += 256) sG[q] = G[q];
+  const double* sG = G;  // 36.9 KB, read through L1/L2 by every wave
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int A = lane >> 3, B = lane & 7;
-  for (long long E0 = (long long)blockIdx.x * 4; E0 < gc.nel; E0 += (long long)gridDim.x * 4) {
+  {
+    const long long E0 = (long long)blockIdx.x * 4;
     const long long E = E0 + wave;
     const bool valid = E < gc.nel;
     int CI = 0, CJ = 0, CK = 0;
@@ -71,51 +71,63 @@ galerkin_fine_fast_kernel(GridDims gf, GridDims gc, const double* __restrict__ G
         smask[wave][lane] = bcm ? bcm[((long long)k * gf.ny + j) * gf.nx + i] : 0u;
       }
     }
-    __syncthreads();  // sG (first pass) and this wave's masks visible
+    __syncthreads();  // this block's masks visible
     if (valid) {
+      // children with a Dirichlet node: lane c < 8 tests child c, one ballot makes it wave-uniform
+      bool mine = false;
+      if (lane < NN) {
+        const int ox = lane & 1, oy = (lane >> 1) & 1, oz = (lane >> 2) & 1;
+#pragma unroll
+        for (int a = 0; a < NN; ++a)
+          mine = mine || smask[wave][(oz + ((a >> 2) & 1)) * 9 + (oy + ((a >> 1) & 1)) * 3 + ox + (a & 1)] != 0u;
+      }
+      const unsigned bcchild = (unsigned)(__ballot(mine) & 0xffull);
+      double xc[NN];
+#pragma unroll
+      for (int c = 0; c < NN; ++c)
+        xc[c] = xe[((long long)(2 * CK + ((c >> 2) & 1)) * gf.ey + (2 * CJ + ((c >> 1) & 1))) * gf.ex + 2 * CI + (c & 1)];
       double acc[3][3];
 #pragma unroll
       for (int d = 0; d < 3; ++d)
 #pragma unroll
         for (int d2 = 0; d2 < 3; ++d2) acc[d][d2] = 0.0;
+#pragma unroll
       for (int c = 0; c < NN; ++c) {
-        const int ox = c & 1, oy = (c >> 1) & 1, oz = (c >> 2) & 1;
-        const int fi = 2 * CI + ox, fj = 2 * CJ + oy, fk = 2 * CK + oz;
-        const double xc = xe[((long long)fk * gf.ey + fj) * gf.ex + fi];
-        unsigned mk[NN];
-        unsigned any = 0u;
+        if ((bcchild >> c) & 1u) continue;
+        const double* g = sG + c * NE * NE;
 #pragma unroll
-        for (int a = 0; a < NN; ++a) {
-          mk[a] = smask[wave][(oz + ((a >> 2) & 1)) * 9 + (oy + ((a >> 1) & 1)) * 3 + ox + (a & 1)];
-          any |= mk[a];
-        }
-        if (!any) {
-          const double* g = sG + c * NE * NE;
+        for (int d = 0; d < 3; ++d)
 #pragma unroll
-          for (int d = 0; d < 3; ++d)
-#pragma unroll
-            for (int d2 = 0; d2 < 3; ++d2) acc[d][d2] = fma(xc, g[(A * 3 + d) * NE + B * 3 + d2], acc[d][d2]);
-          continue;
-        }
-        // child with Dirichlet nodes: block (A, B) of P_c^T (x_c M K0 M + D_c) P_c directly
-        for (int a = 0; a < NN; ++a) {
-          const double wa = pweight<3>(c, a, A);
-          if (wa == 0.0) continue;
-          for (int b = 0; b < NN; ++b) {
-            const double wb = pweight<3>(c, b, B);
-            if (wb == 0.0) continue;
-            for (int d = 0; d < 3; ++d)
-              for (int d2 = 0; d2 < 3; ++d2) {
-                const bool fa = (mk[a] >> d) & 1u, fb = (mk[b] >> d2) & 1u;
-                double v = (fa || fb) ? 0.0 : xc * K0[(a * 3 + d) * NE + b * 3 + d2];
-                if (a == b && d == d2 && fa) {
-                  const int ai = fi + (a & 1), aj = fj + ((a >> 1) & 1), ak = fk + ((a >> 2) & 1);
-                  const int mult = ((ai > 0) + (ai < gf.nx - 1)) * ((aj > 0) + (aj < gf.ny - 1)) *
-                                   ((ak > 0) + (ak < gf.nz - 1));
-                  v += bcdiag / (double)mult;
+          for (int d2 = 0; d2 < 3; ++d2) acc[d][d2] = fma(xc[c], g[(A * 3 + d) * NE + B * 3 + d2], acc[d][d2]);
+      }
+      if (bcchild) {
+        // children with Dirichlet nodes: block (A, B) of P_c^T (x_c M K0 M + D_c) P_c directly
+        for (int c = 0; c < NN; ++c) {
+          if (!((bcchild >> c) & 1u)) continue;
+          const int ox = c & 1, oy = (c >> 1) & 1, oz = (c >> 2) & 1;
+          const int fi = 2 * CI + ox, fj = 2 * CJ + oy, fk = 2 * CK + oz;
+          const double xv = xe[((long long)fk * gf.ey + fj) * gf.ex + fi];
+          for (int a = 0; a < NN; ++a) {
+            const double wa = pweight<3>(c, a, A);
+            if (wa == 0.0) continue;
+            const unsigned ma = smask[wave][(oz + ((a >> 2) & 1)) * 9 + (oy + ((a >> 1) & 1)) * 3 + ox + (a & 1)];
+            for (int b = 0; b < NN; ++b) {
+              const double wb = pweight<3>(c, b, B);
+              if (wb == 0.0) continue;
+              const unsigned mb = smask[wave][(oz + ((b >> 2) & 1)) * 9 + (oy + ((b >> 1) & 1)) * 3 + ox + (b & 1)];
+              for (int d = 0; d < 3; ++d)
+                for (int d2 = 0; d2 < 3; ++d2) {
+                  const bool fa = (ma >> d) & 1u, fb = (mb >> d2) & 1u;
+                  double v = (fa || fb) ? 0.0 : xv * K0[(a * 3 + d) * NE + b * 3 + d2];
+                  if (a == b && d == d2 && fa) {
+                    const int ai = fi + (a & 1), aj = fj + ((a >> 1) & 1), ak = fk + ((a >> 2) & 1);
+                    const int mult = ((ai > 0) + (ai < gf.nx - 1)) * ((aj > 0) + (aj < gf.ny - 1)) *
+                                     ((ak > 0) + (ak < gf.nz - 1));
+                    v += bcdiag / (double)mult;
+                  }
+                  acc[d][d2] = fma(wa * wb, v, acc[d][d2]);
                 }
-                acc[d][d2] = fma(wa * wb, v, acc[d][d2]);
-              }
+            }
           }
         }
       }
@@ -124,7 +136,6 @@ galerkin_fine_fast_kernel(GridDims gf, GridDims gc, const double* __restrict__ G
 #pragma unroll
         for (int d2 = 0; d2 < 3; ++d2) Kc[E * NE * NE + (A * 3 + d) * NE + B * 3 + d2] = acc[d][d2];
     }
-    __syncthreads();  // masks consumed before the next element overwrites them
   }
 }
 
@@ -479,9 +490,7 @@ int launch_galerkin_fine(const GridDims& gf, const double* K0, const double* G, 
                          double bcdiag, double* Kc, hipStream_t s) {
   GridDims gc = make_grid(gf.ex / 2, gf.ey / 2, gf.dim == 3 ? gf.ez / 2 : 0);
   if (G != nullptr && gf.dim == 3) {
-    long long blocks = (gc.nel + 3) / 4;
-    if (blocks > 4096) blocks = 4096;  // each block stages G once and walks its share of coarse elements
-    galerkin_fine_fast_kernel<<<(unsigned)blocks, 256, 0, s>>>(gf, gc, G, K0, xe, bcm, bcdiag, Kc);
+    galerkin_fine_fast_kernel<<<(unsigned)((gc.nel + 3) / 4), 256, 0, s>>>(gf, gc, G, K0, xe, bcm, bcdiag, Kc);
     return check_launch("galerkin_fine_fast_kernel");
   }
   if (gf.dim == 3)

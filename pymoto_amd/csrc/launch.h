@@ -15,6 +15,15 @@ int launch_fine(int mode, const GridDims& g, const double* K0, int k0_pattern, c
                 double bcdiag, const T* u, const T* b, const T* dinv, double omega, T* y, hipStream_t s,
                 const T* up = nullptr, double beta = 0.0);
 int k0_matches_pattern(int dim, const double* K0_host);
+// refl.hip: hex8 operator in the mid-plane reflection basis (K0ext = pmt_k0_extend layout)
+constexpr int kK0ReflOffset64 = 576;  // 72 FP64 block entries after K0
+constexpr int kK0ReflOffset32 = 648;  // 72 FP32 block entries (36 doubles of storage)
+bool k0_reflection_table(const double* K0_host, double* tab);
+int refl_tiles_xy(const GridDims& g, int rows);
+template <typename T>
+int launch_fine_refl(int mode, const GridDims& g, int nzc, int rows, const double* K0ext, const T* xe,
+                     const uint8_t* bcm, double bcdiag, const T* u, const T* b, const T* dinv, double omega, T* y,
+                     hipStream_t s, const T* up, double beta);
 int set_tuning(int param, int value);
 int get_tuning(int param);
 int tuned_stream_planes();  // 0 = automatic

@@ -313,7 +313,8 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
   mg->dim = nelz > 0 ? 3 : 2;
   mg->nlev = nlevels;
   mg->NE = (1 << mg->dim) * mg->dim;
-  mg->k0pat = k0_matches_pattern(mg->dim, K0_host);
+  std::vector<double> K0ext(PMT_K0_EXT_LEN);
+  mg->k0pat = pmt_k0_extend(mg->dim, K0_host, K0ext.data());
   mg->lmax_elem = element_lmax(mg->NE, K0_host);
   int ex = nelx, ey = nely, ez = nelz;
   for (int l = 0; l < nlevels; ++l) {
@@ -346,7 +347,7 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
   if (rc == PMT_OK) rc = dalloc(mg, &(ptr), (n));
   const int NE = mg->NE;
   const int NS = mg->dim == 3 ? 243 : 36;
-  ALLOC(mg->K0, (long long)NE * NE);
+  ALLOC(mg->K0, (long long)PMT_K0_EXT_LEN);  // K0 followed by its reflection-basis blocks
   ALLOC(mg->G, (long long)(1 << mg->dim) * NE * NE);
   for (int l = 0; l < nlevels; ++l) {
     Level& L = mg->L[l];
@@ -384,7 +385,8 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
   if (rc == PMT_OK) rc = hip_status(hipHostMalloc((void**)&mg->host_sc, 16 * sizeof(double), hipHostMallocDefault),
                                     "hipHostMalloc");
   if (rc == PMT_OK)
-    rc = hip_status(hipMemcpy(mg->K0, K0_host, (size_t)NE * NE * sizeof(double), hipMemcpyHostToDevice), "K0 upload");
+    rc = hip_status(hipMemcpy(mg->K0, K0ext.data(), K0ext.size() * sizeof(double), hipMemcpyHostToDevice),
+                    "K0 upload");
   if (rc == PMT_OK) {
     std::vector<double> G((size_t)(1 << mg->dim) * NE * NE);
     galerkin_child_matrices(mg->dim, K0_host, G.data());

@@ -706,7 +706,8 @@ static void launch_fine_t(int mode, const GridDims& g, const double* K0, const T
 }
 
 // Launch-shape tuning of the fine operator (set by pmt_set_tuning; read at every launch).
-static int g_fine_kernel = 1;  // 0 = 3D tiles, 1 = z-streamed columns
+static int g_fine_kernel = 2;  // 0 = 3D tiles, 1 = z-streamed columns, 2 = reflection basis (if available)
+static int g_refl_rows = 16;   // node rows per block of the reflection-basis kernel
 static int g_stream_nzc = 0;   // node planes per streamed block (0 = chosen from the CU count)
 static int g_stream_tile = 0;  // streamed column shape: 0 = 32x8, 1 = 64x4, 2 = 16x16 nodes
 
@@ -715,7 +716,7 @@ static int g_tuning_epoch = 0;  // bumped by every change: captured launches mus
 int tuning_epoch() { return g_tuning_epoch; }
 
 int set_tuning(int param, int value) {
-  if (param == PMT_TUNE_FINE_KERNEL && (value == 0 || value == 1)) {
+  if (param == PMT_TUNE_FINE_KERNEL && value >= 0 && value <= 2) {
     g_fine_kernel = value;
     ++g_tuning_epoch;
     return 0;
@@ -730,6 +731,11 @@ int set_tuning(int param, int value) {
     ++g_tuning_epoch;
     return 0;
   }
+  if (param == PMT_TUNE_REFL_ROWS && (value == 16 || value == 32)) {
+    g_refl_rows = value;
+    ++g_tuning_epoch;
+    return 0;
+  }
   return -1;
 }
 
@@ -739,6 +745,7 @@ int get_tuning(int param) {
   if (param == PMT_TUNE_FINE_KERNEL) return g_fine_kernel;
   if (param == PMT_TUNE_STREAM_PLANES) return g_stream_nzc;
   if (param == PMT_TUNE_STREAM_TILE) return g_stream_tile;
+  if (param == PMT_TUNE_REFL_ROWS) return g_refl_rows;
   return -1;
 }
 
@@ -812,18 +819,23 @@ int launch_fine(int mode, const GridDims& g, const double* K0, int k0_pattern, c
                 double bcdiag, const T* u, const T* b, const T* dinv, double omega, T* y, hipStream_t s, const T* up,
                 double beta) {
   const T bd = (T)bcdiag, om = (T)omega, be = (T)beta;
-  if (g.dim == 3 && g_fine_kernel == 1) {
-    if (k0_pattern)
+  const bool cls = (k0_pattern & PMT_K0_CLASS_PATTERN) != 0;
+  if (g.dim == 3 && g_fine_kernel == 2 && (k0_pattern & PMT_K0_REFLECTION)) {
+    const int nzc = stream_planes(refl_tiles_xy(g, g_refl_rows), g.nz);
+    return launch_fine_refl<T>(mode, g, nzc, g_refl_rows, K0, xe, bcm, bcdiag, u, b, dinv, omega, y, s, up, beta);
+  }
+  if (g.dim == 3 && g_fine_kernel >= 1) {
+    if (cls)
       launch_stream_t<true, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
     else
       launch_stream_t<false, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
   } else if (g.dim == 3) {
-    if (k0_pattern)
+    if (cls)
       launch_fine_t<3, true, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
     else
       launch_fine_t<3, false, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
   } else {
-    if (k0_pattern)
+    if (cls)
       launch_fine_t<2, true, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
     else
       launch_fine_t<2, false, T>(mode, g, K0, xe, bcm, bd, u, b, dinv, om, y, s, up, be);

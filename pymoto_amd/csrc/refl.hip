@@ -1,0 +1,389 @@
+// Fine-level K(x) u for hex8 element matrices that are symmetric under the three mid-plane
+// reflections of the element (every box element of an orthotropic, axis-aligned material - the
+// isotropic unit cube of AssembleStiffness included).
+//
+// Such a K0 commutes with the reflections R_x, R_y, R_z (node bit flipped, the matching displacement
+// component negated), so in the basis of their joint eigenvectors it is block diagonal: the 24 dofs
+// split into 8 characters chi of 3 dofs each, dof (d, k) of the corner-sign transform
+//   U_d[k] = sum_a (-1)^(k.a) u_d[a]   (a, k: 3-bit corner / frequency indices)
+// belonging to chi = k ^ (1 << d).  Then
+//   K0 u = WHT( blockdiag(M_chi) WHT(u) ),   M_chi[d][e] = 1/64 sum_{a,b} (-1)^(k_d.a + k_e.b) K0[a d][b e],
+// with WHT the unnormalised 8-point Walsh-Hadamard transform over the corner bits, per component.
+// An element costs 2*72 add/sub + 72 multiply-adds + 24 multiplies (by x_e) instead of 576
+// multiply-adds, which moves the FP64 operator from the FMA roof to the HBM roof.
+//
+// Kernel layout: a block owns a 16 x SY column of nodes (one 16-lane DPP row per node row) and walks
+// z.  Thread (tx, ty) owns node (i, j) and the element (i, j) whose corner 0 it is.  Per element
+// layer k it forms the 8 Dirichlet-masked corner vectors (own node from registers, the +x neighbour
+// by a DPP row shift, the +y row through LDS), multiplies, combines the four xy corner results of
+// each node (x by DPP, y through LDS) and carries the upper-plane half to the next layer, so every
+// value crosses HBM once per application.  Edge rows/columns of the column are halo: 14 x (SY-2)
+// nodes are written per block.
+#include <type_traits>
+
+#include "launch.h"
+
+namespace pmt {
+
+constexpr int kReflRow = 16;  // nodes per block row = lanes per DPP row
+
+// DPP row shifts (within 16-lane rows; lanes shifted in from outside the row read 0):
+// from_lower: lane n receives lane n-1; from_upper: lane n receives lane n+1.
+template <int CTRL>
+__device__ inline float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, true));
+}
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(bits & 0xffffffffLL), CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), CTRL, 0xf, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <typename T>
+__device__ inline T from_lower(T v) {
+  return dpp_move<0x111>(v);  // row_shr:1
+}
+template <typename T>
+__device__ inline T from_upper(T v) {
+  return dpp_move<0x101>(v);  // row_shl:1
+}
+
+// In-place unnormalised Walsh-Hadamard transform over the corner index, per component.
+template <typename T>
+__device__ inline void wht8(T (&v)[8][3]) {
+#pragma unroll
+  for (int bit = 1; bit < 8; bit <<= 1)
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+      if (!(a & bit)) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const T p = v[a][d], q = v[a | bit][d];
+          v[a][d] = p + q;
+          v[a | bit][d] = p - q;
+        }
+      }
+}
+
+// v[a][d] (masked corner displacements) -> x * (K0 v)[a][d]
+template <typename T>
+__device__ inline void refl_element(const T* __restrict__ tab, T x, T (&v)[8][3]) {
+  wht8(v);
+  T w[8][3];
+#pragma unroll
+  for (int chi = 0; chi < 8; ++chi) {
+    const T in0 = x * v[chi ^ 1][0], in1 = x * v[chi ^ 2][1], in2 = x * v[chi ^ 4][2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T* m = tab + chi * 9 + d * 3;
+      w[chi ^ (1 << d)][d] = fma(m[2], in2, fma(m[1], in1, m[0] * in0));
+    }
+  }
+  wht8(w);
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) v[a][d] = w[a][d];
+}
+
+// Buffer resources: every global access of the kernel goes through a raw buffer descriptor of one
+// node plane (or element layer), whose hardware range check returns 0 for loads and drops stores at
+// an out-of-range offset.  Halo lanes and planes outside the grid therefore issue the same
+// instructions as the others (offset kOOB, or a zero-length plane), no branch surrounds a memory
+// instruction, and the wait counters stay exact: a prefetch is waited for only where it is used,
+// never together with the stores issued after it.
+constexpr int kOOB = 0x7fffffff;
+constexpr int kRsrcWord3 = 0x00020000;  // raw 32-bit buffer (gfx9 family)
+
+template <typename T>
+__device__ inline __amdgpu_buffer_rsrc_t plane_rsrc(const T* base, long long first, long long count, bool valid) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)(valid ? base + first : base), 0,
+                                           valid ? (int)(count * (long long)sizeof(T)) : 0, kRsrcWord3);
+}
+
+typedef unsigned int pmt_u32x2 __attribute__((ext_vector_type(2)));
+
+__device__ inline double bload(__amdgpu_buffer_rsrc_t r, int off, double) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ inline float bload(__amdgpu_buffer_rsrc_t r, int off, float) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ inline void bstore(__amdgpu_buffer_rsrc_t r, int off, double v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(pmt_u32x2, v), r, off, 0, 0);
+}
+__device__ inline void bstore(__amdgpu_buffer_rsrc_t r, int off, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, off, 0, 0);
+}
+
+template <int SY, int MODE, typename T>
+__global__ void __launch_bounds__(kReflRow* SY)
+fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, const T* __restrict__ tab, int zero_stride,
+                 const T* __restrict__ xe, const uint8_t* __restrict__ bcm, T bcdiag, const T* __restrict__ u,
+                 const T* __restrict__ b, const T* __restrict__ dinv, T omega, T* y, const T* up, T beta) {
+  constexpr int RX = kReflRow, NT = RX * SY;
+  constexpr int TS = (int)sizeof(T);
+  __shared__ T su[2][3][NT];  // masked displacements of the block's nodes, planes of both parities
+  __shared__ T so[6][NT];  // corner (., 1, .) results after the x combine, both planes of a layer
+
+  const int work = (int)xcd_block();
+  if (work >= nwork) return;  // whole block leaves together (grid padded for the XCD remap)
+  const int txy = work % ntiles_xy;
+  const int tid = threadIdx.x, tx = tid % RX, ty = tid / RX;
+  const int i = (txy % tiles_x) * (RX - 2) - 1 + tx;
+  const int j = (txy / tiles_x) * (SY - 2) - 1 + ty;
+  const int kbeg = (work / ntiles_xy) * nzc;
+  const int kend = min(kbeg + nzc, g.nz);
+  const bool nin = i >= 0 && i < g.nx && j >= 0 && j < g.ny;
+  const bool ein = i >= 0 && i < g.ex && j >= 0 && j < g.ey;
+  const bool outp = nin && tx >= 1 && tx <= RX - 2 && ty >= 1 && ty <= SY - 2;
+  const int slot_up = min(ty + 1, SY - 1) * RX + tx;  // node (i, j+1); the last row reads itself (halo)
+  const int slot_dn = max(ty - 1, 0) * RX + tx;       // node (i, j-1); the first row reads itself (halo)
+  const long long uplane = (long long)g.nx * g.ny, xplane = (long long)g.ex * g.ey;
+  // byte offsets within a plane (kOOB for lanes outside the grid / without an output node)
+  const int o_node = nin ? (int)((long long)j * g.nx + i) : kOOB;
+  const int o_vec = nin ? 3 * TS * o_node : kOOB;
+  const int o_out = outp ? o_vec : kOOB;
+  const int o_el = ein ? TS * (int)((long long)j * g.ex + i) : kOOB;
+
+  auto load_node = [&](int p, T(&ur)[3], unsigned& mk) {
+    const bool pin = p >= 0 && p < g.nz;
+    const __amdgpu_buffer_rsrc_t ru = plane_rsrc(u, 3 * (long long)p * uplane, 3 * uplane, pin);
+    const __amdgpu_buffer_rsrc_t rm = plane_rsrc(bcm, (long long)p * uplane, uplane, pin);
+    mk = __builtin_amdgcn_raw_buffer_load_b8(rm, o_node, 0, 0);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) ur[d] = bload(ru, o_vec + d * TS, T());
+  };
+  auto load_x = [&](int l) -> T {
+    const __amdgpu_buffer_rsrc_t rx = plane_rsrc(xe, (long long)l * xplane, xplane, l >= 0 && l < g.ez);
+    return bload(rx, o_el, T());
+  };
+  auto load_out = [&](int p, T(&bo)[3], T(&dv)[3], T(&po)[3]) {
+    const bool pin = p >= 0 && p < kend;
+    const long long first = 3 * (long long)p * uplane, cnt = 3 * uplane;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) bo[d] = dv[d] = po[d] = (T)0;
+    if (MODE != MODE_APPLY) {
+      const __amdgpu_buffer_rsrc_t rb = plane_rsrc(b, first, cnt, pin);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) bo[d] = bload(rb, o_out + d * TS, T());
+    }
+    if (MODE == MODE_JACOBI || MODE == MODE_CHEB) {
+      const __amdgpu_buffer_rsrc_t rd = plane_rsrc(dinv, first, cnt, pin);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) dv[d] = bload(rd, o_out + d * TS, T());
+    }
+    if (MODE == MODE_CHEB) {
+      const __amdgpu_buffer_rsrc_t rp = plane_rsrc(up, first, cnt, pin && up != nullptr);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) po[d] = bload(rp, o_out + d * TS, T());
+    }
+  };
+  auto masked = [](const T(&ur)[3], unsigned mk, T(&c)[3]) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) c[d] = ((mk >> d) & 1u) ? (T)0 : ur[d];
+  };
+
+  // Software pipeline over a ring of three slots (plane k in slot (k - kbeg + 1) % 3): the node values
+  // of plane k+3, the densities of layer k+3 and the update operands of plane k+3 are loaded at the
+  // end of layer k, after the stores of plane k, into the slot plane k frees; the node values are
+  // used two layers later, the rest three.  The loop is unrolled by three so every slot is a fixed
+  // register set: no copies, and no wait falls on a memory operation issued within the last layer.
+  T raw[3][3], xs[3], bo[3][3], dv[3][3], po[3][3];
+  unsigned mk[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    load_node(kbeg - 1 + r, raw[r], mk[r]);
+    xs[r] = load_x(kbeg - 1 + r);
+    load_out(kbeg - 1 + r, bo[r], dv[r], po[r]);
+  }
+  {
+    T c[3];
+    masked(raw[0], mk[0], c);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) su[(kbeg - 1) & 1][d][tid] = c[d];
+  }
+  T C[3] = {(T)0, (T)0, (T)0};  // own node, plane k: the sum over element layer k-1
+
+  auto layer = [&](int k, auto slot) {
+    constexpr int SA = decltype(slot)::value, SB = (SA + 1) % 3;
+    T(*const lo)[NT] = su[k & 1];        // plane k (written by the previous layer)
+    T(*const hi)[NT] = su[(k + 1) & 1];  // plane k+1
+    T c0[3], c1[3];
+    masked(raw[SA], mk[SA], c0);
+    masked(raw[SB], mk[SB], c1);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) hi[d][tid] = c1[d];
+    __syncthreads();
+    // corners a = ax + 2 ay + 4 az of element (i, j, k): own node, +x by DPP, +y row from LDS
+    T v[8][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T y0 = lo[d][slot_up], y1 = hi[d][slot_up];
+      v[0][d] = c0[d];
+      v[1][d] = from_upper(c0[d]);
+      v[2][d] = y0;
+      v[3][d] = from_upper(y0);
+      v[4][d] = c1[d];
+      v[5][d] = from_upper(c1[d]);
+      v[6][d] = y1;
+      v[7][d] = from_upper(y1);
+    }
+    // the 72 block entries are re-read through the scalar cache each layer (zero_stride is 0; the
+    // compiler cannot hoist the loads, which would exceed the scalar register file)
+    refl_element(tab + zero_stride * k, xs[SA], v);
+
+    // x combine: s[ay][az] = own corner (0, ay, az) + left neighbour's corner (1, ay, az)
+    T s[2][2][3];
+#pragma unroll
+    for (int ay = 0; ay < 2; ++ay)
+#pragma unroll
+      for (int az = 0; az < 2; ++az)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) s[ay][az][d] = v[2 * ay + 4 * az][d] + from_lower(v[1 + 2 * ay + 4 * az][d]);
+    // y combine through LDS: the row below contributes its corners (., 1, .)
+#pragma unroll
+    for (int az = 0; az < 2; ++az)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) so[3 * az + d][tid] = s[1][az][d];
+    __syncthreads();
+
+    {
+      // plane k is complete (stores of the prologue layer and of halo lanes are dropped)
+      const __amdgpu_buffer_rsrc_t ry = plane_rsrc(y, 3 * (long long)k * uplane, 3 * uplane, k >= kbeg);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const T au = C[d] + s[0][0][d] + so[d][slot_dn];
+        const T uk = raw[SA][d];
+        const T ku = ((mk[SA] >> d) & 1u) ? bcdiag * uk : au;
+        T out;
+        if (MODE == MODE_APPLY)
+          out = ku;
+        else if (MODE == MODE_RESIDUAL)
+          out = bo[SA][d] - ku;
+        else if (MODE == MODE_JACOBI)
+          out = uk + omega * dv[SA][d] * (bo[SA][d] - ku);
+        else
+          out = uk + omega * dv[SA][d] * (bo[SA][d] - ku) + beta * (uk - po[SA][d]);
+        bstore(ry, o_out + d * TS, out);
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) C[d] = s[0][1][d] + so[3 + d][slot_dn];
+
+    __builtin_amdgcn_sched_barrier(0);  // keep the prefetch behind the stores
+    load_node(k + 3, raw[SA], mk[SA]);
+    xs[SA] = load_x(k + 3);
+    load_out(k + 3, bo[SA], dv[SA], po[SA]);
+  };
+
+  for (int k = kbeg - 1;; k += 3) {
+    layer(k, std::integral_constant<int, 0>{});
+    if (k + 1 >= kend) break;
+    layer(k + 1, std::integral_constant<int, 1>{});
+    if (k + 2 >= kend) break;
+    layer(k + 2, std::integral_constant<int, 2>{});
+    if (k + 3 >= kend) break;
+  }
+}
+
+// Block tables of an element matrix in the reflection basis; false if K0 does not commute with the
+// three mid-plane reflections (to 1e-13 of its largest entry).
+bool k0_reflection_table(const double* K0, double* tab) {
+  double scale = 0.0;
+  for (int q = 0; q < 576; ++q) scale = fmax(scale, fabs(K0[q]));
+  if (!(scale > 0.0)) return false;
+  auto sgn = [](int k, int a) { return (__builtin_popcount(k & a) & 1) ? -1.0 : 1.0; };
+  for (int d = 0; d < 3; ++d)
+    for (int k = 0; k < 8; ++k)
+      for (int e = 0; e < 3; ++e)
+        for (int l = 0; l < 8; ++l) {
+          double m = 0.0;
+          for (int a = 0; a < 8; ++a)
+            for (int bb = 0; bb < 8; ++bb) m += sgn(k, a) * sgn(l, bb) * K0[(a * 3 + d) * 24 + bb * 3 + e];
+          m /= 64.0;
+          const int chi = k ^ (1 << d), chi2 = l ^ (1 << e);
+          if (chi != chi2) {
+            if (fabs(m) > 1e-13 * scale) return false;
+          } else {
+            tab[chi * 9 + d * 3 + e] = m;
+          }
+        }
+  return true;
+}
+
+template <int SY, typename T>
+static void launch_refl_shape(int mode, const GridDims& g, int nzc, const T* tab, const T* xe, const uint8_t* bcm,
+                              T bcdiag, const T* u, const T* b, const T* dinv, T omega, T* y, hipStream_t s,
+                              const T* up, T beta) {
+  const int tx = (g.nx + kReflRow - 3) / (kReflRow - 2);
+  const int ty = (g.ny + SY - 3) / (SY - 2);
+  const int nxy = tx * ty;
+  const int nwork = nxy * ((g.nz + nzc - 1) / nzc);
+  dim3 grid((unsigned)((nwork + 7) & ~7));
+#define PMT_REFL(M)                                                                                            \
+  fine_refl_kernel<SY, M, T><<<grid, kReflRow * SY, 0, s>>>(g, tx, nxy, nzc, nwork, tab, 0, xe, bcm, bcdiag, u, \
+                                                              b, dinv, omega, y, up, beta)
+  if (mode == MODE_APPLY)
+    PMT_REFL(MODE_APPLY);
+  else if (mode == MODE_RESIDUAL)
+    PMT_REFL(MODE_RESIDUAL);
+  else if (mode == MODE_JACOBI)
+    PMT_REFL(MODE_JACOBI);
+  else
+    PMT_REFL(MODE_CHEB);
+#undef PMT_REFL
+}
+
+int refl_tiles_xy(const GridDims& g, int rows) {
+  return ((g.nx + kReflRow - 3) / (kReflRow - 2)) * ((g.ny + rows - 3) / (rows - 2));
+}
+
+template <typename T>
+int launch_fine_refl(int mode, const GridDims& g, int nzc, int rows, const double* K0ext, const T* xe,
+                     const uint8_t* bcm, double bcdiag, const T* u, const T* b, const T* dinv, double omega, T* y,
+                     hipStream_t s, const T* up, double beta) {
+  const T* tab = sizeof(T) == 8 ? (const T*)(K0ext + kK0ReflOffset64) : (const T*)(K0ext + kK0ReflOffset32);
+  const T bd = (T)bcdiag, om = (T)omega, be = (T)beta;
+  if (rows == 32)
+    launch_refl_shape<32, T>(mode, g, nzc, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
+  else
+    launch_refl_shape<16, T>(mode, g, nzc, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
+  return check_launch("fine_refl_kernel");
+}
+
+template int launch_fine_refl<double>(int, const GridDims&, int, int, const double*, const double*, const uint8_t*,
+                                      double, const double*, const double*, const double*, double, double*,
+                                      hipStream_t, const double*, double);
+template int launch_fine_refl<float>(int, const GridDims&, int, int, const double*, const float*, const uint8_t*,
+                                     double, const float*, const float*, const float*, double, float*, hipStream_t,
+                                     const float*, double);
+
+}  // namespace pmt
+
+using namespace pmt;
+
+extern "C" {
+
+int pmt_k0_extend(int dim, const double* K0_host, double* ext_host) {
+  if ((dim != 2 && dim != 3) || !K0_host || !ext_host) return arg_error("pmt_k0_extend: bad argument");
+  const int ne = (1 << dim) * dim;
+  for (int q = 0; q < PMT_K0_EXT_LEN; ++q) ext_host[q] = 0.0;
+  for (int q = 0; q < ne * ne; ++q) ext_host[q] = K0_host[q];
+  int bits = k0_matches_pattern(dim, K0_host) ? PMT_K0_CLASS_PATTERN : 0;
+  if (dim == 3) {
+    double tab[72];
+    if (k0_reflection_table(K0_host, tab)) {
+      float* t32 = (float*)(ext_host + kK0ReflOffset32);
+      for (int q = 0; q < 72; ++q) {
+        ext_host[kK0ReflOffset64 + q] = tab[q];
+        t32[q] = (float)tab[q];
+      }
+      bits |= PMT_K0_REFLECTION;
+    }
+  }
+  return bits;
+}
+
+}  // extern "C"

@@ -8,8 +8,9 @@ with HIP events on the launch stream, keeps the fastest, and applies it process-
 Results are cached per (grid, device) for the life of the process.
 
 Tuning parameters (``include/pymoto_amd.h``): ``FINE_KERNEL`` (0 = 3D tiles, 1 = z-streamed
-columns), ``STREAM_TILE`` (0 = 32x8, 1 = 64x4, 2 = 16x16 nodes per plane), ``STREAM_PLANES``
-(z planes per block, 0 = derived from the CU count).
+columns, 2 = z-streamed columns multiplying in the reflection basis of K0), ``STREAM_TILE`` (0 = 32x8,
+1 = 64x4, 2 = 16x16 nodes per plane of kernel 1), ``REFL_ROWS`` (16 or 32 node rows per block of
+kernel 2), ``STREAM_PLANES`` (z planes per block, 0 = derived from the CU count).
 """
 import itertools
 
@@ -17,29 +18,35 @@ import torch
 
 from . import _lib
 
-FINE_KERNEL, STREAM_PLANES, STREAM_TILE = 1, 2, 3
+FINE_KERNEL, STREAM_PLANES, STREAM_TILE, REFL_ROWS = 1, 2, 3, 4
+DEFAULT = dict(fine_kernel=2, stream_tile=0, stream_planes=0, refl_rows=16)
 _cache = {}
 
 
 def get_tuning():
     lib = _lib.load()
     return {"fine_kernel": lib.pmt_get_tuning(FINE_KERNEL), "stream_tile": lib.pmt_get_tuning(STREAM_TILE),
-            "stream_planes": lib.pmt_get_tuning(STREAM_PLANES)}
+            "stream_planes": lib.pmt_get_tuning(STREAM_PLANES), "refl_rows": lib.pmt_get_tuning(REFL_ROWS)}
 
 
-def set_tuning(fine_kernel=None, stream_tile=None, stream_planes=None):
+def set_tuning(fine_kernel=None, stream_tile=None, stream_planes=None, refl_rows=None):
     lib = _lib.lib()
-    for param, value in ((FINE_KERNEL, fine_kernel), (STREAM_TILE, stream_tile), (STREAM_PLANES, stream_planes)):
+    for param, value in ((FINE_KERNEL, fine_kernel), (STREAM_TILE, stream_tile), (STREAM_PLANES, stream_planes),
+                         (REFL_ROWS, refl_rows)):
         if value is not None:
             _lib.check(lib.pmt_set_tuning(param, int(value)), "set tuning")
 
 
-def candidates(nz):
-    """Launch shapes tried for a grid with nz node planes."""
-    out = [dict(fine_kernel=0, stream_tile=0, stream_planes=0)]
+def candidates(nz, reflection=True):
+    """Launch shapes tried for a grid with nz node planes (kernel 2 only when K0 has the
+    reflection form)."""
+    out = [dict(fine_kernel=0, stream_tile=0, stream_planes=0, refl_rows=16)]
     planes = sorted({0, 4, 8, 16, max(2, nz // 4)})
     for tile, p in itertools.product((0, 1, 2), planes):
-        out.append(dict(fine_kernel=1, stream_tile=tile, stream_planes=p))
+        out.append(dict(fine_kernel=1, stream_tile=tile, stream_planes=p, refl_rows=16))
+    if reflection:
+        for rows, p in itertools.product((16, 32), planes):
+            out.append(dict(fine_kernel=2, stream_tile=0, stream_planes=p, refl_rows=rows))
     return out
 
 
@@ -76,8 +83,8 @@ def autotune_fine_operator(K, reps: int = 10, precision: str = "fp64", verbose: 
     before = get_tuning()
     table = {}
     try:
-        for c in candidates(g.nel3[2] + 1 if g.nel3[2] else 1):
-            if g.nel3[2] == 0 and c["fine_kernel"] == 1:
+        for c in candidates(g.nel3[2] + 1 if g.nel3[2] else 1, reflection=bool(g.k0_pattern & 2)):
+            if g.nel3[2] == 0 and c["fine_kernel"] != 0:
                 continue  # 2D grids use the tiled kernel
             if precision == "bf16" and c["fine_kernel"] == 0:
                 continue  # the matrix-core operator is always streamed

@@ -94,8 +94,8 @@ def test_network_forward_backward_vs_oracle(gpu_lib):
 def test_training_reduces_loss_and_warm_start_beats_zero_start(gpu_lib):
     """Train on solver-generated pairs along seeded OC trajectories (32x16x16 cantilever, move 0.1);
     on held-out designs of the same trajectories (every third design, not trained on) the network's
-    start has a smaller initial residual than the zero start and the MG-PCG solve from it needs
-    fewer iterations in total."""
+    start has a smaller energy-norm error than the zero start (whose relative error is 1) and the
+    MG-PCG solve from it needs fewer iterations in total."""
     from pymoto_amd import learned as PL
     nel = (32, 16, 16)
     d = pm.VoxelDomain(*nel)
@@ -124,19 +124,20 @@ def test_training_reduces_loss_and_warm_start_beats_zero_start(gpu_lib):
     hist = PL.train(net, train, epochs=20, lr=1e-3)
     assert hist[-1] < 0.5 * hist[0]
     cg = pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(d, 3, omega=0.5, smooth_steps=2), tol=1e-8)
-    res_net, its_net, its_zero = [], [], []
+    err_net, its_net, its_zero = [], [], []
     for xf in held:
         xs = torch.as_tensor(fe.simp(xf.cpu().numpy())).cuda()
         K = pm.AssembleStiffness(d, bc=bc)(xs)
         cg.update(K)
         g = PL.predict_displacement(net, d, K, xf, fd)
-        res_net.append(float(K.residual(fd, g).norm() / fd.norm()))
         cg.solve(fd, x0=g)
         its_net.append(cg.iterations)
-        cg.solve(fd)
+        us = cg.solve(fd)
         its_zero.append(cg.iterations)
-    print(f"held-out: initial residual {np.round(res_net, 3)}, iterations network {its_net} vs zero start {its_zero}")
-    assert np.mean(res_net) < 1.0
+        e = us - g
+        err_net.append(float(torch.sqrt(torch.dot(e, K @ e) / torch.dot(us, K @ us))))
+    print(f"held-out: energy-norm error {np.round(err_net, 3)}, iterations network {its_net} vs zero start {its_zero}")
+    assert np.mean(err_net) < 0.9
     assert sum(its_net) < sum(its_zero)
 
 

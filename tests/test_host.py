@@ -33,6 +33,63 @@ def test_host_helpers_of_the_library():
     assert lib.pmt_reduce_work_size() >= 1024
 
 
+def _k0_extend(K0):
+    from pymoto_amd import _lib
+    lib = _lib.load()
+    dim = 3 if K0.shape[0] == 24 else 2
+    K0 = np.ascontiguousarray(K0, dtype=np.float64)
+    ext = np.zeros(_lib.K0_EXT_LEN)
+    bits = lib.pmt_k0_extend(dim, K0.ctypes.data_as(_lib.c_vp), ext.ctypes.data_as(_lib.c_vp))
+    return bits, ext
+
+
+def _reflection_product(tab, u):
+    """K0 u from the 8 reflection-basis blocks: WHT over the corner bits, 3x3 block per character
+    chi (dof (d, k) belongs to chi = k ^ 2^d), WHT back."""
+    H = np.array([[(-1.0) ** bin(k & a).count("1") for a in range(8)] for k in range(8)])
+    U = H @ u.reshape(8, 3)  # U[k, d]
+    W = np.zeros_like(U)
+    for chi in range(8):
+        vin = np.array([U[chi ^ (1 << d), d] for d in range(3)])
+        out = tab[chi * 9:(chi + 1) * 9].reshape(3, 3) @ vin
+        for d in range(3):
+            W[chi ^ (1 << d), d] = out[d]
+    return (H @ W).reshape(-1)
+
+
+@pytest.mark.parametrize("esize,nu", [((1.0, 1.0, 1.0), 0.3), ((1.0, 2.0, 0.5), 0.3), ((1.0, 1.0, 1.0), 0.45)])
+def test_k0_reflection_form(esize, nu):
+    """pmt_k0_extend: the block tables reproduce K0 u (identity of the mid-plane reflection basis) for
+    box elements; the FP32 copy is the rounded FP64 table; K0 itself leads the buffer."""
+    K0 = fe.element_stiffness(3, esize=esize, nu=nu)
+    bits, ext = _k0_extend(K0)
+    assert bits == 3  # box elements: class pattern and reflection form
+    npt.assert_array_equal(ext[:576], K0.ravel())
+    tab = ext[576:648]
+    npt.assert_array_equal(ext[648:684].view(np.float32), tab.astype(np.float32))
+    rng = np.random.default_rng(5)
+    for _ in range(4):
+        u = rng.standard_normal(24)
+        ref = K0 @ u
+        npt.assert_allclose(_reflection_product(tab, u), ref, rtol=0, atol=1e-14 * np.abs(K0).max() * np.abs(u).sum())
+    # rigid translations are in the null space: their blocks have an exact zero row
+    for d in range(3):
+        t = np.zeros((8, 3))
+        t[:, d] = 1.0
+        assert np.abs(_reflection_product(tab, t.ravel())).max() < 1e-14 * np.abs(K0).max()
+
+
+def test_k0_without_reflection_symmetry_is_refused_the_form():
+    K0 = fe.element_stiffness(3)
+    rng = np.random.default_rng(1)
+    P = rng.standard_normal((24, 24)) * 1e-6
+    bits, ext = _k0_extend(K0 + P + P.T)
+    assert bits & 2 == 0
+    assert not ext[576:].any()
+    bits2, ext2 = _k0_extend(fe.element_stiffness(2))
+    assert bits2 == 1 and not ext2[64:].any()
+
+
 def test_errors_are_reported_without_gpu():
     from pymoto_amd import _lib
     lib = _lib.load()
