@@ -43,99 +43,132 @@ __device__ inline bool touches_bc(const GridDims& gf, const uint8_t* __restrict_
 
 // Fine -> first coarse level (hex8): for a child without Dirichlet nodes (M_c = I, no bcdiag terms)
 // P_c^T (x_c K0) P_c = x_c G_c with G_c = P_c^T K0 P_c fixed, so K_E = sum_c x_c G_c (576*8
-// multiply-adds per coarse element instead of the triple products); the few children with Dirichlet
-// nodes get the masked triple product of their block directly.  One coarse element per wave, thread
-// = (A, B) node block of K_E; the 27 Dirichlet bytes of the element's fine nodes come through LDS and
-// one ballot marks the children that touch them (the branch is wave-uniform).
-__global__ void __launch_bounds__(256)
-galerkin_fine_fast_kernel(GridDims gf, GridDims gc, const double* __restrict__ G, const double* __restrict__ K0,
-                          const double* __restrict__ xe, const uint8_t* __restrict__ bcm, double bcdiag,
-                          double* __restrict__ Kc) {
-  constexpr int NN = 8, NE = 24;
-  __shared__ unsigned smask[4][27];
-  const double* sG = G;  // 36.9 KB, read through L1/L2 by every wave
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int A = lane >> 3, B = lane & 7;
-  {
-    const long long E0 = (long long)blockIdx.x * 4;
-    const long long E = E0 + wave;
-    const bool valid = E < gc.nel;
-    int CI = 0, CJ = 0, CK = 0;
-    if (valid) {
-      CI = (int)(E % gc.ex);
-      const long long tt = E / gc.ex;
-      CJ = (int)(tt % gc.ey);
-      CK = (int)(tt / gc.ey);
-      if (lane < 27) {
-        const int i = 2 * CI + lane % 3, j = 2 * CJ + (lane / 3) % 3, k = 2 * CK + lane / 9;
-        smask[wave][lane] = bcm ? bcm[((long long)k * gf.ny + j) * gf.nx + i] : 0u;
-      }
+// multiply-adds per coarse element instead of the triple products).  Three passes:
+//  1. galerkin_bc_flags_kernel: per coarse element, the bit set of children with a Dirichlet node,
+//     and the list of coarse elements with any (order of the list is irrelevant: each entry only
+//     touches its own element matrix);
+//  2. galerkin_fine_fast_kernel: K_E = sum over the other children of x_c G_c (thread = (E, A, B));
+//  3. galerkin_fine_bc_kernel: adds the masked triple products of the flagged children (one wave per
+//     listed element, lane = (A, B)).
+__global__ void galerkin_bc_flags_kernel(GridDims gf, GridDims gc, const uint8_t* __restrict__ bcm,
+                                         uint8_t* __restrict__ cflag, int* __restrict__ list, int* __restrict__ count) {
+  const long long E = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (E >= gc.nel) return;
+  const int CI = (int)(E % gc.ex);
+  const long long tt = E / gc.ex;
+  const int CJ = (int)(tt % gc.ey), CK = (int)(tt / gc.ey);
+  unsigned m27 = 0;  // bit q: fine node (q%3, q/3%3, q/9) of the element is Dirichlet
+  for (int q = 0; q < 27; ++q) {
+    const int i = 2 * CI + q % 3, j = 2 * CJ + (q / 3) % 3, k = 2 * CK + q / 9;
+    if (bcm[((long long)k * gf.ny + j) * gf.nx + i]) m27 |= 1u << q;
+  }
+  unsigned flags = 0;
+  for (int c = 0; c < 8; ++c)
+    for (int a = 0; a < 8; ++a) {
+      const int q = ((c >> 2) + (a >> 2)) * 9 + (((c >> 1) & 1) + ((a >> 1) & 1)) * 3 + (c & 1) + (a & 1);
+      if ((m27 >> q) & 1u) flags |= 1u << c;
     }
-    __syncthreads();  // this block's masks visible
-    if (valid) {
-      // children with a Dirichlet node: lane c < 8 tests child c, one ballot makes it wave-uniform
-      bool mine = false;
-      if (lane < NN) {
-        const int ox = lane & 1, oy = (lane >> 1) & 1, oz = (lane >> 2) & 1;
+  cflag[E] = (uint8_t)flags;
+  if (flags) list[atomicAdd(count, 1)] = (int)E;
+}
+
+__global__ void zero_int_kernel(int* p) { *p = 0; }
+
+__global__ void __launch_bounds__(256)
+galerkin_fine_fast_kernel(GridDims gf, GridDims gc, const double* __restrict__ G, const double* __restrict__ xe,
+                          const uint8_t* __restrict__ cflag, double* __restrict__ Kc) {
+  constexpr int NE = 24;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= gc.nel * 64) return;
+  const long long E = t >> 6;  // one coarse element per wave
+  const int A = (int)(t >> 3) & 7, B = (int)t & 7;
+  const int CI = (int)(E % gc.ex);
+  const long long tt = E / gc.ex;
+  const int CJ = (int)(tt % gc.ey), CK = (int)(tt / gc.ey);
+  const unsigned flags = cflag[E];
+  double xc[8];
 #pragma unroll
-        for (int a = 0; a < NN; ++a)
-          mine = mine || smask[wave][(oz + ((a >> 2) & 1)) * 9 + (oy + ((a >> 1) & 1)) * 3 + ox + (a & 1)] != 0u;
-      }
-      const unsigned bcchild = (unsigned)(__ballot(mine) & 0xffull);
-      double xc[NN];
+  for (int c = 0; c < 8; ++c) {
+    const double v = xe[((long long)(2 * CK + ((c >> 2) & 1)) * gf.ey + (2 * CJ + ((c >> 1) & 1))) * gf.ex + 2 * CI + (c & 1)];
+    xc[c] = ((flags >> c) & 1u) ? 0.0 : v;
+  }
+  double acc[3][3];
 #pragma unroll
-      for (int c = 0; c < NN; ++c)
-        xc[c] = xe[((long long)(2 * CK + ((c >> 2) & 1)) * gf.ey + (2 * CJ + ((c >> 1) & 1))) * gf.ex + 2 * CI + (c & 1)];
-      double acc[3][3];
+  for (int d = 0; d < 3; ++d)
 #pragma unroll
-      for (int d = 0; d < 3; ++d)
+    for (int d2 = 0; d2 < 3; ++d2) acc[d][d2] = 0.0;
 #pragma unroll
-        for (int d2 = 0; d2 < 3; ++d2) acc[d][d2] = 0.0;
+  for (int c = 0; c < 8; ++c) {
+    const double* g = G + c * NE * NE + (A * 3) * NE + B * 3;
 #pragma unroll
-      for (int c = 0; c < NN; ++c) {
-        if ((bcchild >> c) & 1u) continue;
-        const double* g = sG + c * NE * NE;
+    for (int d = 0; d < 3; ++d)
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
+      for (int d2 = 0; d2 < 3; ++d2) acc[d][d2] = fma(xc[c], g[d * NE + d2], acc[d][d2]);
+  }
 #pragma unroll
-          for (int d2 = 0; d2 < 3; ++d2) acc[d][d2] = fma(xc[c], g[(A * 3 + d) * NE + B * 3 + d2], acc[d][d2]);
-      }
-      if (bcchild) {
-        // children with Dirichlet nodes: block (A, B) of P_c^T (x_c M K0 M + D_c) P_c directly
-        for (int c = 0; c < NN; ++c) {
-          if (!((bcchild >> c) & 1u)) continue;
-          const int ox = c & 1, oy = (c >> 1) & 1, oz = (c >> 2) & 1;
-          const int fi = 2 * CI + ox, fj = 2 * CJ + oy, fk = 2 * CK + oz;
-          const double xv = xe[((long long)fk * gf.ey + fj) * gf.ex + fi];
-          for (int a = 0; a < NN; ++a) {
-            const double wa = pweight<3>(c, a, A);
-            if (wa == 0.0) continue;
-            const unsigned ma = smask[wave][(oz + ((a >> 2) & 1)) * 9 + (oy + ((a >> 1) & 1)) * 3 + ox + (a & 1)];
-            for (int b = 0; b < NN; ++b) {
-              const double wb = pweight<3>(c, b, B);
-              if (wb == 0.0) continue;
-              const unsigned mb = smask[wave][(oz + ((b >> 2) & 1)) * 9 + (oy + ((b >> 1) & 1)) * 3 + ox + (b & 1)];
-              for (int d = 0; d < 3; ++d)
-                for (int d2 = 0; d2 < 3; ++d2) {
-                  const bool fa = (ma >> d) & 1u, fb = (mb >> d2) & 1u;
-                  double v = (fa || fb) ? 0.0 : xv * K0[(a * 3 + d) * NE + b * 3 + d2];
-                  if (a == b && d == d2 && fa) {
-                    const int ai = fi + (a & 1), aj = fj + ((a >> 1) & 1), ak = fk + ((a >> 2) & 1);
-                    const int mult = ((ai > 0) + (ai < gf.nx - 1)) * ((aj > 0) + (aj < gf.ny - 1)) *
-                                     ((ak > 0) + (ak < gf.nz - 1));
-                    v += bcdiag / (double)mult;
-                  }
-                  acc[d][d2] = fma(wa * wb, v, acc[d][d2]);
-                }
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int d2 = 0; d2 < 3; ++d2) Kc[E * NE * NE + (A * 3 + d) * NE + B * 3 + d2] = acc[d][d2];
+}
+
+// Children with Dirichlet nodes: block (A, B) of P_c^T (x_c M K0 M + D_c) P_c added to K_E.  A fixed
+// grid of waves walks the list (its length is read on the device).
+__global__ void __launch_bounds__(256)
+galerkin_fine_bc_kernel(GridDims gf, GridDims gc, const double* __restrict__ K0, const double* __restrict__ xe,
+                        const uint8_t* __restrict__ bcm, double bcdiag, const uint8_t* __restrict__ cflag,
+                        const int* __restrict__ list, const int* __restrict__ count, double* __restrict__ Kc) {
+  constexpr int NE = 24;
+  __shared__ double sK0[NE * NE];
+  const int n = *count;
+  if (blockIdx.x * (blockDim.x >> 6) >= n) return;  // whole block leaves together
+  for (int q = threadIdx.x; q < NE * NE; q += blockDim.x) sK0[q] = K0[q];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int A = lane >> 3, B = lane & 7;
+  const int nwaves = gridDim.x * (blockDim.x >> 6);
+  for (int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < n; w += nwaves) {
+    const long long E = list[w];
+    const int CI = (int)(E % gc.ex);
+    const long long tt = E / gc.ex;
+    const int CJ = (int)(tt % gc.ey), CK = (int)(tt / gc.ey);
+    const unsigned flags = cflag[E];
+    double acc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int c = 0; c < 8; ++c) {
+      if (!((flags >> c) & 1u)) continue;
+      const int fi = 2 * CI + (c & 1), fj = 2 * CJ + ((c >> 1) & 1), fk = 2 * CK + ((c >> 2) & 1);
+      const double xv = xe[((long long)fk * gf.ey + fj) * gf.ex + fi];
+#pragma unroll 1
+      for (int a = 0; a < 8; ++a) {
+        const double wa = pweight<3>(c, a, A);
+        if (wa == 0.0) continue;
+        const int ai = fi + (a & 1), aj = fj + ((a >> 1) & 1), ak = fk + ((a >> 2) & 1);
+        const unsigned ma = bcm[((long long)ak * gf.ny + aj) * gf.nx + ai];
+#pragma unroll 1
+        for (int b = 0; b < 8; ++b) {
+          const double wb = pweight<3>(c, b, B);
+          if (wb == 0.0) continue;
+          const unsigned mb =
+              bcm[((long long)(fk + ((b >> 2) & 1)) * gf.ny + fj + ((b >> 1) & 1)) * gf.nx + fi + (b & 1)];
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int d2 = 0; d2 < 3; ++d2) {
+              const bool fa = (ma >> d) & 1u, fb = (mb >> d2) & 1u;
+              double v = (fa || fb) ? 0.0 : xv * sK0[(a * 3 + d) * NE + b * 3 + d2];
+              if (a == b && d == d2 && fa) {
+                const int mult = ((ai > 0) + (ai < gf.nx - 1)) * ((aj > 0) + (aj < gf.ny - 1)) *
+                                 ((ak > 0) + (ak < gf.nz - 1));
+                v += bcdiag / (double)mult;
+              }
+              acc[d][d2] = fma(wa * wb, v, acc[d][d2]);
             }
-          }
         }
       }
-#pragma unroll
-      for (int d = 0; d < 3; ++d)
-#pragma unroll
-        for (int d2 = 0; d2 < 3; ++d2) Kc[E * NE * NE + (A * 3 + d) * NE + B * 3 + d2] = acc[d][d2];
     }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int d2 = 0; d2 < 3; ++d2) Kc[E * NE * NE + (A * 3 + d) * NE + B * 3 + d2] += acc[d][d2];
   }
 }
 
@@ -487,10 +520,16 @@ void galerkin_child_matrices(int dim, const double* K0, double* G) {
 }
 
 int launch_galerkin_fine(const GridDims& gf, const double* K0, const double* G, const double* xe, const uint8_t* bcm,
-                         double bcdiag, double* Kc, hipStream_t s) {
+                         double bcdiag, double* Kc, hipStream_t s, uint8_t* cflag, int* list) {
   GridDims gc = make_grid(gf.ex / 2, gf.ey / 2, gf.dim == 3 ? gf.ez / 2 : 0);
-  if (G != nullptr && gf.dim == 3) {
-    galerkin_fine_fast_kernel<<<(unsigned)((gc.nel + 3) / 4), 256, 0, s>>>(gf, gc, G, K0, xe, bcm, bcdiag, Kc);
+  if (G != nullptr && gf.dim == 3 && cflag != nullptr && list != nullptr) {
+    int* count = list + gc.nel;  // one int after the list
+    zero_int_kernel<<<1, 1, 0, s>>>(count);
+    galerkin_bc_flags_kernel<<<blocks_for(gc.nel, 256), 256, 0, s>>>(gf, gc, bcm, cflag, list, count);
+    galerkin_fine_fast_kernel<<<blocks_for(gc.nel * 64, 256), 256, 0, s>>>(gf, gc, G, xe, cflag, Kc);
+    // one wave per listed element for up to 32768 of them (the rest loop); surplus blocks leave at once
+    const long long bc_blocks = (gc.nel + 3) / 4 < 8192 ? (gc.nel + 3) / 4 : 8192;
+    galerkin_fine_bc_kernel<<<(unsigned)bc_blocks, 256, 0, s>>>(gf, gc, K0, xe, bcm, bcdiag, cflag, list, count, Kc);
     return check_launch("galerkin_fine_fast_kernel");
   }
   if (gf.dim == 3)

@@ -19,9 +19,9 @@ int k0_matches_pattern(int dim, const double* K0_host);
 constexpr int kK0ReflOffset64 = 576;  // 72 FP64 block entries after K0
 constexpr int kK0ReflOffset32 = 648;  // 72 FP32 block entries (36 doubles of storage)
 bool k0_reflection_table(const double* K0_host, double* tab);
-int refl_tiles_xy(const GridDims& g, int rows);
 template <typename T>
-int launch_fine_refl(int mode, const GridDims& g, int nzc, int rows, const double* K0ext, const T* xe,
+int launch_fine_refl(int mode, const GridDims& g, int planes /* 0 = from the occupancy */, int rows,
+                     const double* K0ext, const T* xe,
                      const uint8_t* bcm, double bcdiag, const T* u, const T* b, const T* dinv, double omega, T* y,
                      hipStream_t s, const T* up, double beta);
 int set_tuning(int param, int value);
@@ -72,8 +72,9 @@ int launch_to_double(long long n, const float* x, double* y, hipStream_t s);
 // G (optional, 2^D fixed matrices P_c^T K0 P_c from galerkin_child_matrices): fast path away from
 // Dirichlet nodes
 void galerkin_child_matrices(int dim, const double* K0_host, double* G_host);
+// cflag (nel of the coarse grid) and list (nel + 1 ints) are work space of the fast path.
 int launch_galerkin_fine(const GridDims& gf, const double* K0, const double* G, const double* xe, const uint8_t* bcm,
-                         double bcdiag, double* Kc, hipStream_t s);
+                         double bcdiag, double* Kc, hipStream_t s, uint8_t* cflag = nullptr, int* list = nullptr);
 int launch_galerkin_coarse(const GridDims& gf, const double* Kf, double* Kc, hipStream_t s);
 int launch_stencil_from_elements(const GridDims& g, const double* Ke, double* S, hipStream_t s);
 int launch_dense_from_stencil(const GridDims& g, const double* S, double* A, hipStream_t s);

@@ -349,6 +349,8 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
   const int NS = mg->dim == 3 ? 243 : 36;
   ALLOC(mg->K0, (long long)PMT_K0_EXT_LEN);  // K0 followed by its reflection-basis blocks
   ALLOC(mg->G, (long long)(1 << mg->dim) * NE * NE);
+  if (rc == PMT_OK) rc = dalloc(mg, &mg->gflag, mg->L[1].g.nel);
+  if (rc == PMT_OK) rc = dalloc(mg, &mg->glist, mg->L[1].g.nel + 1);
   for (int l = 0; l < nlevels; ++l) {
     Level& L = mg->L[l];
     if (l == nlevels - 1) {
@@ -453,7 +455,7 @@ int pmt_mg_update(pmt_mg* mg, const double* xe, const uint8_t* bcmask, double bc
     Level& L = mg->L[l];
     Level& P = mg->L[l - 1];
     if (l == 1)
-      PMT_TRY(launch_galerkin_fine(P.g, mg->K0, mg->G, xe, bcmask, bcdiag, L.Ke, s));
+      PMT_TRY(launch_galerkin_fine(P.g, mg->K0, mg->G, xe, bcmask, bcdiag, L.Ke, s, mg->gflag, mg->glist));
     else
       PMT_TRY(launch_galerkin_coarse(P.g, P.Ke, L.Ke, s));
     PMT_TRY(launch_stencil_from_elements(L.g, L.Ke, L.d.S, s));

@@ -56,6 +56,8 @@ struct pmt_mg {
   int NE = 24;
   double* K0 = nullptr;
   double* G = nullptr;  // P_c^T K0 P_c per child position (fast fine -> level 1 Galerkin)
+  uint8_t* gflag = nullptr;  // per level-1 element: children with Dirichlet nodes (work space)
+  int* glist = nullptr;      // level-1 elements with such children, then their count (work space)
   int k0pat = 0;
   const double* xe = nullptr;
   float* xe32 = nullptr;

@@ -160,7 +160,7 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
     return bload(rx, o_el, T());
   };
   auto load_out = [&](int p, T(&bo)[3], T(&dv)[3], T(&po)[3]) {
-    const bool pin = p >= 0 && p < kend;
+    const bool pin = p >= kbeg && p < kend;
     const long long first = 3 * (long long)p * uplane, cnt = 3 * uplane;
 #pragma unroll
     for (int d = 0; d < 3; ++d) bo[d] = dv[d] = po[d] = (T)0;
@@ -186,17 +186,17 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
   };
 
   // Software pipeline over a ring of three slots (plane k in slot (k - kbeg + 1) % 3): the node values
-  // of plane k+3, the densities of layer k+3 and the update operands of plane k+3 are loaded at the
-  // end of layer k, after the stores of plane k, into the slot plane k frees; the node values are
-  // used two layers later, the rest three.  The loop is unrolled by three so every slot is a fixed
-  // register set: no copies, and no wait falls on a memory operation issued within the last layer.
-  T raw[3][3], xs[3], bo[3][3], dv[3][3], po[3][3];
+  // of plane k+3 and the densities of layer k+3 are loaded at the end of layer k, after the stores of
+  // plane k, into the slot plane k frees; the node values are used two layers later, the densities
+  // three.  The loop is unrolled by three so every slot is a fixed register set: no copies, and no
+  // wait falls on a memory operation issued within the last layer.  The update operands (b, D^-1,
+  // previous iterate) of plane k are loaded at the start of layer k and used at its end.
+  T raw[3][3], xs[3];
   unsigned mk[3];
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
     load_node(kbeg - 1 + r, raw[r], mk[r]);
     xs[r] = load_x(kbeg - 1 + r);
-    load_out(kbeg - 1 + r, bo[r], dv[r], po[r]);
   }
   {
     T c[3];
@@ -208,6 +208,8 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
 
   auto layer = [&](int k, auto slot) {
     constexpr int SA = decltype(slot)::value, SB = (SA + 1) % 3;
+    T bo[3], dv[3], po[3];
+    load_out(k, bo, dv, po);
     T(*const lo)[NT] = su[k & 1];        // plane k (written by the previous layer)
     T(*const hi)[NT] = su[(k + 1) & 1];  // plane k+1
     T c0[3], c1[3];
@@ -261,11 +263,11 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
         if (MODE == MODE_APPLY)
           out = ku;
         else if (MODE == MODE_RESIDUAL)
-          out = bo[SA][d] - ku;
+          out = bo[d] - ku;
         else if (MODE == MODE_JACOBI)
-          out = uk + omega * dv[SA][d] * (bo[SA][d] - ku);
+          out = uk + omega * dv[d] * (bo[d] - ku);
         else
-          out = uk + omega * dv[SA][d] * (bo[SA][d] - ku) + beta * (uk - po[SA][d]);
+          out = uk + omega * dv[d] * (bo[d] - ku) + beta * (uk - po[d]);
         bstore(ry, o_out + d * TS, out);
       }
     }
@@ -275,7 +277,6 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
     __builtin_amdgcn_sched_barrier(0);  // keep the prefetch behind the stores
     load_node(k + 3, raw[SA], mk[SA]);
     xs[SA] = load_x(k + 3);
-    load_out(k + 3, bo[SA], dv[SA], po[SA]);
   };
 
   for (int k = kbeg - 1;; k += 3) {
@@ -313,18 +314,59 @@ bool k0_reflection_table(const double* K0, double* tab) {
   return true;
 }
 
+// Resident blocks per CU of one kernel instance (cached per device).
+template <int SY, int MODE, typename T>
+static int refl_resident_blocks() {
+  static int cache[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 2;
+  if (cache[dev] == 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fine_refl_kernel<SY, MODE, T>, kReflRow * SY, 0) !=
+            hipSuccess ||
+        nb < 1)
+      nb = 2;
+    cache[dev] = nb;
+  }
+  return cache[dev];
+}
+
+// Planes per block.  A CU's share of the work is ceil(blocks / CUs) blocks of (planes + 1) layers
+// (one prologue layer each); minimise that, among the chunkings that still give every CU at least
+// two resident blocks (latency hiding) whenever the grid allows it.
+static int refl_planes(int nxy, int nz, int resident) {
+  const long long cus = device_cu_count();
+  const long long want = cus * (resident < 2 ? resident : 2);
+  int best = 0;
+  long long best_cost = -1;
+  for (int pass = 0; pass < 2 && best == 0; ++pass)
+    for (int c = 2; c <= nz; ++c) {
+      const long long blocks = (long long)nxy * ((nz + c - 1) / c);
+      if (pass == 0 && blocks < want) continue;
+      const long long cost = ((blocks + cus - 1) / cus) * (c + 1);
+      if (best_cost < 0 || cost < best_cost) {
+        best_cost = cost;
+        best = c;
+      }
+    }
+  return best > 0 ? best : nz;
+}
+
 template <int SY, typename T>
-static void launch_refl_shape(int mode, const GridDims& g, int nzc, const T* tab, const T* xe, const uint8_t* bcm,
+static void launch_refl_shape(int mode, const GridDims& g, int planes, const T* tab, const T* xe, const uint8_t* bcm,
                               T bcdiag, const T* u, const T* b, const T* dinv, T omega, T* y, hipStream_t s,
                               const T* up, T beta) {
   const int tx = (g.nx + kReflRow - 3) / (kReflRow - 2);
   const int ty = (g.ny + SY - 3) / (SY - 2);
   const int nxy = tx * ty;
-  const int nwork = nxy * ((g.nz + nzc - 1) / nzc);
-  dim3 grid((unsigned)((nwork + 7) & ~7));
-#define PMT_REFL(M)                                                                                            \
-  fine_refl_kernel<SY, M, T><<<grid, kReflRow * SY, 0, s>>>(g, tx, nxy, nzc, nwork, tab, 0, xe, bcm, bcdiag, u, \
-                                                              b, dinv, omega, y, up, beta)
+#define PMT_REFL(M)                                                                                             \
+  do {                                                                                                          \
+    const int nzc = planes > 0 ? (planes < g.nz ? planes : g.nz)                                               \
+                               : refl_planes(nxy, g.nz, refl_resident_blocks<SY, M, T>());                      \
+    const int nwork = nxy * ((g.nz + nzc - 1) / nzc);                                                          \
+    fine_refl_kernel<SY, M, T><<<(unsigned)((nwork + 7) & ~7), kReflRow * SY, 0, s>>>(                         \
+        g, tx, nxy, nzc, nwork, tab, 0, xe, bcm, bcdiag, u, b, dinv, omega, y, up, beta);                        \
+  } while (0)
   if (mode == MODE_APPLY)
     PMT_REFL(MODE_APPLY);
   else if (mode == MODE_RESIDUAL)
@@ -336,20 +378,16 @@ static void launch_refl_shape(int mode, const GridDims& g, int nzc, const T* tab
 #undef PMT_REFL
 }
 
-int refl_tiles_xy(const GridDims& g, int rows) {
-  return ((g.nx + kReflRow - 3) / (kReflRow - 2)) * ((g.ny + rows - 3) / (rows - 2));
-}
-
 template <typename T>
-int launch_fine_refl(int mode, const GridDims& g, int nzc, int rows, const double* K0ext, const T* xe,
+int launch_fine_refl(int mode, const GridDims& g, int planes, int rows, const double* K0ext, const T* xe,
                      const uint8_t* bcm, double bcdiag, const T* u, const T* b, const T* dinv, double omega, T* y,
                      hipStream_t s, const T* up, double beta) {
   const T* tab = sizeof(T) == 8 ? (const T*)(K0ext + kK0ReflOffset64) : (const T*)(K0ext + kK0ReflOffset32);
   const T bd = (T)bcdiag, om = (T)omega, be = (T)beta;
   if (rows == 32)
-    launch_refl_shape<32, T>(mode, g, nzc, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
+    launch_refl_shape<32, T>(mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
   else
-    launch_refl_shape<16, T>(mode, g, nzc, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
+    launch_refl_shape<16, T>(mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
   return check_launch("fine_refl_kernel");
 }
 

@@ -239,13 +239,36 @@ def _hierarchies(nel, levels, x, bc, omega=0.6, steps=2):
     return Kref, omg, K, mg
 
 
-@pytest.mark.parametrize("nel,levels", [((16, 8, 0), 3), ((8, 4, 4), 3), ((16, 8, 8), 4)])
-def test_galerkin_levels_and_vcycle_vs_oracle(gpu_lib, nel, levels):
+def _mixed_bc3d(nel):
+    """Partial Dirichlet sets on several faces: x-dof on x = 0, y-dof on y = 0, z-dof on z = max,
+    all dofs on one interior node line - children with every mask pattern."""
+    nx, ny, nz = nel[0] + 1, nel[1] + 1, nel[2] + 1
+    node = lambda i, j, k: (k * ny + j) * nx + i
+    dofs = []
+    for k in range(nz):
+        for j in range(ny):
+            dofs.append(3 * node(0, j, k))
+    for k in range(nz):
+        for i in range(nx):
+            dofs.append(3 * node(i, 0, k) + 1)
+    for j in range(ny):
+        for i in range(nx):
+            dofs.append(3 * node(i, j, nz - 1) + 2)
+    for k in range(nz):
+        dofs += [3 * node(nx // 2, ny // 2, k) + d for d in range(3)]
+    return np.unique(np.array(dofs))
+
+
+@pytest.mark.parametrize("nel,levels,bcs", [((16, 8, 0), 3, "std"), ((8, 4, 4), 3, "std"), ((16, 8, 8), 4, "std"),
+                                            ((16, 8, 8), 3, "mixed")])
+def test_galerkin_levels_and_vcycle_vs_oracle(gpu_lib, nel, levels, bcs):
     dim = 3 if nel[2] else 2
     rng = np.random.default_rng(11)
     ne = nel[0] * nel[1] * max(nel[2], 1)
     x = fe.simp(rng.uniform(0.05, 1.0, ne))
     bc = problems.cantilever3d(*nel)[0] if dim == 3 else problems.mbb2d(nel[0], nel[1])[0]
+    if bcs == "mixed":
+        bc = _mixed_bc3d(nel)
     Kref, omg, K, mg = _hierarchies(nel, levels, x, bc)
     # Galerkin operators level by level
     ops, lvl = [Kref], omg
