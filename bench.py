@@ -57,6 +57,9 @@ def parse():
     p.add_argument("--learned-samples", type=int, default=24, help="training pairs for '+learned' variants")
     p.add_argument("--learned-epochs", type=int, default=4)
     p.add_argument("--cpu-nel", type=int, nargs=3, default=[48, 24, 24])
+    p.add_argument("--ku-only", action="store_true",
+                   help="kernel-only driver for hardware counters: 5 fine FP64 K.u launches, then 5 copies of one "
+                        "nodal vector (calibration of the byte counters); prints no JSON line")
     return p.parse_args()
 
 
@@ -175,6 +178,23 @@ def kernel_roofline(pm, torch, nel, reps=20):
         if tuple(rec.get("nel", ())) == tuple(nel):
             out["traffic"] = rec["hbm_bytes_per_launch"]
     return out
+
+
+def ku_driver(pm, torch, nel, reps=5):
+    """Kernel-only run for rocprofv3 --pmc: the fine FP64 K.u, then a plain copy of one nodal vector."""
+    from pymoto_amd import problems
+    d = pm.VoxelDomain(*nel)
+    bc, _ = problems.cantilever3d(d)
+    x = torch.as_tensor(np.random.default_rng(0).uniform(1e-3, 1.0, d.nel), device="cuda")
+    K = pm.AssembleStiffness(d, bc=bc)(x)
+    u = torch.randn(3 * d.nnodes, dtype=torch.float64, device="cuda")
+    for _ in range(reps):
+        K @ u
+    c = torch.empty_like(u)
+    for _ in range(reps):
+        c.copy_(u)
+    torch.cuda.synchronize()
+    print(f"K.u x{reps}, copy x{reps}: vector of {u.numel()} FP64 = {u.numel() * 8 / 1024:.1f} KiB", flush=True)
 
 
 def batched_workload(pm, torch, nel=(128, 64, 64), beta=8.0, etas=(0.3, 0.5, 0.7), nloads=4, nuneven=16, seed=7,
@@ -326,6 +346,9 @@ def main():
 
     nel = tuple(args.nel)
     nelt = nel[0] * nel[1] * nel[2]
+    if args.ku_only:
+        ku_driver(pm, torch, nel)
+        return
     nsteps = args.warmup + args.steps
     xs = [torch.as_tensor(np.random.default_rng(1000 + k).uniform(0.0, 1.0, nelt), device="cuda")
           for k in range(nsteps)]

@@ -264,6 +264,81 @@ galerkin_kernel(GridDims gf, GridDims gc, const double* __restrict__ K0, const d
     for (int d2 = 0; d2 < DIM; ++d2) Kc[E * NE * NE + (tA * DIM + d) * NE + tB * DIM + d2] = acc[d][d2];
 }
 
+// Level-1 stencil straight from the fine densities (hex8): S[o](n) = sum over the coarse elements E
+// holding n and n+o of K_E(loc n, loc n+o), with K_E = sum_c x_c G_c for elements whose children
+// have no Dirichlet node - read from the fine densities, never from the stored K_E - and the stored
+// K_E (fast part plus Dirichlet children) for the few flagged elements.  Thread = coarse node; the
+// 64 child densities of its 8 elements are held in registers, the offset/element/child loops are
+// unrolled so every G entry is a compile-time offset (scalar loads).
+__global__ void __launch_bounds__(256)
+stencil_from_fine_kernel(GridDims gf, GridDims gc, const double* __restrict__ G, const double* __restrict__ xe,
+                         const uint8_t* __restrict__ cflag, const double* __restrict__ Ke, double* __restrict__ S) {
+  constexpr int NE = 24;
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= gc.nnodes) return;
+  const int i = (int)(n % gc.nx);
+  const long long t = n / gc.nx;
+  const int j = (int)(t % gc.ny);
+  const int k = (int)(t / gc.ny);
+  const long long nn = gc.nnodes;
+  double xv[8][8];
+  unsigned fl[8];
+  long long Eix[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int ei = i - 1 + (e & 1), ej = j - 1 + ((e >> 1) & 1), ek = k - 1 + ((e >> 2) & 1);
+    const bool ok = ei >= 0 && ei < gc.ex && ej >= 0 && ej < gc.ey && ek >= 0 && ek < gc.ez;
+    const long long E = ok ? ((long long)ek * gc.ey + ej) * gc.ex + ei : 0;
+    Eix[e] = E;
+    fl[e] = ok ? cflag[E] : 0u;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      double v = 0.0;
+      if (ok) v = xe[((long long)(2 * ek + ((c >> 2) & 1)) * gf.ey + (2 * ej + ((c >> 1) & 1))) * gf.ex + 2 * ei + (c & 1)];
+      xv[e][c] = fl[e] ? 0.0 : v;  // flagged elements come from the stored K_E below
+    }
+  }
+  int o = 0;
+#pragma unroll
+  for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx, ++o) {
+        double blk[3][3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+          for (int d2 = 0; d2 < 3; ++d2) blk[d][d2] = 0.0;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int ex = e & 1, ey = (e >> 1) & 1, ez = (e >> 2) & 1;
+          const int bx = dx + 1 - ex, by = dy + 1 - ey, bz = dz + 1 - ez;
+          if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;  // compile time
+          const int A = (1 - ex) + 2 * (1 - ey) + 4 * (1 - ez), B = bx + 2 * by + 4 * bz;
+#pragma unroll
+          for (int c = 0; c < 8; ++c) {
+            const double* g = G + c * NE * NE + (A * 3) * NE + B * 3;
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+              for (int d2 = 0; d2 < 3; ++d2) blk[d][d2] = fma(xv[e][c], g[d * NE + d2], blk[d][d2]);
+          }
+          if (fl[e]) {
+            const double* KE = Ke + Eix[e] * NE * NE + (A * 3) * NE + B * 3;
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+              for (int d2 = 0; d2 < 3; ++d2) blk[d][d2] += KE[d * NE + d2];
+          }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+          for (int d2 = 0; d2 < 3; ++d2) S[((long long)(o * 3 + d) * 3 + d2) * nn + n] = blk[d][d2];
+      }
+}
+
 // Per-node block stencil S[(o*D+d)*D+d2][n] = sum over elements E holding n and n+o of K_E(loc n, loc n+o)
 template <int DIM>
 __global__ void stencil_from_elements_kernel(GridDims g, const double* __restrict__ Ke, double* __restrict__ S) {
@@ -546,6 +621,13 @@ int launch_galerkin_coarse(const GridDims& gf, const double* Kf, double* Kc, hip
   else
     galerkin_kernel<2, false><<<(unsigned)gc.nel, 16, 0, s>>>(gf, gc, nullptr, nullptr, nullptr, 0.0, Kf, Kc, 0);
   return check_launch("galerkin_kernel<coarse>");
+}
+
+int launch_stencil_from_fine(const GridDims& gf, const double* G, const double* xe, const uint8_t* cflag,
+                             const double* Ke, double* S, hipStream_t s) {
+  GridDims gc = make_grid(gf.ex / 2, gf.ey / 2, gf.ez / 2);
+  stencil_from_fine_kernel<<<blocks_for(gc.nnodes, 256), 256, 0, s>>>(gf, gc, G, xe, cflag, Ke, S);
+  return check_launch("stencil_from_fine_kernel");
 }
 
 int launch_stencil_from_elements(const GridDims& g, const double* Ke, double* S, hipStream_t s) {

@@ -66,6 +66,34 @@ __device__ inline long long xcd_block() {
   return (long long)(blockIdx.x & 7u) * per + (blockIdx.x >> 3);
 }
 
+// Raw buffer descriptors (gfx9 family): the hardware range check returns 0 for a load and drops a
+// store whose byte offset is outside [0, records), so a lane with nothing to access uses offset kOOB
+// and every lane issues the same memory instructions - no branch around them, exact wait counters.
+// A descriptor covers at most 2 GiB: callers build one per node plane or per coefficient row.
+constexpr int kOOB = 0x7fffffff;
+constexpr int kRsrcWord3 = 0x00020000;  // raw 32-bit buffer
+
+template <typename T>
+__device__ inline __amdgpu_buffer_rsrc_t plane_rsrc(const T* base, long long first, long long count, bool valid) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)(valid ? base + first : base), 0,
+                                           valid ? (int)(count * (long long)sizeof(T)) : 0, kRsrcWord3);
+}
+
+typedef unsigned int pmt_u32x2 __attribute__((ext_vector_type(2)));
+
+__device__ inline double bload(__amdgpu_buffer_rsrc_t r, int off, double) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ inline float bload(__amdgpu_buffer_rsrc_t r, int off, float) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ inline void bstore(__amdgpu_buffer_rsrc_t r, int off, double v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(pmt_u32x2, v), r, off, 0, 0);
+}
+__device__ inline void bstore(__amdgpu_buffer_rsrc_t r, int off, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, off, 0, 0);
+}
+
 constexpr int kReduceBlocks = 1024;  // fixed grid for deterministic two-stage reductions
 constexpr int kReduceThreads = 256;
 

@@ -21,7 +21,8 @@ constexpr int kK0ReflOffset32 = 648;  // 72 FP32 block entries (36 doubles of st
 bool k0_reflection_table(const double* K0_host, double* tab);
 template <typename T>
 int launch_fine_refl(int mode, const GridDims& g, int planes /* 0 = from the occupancy */, int rows,
-                     const double* K0ext, const T* xe,
+                     int pattern /* K0 has the box class pattern: skip its structural zeros */, const double* K0ext,
+                     const T* xe,
                      const uint8_t* bcm, double bcdiag, const T* u, const T* b, const T* dinv, double omega, T* y,
                      hipStream_t s, const T* up, double beta);
 int set_tuning(int param, int value);
@@ -77,6 +78,10 @@ int launch_galerkin_fine(const GridDims& gf, const double* K0, const double* G, 
                          double bcdiag, double* Kc, hipStream_t s, uint8_t* cflag = nullptr, int* list = nullptr);
 int launch_galerkin_coarse(const GridDims& gf, const double* Kf, double* Kc, hipStream_t s);
 int launch_stencil_from_elements(const GridDims& g, const double* Ke, double* S, hipStream_t s);
+// level-1 stencil of a hex8 grid from the fine densities and G; Ke (after launch_galerkin_fine with the
+// same cflag) is read only for elements with Dirichlet children
+int launch_stencil_from_fine(const GridDims& gf, const double* G, const double* xe, const uint8_t* cflag,
+                             const double* Ke, double* S, hipStream_t s);
 int launch_dense_from_stencil(const GridDims& g, const double* S, double* A, hipStream_t s);
 int gj_invert(long long n, double* A, double* wsP, double* wsC, hipStream_t s);
 template <typename T>

@@ -458,7 +458,10 @@ int pmt_mg_update(pmt_mg* mg, const double* xe, const uint8_t* bcmask, double bc
       PMT_TRY(launch_galerkin_fine(P.g, mg->K0, mg->G, xe, bcmask, bcdiag, L.Ke, s, mg->gflag, mg->glist));
     else
       PMT_TRY(launch_galerkin_coarse(P.g, P.Ke, L.Ke, s));
-    PMT_TRY(launch_stencil_from_elements(L.g, L.Ke, L.d.S, s));
+    if (l == 1 && L.g.dim == 3)
+      PMT_TRY(launch_stencil_from_fine(P.g, mg->G, xe, mg->gflag, L.Ke, L.d.S, s));
+    else
+      PMT_TRY(launch_stencil_from_elements(L.g, L.Ke, L.d.S, s));
     if (l < mg->nlev - 1) {
       PMT_TRY(launch_stencil_dinv<double>(L.g, L.d.S, L.d.dinv, s));
       PMT_TRY(launch_stencil_compress<double>(L.g, L.d.S, L.d.H, s));

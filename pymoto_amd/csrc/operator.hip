@@ -509,6 +509,10 @@ __global__ void stencil_compress_kernel(GridDims g, const double* __restrict__ S
     H[(long long)c * g.nnodes + n] = (T)S[((long long)C * DIM * DIM + c) * g.nnodes + n];
 }
 
+// Loads go through buffer descriptors (one per coefficient row, one per vector): a neighbour outside
+// the grid uses offset kOOB and reads 0, so every lane issues the same loads with no branch around
+// them (the bounds-checked version waited on each neighbour).  The neighbours are processed in rows
+// of three (fixed dz, dy); the loads of the next row are issued before the current row is multiplied.
 template <int DIM, int MODE, typename T>
 __global__ void __launch_bounds__(256)
 stencil_half_kernel(GridDims g, const T* __restrict__ H, const T* __restrict__ u, const T* __restrict__ b,
@@ -520,38 +524,60 @@ stencil_half_kernel(GridDims g, const T* __restrict__ H, const T* __restrict__ u
   const int j = (int)(t % g.ny);
   const int k = (int)(t / g.ny);
   const long long nn = g.nnodes;
+  constexpr int TS = (int)sizeof(T);
   constexpr int C = Half<DIM>::C;
+  constexpr int NROW = DIM == 3 ? 9 : 3;  // rows of three neighbours
+  constexpr int NV = DIM + DIM * DIM;     // values per neighbour: its dofs and one block
+  const __amdgpu_buffer_rsrc_t ru = plane_rsrc(u, 0, nn * DIM, true);
+
+  // loads of row r (dz = r / 3 - 1, dy = r % 3 - 1; DIM == 2: dy = r - 1) into v[3][NV]
+  auto load_row = [&](int r, T(&v)[3][NV]) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int dx = q - 1, dy = (DIM == 3 ? r % 3 : r) - 1, dz = DIM == 3 ? r / 3 - 1 : 0;
+      const int o = r * 3 + q;
+      const int ii = i + dx, jj = j + dy, kk = k + dz;
+      const bool in = ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny && kk >= 0 && kk < g.nz;
+      const long long m = n + dx + dy * (long long)g.nx + dz * (long long)g.nx * g.ny;
+      const int om = in ? (int)m * TS : kOOB;          // byte offset of node m in a coefficient row
+      const int ou = in ? (int)(m * DIM) * TS : kOOB;  // byte offset of node m's dofs
+#pragma unroll
+      for (int d2 = 0; d2 < DIM; ++d2) v[q][d2] = bload(ru, ou + d2 * TS, T());
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+#pragma unroll
+        for (int d2 = 0; d2 < DIM; ++d2) {
+          if (o >= C) {  // stored at n: block (n, n+o), entry (d, d2)
+            const __amdgpu_buffer_rsrc_t rh = plane_rsrc(H, (long long)(((o - C) * DIM + d) * DIM + d2) * nn, nn, true);
+            v[q][DIM + d * DIM + d2] = bload(rh, (int)n * TS, T());
+          } else {  // backward: (n, m) = (m, m + o')^T with o' the mirrored offset, stored at m
+            const __amdgpu_buffer_rsrc_t rh =
+                plane_rsrc(H, (long long)((((2 * C - o) - C) * DIM + d2) * DIM + d) * nn, nn, true);
+            v[q][DIM + d * DIM + d2] = bload(rh, om, T());
+          }
+        }
+    }
+  };
   T out[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) out[d] = 0.0;
-  constexpr int ZLO = DIM == 3 ? -1 : 0, ZHI = DIM == 3 ? 1 : 0;
-  int o = 0;
+  auto mul_row = [&](const T(&v)[3][NV]) {
 #pragma unroll
-  for (int dz = ZLO; dz <= ZHI; ++dz) {
+    for (int q = 0; q < 3; ++q)
 #pragma unroll
-    for (int dy = -1; dy <= 1; ++dy) {
+      for (int d = 0; d < DIM; ++d)
 #pragma unroll
-      for (int dx = -1; dx <= 1; ++dx, ++o) {
-        const int ii = i + dx, jj = j + dy, kk = k + dz;
-        if (ii < 0 || ii >= g.nx || jj < 0 || jj >= g.ny || kk < 0 || kk >= g.nz) continue;
-        const long long m = n + dx + dy * (long long)g.nx + dz * (long long)g.nx * g.ny;
-        T um[DIM];
+        for (int d2 = 0; d2 < DIM; ++d2) out[d] = fma(v[q][DIM + d * DIM + d2], v[q][d2], out[d]);
+  };
+  T va[3][NV], vb[3][NV];
+  load_row(0, va);
 #pragma unroll
-        for (int d2 = 0; d2 < DIM; ++d2) um[d2] = u[m * DIM + d2];
-        if (o >= C) {  // stored at n: block (n, n+o)
-          const int ob = o - C;
-#pragma unroll
-          for (int d = 0; d < DIM; ++d)
-#pragma unroll
-            for (int d2 = 0; d2 < DIM; ++d2) out[d] = fma(H[((long long)(ob * DIM + d) * DIM + d2) * nn + n], um[d2], out[d]);
-        } else {  // backward: (n, m) = (m, m + o')^T with o' the mirrored offset, stored at m
-          const int ob = (2 * C - o) - C;
-#pragma unroll
-          for (int d = 0; d < DIM; ++d)
-#pragma unroll
-            for (int d2 = 0; d2 < DIM; ++d2) out[d] = fma(H[((long long)(ob * DIM + d2) * DIM + d) * nn + m], um[d2], out[d]);
-        }
-      }
+  for (int r = 0; r < NROW; r += 2) {
+    if (r + 1 < NROW) load_row(r + 1, vb);
+    mul_row(va);
+    if (r + 1 < NROW) {
+      if (r + 2 < NROW) load_row(r + 2, va);
+      mul_row(vb);
     }
   }
 #pragma unroll
@@ -821,8 +847,8 @@ int launch_fine(int mode, const GridDims& g, const double* K0, int k0_pattern, c
   const T bd = (T)bcdiag, om = (T)omega, be = (T)beta;
   const bool cls = (k0_pattern & PMT_K0_CLASS_PATTERN) != 0;
   if (g.dim == 3 && g_fine_kernel == 2 && (k0_pattern & PMT_K0_REFLECTION)) {
-    return launch_fine_refl<T>(mode, g, g_stream_nzc, g_refl_rows, K0, xe, bcm, bcdiag, u, b, dinv, omega, y, s, up,
-                               beta);
+    return launch_fine_refl<T>(mode, g, g_stream_nzc, g_refl_rows, cls ? 1 : 0, K0, xe, bcm, bcdiag, u, b, dinv, omega,
+                               y, s, up, beta);
   }
   if (g.dim == 3 && g_fine_kernel >= 1) {
     if (cls)

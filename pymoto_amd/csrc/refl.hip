@@ -66,10 +66,41 @@ __device__ inline void wht8(T (&v)[8][3]) {
       }
 }
 
-// v[a][d] (masked corner displacements) -> x * (K0 v)[a][d]
+// The x and y stages of the transform for the four corners (a = ax + 2 ay) of one node plane.
 template <typename T>
+__device__ inline void wht_plane(T (&p)[4][3]) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const T s0 = p[0][d] + p[1][d], d0 = p[0][d] - p[1][d], s1 = p[2][d] + p[3][d], d1 = p[2][d] - p[3][d];
+    p[0][d] = s0 + s1;
+    p[2][d] = s0 - s1;
+    p[1][d] = d0 + d1;
+    p[3][d] = d0 - d1;
+  }
+}
+
+// Entry (d, e) of block chi vanishes for every K0 with the box sign/class pattern: characters 1..6
+// split into a single dof (iso) and a coupled pair.
+__host__ __device__ constexpr int refl_iso(int chi) {
+  return chi == 1 || chi == 6 ? 0 : (chi == 2 || chi == 5 ? 1 : 2);
+}
+__host__ __device__ constexpr bool refl_zero(int chi, int d, int e) {
+  return chi != 0 && chi != 7 && d != e && (d == refl_iso(chi) || e == refl_iso(chi));
+}
+
+// v[0..3] / v[4..7]: the lower / upper plane's masked corner displacements after wht_plane.
+// Returns x * (K0 u_e)[a][d] in v (corner order a = ax + 2 ay + 4 az).  PAT skips the entries that
+// the box pattern makes zero.
+template <bool PAT, typename T>
 __device__ inline void refl_element(const T* __restrict__ tab, T x, T (&v)[8][3]) {
-  wht8(v);
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T p = v[a][d], q = v[a + 4][d];
+      v[a][d] = p + q;
+      v[a + 4][d] = p - q;
+    }
   T w[8][3];
 #pragma unroll
   for (int chi = 0; chi < 8; ++chi) {
@@ -77,7 +108,10 @@ __device__ inline void refl_element(const T* __restrict__ tab, T x, T (&v)[8][3]
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const T* m = tab + chi * 9 + d * 3;
-      w[chi ^ (1 << d)][d] = fma(m[2], in2, fma(m[1], in1, m[0] * in0));
+      T acc = (PAT && refl_zero(chi, d, 0)) ? (T)0 : m[0] * in0;
+      if (!(PAT && refl_zero(chi, d, 1))) acc = fma(m[1], in1, acc);
+      if (!(PAT && refl_zero(chi, d, 2))) acc = fma(m[2], in2, acc);
+      w[chi ^ (1 << d)][d] = acc;
     }
   }
   wht8(w);
@@ -87,43 +121,21 @@ __device__ inline void refl_element(const T* __restrict__ tab, T x, T (&v)[8][3]
     for (int d = 0; d < 3; ++d) v[a][d] = w[a][d];
 }
 
-// Buffer resources: every global access of the kernel goes through a raw buffer descriptor of one
-// node plane (or element layer), whose hardware range check returns 0 for loads and drops stores at
-// an out-of-range offset.  Halo lanes and planes outside the grid therefore issue the same
+// Every global access of the kernel goes through a raw buffer descriptor of one node plane (or element
+// layer), whose hardware range check returns 0 for loads and drops stores at an out-of-range offset
+// (buffer helpers in common.h).  Halo lanes and planes outside the grid therefore issue the same
 // instructions as the others (offset kOOB, or a zero-length plane), no branch surrounds a memory
 // instruction, and the wait counters stay exact: a prefetch is waited for only where it is used,
 // never together with the stores issued after it.
-constexpr int kOOB = 0x7fffffff;
-constexpr int kRsrcWord3 = 0x00020000;  // raw 32-bit buffer (gfx9 family)
 
-template <typename T>
-__device__ inline __amdgpu_buffer_rsrc_t plane_rsrc(const T* base, long long first, long long count, bool valid) {
-  return __builtin_amdgcn_make_buffer_rsrc((void*)(valid ? base + first : base), 0,
-                                           valid ? (int)(count * (long long)sizeof(T)) : 0, kRsrcWord3);
-}
-
-typedef unsigned int pmt_u32x2 __attribute__((ext_vector_type(2)));
-
-__device__ inline double bload(__amdgpu_buffer_rsrc_t r, int off, double) {
-  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-}
-__device__ inline float bload(__amdgpu_buffer_rsrc_t r, int off, float) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
-}
-__device__ inline void bstore(__amdgpu_buffer_rsrc_t r, int off, double v) {
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(pmt_u32x2, v), r, off, 0, 0);
-}
-__device__ inline void bstore(__amdgpu_buffer_rsrc_t r, int off, float v) {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, off, 0, 0);
-}
-
-template <int SY, int MODE, typename T>
+template <int SY, int MODE, bool PAT, typename T>
 __global__ void __launch_bounds__(kReflRow* SY)
 fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, const T* __restrict__ tab, int zero_stride,
                  const T* __restrict__ xe, const uint8_t* __restrict__ bcm, T bcdiag, const T* __restrict__ u,
                  const T* __restrict__ b, const T* __restrict__ dinv, T omega, T* y, const T* up, T beta) {
   constexpr int RX = kReflRow, NT = RX * SY;
   constexpr int TS = (int)sizeof(T);
+  constexpr bool CARRY = sizeof(T) == 4;  // carry the transformed plane to the next layer
   __shared__ T su[2][3][NT];  // masked displacements of the block's nodes, planes of both parities
   __shared__ T so[6][NT];  // corner (., 1, .) results after the x combine, both planes of a layer
 
@@ -198,11 +210,27 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
     load_node(kbeg - 1 + r, raw[r], mk[r]);
     xs[r] = load_x(kbeg - 1 + r);
   }
+  // corners of a node plane after the x and y transform stages, from the own masked values c (written
+  // to LDS plane buffer pl before the barrier inside)
+  auto plane_corners = [&](const T(&c)[3], T(*const pl)[NT], T(&P)[4][3]) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) pl[d][tid] = c[d];
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T yv = pl[d][slot_up];
+      P[0][d] = c[d];
+      P[1][d] = from_upper(c[d]);
+      P[2][d] = yv;
+      P[3][d] = from_upper(yv);
+    }
+    wht_plane(P);
+  };
+  T Wlo[4][3];  // plane k, transformed (FP32: carried from the previous layer's upper plane)
   {
     T c[3];
     masked(raw[0], mk[0], c);
-#pragma unroll
-    for (int d = 0; d < 3; ++d) su[(kbeg - 1) & 1][d][tid] = c[d];
+    plane_corners(c, su[(kbeg - 1) & 1], Wlo);
   }
   T C[3] = {(T)0, (T)0, (T)0};  // own node, plane k: the sum over element layer k-1
 
@@ -210,31 +238,38 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
     constexpr int SA = decltype(slot)::value, SB = (SA + 1) % 3;
     T bo[3], dv[3], po[3];
     load_out(k, bo, dv, po);
-    T(*const lo)[NT] = su[k & 1];        // plane k (written by the previous layer)
-    T(*const hi)[NT] = su[(k + 1) & 1];  // plane k+1
-    T c0[3], c1[3];
-    masked(raw[SA], mk[SA], c0);
+    T c1[3];
     masked(raw[SB], mk[SB], c1);
+    T Whi[4][3];
+    plane_corners(c1, su[(k + 1) & 1], Whi);
+    if (!CARRY) {
+      // FP64: rebuild the lower plane from its own values and the LDS plane the previous layer wrote
+      // (24 fewer live registers than carrying it)
+      T c0[3];
+      masked(raw[SA], mk[SA], c0);
+      T(*const pl)[NT] = su[k & 1];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) hi[d][tid] = c1[d];
-    __syncthreads();
-    // corners a = ax + 2 ay + 4 az of element (i, j, k): own node, +x by DPP, +y row from LDS
+      for (int d = 0; d < 3; ++d) {
+        const T yv = pl[d][slot_up];
+        Wlo[0][d] = c0[d];
+        Wlo[1][d] = from_upper(c0[d]);
+        Wlo[2][d] = yv;
+        Wlo[3][d] = from_upper(yv);
+      }
+      wht_plane(Wlo);
+    }
     T v[8][3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const T y0 = lo[d][slot_up], y1 = hi[d][slot_up];
-      v[0][d] = c0[d];
-      v[1][d] = from_upper(c0[d]);
-      v[2][d] = y0;
-      v[3][d] = from_upper(y0);
-      v[4][d] = c1[d];
-      v[5][d] = from_upper(c1[d]);
-      v[6][d] = y1;
-      v[7][d] = from_upper(y1);
-    }
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        v[a][d] = Wlo[a][d];
+        v[4 + a][d] = Whi[a][d];
+        if (CARRY) Wlo[a][d] = Whi[a][d];
+      }
     // the 72 block entries are re-read through the scalar cache each layer (zero_stride is 0; the
     // compiler cannot hoist the loads, which would exceed the scalar register file)
-    refl_element(tab + zero_stride * k, xs[SA], v);
+    refl_element<PAT>(tab + zero_stride * k, xs[SA], v);
 
     // x combine: s[ay][az] = own corner (0, ay, az) + left neighbour's corner (1, ay, az)
     T s[2][2][3];
@@ -315,14 +350,14 @@ bool k0_reflection_table(const double* K0, double* tab) {
 }
 
 // Resident blocks per CU of one kernel instance (cached per device).
-template <int SY, int MODE, typename T>
+template <int SY, int MODE, bool PAT, typename T>
 static int refl_resident_blocks() {
   static int cache[64] = {0};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 2;
   if (cache[dev] == 0) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fine_refl_kernel<SY, MODE, T>, kReflRow * SY, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fine_refl_kernel<SY, MODE, PAT, T>, kReflRow * SY, 0) !=
             hipSuccess ||
         nb < 1)
       nb = 2;
@@ -352,7 +387,7 @@ static int refl_planes(int nxy, int nz, int resident) {
   return best > 0 ? best : nz;
 }
 
-template <int SY, typename T>
+template <int SY, bool PAT, typename T>
 static void launch_refl_shape(int mode, const GridDims& g, int planes, const T* tab, const T* xe, const uint8_t* bcm,
                               T bcdiag, const T* u, const T* b, const T* dinv, T omega, T* y, hipStream_t s,
                               const T* up, T beta) {
@@ -362,9 +397,9 @@ static void launch_refl_shape(int mode, const GridDims& g, int planes, const T* 
 #define PMT_REFL(M)                                                                                             \
   do {                                                                                                          \
     const int nzc = planes > 0 ? (planes < g.nz ? planes : g.nz)                                               \
-                               : refl_planes(nxy, g.nz, refl_resident_blocks<SY, M, T>());                      \
+                               : refl_planes(nxy, g.nz, refl_resident_blocks<SY, M, PAT, T>());                 \
     const int nwork = nxy * ((g.nz + nzc - 1) / nzc);                                                          \
-    fine_refl_kernel<SY, M, T><<<(unsigned)((nwork + 7) & ~7), kReflRow * SY, 0, s>>>(                         \
+    fine_refl_kernel<SY, M, PAT, T><<<(unsigned)((nwork + 7) & ~7), kReflRow * SY, 0, s>>>(                    \
         g, tx, nxy, nzc, nwork, tab, 0, xe, bcm, bcdiag, u, b, dinv, omega, y, up, beta);                        \
   } while (0)
   if (mode == MODE_APPLY)
@@ -379,22 +414,30 @@ static void launch_refl_shape(int mode, const GridDims& g, int planes, const T* 
 }
 
 template <typename T>
-int launch_fine_refl(int mode, const GridDims& g, int planes, int rows, const double* K0ext, const T* xe,
+int launch_fine_refl(int mode, const GridDims& g, int planes, int rows, int pattern, const double* K0ext, const T* xe,
                      const uint8_t* bcm, double bcdiag, const T* u, const T* b, const T* dinv, double omega, T* y,
                      hipStream_t s, const T* up, double beta) {
   const T* tab = sizeof(T) == 8 ? (const T*)(K0ext + kK0ReflOffset64) : (const T*)(K0ext + kK0ReflOffset32);
   const T bd = (T)bcdiag, om = (T)omega, be = (T)beta;
-  if (rows == 32)
-    launch_refl_shape<32, T>(mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
-  else
-    launch_refl_shape<16, T>(mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
+  pattern = pattern && sizeof(T) == 4;  // FP64: skipping the zeros costs more scalar-register spills than it saves
+  if (rows == 32) {
+    if (pattern)
+      launch_refl_shape<32, true, T>(mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
+    else
+      launch_refl_shape<32, false, T>(mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
+  } else {
+    if (pattern)
+      launch_refl_shape<16, true, T>(mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
+    else
+      launch_refl_shape<16, false, T>(mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
+  }
   return check_launch("fine_refl_kernel");
 }
 
-template int launch_fine_refl<double>(int, const GridDims&, int, int, const double*, const double*, const uint8_t*,
+template int launch_fine_refl<double>(int, const GridDims&, int, int, int, const double*, const double*, const uint8_t*,
                                       double, const double*, const double*, const double*, double, double*,
                                       hipStream_t, const double*, double);
-template int launch_fine_refl<float>(int, const GridDims&, int, int, const double*, const float*, const uint8_t*,
+template int launch_fine_refl<float>(int, const GridDims&, int, int, int, const double*, const float*, const uint8_t*,
                                      double, const float*, const float*, const float*, double, float*, hipStream_t,
                                      const float*, double);
 
