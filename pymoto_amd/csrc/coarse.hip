@@ -17,18 +17,6 @@
 
 namespace pmt {
 
-__host__ __device__ inline double interp_w(int f, int Abit) {
-  return f == 1 ? 0.5 : ((f == 0) == (Abit == 0) ? 1.0 : 0.0);
-}
-
-template <int DIM>
-__host__ __device__ inline double pweight(int c, int a, int A) {
-  double w = 1.0;
-#pragma unroll
-  for (int b = 0; b < DIM; ++b) w *= interp_w(((c >> b) & 1) + ((a >> b) & 1), (A >> b) & 1);
-  return w;
-}
-
 // One block per coarse element; thread t = (A, B) node pair owns the D x D block (A, B) of K_E.
 // FINE: A_c built from x_c, K0, the Dirichlet mask and bcdiag/mult; else read from Kf.
 // Does coarse element (CI, CJ, CK) contain a Dirichlet node among its 3^D fine nodes?
@@ -41,17 +29,16 @@ __device__ inline bool touches_bc(const GridDims& gf, const uint8_t* __restrict_
   return false;
 }
 
-// Fine -> first coarse level (hex8): for a child without Dirichlet nodes (M_c = I, no bcdiag terms)
-// P_c^T (x_c K0) P_c = x_c G_c with G_c = P_c^T K0 P_c fixed, so K_E = sum_c x_c G_c (576*8
-// multiply-adds per coarse element instead of the triple products).  Three passes:
-//  1. galerkin_bc_flags_kernel: per coarse element, the bit set of children with a Dirichlet node,
-//     and the list of coarse elements with any (order of the list is irrelevant: each entry only
-//     touches its own element matrix);
-//  2. galerkin_fine_fast_kernel: K_E = sum over the other children of x_c G_c (thread = (E, A, B));
-//  3. galerkin_fine_bc_kernel: adds the masked triple products of the flagged children (one wave per
-//     listed element, lane = (A, B)).
+// Fine -> first coarse level (hex8).  A child without Dirichlet nodes contributes x_c G_c with
+// G_c = P_c^T K0 P_c fixed (P_c^T (x_c K0) P_c); the children with Dirichlet nodes contribute the masked
+// P_c^T (x_c M K0 M + D_c) P_c, collected per flagged coarse element as a correction dK in a compact
+// list (the order of the list is irrelevant: each entry only touches its own element's correction).
+//  1. galerkin_bc_flags_kernel: per coarse element the bit set of children with a Dirichlet node, the
+//     list of flagged elements and the element -> list position map (-1: not flagged);
+//  2. galerkin_fine_bc_kernel: dK of the listed elements (one wave per element, lane = (A, B)).
 __global__ void galerkin_bc_flags_kernel(GridDims gf, GridDims gc, const uint8_t* __restrict__ bcm,
-                                         uint8_t* __restrict__ cflag, int* __restrict__ list, int* __restrict__ count) {
+                                         uint8_t* __restrict__ cflag, int* __restrict__ list, int* __restrict__ count,
+                                         int* __restrict__ map) {
   const long long E = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (E >= gc.nel) return;
   const int CI = (int)(E % gc.ex);
@@ -69,54 +56,24 @@ __global__ void galerkin_bc_flags_kernel(GridDims gf, GridDims gc, const uint8_t
       if ((m27 >> q) & 1u) flags |= 1u << c;
     }
   cflag[E] = (uint8_t)flags;
-  if (flags) list[atomicAdd(count, 1)] = (int)E;
+  int idx = -1;
+  if (flags) {
+    idx = atomicAdd(count, 1);
+    list[idx] = (int)E;
+  }
+  map[E] = idx;
 }
 
 __global__ void zero_int_kernel(int* p) { *p = 0; }
 
-__global__ void __launch_bounds__(256)
-galerkin_fine_fast_kernel(GridDims gf, GridDims gc, const double* __restrict__ G, const double* __restrict__ xe,
-                          const uint8_t* __restrict__ cflag, double* __restrict__ Kc) {
-  constexpr int NE = 24;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= gc.nel * 64) return;
-  const long long E = t >> 6;  // one coarse element per wave
-  const int A = (int)(t >> 3) & 7, B = (int)t & 7;
-  const int CI = (int)(E % gc.ex);
-  const long long tt = E / gc.ex;
-  const int CJ = (int)(tt % gc.ey), CK = (int)(tt / gc.ey);
-  const unsigned flags = cflag[E];
-  double xc[8];
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const double v = xe[((long long)(2 * CK + ((c >> 2) & 1)) * gf.ey + (2 * CJ + ((c >> 1) & 1))) * gf.ex + 2 * CI + (c & 1)];
-    xc[c] = ((flags >> c) & 1u) ? 0.0 : v;
-  }
-  double acc[3][3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d)
-#pragma unroll
-    for (int d2 = 0; d2 < 3; ++d2) acc[d][d2] = 0.0;
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const double* g = G + c * NE * NE + (A * 3) * NE + B * 3;
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-      for (int d2 = 0; d2 < 3; ++d2) acc[d][d2] = fma(xc[c], g[d * NE + d2], acc[d][d2]);
-  }
-#pragma unroll
-  for (int d = 0; d < 3; ++d)
-#pragma unroll
-    for (int d2 = 0; d2 < 3; ++d2) Kc[E * NE * NE + (A * 3 + d) * NE + B * 3 + d2] = acc[d][d2];
-}
+void zero_int(int* p, hipStream_t s) { zero_int_kernel<<<1, 1, 0, s>>>(p); }
 
-// Children with Dirichlet nodes: block (A, B) of P_c^T (x_c M K0 M + D_c) P_c added to K_E.  A fixed
-// grid of waves walks the list (its length is read on the device).
+// dK of the listed elements: block (A, B) of sum over flagged children of P_c^T (x_c M K0 M + D_c) P_c.
+// A fixed grid of waves walks the list (its length is read on the device).
 __global__ void __launch_bounds__(256)
 galerkin_fine_bc_kernel(GridDims gf, GridDims gc, const double* __restrict__ K0, const double* __restrict__ xe,
                         const uint8_t* __restrict__ bcm, double bcdiag, const uint8_t* __restrict__ cflag,
-                        const int* __restrict__ list, const int* __restrict__ count, double* __restrict__ Kc) {
+                        const int* __restrict__ list, const int* __restrict__ count, double* __restrict__ dK) {
   constexpr int NE = 24;
   __shared__ double sK0[NE * NE];
   const int n = *count;
@@ -168,7 +125,7 @@ galerkin_fine_bc_kernel(GridDims gf, GridDims gc, const double* __restrict__ K0,
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
-      for (int d2 = 0; d2 < 3; ++d2) Kc[E * NE * NE + (A * 3 + d) * NE + B * 3 + d2] += acc[d][d2];
+      for (int d2 = 0; d2 < 3; ++d2) dK[(long long)w * NE * NE + (A * 3 + d) * NE + B * 3 + d2] = acc[d][d2];
   }
 }
 
@@ -265,14 +222,15 @@ galerkin_kernel(GridDims gf, GridDims gc, const double* __restrict__ K0, const d
 }
 
 // Level-1 stencil straight from the fine densities (hex8): S[o](n) = sum over the coarse elements E
-// holding n and n+o of K_E(loc n, loc n+o), with K_E = sum_c x_c G_c for elements whose children
-// have no Dirichlet node - read from the fine densities, never from the stored K_E - and the stored
-// K_E (fast part plus Dirichlet children) for the few flagged elements.  Thread = coarse node; the
-// 64 child densities of its 8 elements are held in registers, the offset/element/child loops are
-// unrolled so every G entry is a compile-time offset (scalar loads).
+// holding n and n+o of K_E(loc n, loc n+o), K_E = sum over unflagged children of x_c G_c plus the
+// correction dK of flagged elements.  Thread = coarse node; the 64 child densities of its 8 elements
+// are held in registers, the offset/element/child loops are unrolled so every G entry is a
+// compile-time offset (scalar loads).
+template <bool HALF>
 __global__ void __launch_bounds__(256)
 stencil_from_fine_kernel(GridDims gf, GridDims gc, const double* __restrict__ G, const double* __restrict__ xe,
-                         const uint8_t* __restrict__ cflag, const double* __restrict__ Ke, double* __restrict__ S) {
+                         const uint8_t* __restrict__ cflag, const int* __restrict__ map, const double* __restrict__ dK,
+                         StencilOut out) {
   constexpr int NE = 24;
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= gc.nnodes) return;
@@ -295,7 +253,7 @@ stencil_from_fine_kernel(GridDims gf, GridDims gc, const double* __restrict__ G,
     for (int c = 0; c < 8; ++c) {
       double v = 0.0;
       if (ok) v = xe[((long long)(2 * ek + ((c >> 2) & 1)) * gf.ey + (2 * ej + ((c >> 1) & 1))) * gf.ex + 2 * ei + (c & 1)];
-      xv[e][c] = fl[e] ? 0.0 : v;  // flagged elements come from the stored K_E below
+      xv[e][c] = ((fl[e] >> c) & 1u) ? 0.0 : v;  // flagged children come from dK below
     }
   }
   int o = 0;
@@ -305,6 +263,7 @@ stencil_from_fine_kernel(GridDims gf, GridDims gc, const double* __restrict__ G,
     for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
       for (int dx = -1; dx <= 1; ++dx, ++o) {
+        if (HALF && o < 13) continue;  // backward blocks: the neighbours' forward blocks transposed
         double blk[3][3];
 #pragma unroll
         for (int d = 0; d < 3; ++d)
@@ -325,17 +284,14 @@ stencil_from_fine_kernel(GridDims gf, GridDims gc, const double* __restrict__ G,
               for (int d2 = 0; d2 < 3; ++d2) blk[d][d2] = fma(xv[e][c], g[d * NE + d2], blk[d][d2]);
           }
           if (fl[e]) {
-            const double* KE = Ke + Eix[e] * NE * NE + (A * 3) * NE + B * 3;
+            const double* K = dK + (long long)map[Eix[e]] * NE * NE + (A * 3) * NE + B * 3;
 #pragma unroll
             for (int d = 0; d < 3; ++d)
 #pragma unroll
-              for (int d2 = 0; d2 < 3; ++d2) blk[d][d2] += KE[d * NE + d2];
+              for (int d2 = 0; d2 < 3; ++d2) blk[d][d2] += K[d * NE + d2];
           }
         }
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-#pragma unroll
-          for (int d2 = 0; d2 < 3; ++d2) S[((long long)(o * 3 + d) * 3 + d2) * nn + n] = blk[d][d2];
+        emit_block<HALF>(out, o, blk, n, nn);
       }
 }
 
@@ -414,46 +370,47 @@ constexpr int GJB = 64;
 // 16 x 16 threads, each owning a 4 x 4 set of entries (rows tr + 16 i, columns tc + 16 j); the block
 // is padded with the identity beyond kk, so every pivot step is the same 16 multiply-adds per thread
 // between two barriers (in-place Gauss-Jordan, no pivoting: the Galerkin operator is SPD).
-__global__ void __launch_bounds__(256) gj_diag_kernel(long long n, long long k0, int kk, const double* __restrict__ A,
-                                                       double* __restrict__ P) {
-  __shared__ double sM[GJB][GJB + 1];
-  const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
+__global__ void __launch_bounds__(1024) gj_diag_kernel(long long n, long long k0, int kk, const double* __restrict__ A,
+                                                        double* __restrict__ P) {
+  // 32 x 32 threads, each owning a 2 x 2 sub-block; two LDS copies of the block alternate between
+  // pivot steps, so one barrier separates a step's reads from the next step's writes
+  __shared__ double sM[2][GJB][GJB + 1];
+  const int tr = threadIdx.x / 32, tc = threadIdx.x % 32;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int r = tr + 16 * i, c = tc + 16 * j;
-      sM[r][c] = (r < kk && c < kk) ? A[(k0 + r) * n + k0 + c] : (r == c ? 1.0 : 0.0);
+    for (int j = 0; j < 2; ++j) {
+      const int r = tr + 32 * i, c = tc + 32 * j;
+      sM[0][r][c] = (r < kk && c < kk) ? A[(k0 + r) * n + k0 + c] : (r == c ? 1.0 : 0.0);
     }
   __syncthreads();
   for (int p = 0; p < kk; ++p) {
-    const double inv = 1.0 / sM[p][p];
-    double mrp[4], mpc[4], v[4][4];
+    double(*const src)[GJB + 1] = sM[p & 1];
+    double(*const dst)[GJB + 1] = sM[(p + 1) & 1];
+    const double inv = 1.0 / src[p][p];
+    double mrp[2], mpc[2];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) mrp[i] = sM[tr + 16 * i][p];
+    for (int i = 0; i < 2; ++i) mrp[i] = src[tr + 32 * i][p];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) mpc[j] = sM[p][tc + 16 * j];
+    for (int j = 0; j < 2; ++j) mpc[j] = src[p][tc + 32 * j];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int r = tr + 16 * i, c = tc + 16 * j;
-        const double m = sM[r][c];
+      for (int j = 0; j < 2; ++j) {
+        const int r = tr + 32 * i, c = tc + 32 * j;
+        const double m = src[r][c];
+        double v;
         if (r == p)
-          v[i][j] = c == p ? inv : m * inv;
+          v = c == p ? inv : m * inv;
         else
-          v[i][j] = c == p ? -mrp[i] * inv : m - mrp[i] * (mpc[j] * inv);
+          v = c == p ? -mrp[i] * inv : m - mrp[i] * (mpc[j] * inv);
+        dst[r][c] = v;
       }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) sM[tr + 16 * i][tc + 16 * j] = v[i][j];
     __syncthreads();
   }
   for (int q = threadIdx.x; q < GJB * GJB; q += blockDim.x) {  // zero outside kk x kk: consumers read full tiles
     const int r = q / GJB, c = q % GJB;
-    P[q] = (r < kk && c < kk) ? sM[r][c] : 0.0;
+    P[q] = (r < kk && c < kk) ? sM[kk & 1][r][c] : 0.0;
   }
 }
 
@@ -594,24 +551,31 @@ void galerkin_child_matrices(int dim, const double* K0, double* G) {
   }
 }
 
-int launch_galerkin_fine(const GridDims& gf, const double* K0, const double* G, const double* xe, const uint8_t* bcm,
-                         double bcdiag, double* Kc, hipStream_t s, uint8_t* cflag, int* list) {
+int launch_galerkin_fine(const GridDims& gf, const double* K0, const double* xe, const uint8_t* bcm, double bcdiag,
+                         double* Kc, hipStream_t s) {
   GridDims gc = make_grid(gf.ex / 2, gf.ey / 2, gf.dim == 3 ? gf.ez / 2 : 0);
-  if (G != nullptr && gf.dim == 3 && cflag != nullptr && list != nullptr) {
-    int* count = list + gc.nel;  // one int after the list
-    zero_int_kernel<<<1, 1, 0, s>>>(count);
-    galerkin_bc_flags_kernel<<<blocks_for(gc.nel, 256), 256, 0, s>>>(gf, gc, bcm, cflag, list, count);
-    galerkin_fine_fast_kernel<<<blocks_for(gc.nel * 64, 256), 256, 0, s>>>(gf, gc, G, xe, cflag, Kc);
-    // one wave per listed element for up to 32768 of them (the rest loop); surplus blocks leave at once
-    const long long bc_blocks = (gc.nel + 3) / 4 < 8192 ? (gc.nel + 3) / 4 : 8192;
-    galerkin_fine_bc_kernel<<<(unsigned)bc_blocks, 256, 0, s>>>(gf, gc, K0, xe, bcm, bcdiag, cflag, list, count, Kc);
-    return check_launch("galerkin_fine_fast_kernel");
-  }
   if (gf.dim == 3)
     galerkin_kernel<3, true><<<(unsigned)gc.nel, 64, 0, s>>>(gf, gc, K0, xe, bcm, bcdiag, nullptr, Kc, 0);
   else
     galerkin_kernel<2, true><<<(unsigned)gc.nel, 16, 0, s>>>(gf, gc, K0, xe, bcm, bcdiag, nullptr, Kc, 0);
   return check_launch("galerkin_kernel<fine>");
+}
+
+int launch_level1_flags(const GridDims& gf, const uint8_t* bcm, uint8_t* cflag, int* list, int* map, hipStream_t s) {
+  GridDims gc = make_grid(gf.ex / 2, gf.ey / 2, gf.ez / 2);
+  int* count = list + gc.nel;
+  zero_int(count, s);
+  galerkin_bc_flags_kernel<<<blocks_for(gc.nel, 256), 256, 0, s>>>(gf, gc, bcm, cflag, list, count, map);
+  return check_launch("galerkin_bc_flags_kernel");
+}
+
+int launch_level1_dk(const GridDims& gf, const double* K0, const double* xe, const uint8_t* bcm, double bcdiag,
+                     const uint8_t* cflag, const int* list, double* dK, hipStream_t s) {
+  GridDims gc = make_grid(gf.ex / 2, gf.ey / 2, gf.ez / 2);
+  // one wave per listed element for up to 32768 of them (the rest loop); surplus blocks leave at once
+  const long long blocks = (gc.nel + 3) / 4 < 8192 ? (gc.nel + 3) / 4 : 8192;
+  galerkin_fine_bc_kernel<<<(unsigned)blocks, 256, 0, s>>>(gf, gc, K0, xe, bcm, bcdiag, cflag, list, list + gc.nel, dK);
+  return check_launch("galerkin_fine_bc_kernel");
 }
 
 int launch_galerkin_coarse(const GridDims& gf, const double* Kf, double* Kc, hipStream_t s) {
@@ -624,9 +588,12 @@ int launch_galerkin_coarse(const GridDims& gf, const double* Kf, double* Kc, hip
 }
 
 int launch_stencil_from_fine(const GridDims& gf, const double* G, const double* xe, const uint8_t* cflag,
-                             const double* Ke, double* S, hipStream_t s) {
+                             const int* map, const double* dK, const StencilOut& out, hipStream_t s) {
   GridDims gc = make_grid(gf.ex / 2, gf.ey / 2, gf.ez / 2);
-  stencil_from_fine_kernel<<<blocks_for(gc.nnodes, 256), 256, 0, s>>>(gf, gc, G, xe, cflag, Ke, S);
+  if (out.S)
+    stencil_from_fine_kernel<false><<<blocks_for(gc.nnodes, 256), 256, 0, s>>>(gf, gc, G, xe, cflag, map, dK, out);
+  else
+    stencil_from_fine_kernel<true><<<blocks_for(gc.nnodes, 256), 256, 0, s>>>(gf, gc, G, xe, cflag, map, dK, out);
   return check_launch("stencil_from_fine_kernel");
 }
 
@@ -655,7 +622,7 @@ int gj_invert(long long n, double* A, double* wsP, double* wsC, hipStream_t s) {
   for (long long kb = 0; kb < nb; ++kb) {
     const long long k0 = kb * GJB;
     const int kk = (int)((n - k0) < GJB ? (n - k0) : GJB);
-    gj_diag_kernel<<<1, 256, 0, s>>>(n, k0, kk, A, wsP);
+    gj_diag_kernel<<<1, 1024, 0, s>>>(n, k0, kk, A, wsP);
     PMT_TRY(check_launch("gj_diag_kernel"));
     gj_colcopy_kernel<<<blocks_for(n * kk, 256), 256, 0, s>>>(n, k0, kk, A, wsC);
     PMT_TRY(check_launch("gj_colcopy_kernel"));

@@ -66,6 +66,58 @@ __device__ inline long long xcd_block() {
   return (long long)(blockIdx.x & 7u) * per + (blockIdx.x >> 3);
 }
 
+// Trilinear (bilinear) prolongation weight of coarse corner A at local fine node a of child c of a
+// coarse element: per dimension f = child bit + node bit in {0,1,2}, w(0,0) = w(2,1) = 1, w(1,*) = 1/2.
+__host__ __device__ inline double interp_w(int f, int Abit) {
+  return f == 1 ? 0.5 : ((f == 0) == (Abit == 0) ? 1.0 : 0.0);
+}
+template <int DIM>
+__host__ __device__ inline double pweight(int c, int a, int A) {
+  double w = 1.0;
+#pragma unroll
+  for (int b = 0; b < DIM; ++b) w *= interp_w(((c >> b) & 1) + ((a >> b) & 1), (A >> b) & 1);
+  return w;
+}
+__host__ __device__ inline double pweight3(int c, int a, int A) { return pweight<3>(c, a, A); }
+
+// Output of one 3 x 3 block (offset o = (dz+1)*9 + (dy+1)*3 + dx+1) of a hex8 node stencil.  Full form:
+// S[(o*3+d)*3+d2][n].  Half form (levels that smooth): the centre and forward blocks o >= 13 as
+// H[((o-13)*3+d)*3+d2][n] (FP64 and, if H32 is given, FP32) and D^-1 from the centre block.
+struct StencilOut {
+  double* S = nullptr;
+  double* H = nullptr;
+  double* dinv = nullptr;
+  float* H32 = nullptr;
+  float* dinv32 = nullptr;
+};
+template <bool HALF>
+__device__ inline void emit_block(const StencilOut& out, int o, const double (&blk)[3][3], long long n, long long nn) {
+  if (!HALF) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int d2 = 0; d2 < 3; ++d2) out.S[((long long)(o * 3 + d) * 3 + d2) * nn + n] = blk[d][d2];
+    return;
+  }
+  if (o < 13) return;
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int d2 = 0; d2 < 3; ++d2) {
+      const long long c = (long long)((o - 13) * 3 + d) * 3 + d2;
+      out.H[c * nn + n] = blk[d][d2];
+      if (out.H32) out.H32[c * nn + n] = (float)blk[d][d2];
+    }
+  if (o == 13) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double v = 1.0 / blk[d][d];
+      out.dinv[n * 3 + d] = v;
+      if (out.dinv32) out.dinv32[n * 3 + d] = (float)v;
+    }
+  }
+}
+
 // Raw buffer descriptors (gfx9 family): the hardware range check returns 0 for a load and drops a
 // store whose byte offset is outside [0, records), so a lane with nothing to access uses offset kOOB
 // and every lane issues the same memory instructions - no branch around them, exact wait counters.

@@ -70,18 +70,34 @@ int launch_fill(long long n, double v, double* y, hipStream_t s);
 int launch_to_double(long long n, const float* x, double* y, hipStream_t s);
 
 // coarse.hip
-// G (optional, 2^D fixed matrices P_c^T K0 P_c from galerkin_child_matrices): fast path away from
-// Dirichlet nodes
+// G: the 2^D fixed matrices P_c^T K0 P_c (galerkin_child_matrices), for the level-1 stencil of hex8 grids
 void galerkin_child_matrices(int dim, const double* K0_host, double* G_host);
-// cflag (nel of the coarse grid) and list (nel + 1 ints) are work space of the fast path.
-int launch_galerkin_fine(const GridDims& gf, const double* K0, const double* G, const double* xe, const uint8_t* bcm,
-                         double bcdiag, double* Kc, hipStream_t s, uint8_t* cflag = nullptr, int* list = nullptr);
+// element matrices of the first coarse level by triple products (Q4 grids)
+int launch_galerkin_fine(const GridDims& gf, const double* K0, const double* xe, const uint8_t* bcm, double bcdiag,
+                         double* Kc, hipStream_t s);
+void zero_int(int* p, hipStream_t s);
+// hex8 first coarse level: children flags, list (nel + 1 ints, count last) and map of flagged elements
+int launch_level1_flags(const GridDims& gf, const uint8_t* bcm, uint8_t* cflag, int* list, int* map, hipStream_t s);
+// corrections dK (576 per listed element) of the flagged level-1 elements
+int launch_level1_dk(const GridDims& gf, const double* K0, const double* xe, const uint8_t* bcm, double bcdiag,
+                     const uint8_t* cflag, const int* list, double* dK, hipStream_t s);
 int launch_galerkin_coarse(const GridDims& gf, const double* Kf, double* Kc, hipStream_t s);
 int launch_stencil_from_elements(const GridDims& g, const double* Ke, double* S, hipStream_t s);
-// level-1 stencil of a hex8 grid from the fine densities and G; Ke (after launch_galerkin_fine with the
-// same cflag) is read only for elements with Dirichlet children
+// level-1 stencil of a hex8 grid from the fine densities, G and the corrections of flagged elements
+// (out.S set: full stencil, for the coarsest level; else half stencil and D^-1, see StencilOut)
 int launch_stencil_from_fine(const GridDims& gf, const double* G, const double* xe, const uint8_t* cflag,
-                             const double* Ke, double* S, hipStream_t s);
+                             const int* map, const double* dK, const StencilOut& out, hipStream_t s);
+
+// moments.hip: Galerkin operators of hex8 levels >= 2 from density moments (27 per element)
+void moment_tables(int level, const double* K0_host, double* C_host);  // 64 * 27 * 9 doubles
+int launch_moments1(const GridDims& gf, const double* xe, const uint8_t* cflag, double* M, hipStream_t s);
+int launch_moments_up(const GridDims& gf, const double* Mf, double* M, hipStream_t s);
+int launch_flags_up(const GridDims& gf, const uint8_t* cflag_f, uint8_t* cflag, int* list, int* map, hipStream_t s);
+int launch_dk_up(const GridDims& gf, const uint8_t* cflag, const int* list, const int* map_f, const double* dKf,
+                 double* dK, hipStream_t s);
+// element matrices (Ke: 576 x nel, structure of arrays) from the moments plus the corrections, then S
+int launch_stencil_from_moments(const GridDims& g, const double* C, const double* M, const int* list,
+                                const double* dK, double* Ke, const StencilOut& out, hipStream_t s);
 int launch_dense_from_stencil(const GridDims& g, const double* S, double* A, hipStream_t s);
 int gj_invert(long long n, double* A, double* wsP, double* wsC, hipStream_t s);
 template <typename T>

@@ -16,6 +16,7 @@
 // pmt_mg_set_precision(mg, 16) the damped-Jacobi sweeps of the fine grid run on the matrix cores
 // with BF16 operands and FP32 accumulation (hex8 only); the fine residual that is restricted to the
 // coarse grid stays FP32, so the coarse correction sees no BF16 rounding.
+#include <algorithm>
 #include <cmath>
 #include <new>
 #include <type_traits>
@@ -42,6 +43,20 @@ static void release(pmt_mg* mg) {
   mg->allocs.clear();
   if (mg->host_sc) (void)hipHostFree(mg->host_sc);
   mg->host_sc = nullptr;
+  if (mg->host_counts) (void)hipHostFree(mg->host_counts);
+  mg->host_counts = nullptr;
+}
+
+// frees one allocation of the handle (bytes: its size, for the accounting)
+static int mg_free(pmt_mg* mg, void* p, long long bytes) {
+  for (size_t q = 0; q < mg->allocs.size(); ++q)
+    if (mg->allocs[q] == p) {
+      mg->allocs.erase(mg->allocs.begin() + q);
+      mg->bytes -= bytes;
+      PMT_HIP(hipFree(p));
+      return PMT_OK;
+    }
+  return arg_error("mg_free: not an allocation of this handle");
 }
 
 template <typename T>
@@ -297,6 +312,62 @@ int mg_alloc(pmt_mg* mg, void** p, long long bytes) {
   return PMT_OK;
 }
 
+// Coarse stencils of a hex8 hierarchy without element matrices (moments.hip): flags of every level
+// (they depend on the mask only), one read of the flagged counts (corrections are grown if needed),
+// corrections, moments, then each level's stencil.
+static int hex_coarse_stencils(pmt_mg* mg, hipStream_t s) {
+  const int nl = mg->nlev;
+  PMT_TRY(launch_level1_flags(mg->L[0].g, mg->bcm, mg->L[1].cflag, mg->L[1].list, mg->L[1].map, s));
+  for (int l = 2; l < nl; ++l)
+    PMT_TRY(launch_flags_up(mg->L[l - 1].g, mg->L[l - 1].cflag, mg->L[l].cflag, mg->L[l].list, mg->L[l].map, s));
+  for (int l = 1; l < nl; ++l)
+    PMT_HIP(hipMemcpyAsync(mg->host_counts + l, mg->L[l].list + mg->L[l].g.nel, sizeof(int), hipMemcpyDeviceToHost,
+                           s));
+  PMT_HIP(hipStreamSynchronize(s));
+  for (int l = 1; l < nl; ++l) {
+    Level& L = mg->L[l];
+    const long long need = mg->host_counts[l];
+    if (need > L.dk_cap) {
+      PMT_TRY(mg_free(mg, L.dK, L.dk_cap * 576 * (long long)sizeof(double)));
+      L.dK = nullptr;
+      L.dk_cap = 0;
+      PMT_TRY(mg_alloc(mg, (void**)&L.dK, need * 576 * (long long)sizeof(double)));
+      L.dk_cap = need;
+    }
+  }
+  PMT_TRY(launch_level1_dk(mg->L[0].g, mg->K0, mg->xe, mg->bcm, mg->bcdiag, mg->L[1].cflag, mg->L[1].list,
+                           mg->L[1].dK, s));
+  for (int l = 2; l < nl; ++l)
+    PMT_TRY(launch_dk_up(mg->L[l - 1].g, mg->L[l].cflag, mg->L[l].list, mg->L[l - 1].map, mg->L[l - 1].dK,
+                         mg->L[l].dK, s));
+  PMT_TRY(launch_moments1(mg->L[0].g, mg->xe, mg->L[1].cflag, mg->L[1].M, s));
+  for (int l = 2; l < nl; ++l) PMT_TRY(launch_moments_up(mg->L[l - 1].g, mg->L[l - 1].M, mg->L[l].M, s));
+  // smoothing levels get the half stencil and D^-1 (FP64, and FP32 for reduced-precision cycles); the
+  // coarsest level the full stencil for its dense inverse
+  auto target = [&](int l) {
+    Level& L = mg->L[l];
+    StencilOut o;
+    if (l == nl - 1) {
+      o.S = L.d.S;
+    } else {
+      o.H = L.d.H;
+      o.dinv = L.d.dinv;
+      if (mg->prec != 64) {
+        o.H32 = L.f.H;
+        o.dinv32 = L.f.dinv;
+      }
+    }
+    return o;
+  };
+  PMT_TRY(launch_stencil_from_fine(mg->L[0].g, mg->G, mg->xe, mg->L[1].cflag, mg->L[1].map, mg->L[1].dK, target(1),
+                                   s));
+  for (int l = 2; l < nl; ++l) {
+    Level& L = mg->L[l];
+    PMT_TRY(launch_stencil_from_moments(L.g, L.C, L.M, L.list, L.dK, L.Ke, target(l), s));
+  }
+  return PMT_OK;
+}
+
 }  // namespace pmt
 
 using namespace pmt;
@@ -349,13 +420,28 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
   const int NS = mg->dim == 3 ? 243 : 36;
   ALLOC(mg->K0, (long long)PMT_K0_EXT_LEN);  // K0 followed by its reflection-basis blocks
   ALLOC(mg->G, (long long)(1 << mg->dim) * NE * NE);
-  if (rc == PMT_OK) rc = dalloc(mg, &mg->gflag, mg->L[1].g.nel);
-  if (rc == PMT_OK) rc = dalloc(mg, &mg->glist, mg->L[1].g.nel + 1);
+  const bool hex = mg->dim == 3;
+  for (int l = 1; l < nlevels && hex; ++l) {
+    Level& L = mg->L[l];
+    if (rc == PMT_OK) rc = dalloc(mg, &L.cflag, L.g.nel);
+    if (rc == PMT_OK) rc = dalloc(mg, &L.list, L.g.nel + 1);
+    if (rc == PMT_OK) rc = dalloc(mg, &L.map, L.g.nel);
+    ALLOC(L.M, 27 * L.g.nel);
+    if (l >= 2) ALLOC(L.C, 64 * 27 * 9);
+    if (l >= 2) ALLOC(L.Ke, L.g.nel * NE * NE);  // structure-of-arrays element matrices
+    // corrections for the elements along the largest face of the level's grid; grown at update if needed
+    const long long face = std::max({(long long)L.g.ex * L.g.ey, (long long)L.g.ey * L.g.ez, (long long)L.g.ex * L.g.ez});
+    L.dk_cap = std::max(1LL, std::min(L.g.nel, face));
+    ALLOC(L.dK, L.dk_cap * NE * NE);
+  }
+  if (rc == PMT_OK)
+    rc = hip_status(hipHostMalloc((void**)&mg->host_counts, kMaxLevels * sizeof(int), hipHostMallocDefault),
+                    "hipHostMalloc");
   for (int l = 0; l < nlevels; ++l) {
     Level& L = mg->L[l];
     if (l == nlevels - 1) {
       if (l > 0) {
-        ALLOC(L.Ke, L.g.nel * NE * NE);
+        if (!hex) ALLOC(L.Ke, L.g.nel * NE * NE);
         ALLOC(L.d.S, L.g.nnodes * NS);
         ALLOC(L.d.u, L.ndof);
         ALLOC(L.d.b, L.ndof);
@@ -366,8 +452,8 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
     ALLOC(L.d.t, L.ndof);
     ALLOC(L.d.r, L.ndof);
     if (l > 0) {
-      ALLOC(L.Ke, L.g.nel * NE * NE);
-      ALLOC(L.d.S, L.g.nnodes * NS);
+      if (!hex) ALLOC(L.Ke, L.g.nel * NE * NE);
+      if (!hex) ALLOC(L.d.S, L.g.nnodes * NS);  // hex8: half stencil only above the coarsest level
       ALLOC(L.d.H, L.g.nnodes * half_stencil_coefficients(mg->dim));
       ALLOC(L.d.u, L.ndof);
       ALLOC(L.d.b, L.ndof);
@@ -393,6 +479,12 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
     std::vector<double> G((size_t)(1 << mg->dim) * NE * NE);
     galerkin_child_matrices(mg->dim, K0_host, G.data());
     rc = hip_status(hipMemcpy(mg->G, G.data(), G.size() * sizeof(double), hipMemcpyHostToDevice), "G upload");
+  }
+  for (int l = 2; l < nlevels && hex && rc == PMT_OK; ++l) {
+    std::vector<double> C(64 * 27 * 9);
+    moment_tables(l, K0_host, C.data());
+    rc = hip_status(hipMemcpy(mg->L[l].C, C.data(), C.size() * sizeof(double), hipMemcpyHostToDevice),
+                    "moment table upload");
   }
   if (rc != PMT_OK) {
     release(mg);
@@ -451,17 +543,21 @@ int pmt_mg_update(pmt_mg* mg, const double* xe, const uint8_t* bcmask, double bc
   block_cg_invalidate(mg);
   Level& F = mg->L[0];
   PMT_TRY(launch_fine_diag(F.g, mg->K0, xe, bcmask, bcdiag, F.d.dinv, 1, s));
-  for (int l = 1; l < mg->nlev; ++l) {
-    Level& L = mg->L[l];
-    Level& P = mg->L[l - 1];
-    if (l == 1)
-      PMT_TRY(launch_galerkin_fine(P.g, mg->K0, mg->G, xe, bcmask, bcdiag, L.Ke, s, mg->gflag, mg->glist));
-    else
-      PMT_TRY(launch_galerkin_coarse(P.g, P.Ke, L.Ke, s));
-    if (l == 1 && L.g.dim == 3)
-      PMT_TRY(launch_stencil_from_fine(P.g, mg->G, xe, mg->gflag, L.Ke, L.d.S, s));
-    else
+  if (mg->dim == 3) {
+    PMT_TRY(hex_coarse_stencils(mg, s));
+  } else {
+    for (int l = 1; l < mg->nlev; ++l) {
+      Level& L = mg->L[l];
+      Level& P = mg->L[l - 1];
+      if (l == 1)
+        PMT_TRY(launch_galerkin_fine(P.g, mg->K0, xe, bcmask, bcdiag, L.Ke, s));
+      else
+        PMT_TRY(launch_galerkin_coarse(P.g, P.Ke, L.Ke, s));
       PMT_TRY(launch_stencil_from_elements(L.g, L.Ke, L.d.S, s));
+    }
+  }
+  for (int l = 1; l < mg->nlev && mg->dim != 3; ++l) {  // hex8: written by the stencil kernels
+    Level& L = mg->L[l];
     if (l < mg->nlev - 1) {
       PMT_TRY(launch_stencil_dinv<double>(L.g, L.d.S, L.d.dinv, s));
       PMT_TRY(launch_stencil_compress<double>(L.g, L.d.S, L.d.H, s));
@@ -482,7 +578,7 @@ int pmt_mg_update(pmt_mg* mg, const double* xe, const uint8_t* bcmask, double bc
   if (mg->prec != 64) {
     PMT_TRY(launch_to_float(F.g.nel, xe, mg->xe32, s));
     PMT_TRY(launch_to_float(F.ndof, F.d.dinv, F.f.dinv, s));
-    for (int l = 1; l < mg->nlev - 1; ++l) {
+    for (int l = 1; l < mg->nlev - 1 && mg->dim != 3; ++l) {
       Level& L = mg->L[l];
       PMT_TRY(launch_stencil_compress<float>(L.g, L.d.S, L.f.H, s));
       PMT_TRY(launch_stencil_dinv<float>(L.g, L.d.S, L.f.dinv, s));

@@ -27,7 +27,17 @@ struct Level {
   double omega = 0.5;
   int steps = 1;
   double cheb_lo = 0.0, cheb_hi = 0.0;  // Chebyshev interval of D^-1 A (set at update)
-  double* Ke = nullptr;  // Galerkin element matrices (levels >= 1, FP64)
+  double* Ke = nullptr;  // Galerkin element matrices (Q4: levels >= 1, [E][576]; hex8: levels >= 2, [576][E])
+  // hex8 levels >= 1: children with Dirichlet descendants per element, list (+ count) and map of the
+  // flagged elements, their corrections dK (dk_cap x 576), density moments (27 x nel) and, levels >= 2,
+  // the moment tables C (64 x 27 x 9)
+  uint8_t* cflag = nullptr;
+  int* list = nullptr;
+  int* map = nullptr;
+  double* dK = nullptr;
+  long long dk_cap = 0;
+  double* M = nullptr;
+  double* C = nullptr;
   LevelBuf<double> d;
   LevelBuf<float> f;
   template <typename T>
@@ -55,9 +65,8 @@ struct pmt_mg {
   pmt::Level L[pmt::kMaxLevels];
   int NE = 24;
   double* K0 = nullptr;
-  double* G = nullptr;  // P_c^T K0 P_c per child position (fast fine -> level 1 Galerkin)
-  uint8_t* gflag = nullptr;  // per level-1 element: children with Dirichlet nodes (work space)
-  int* glist = nullptr;      // level-1 elements with such children, then their count (work space)
+  double* G = nullptr;  // P_c^T K0 P_c per child position (level-1 stencil of hex8 grids)
+  int* host_counts = nullptr;  // pinned: flagged elements per level (read once per update)
   int k0pat = 0;
   const double* xe = nullptr;
   float* xe32 = nullptr;

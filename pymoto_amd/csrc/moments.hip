@@ -1,0 +1,299 @@
+// Coarse-grid Galerkin operators of hex8 hierarchies from density moments (no level-1 element matrices).
+//
+// Composite trilinear prolongation from a level-l element (h = 2^l fine elements per side) to one of
+// its fine elements f is again trilinear, so R^T A R = assemble_E sum_{f in E} P_f^T A_f P_f with
+// P_f[a][A] = prod_d w(f_d + a_d, A_d), w(p, 0) = 1 - p/h, w(p, 1) = p/h.  For a fine element without
+// Dirichlet nodes A_f = x_f K0.  In the centred coordinate s_d = (f_d + 1/2)/h - 1/2 each factor
+// w(f+a, A) w(f+b, B) is a quadratic in s_d, hence
+//   sum_f x_f P_f^T K0 P_f = sum_{m = (i,j,k) in {0,1,2}^3} M_m(E) C^(l)_m,
+//   M_m(E) = sum_{f in E} x_f s_x^i s_y^j s_z^k,
+// with 27 fixed 24x24 tables C^(l)_m per level (moment_tables).  Moments of a level-(l+1) element
+// follow from its children's by the binomial shift s' = s/2 + t, t = +-1/4 per dimension.
+// Fine elements touching Dirichlet nodes are left out of the moments; their masked products
+// P_f^T (x_f M K0 M + D_f) P_f are kept per flagged element as a correction dK_E (level 1 directly,
+// level l+1 as the Galerkin product of the children's corrections), in a compact list.
+// Level-1 stencils are summed from the densities directly (coarse.hip); levels >= 2 materialise their
+// (small) element matrices in structure-of-arrays layout and sum those.
+#include "launch.h"
+
+namespace pmt {
+
+constexpr int kNM = 27;  // moments s_x^i s_y^j s_z^k, m = i + 3 j + 9 k
+
+void moment_tables(int level, const double* K0, double* C) {
+  const double h = (double)(1 << level);
+  for (int q = 0; q < 64 * kNM * 9; ++q) C[q] = 0.0;
+  auto coefs = [h](int a, int A, int b, int B, double* c) {
+    const double sA = A ? 1.0 : -1.0, sB = B ? 1.0 : -1.0;
+    const double aa = (2 * a - 1) / (2.0 * h), ab = (2 * b - 1) / (2.0 * h);
+    c[0] = 0.25 + (sA * aa + sB * ab) / 2.0 + sA * sB * aa * ab;
+    c[1] = (sA + sB) / 2.0 + sA * sB * (aa + ab);
+    c[2] = sA * sB;
+  };
+  for (int A = 0; A < 8; ++A)
+    for (int B = 0; B < 8; ++B)
+      for (int a = 0; a < 8; ++a)
+        for (int b = 0; b < 8; ++b) {
+          double cx[3], cy[3], cz[3];
+          coefs(a & 1, A & 1, b & 1, B & 1, cx);
+          coefs((a >> 1) & 1, (A >> 1) & 1, (b >> 1) & 1, (B >> 1) & 1, cy);
+          coefs((a >> 2) & 1, (A >> 2) & 1, (b >> 2) & 1, (B >> 2) & 1, cz);
+          for (int k = 0; k < 3; ++k)
+            for (int j = 0; j < 3; ++j)
+              for (int i = 0; i < 3; ++i) {
+                const double w = cx[i] * cy[j] * cz[k];
+                double* c = C + ((A * 8 + B) * kNM + i + 3 * j + 9 * k) * 9;
+                for (int d = 0; d < 3; ++d)
+                  for (int d2 = 0; d2 < 3; ++d2) c[d * 3 + d2] += w * K0[(a * 3 + d) * 24 + b * 3 + d2];
+              }
+        }
+}
+
+__device__ inline long long child_index(const GridDims& gf, long long E, const GridDims& gc, int c) {
+  const int CI = (int)(E % gc.ex);
+  const long long tt = E / gc.ex;
+  const int CJ = (int)(tt % gc.ey), CK = (int)(tt / gc.ey);
+  return ((long long)(2 * CK + ((c >> 2) & 1)) * gf.ey + (2 * CJ + ((c >> 1) & 1))) * gf.ex + 2 * CI + (c & 1);
+}
+
+// level-1 moments from the fine densities (flagged children left out); M[m * nel + E]
+__global__ void moments1_kernel(GridDims gf, GridDims gc, const double* __restrict__ xe,
+                                const uint8_t* __restrict__ cflag, double* __restrict__ M) {
+  const long long E = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (E >= gc.nel) return;
+  const unsigned fl = cflag[E];
+  double m[kNM];
+#pragma unroll
+  for (int q = 0; q < kNM; ++q) m[q] = 0.0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const double x = ((fl >> c) & 1u) ? 0.0 : xe[child_index(gf, E, gc, c)];
+    const double tx = (c & 1) ? 0.25 : -0.25, ty = ((c >> 1) & 1) ? 0.25 : -0.25, tz = ((c >> 2) & 1) ? 0.25 : -0.25;
+    const double px[3] = {1.0, tx, tx * tx}, py[3] = {1.0, ty, ty * ty}, pz[3] = {1.0, tz, tz * tz};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) m[i + 3 * j + 9 * k] = fma(x, px[i] * py[j] * pz[k], m[i + 3 * j + 9 * k]);
+  }
+#pragma unroll
+  for (int q = 0; q < kNM; ++q) M[q * gc.nel + E] = m[q];
+}
+
+// level-l moments from the level-(l-1) moments of the 8 children: s' = s/2 + t per dimension
+__global__ void moments_up_kernel(GridDims gf, GridDims gc, const double* __restrict__ Mf, double* __restrict__ M) {
+  const long long E = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (E >= gc.nel) return;
+  double m[kNM];
+#pragma unroll
+  for (int q = 0; q < kNM; ++q) m[q] = 0.0;
+  for (int c = 0; c < 8; ++c) {
+    const long long ch = child_index(gf, E, gc, c);
+    double v[kNM];
+#pragma unroll
+    for (int q = 0; q < kNM; ++q) v[q] = Mf[q * gf.nel + ch];
+    // (s/2 + t)^i = sum_{i' <= i} binom(i, i') 2^-i' t^(i-i') s^i', one dimension at a time
+#pragma unroll
+    for (int dim = 0; dim < 3; ++dim) {
+      const double t = ((c >> dim) & 1) ? 0.25 : -0.25;
+      const int st = dim == 0 ? 1 : (dim == 1 ? 3 : 9);
+#pragma unroll
+      for (int q = 0; q < kNM; ++q) {
+        const int i = (q / st) % 3;
+        if (i != 0) continue;  // handle each line (q, q + st, q + 2 st) once
+        const double v0 = v[q], v1 = v[q + st], v2 = v[q + 2 * st];
+        v[q] = v0;
+        v[q + st] = t * v0 + 0.5 * v1;
+        v[q + 2 * st] = t * t * v0 + t * v1 + 0.25 * v2;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kNM; ++q) m[q] += v[q];
+  }
+#pragma unroll
+  for (int q = 0; q < kNM; ++q) M[q * gc.nel + E] = m[q];
+}
+
+// flags of level l from level l-1: bit c set when child c has Dirichlet descendants; list and map
+__global__ void flags_up_kernel(GridDims gf, GridDims gc, const uint8_t* __restrict__ cflag_f,
+                                uint8_t* __restrict__ cflag, int* __restrict__ list, int* __restrict__ count,
+                                int* __restrict__ map) {
+  const long long E = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (E >= gc.nel) return;
+  unsigned bits = 0;
+  for (int c = 0; c < 8; ++c)
+    if (cflag_f[child_index(gf, E, gc, c)]) bits |= 1u << c;
+  cflag[E] = (uint8_t)bits;
+  int idx = -1;
+  if (bits) {
+    idx = atomicAdd(count, 1);
+    list[idx] = (int)E;
+  }
+  map[E] = idx;
+}
+
+// dK of a listed level-l element: sum over flagged children c of P_c^T dK_c P_c (lane = (A, B))
+__global__ void __launch_bounds__(256)
+dk_up_kernel(GridDims gf, GridDims gc, const uint8_t* __restrict__ cflag, const int* __restrict__ list,
+             const int* __restrict__ count, const int* __restrict__ map_f, const double* __restrict__ dKf,
+             double* __restrict__ dK) {
+  const int n = *count;
+  if (blockIdx.x * (blockDim.x >> 6) >= n) return;  // whole block leaves together
+  const int lane = threadIdx.x & 63;
+  const int A = lane >> 3, B = lane & 7;
+  const int nwaves = gridDim.x * (blockDim.x >> 6);
+  for (int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < n; w += nwaves) {
+    const long long E = list[w];
+    const unsigned bits = cflag[E];
+    double acc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int c = 0; c < 8; ++c) {
+      if (!((bits >> c) & 1u)) continue;
+      const double* K = dKf + (long long)map_f[child_index(gf, E, gc, c)] * 576;
+#pragma unroll 1
+      for (int a = 0; a < 8; ++a) {
+        const double wa = pweight3(c, a, A);
+        if (wa == 0.0) continue;
+#pragma unroll 1
+        for (int b = 0; b < 8; ++b) {
+          const double wb = pweight3(c, b, B);
+          if (wb == 0.0) continue;
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int d2 = 0; d2 < 3; ++d2) acc[d][d2] = fma(wa * wb, K[(a * 3 + d) * 24 + b * 3 + d2], acc[d][d2]);
+        }
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int d2 = 0; d2 < 3; ++d2) dK[(long long)w * 576 + (A * 3 + d) * 24 + B * 3 + d2] = acc[d][d2];
+  }
+}
+
+// Element matrices of level l >= 2 in structure-of-arrays layout Ke[q * nel + E] (q = row * 24 + col):
+// K_E = sum_m M_m(E) C_m.  Thread = (element, node row block A): coalesced moment loads and stores; the
+// 27 x 9 entries of each (A, B) block of C are uniform across a wave (scalar loads).
+__global__ void __launch_bounds__(256)
+elem_from_moments_kernel(GridDims g, const double* __restrict__ C, const double* __restrict__ M,
+                         double* __restrict__ Ke) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nel * 8) return;
+  const long long E = t % g.nel;
+  const int A = (int)(t / g.nel);  // uniform across a wave unless the wave straddles two row blocks
+  double m[kNM];
+#pragma unroll
+  for (int q = 0; q < kNM; ++q) m[q] = M[q * g.nel + E];
+  for (int B = 0; B < 8; ++B) {
+    const int AB = A * 8 + B;
+    const double* Cb = C + AB * kNM * 9;
+    double blk[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) blk[q] = 0.0;
+#pragma unroll
+    for (int mm = 0; mm < kNM; ++mm)
+#pragma unroll
+      for (int q = 0; q < 9; ++q) blk[q] = fma(m[mm], Cb[mm * 9 + q], blk[q]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int d2 = 0; d2 < 3; ++d2) Ke[(long long)((A * 3 + d) * 24 + B * 3 + d2) * g.nel + E] = blk[d * 3 + d2];
+  }
+}
+
+// Ke[q * nel + list[w]] += dK[w][q] for the flagged elements
+__global__ void dk_add_kernel(GridDims g, const int* __restrict__ list, const int* __restrict__ count,
+                              const double* __restrict__ dK, double* __restrict__ Ke) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long w = t / 576;
+  if (w >= *count) return;
+  const int q = (int)(t % 576);
+  Ke[(long long)q * g.nel + list[w]] += dK[w * 576 + q];
+}
+
+// Per-node block stencil from structure-of-arrays element matrices (lanes = consecutive nodes read
+// consecutive elements: coalesced)
+template <bool HALF>
+__global__ void __launch_bounds__(256)
+stencil_from_soa_kernel(GridDims g, const double* __restrict__ Ke, StencilOut out) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.nnodes) return;
+  const int i = (int)(n % g.nx);
+  const long long t = n / g.nx;
+  const int j = (int)(t % g.ny);
+  const int k = (int)(t / g.ny);
+  const long long nn = g.nnodes;
+  int o = 0;
+#pragma unroll
+  for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx, ++o) {
+        if (HALF && o < 13) continue;  // backward blocks: the neighbours' forward blocks transposed
+        double blk[3][3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+          for (int d2 = 0; d2 < 3; ++d2) blk[d][d2] = 0.0;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int ex = e & 1, ey = (e >> 1) & 1, ez = (e >> 2) & 1;
+          const int bx = dx + 1 - ex, by = dy + 1 - ey, bz = dz + 1 - ez;
+          if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;  // compile time
+          const int ei = i - 1 + ex, ej = j - 1 + ey, ek = k - 1 + ez;
+          if (ei < 0 || ei >= g.ex || ej < 0 || ej >= g.ey || ek < 0 || ek >= g.ez) continue;
+          const long long E = ((long long)ek * g.ey + ej) * g.ex + ei;
+          const int A = (1 - ex) + 2 * (1 - ey) + 4 * (1 - ez), B = bx + 2 * by + 4 * bz;
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int d2 = 0; d2 < 3; ++d2) blk[d][d2] += Ke[(long long)((A * 3 + d) * 24 + B * 3 + d2) * g.nel + E];
+        }
+        emit_block<HALF>(out, o, blk, n, nn);
+      }
+}
+
+int launch_moments1(const GridDims& gf, const double* xe, const uint8_t* cflag, double* M, hipStream_t s) {
+  GridDims gc = make_grid(gf.ex / 2, gf.ey / 2, gf.ez / 2);
+  moments1_kernel<<<blocks_for(gc.nel, 256), 256, 0, s>>>(gf, gc, xe, cflag, M);
+  return check_launch("moments1_kernel");
+}
+
+int launch_moments_up(const GridDims& gf, const double* Mf, double* M, hipStream_t s) {
+  GridDims gc = make_grid(gf.ex / 2, gf.ey / 2, gf.ez / 2);
+  moments_up_kernel<<<blocks_for(gc.nel, 256), 256, 0, s>>>(gf, gc, Mf, M);
+  return check_launch("moments_up_kernel");
+}
+
+int launch_flags_up(const GridDims& gf, const uint8_t* cflag_f, uint8_t* cflag, int* list, int* map, hipStream_t s) {
+  GridDims gc = make_grid(gf.ex / 2, gf.ey / 2, gf.ez / 2);
+  int* count = list + gc.nel;
+  zero_int(count, s);
+  flags_up_kernel<<<blocks_for(gc.nel, 256), 256, 0, s>>>(gf, gc, cflag_f, cflag, list, count, map);
+  return check_launch("flags_up_kernel");
+}
+
+int launch_dk_up(const GridDims& gf, const uint8_t* cflag, const int* list, const int* map_f, const double* dKf,
+                 double* dK, hipStream_t s) {
+  GridDims gc = make_grid(gf.ex / 2, gf.ey / 2, gf.ez / 2);
+  const long long blocks = (gc.nel + 3) / 4 < 8192 ? (gc.nel + 3) / 4 : 8192;
+  dk_up_kernel<<<(unsigned)blocks, 256, 0, s>>>(gf, gc, cflag, list, list + gc.nel, map_f, dKf, dK);
+  return check_launch("dk_up_kernel");
+}
+
+int launch_stencil_from_moments(const GridDims& g, const double* C, const double* M, const int* list,
+                                const double* dK, double* Ke, const StencilOut& out, hipStream_t s) {
+  elem_from_moments_kernel<<<blocks_for(g.nel * 8, 256), 256, 0, s>>>(g, C, M, Ke);
+  PMT_TRY(check_launch("elem_from_moments_kernel"));
+  dk_add_kernel<<<blocks_for(g.nel * 576, 256), 256, 0, s>>>(g, list, list + g.nel, dK, Ke);
+  PMT_TRY(check_launch("dk_add_kernel"));
+  if (out.S)
+    stencil_from_soa_kernel<false><<<blocks_for(g.nnodes, 256), 256, 0, s>>>(g, Ke, out);
+  else
+    stencil_from_soa_kernel<true><<<blocks_for(g.nnodes, 256), 256, 0, s>>>(g, Ke, out);
+  return check_launch("stencil_from_soa_kernel");
+}
+
+}  // namespace pmt
