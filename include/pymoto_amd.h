@@ -135,12 +135,17 @@ int pmt_k0_extend(int dim, const double* K0_host, double* ext_host);
  * PMT_TUNE_REFL_ROWS: node rows per block of the reflection-basis kernel, 16 (default) or 32.
  * PMT_TUNE_STREAM_PLANES: node planes per streamed column, 0 = chosen from the CU count (default).
  * PMT_TUNE_STREAM_TILE: streamed column shape, 0 = 32x8 (default), 1 = 64x4, 2 = 16x16 nodes.
+ * PMT_TUNE_COARSE1: operator of the first coarse level of hex8 hierarchies, 1 = element by element in
+ * the reflection basis from the transformed child densities, no stencil stored (default; used when
+ * k0_pattern has PMT_K0_REFLECTION and the level is not the coarsest), 0 = half stencil.  Read by
+ * pmt_mg_create: it applies to hierarchies created afterwards.
  * pmt_get_tuning returns the current value or a negative status.
  * (No reference counterpart: the launch shapes picked by the autotuner, pymoto_amd/tune.py.) */
 #define PMT_TUNE_FINE_KERNEL 1
 #define PMT_TUNE_STREAM_PLANES 2
 #define PMT_TUNE_STREAM_TILE 3
 #define PMT_TUNE_REFL_ROWS 4
+#define PMT_TUNE_COARSE1 5
 int pmt_set_tuning(int param, int value);
 int pmt_get_tuning(int param);
 /* diag(K) (DampedJacobi.update, iterative.py:38-39) */

@@ -225,8 +225,9 @@ galerkin_kernel(GridDims gf, GridDims gc, const double* __restrict__ K0, const d
 // holding n and n+o of K_E(loc n, loc n+o), K_E = sum over unflagged children of x_c G_c plus the
 // correction dK of flagged elements.  Thread = coarse node; the 64 child densities of its 8 elements
 // are held in registers, the offset/element/child loops are unrolled so every G entry is a
-// compile-time offset (scalar loads).
-template <bool HALF>
+// compile-time offset (scalar loads).  KIND 0: full stencil (out.S); 1: half stencil and D^-1; 2: D^-1
+// only (the level's operator is applied without a stencil).
+template <int KIND>
 __global__ void __launch_bounds__(256)
 stencil_from_fine_kernel(GridDims gf, GridDims gc, const double* __restrict__ G, const double* __restrict__ xe,
                          const uint8_t* __restrict__ cflag, const int* __restrict__ map, const double* __restrict__ dK,
@@ -263,7 +264,8 @@ stencil_from_fine_kernel(GridDims gf, GridDims gc, const double* __restrict__ G,
     for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
       for (int dx = -1; dx <= 1; ++dx, ++o) {
-        if (HALF && o < 13) continue;  // backward blocks: the neighbours' forward blocks transposed
+        if (KIND == 1 && o < 13) continue;  // backward blocks: the neighbours' forward blocks transposed
+        if (KIND == 2 && o != 13) continue;
         double blk[3][3];
 #pragma unroll
         for (int d = 0; d < 3; ++d)
@@ -291,7 +293,16 @@ stencil_from_fine_kernel(GridDims gf, GridDims gc, const double* __restrict__ G,
               for (int d2 = 0; d2 < 3; ++d2) blk[d][d2] += K[d * NE + d2];
           }
         }
-        emit_block<HALF>(out, o, blk, n, nn);
+        if (KIND == 2) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const double v = 1.0 / blk[d][d];
+            out.dinv[n * 3 + d] = v;
+            if (out.dinv32) out.dinv32[n * 3 + d] = (float)v;
+          }
+        } else {
+          emit_block<KIND == 1>(out, o, blk, n, nn);
+        }
       }
 }
 
@@ -591,9 +602,11 @@ int launch_stencil_from_fine(const GridDims& gf, const double* G, const double* 
                              const int* map, const double* dK, const StencilOut& out, hipStream_t s) {
   GridDims gc = make_grid(gf.ex / 2, gf.ey / 2, gf.ez / 2);
   if (out.S)
-    stencil_from_fine_kernel<false><<<blocks_for(gc.nnodes, 256), 256, 0, s>>>(gf, gc, G, xe, cflag, map, dK, out);
+    stencil_from_fine_kernel<0><<<blocks_for(gc.nnodes, 256), 256, 0, s>>>(gf, gc, G, xe, cflag, map, dK, out);
+  else if (out.H)
+    stencil_from_fine_kernel<1><<<blocks_for(gc.nnodes, 256), 256, 0, s>>>(gf, gc, G, xe, cflag, map, dK, out);
   else
-    stencil_from_fine_kernel<true><<<blocks_for(gc.nnodes, 256), 256, 0, s>>>(gf, gc, G, xe, cflag, map, dK, out);
+    stencil_from_fine_kernel<2><<<blocks_for(gc.nnodes, 256), 256, 0, s>>>(gf, gc, G, xe, cflag, map, dK, out);
   return check_launch("stencil_from_fine_kernel");
 }
 

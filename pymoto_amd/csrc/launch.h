@@ -19,6 +19,17 @@ int k0_matches_pattern(int dim, const double* K0_host);
 constexpr int kK0ReflOffset64 = 576;  // 72 FP64 block entries after K0
 constexpr int kK0ReflOffset32 = 648;  // 72 FP32 block entries (36 doubles of storage)
 bool k0_reflection_table(const double* K0_host, double* tab);
+// first coarse level of a hex8 hierarchy in the same basis (K0 reflection symmetric), without a stencil:
+// G0ext = coarse1_table (864 doubles), Xh[layer][8][element] the transformed densities of the unflagged
+// children (launch_coarse1_xhat), list (nlist entries) / map / dK the elements with Dirichlet children and
+// their corrections, dku workspace of 24 * nlist doubles
+void coarse1_table(const double* G0_host, double* ext_host);
+int launch_coarse1_xhat(const GridDims& gf, const GridDims& gc, const double* xe, const uint8_t* cflag, double* X64,
+                        float* X32, hipStream_t s);
+template <typename T>
+int launch_coarse1(int mode, const GridDims& g, int pattern, const double* G0ext, const T* Xh, const int* list,
+                   int nlist, const int* map, const double* dK, double* dku, const T* u, const T* b, const T* dinv,
+                   double omega, T* y, hipStream_t s, const T* up = nullptr, double beta = 0.0);
 template <typename T>
 int launch_fine_refl(int mode, const GridDims& g, int planes /* 0 = from the occupancy */, int rows,
                      int pattern /* K0 has the box class pattern: skip its structural zeros */, const double* K0ext,
@@ -84,7 +95,8 @@ int launch_level1_dk(const GridDims& gf, const double* K0, const double* xe, con
 int launch_galerkin_coarse(const GridDims& gf, const double* Kf, double* Kc, hipStream_t s);
 int launch_stencil_from_elements(const GridDims& g, const double* Ke, double* S, hipStream_t s);
 // level-1 stencil of a hex8 grid from the fine densities, G and the corrections of flagged elements
-// (out.S set: full stencil, for the coarsest level; else half stencil and D^-1, see StencilOut)
+// (out.S set: full stencil, for the coarsest level; else out.H set: half stencil and D^-1, see StencilOut;
+// else D^-1 only)
 int launch_stencil_from_fine(const GridDims& gf, const double* G, const double* xe, const uint8_t* cflag,
                              const int* map, const double* dK, const StencilOut& out, hipStream_t s);
 

@@ -106,6 +106,9 @@ static int level_apply(pmt_mg* mg, int l, int mode, const T* u, const T* b, T* y
       if (mode != MODE_CHEB) return launch_stencil<double>(mode, L.g, L.d.S, u, b, B.dinv, omega, y, s);
     return arg_error("level_apply: the coarsest level is solved directly");
   }
+  if (l == 1 && mg->l1free)
+    return launch_coarse1<T>(mode, L.g, mg->k0pat & PMT_K0_CLASS_PATTERN, mg->G0ext, B.Xh, L.list, mg->host_counts[1],
+                             L.map, L.dK, L.dku, u, b, B.dinv, omega, y, s, up, beta);
   return launch_stencil_half<T>(mode, L.g, B.H, u, b, B.dinv, omega, y, s, up, beta);
 }
 
@@ -216,7 +219,11 @@ static int alloc_fp32(pmt_mg* mg) {
     if (rc == PMT_OK) rc = dalloc(mg, &B.r, L.ndof);
     if (rc == PMT_OK) rc = dalloc(mg, &B.b, L.ndof);
     if (l > 0) {
-      if (rc == PMT_OK) rc = dalloc(mg, &B.H, L.g.nnodes * half_stencil_coefficients(mg->dim));
+      if (l == 1 && mg->l1free) {
+        if (rc == PMT_OK) rc = dalloc(mg, &B.Xh, 8 * L.g.nel);
+      } else {
+        if (rc == PMT_OK) rc = dalloc(mg, &B.H, L.g.nnodes * half_stencil_coefficients(mg->dim));
+      }
       if (rc == PMT_OK) rc = dalloc(mg, &B.u, L.ndof);
     }
   }
@@ -330,8 +337,11 @@ static int hex_coarse_stencils(pmt_mg* mg, hipStream_t s) {
     if (need > L.dk_cap) {
       PMT_TRY(mg_free(mg, L.dK, L.dk_cap * 576 * (long long)sizeof(double)));
       L.dK = nullptr;
+      if (L.dku) PMT_TRY(mg_free(mg, L.dku, L.dk_cap * 24 * (long long)sizeof(double)));
+      L.dku = nullptr;
       L.dk_cap = 0;
       PMT_TRY(mg_alloc(mg, (void**)&L.dK, need * 576 * (long long)sizeof(double)));
+      if (l == 1 && mg->l1free) PMT_TRY(mg_alloc(mg, (void**)&L.dku, need * 24 * (long long)sizeof(double)));
       L.dk_cap = need;
     }
   }
@@ -350,7 +360,7 @@ static int hex_coarse_stencils(pmt_mg* mg, hipStream_t s) {
     if (l == nl - 1) {
       o.S = L.d.S;
     } else {
-      o.H = L.d.H;
+      o.H = L.d.H;  // null for a level without a stencil: D^-1 only
       o.dinv = L.d.dinv;
       if (mg->prec != 64) {
         o.H32 = L.f.H;
@@ -359,6 +369,9 @@ static int hex_coarse_stencils(pmt_mg* mg, hipStream_t s) {
     }
     return o;
   };
+  if (mg->l1free)
+    PMT_TRY(launch_coarse1_xhat(mg->L[0].g, mg->L[1].g, mg->xe, mg->L[1].cflag, mg->L[1].d.Xh,
+                                mg->prec != 64 ? mg->L[1].f.Xh : nullptr, s));
   PMT_TRY(launch_stencil_from_fine(mg->L[0].g, mg->G, mg->xe, mg->L[1].cflag, mg->L[1].map, mg->L[1].dK, target(1),
                                    s));
   for (int l = 2; l < nl; ++l) {
@@ -386,6 +399,7 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
   mg->NE = (1 << mg->dim) * mg->dim;
   std::vector<double> K0ext(PMT_K0_EXT_LEN);
   mg->k0pat = pmt_k0_extend(mg->dim, K0_host, K0ext.data());
+  mg->l1free = mg->dim == 3 && nlevels >= 3 && (mg->k0pat & PMT_K0_REFLECTION) && get_tuning(PMT_TUNE_COARSE1) == 1;
   mg->lmax_elem = element_lmax(mg->NE, K0_host);
   int ex = nelx, ey = nely, ez = nelz;
   for (int l = 0; l < nlevels; ++l) {
@@ -420,6 +434,7 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
   const int NS = mg->dim == 3 ? 243 : 36;
   ALLOC(mg->K0, (long long)PMT_K0_EXT_LEN);  // K0 followed by its reflection-basis blocks
   ALLOC(mg->G, (long long)(1 << mg->dim) * NE * NE);
+  if (mg->l1free) ALLOC(mg->G0ext, 864);
   const bool hex = mg->dim == 3;
   for (int l = 1; l < nlevels && hex; ++l) {
     Level& L = mg->L[l];
@@ -433,6 +448,7 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
     const long long face = std::max({(long long)L.g.ex * L.g.ey, (long long)L.g.ey * L.g.ez, (long long)L.g.ex * L.g.ez});
     L.dk_cap = std::max(1LL, std::min(L.g.nel, face));
     ALLOC(L.dK, L.dk_cap * NE * NE);
+    if (l == 1 && mg->l1free) ALLOC(L.dku, L.dk_cap * NE);
   }
   if (rc == PMT_OK)
     rc = hip_status(hipHostMalloc((void**)&mg->host_counts, kMaxLevels * sizeof(int), hipHostMallocDefault),
@@ -454,7 +470,11 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
     if (l > 0) {
       if (!hex) ALLOC(L.Ke, L.g.nel * NE * NE);
       if (!hex) ALLOC(L.d.S, L.g.nnodes * NS);  // hex8: half stencil only above the coarsest level
-      ALLOC(L.d.H, L.g.nnodes * half_stencil_coefficients(mg->dim));
+      if (l == 1 && mg->l1free) {
+        ALLOC(L.d.Xh, 8 * L.g.nel);
+      } else {
+        ALLOC(L.d.H, L.g.nnodes * half_stencil_coefficients(mg->dim));
+      }
       ALLOC(L.d.u, L.ndof);
       ALLOC(L.d.b, L.ndof);
     }
@@ -479,6 +499,12 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
     std::vector<double> G((size_t)(1 << mg->dim) * NE * NE);
     galerkin_child_matrices(mg->dim, K0_host, G.data());
     rc = hip_status(hipMemcpy(mg->G, G.data(), G.size() * sizeof(double), hipMemcpyHostToDevice), "G upload");
+    if (rc == PMT_OK && mg->l1free) {
+      std::vector<double> ext(864);
+      coarse1_table(G.data(), ext.data());  // from G_0, the child at the element's corner 0
+      rc = hip_status(hipMemcpy(mg->G0ext, ext.data(), ext.size() * sizeof(double), hipMemcpyHostToDevice),
+                      "level-1 table upload");
+    }
   }
   for (int l = 2; l < nlevels && hex && rc == PMT_OK; ++l) {
     std::vector<double> C(64 * 27 * 9);

@@ -19,6 +19,7 @@ struct LevelBuf {
   T* t = nullptr;     // ping-pong buffer
   T* b = nullptr;     // restricted right-hand side (levels >= 1; level 0 in FP32: converted input)
   T* r = nullptr;     // residual
+  T* Xh = nullptr;    // hex8 level 1 without a stencil: transformed child densities [layer][8][element]
 };
 
 struct Level {
@@ -35,6 +36,7 @@ struct Level {
   int* list = nullptr;
   int* map = nullptr;
   double* dK = nullptr;
+  double* dku = nullptr;  // level 1 without a stencil: dK_E u_E per listed element (dk_cap x 24)
   long long dk_cap = 0;
   double* M = nullptr;
   double* C = nullptr;
@@ -66,6 +68,8 @@ struct pmt_mg {
   int NE = 24;
   double* K0 = nullptr;
   double* G = nullptr;  // P_c^T K0 P_c per child position (level-1 stencil of hex8 grids)
+  bool l1free = false;     // hex8 level 1 applied element by element in the reflection basis (no stencil)
+  double* G0ext = nullptr;  // its 24 x 24 table (coarse1_table), FP64 then FP32
   int* host_counts = nullptr;  // pinned: flagged elements per level (read once per update)
   int k0pat = 0;
   const double* xe = nullptr;

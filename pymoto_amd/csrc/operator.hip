@@ -734,6 +734,7 @@ static void launch_fine_t(int mode, const GridDims& g, const double* K0, const T
 // Launch-shape tuning of the fine operator (set by pmt_set_tuning; read at every launch).
 static int g_fine_kernel = 2;  // 0 = 3D tiles, 1 = z-streamed columns, 2 = reflection basis (if available)
 static int g_refl_rows = 16;   // node rows per block of the reflection-basis kernel
+static int g_coarse1 = 1;      // hex8 level 1 without a stencil (reflection basis), new hierarchies
 static int g_stream_nzc = 0;   // node planes per streamed block (0 = chosen from the CU count)
 static int g_stream_tile = 0;  // streamed column shape: 0 = 32x8, 1 = 64x4, 2 = 16x16 nodes
 
@@ -762,6 +763,11 @@ int set_tuning(int param, int value) {
     ++g_tuning_epoch;
     return 0;
   }
+  if (param == PMT_TUNE_COARSE1 && (value == 0 || value == 1)) {
+    g_coarse1 = value;
+    ++g_tuning_epoch;
+    return 0;
+  }
   return -1;
 }
 
@@ -772,6 +778,7 @@ int get_tuning(int param) {
   if (param == PMT_TUNE_STREAM_PLANES) return g_stream_nzc;
   if (param == PMT_TUNE_STREAM_TILE) return g_stream_tile;
   if (param == PMT_TUNE_REFL_ROWS) return g_refl_rows;
+  if (param == PMT_TUNE_COARSE1) return g_coarse1;
   return -1;
 }
 

@@ -121,6 +121,88 @@ __device__ inline void refl_element(const T* __restrict__ tab, T x, T (&v)[8][3]
     for (int d = 0; d < 3; ++d) v[a][d] = w[a][d];
 }
 
+// First coarse level of a hex8 hierarchy, element by element in the same basis.  With K0 reflection
+// symmetric, the fixed child matrices G_c = P_c^T K0 P_c are G_0 mirrored: G_c = S_c G_0 S_c.  In the
+// transformed basis S_c is diagonal with the characters of class chi, so
+//   K^_E[a][b] = G^_0[a][b] X^[chi(a) ^ chi(b)],   X^[m] = sum_c (-1)^(m.c) x_c   (children c of E),
+// a fixed 24 x 24 table (G0tab, the 1/64 of both transforms folded in) modulated by the 8 transformed
+// child densities.  Entry (a, b) (a = k*3 + d) can only be nonzero if some k' within k and k'' within
+// the k of b (bitwise) share a class (G_0 = P_0^T K0 P_0 with P_0 bit-triangular in this basis); with
+// the box class pattern also the pattern's zero entries of K0's blocks drop out.
+__host__ __device__ constexpr bool g0_nonzero(int a, int b, bool pat) {
+  const int K = a / 3, d = a % 3, K2 = b / 3, d2 = b % 3;
+  for (int k = 0; k < 8; ++k) {
+    if (k & ~K) continue;
+    for (int k2 = 0; k2 < 8; ++k2) {
+      if (k2 & ~K2) continue;
+      const int c1 = k ^ (1 << d), c2 = k2 ^ (1 << d2);
+      if (c1 == c2 && !(pat && refl_zero(c1, d, d2))) return true;
+    }
+  }
+  return false;
+}
+
+struct G0Pattern {
+  bool nz[2][576];
+  constexpr G0Pattern() : nz() {
+    for (int p = 0; p < 2; ++p)
+      for (int q = 0; q < 576; ++q) nz[p][q] = g0_nonzero(q / 24, q % 24, p == 1);
+  }
+};
+constexpr G0Pattern kG0Pattern{};
+
+// v: the element's corners after the plane stages (as for refl_element); X: its X^.  Returns
+// (K_E u_e)[a][d] in v, without the Dirichlet corrections.
+template <bool PAT, typename T>
+__device__ inline void coarse1_element(const T* tab, const T (&X)[8], T (&v)[8][3]) {
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T p = v[a][d], q = v[a + 4][d];
+      v[a][d] = p + q;
+      v[a + 4][d] = p - q;
+    }
+  T w[8][3];
+  int zoff = 0;  // 0, laundered through the previous class's result (see below)
+#pragma unroll
+  for (int ca = 0; ca < 8; ++ca) {
+    // One output class at a time.  Its table entries are read through an offset the compiler cannot
+    // see is 0 and that depends on the previous class's outputs, so the reads of all ~400 entries are
+    // not hoisted to the start (they would not fit the register file).
+    const T* tb = tab + zoff;
+    T acc[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int al = (ca ^ (1 << d)) * 3 + d;
+      acc[d] = (T)0;
+#pragma unroll
+      for (int cb = 0; cb < 8; ++cb) {
+        T t = (T)0;
+        bool any = false;
+#pragma unroll
+        for (int d2 = 0; d2 < 3; ++d2) {
+          const int be = (cb ^ (1 << d2)) * 3 + d2;  // the dof of class cb with component d2
+          if (kG0Pattern.nz[PAT ? 1 : 0][al * 24 + be]) {
+            t = any ? fma(tb[al * 24 + be], v[be / 3][d2], t) : tb[al * 24 + be] * v[be / 3][d2];
+            any = true;
+          }
+        }
+        if (any) acc[d] = fma(X[ca ^ cb], t, acc[d]);
+      }
+    }
+    // the class's three results are complete before the next class's offset exists
+    asm volatile("v_mov_b32 %0, 0" : "=v"(zoff), "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]));
+#pragma unroll
+    for (int d = 0; d < 3; ++d) w[ca ^ (1 << d)][d] = acc[d];
+  }
+  wht8(w);
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) v[a][d] = w[a][d];
+}
+
 // Every global access of the kernel goes through a raw buffer descriptor of one node plane (or element
 // layer), whose hardware range check returns 0 for loads and drops stores at an out-of-range offset
 // (buffer helpers in common.h).  Halo lanes and planes outside the grid therefore issue the same
@@ -128,21 +210,28 @@ __device__ inline void refl_element(const T* __restrict__ tab, T x, T (&v)[8][3]
 // instruction, and the wait counters stay exact: a prefetch is waited for only where it is used,
 // never together with the stores issued after it.
 
-template <int SY, int MODE, bool PAT, typename T>
+// OP 0: the fine grid (element factor x_e, Dirichlet-masked nodes, bcdiag rows).  OP 1: the first
+// coarse level of a hex8 hierarchy (element factors X^[8], layout [layer][m][element]; no mask: the
+// elements with Dirichlet children get their corrections from coarse1_correct_kernel afterwards).
+template <int OP, int SY, int MODE, bool PAT, typename T>
 __global__ void __launch_bounds__(kReflRow* SY)
 fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, const T* __restrict__ tab, int zero_stride,
                  const T* __restrict__ xe, const uint8_t* __restrict__ bcm, T bcdiag, const T* __restrict__ u,
                  const T* __restrict__ b, const T* __restrict__ dinv, T omega, T* y, const T* up, T beta) {
   constexpr int RX = kReflRow, NT = RX * SY;
+  constexpr int XN = OP == 0 ? 1 : 8;  // element factors per element
   constexpr int TS = (int)sizeof(T);
   constexpr bool CARRY = sizeof(T) == 4;  // carry the transformed plane to the next layer
   __shared__ T su[2][3][NT];  // masked displacements of the block's nodes, planes of both parities
   __shared__ T so[6][NT];  // corner (., 1, .) results after the x combine, both planes of a layer
+  __shared__ T sG0[OP == 1 ? 576 : 1];  // OP 1: the 24 x 24 table (read as broadcasts; too large for SGPRs)
 
   const int work = (int)xcd_block();
   if (work >= nwork) return;  // whole block leaves together (grid padded for the XCD remap)
   const int txy = work % ntiles_xy;
   const int tid = threadIdx.x, tx = tid % RX, ty = tid / RX;
+  if (OP == 1)
+    for (int q = tid; q < 576; q += NT) sG0[q] = tab[q];  // visible after the first barrier below
   const int i = (txy % tiles_x) * (RX - 2) - 1 + tx;
   const int j = (txy / tiles_x) * (SY - 2) - 1 + ty;
   const int kbeg = (work / ntiles_xy) * nzc;
@@ -162,14 +251,22 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
   auto load_node = [&](int p, T(&ur)[3], unsigned& mk) {
     const bool pin = p >= 0 && p < g.nz;
     const __amdgpu_buffer_rsrc_t ru = plane_rsrc(u, 3 * (long long)p * uplane, 3 * uplane, pin);
-    const __amdgpu_buffer_rsrc_t rm = plane_rsrc(bcm, (long long)p * uplane, uplane, pin);
-    mk = __builtin_amdgcn_raw_buffer_load_b8(rm, o_node, 0, 0);
+    mk = 0u;
+    if (OP == 0) {
+      const __amdgpu_buffer_rsrc_t rm = plane_rsrc(bcm, (long long)p * uplane, uplane, pin);
+      mk = __builtin_amdgcn_raw_buffer_load_b8(rm, o_node, 0, 0);
+    }
 #pragma unroll
     for (int d = 0; d < 3; ++d) ur[d] = bload(ru, o_vec + d * TS, T());
   };
-  auto load_x = [&](int l) -> T {
-    const __amdgpu_buffer_rsrc_t rx = plane_rsrc(xe, (long long)l * xplane, xplane, l >= 0 && l < g.ez);
-    return bload(rx, o_el, T());
+  // element factors of layer l
+  auto load_x = [&](int l, T(&xv)[XN]) {
+    const bool lin = l >= 0 && l < g.ez;
+#pragma unroll
+    for (int m = 0; m < XN; ++m) {
+      const __amdgpu_buffer_rsrc_t rx = plane_rsrc(xe, ((long long)l * XN + m) * xplane, xplane, lin);
+      xv[m] = bload(rx, o_el, T());
+    }
   };
   auto load_out = [&](int p, T(&bo)[3], T(&dv)[3], T(&po)[3]) {
     const bool pin = p >= kbeg && p < kend;
@@ -203,12 +300,12 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
   // three.  The loop is unrolled by three so every slot is a fixed register set: no copies, and no
   // wait falls on a memory operation issued within the last layer.  The update operands (b, D^-1,
   // previous iterate) of plane k are loaded at the start of layer k and used at its end.
-  T raw[3][3], xs[3];
+  T raw[3][3], xs[3][XN];
   unsigned mk[3];
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
     load_node(kbeg - 1 + r, raw[r], mk[r]);
-    xs[r] = load_x(kbeg - 1 + r);
+    load_x(kbeg - 1 + r, xs[r]);
   }
   // corners of a node plane after the x and y transform stages, from the own masked values c (written
   // to LDS plane buffer pl before the barrier inside)
@@ -267,9 +364,13 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
         v[4 + a][d] = Whi[a][d];
         if (CARRY) Wlo[a][d] = Whi[a][d];
       }
-    // the 72 block entries are re-read through the scalar cache each layer (zero_stride is 0; the
+    // the table entries are re-read through the scalar cache each layer (zero_stride is 0; the
     // compiler cannot hoist the loads, which would exceed the scalar register file)
-    refl_element<PAT>(tab + zero_stride * k, xs[SA], v);
+    if constexpr (OP == 0) {
+      refl_element<PAT>(tab + zero_stride * k, xs[SA][0], v);
+    } else {
+      coarse1_element<PAT>(sG0 + zero_stride * k, xs[SA], v);  // (re-read per layer: not hoisted)
+    }
 
     // x combine: s[ay][az] = own corner (0, ay, az) + left neighbour's corner (1, ay, az)
     T s[2][2][3];
@@ -311,7 +412,7 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
 
     __builtin_amdgcn_sched_barrier(0);  // keep the prefetch behind the stores
     load_node(k + 3, raw[SA], mk[SA]);
-    xs[SA] = load_x(k + 3);
+    load_x(k + 3, xs[SA]);
   };
 
   for (int k = kbeg - 1;; k += 3) {
@@ -350,14 +451,14 @@ bool k0_reflection_table(const double* K0, double* tab) {
 }
 
 // Resident blocks per CU of one kernel instance (cached per device).
-template <int SY, int MODE, bool PAT, typename T>
+template <int OP, int SY, int MODE, bool PAT, typename T>
 static int refl_resident_blocks() {
   static int cache[64] = {0};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 2;
   if (cache[dev] == 0) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fine_refl_kernel<SY, MODE, PAT, T>, kReflRow * SY, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fine_refl_kernel<OP, SY, MODE, PAT, T>, kReflRow * SY, 0) !=
             hipSuccess ||
         nb < 1)
       nb = 2;
@@ -387,7 +488,7 @@ static int refl_planes(int nxy, int nz, int resident) {
   return best > 0 ? best : nz;
 }
 
-template <int SY, bool PAT, typename T>
+template <int OP, int SY, bool PAT, typename T>
 static void launch_refl_shape(int mode, const GridDims& g, int planes, const T* tab, const T* xe, const uint8_t* bcm,
                               T bcdiag, const T* u, const T* b, const T* dinv, T omega, T* y, hipStream_t s,
                               const T* up, T beta) {
@@ -397,9 +498,9 @@ static void launch_refl_shape(int mode, const GridDims& g, int planes, const T* 
 #define PMT_REFL(M)                                                                                             \
   do {                                                                                                          \
     const int nzc = planes > 0 ? (planes < g.nz ? planes : g.nz)                                               \
-                               : refl_planes(nxy, g.nz, refl_resident_blocks<SY, M, PAT, T>());                 \
+                               : refl_planes(nxy, g.nz, refl_resident_blocks<OP, SY, M, PAT, T>());             \
     const int nwork = nxy * ((g.nz + nzc - 1) / nzc);                                                          \
-    fine_refl_kernel<SY, M, PAT, T><<<(unsigned)((nwork + 7) & ~7), kReflRow * SY, 0, s>>>(                    \
+    fine_refl_kernel<OP, SY, M, PAT, T><<<(unsigned)((nwork + 7) & ~7), kReflRow * SY, 0, s>>>(                \
         g, tx, nxy, nzc, nwork, tab, 0, xe, bcm, bcdiag, u, b, dinv, omega, y, up, beta);                        \
   } while (0)
   if (mode == MODE_APPLY)
@@ -422,16 +523,154 @@ int launch_fine_refl(int mode, const GridDims& g, int planes, int rows, int patt
   pattern = pattern && sizeof(T) == 4;  // FP64: skipping the zeros costs more scalar-register spills than it saves
   if (rows == 32) {
     if (pattern)
-      launch_refl_shape<32, true, T>(mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
+      launch_refl_shape<0, 32, true, T>(mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
     else
-      launch_refl_shape<32, false, T>(mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
+      launch_refl_shape<0, 32, false, T>(mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
   } else {
     if (pattern)
-      launch_refl_shape<16, true, T>(mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
+      launch_refl_shape<0, 16, true, T>(mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
     else
-      launch_refl_shape<16, false, T>(mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
+      launch_refl_shape<0, 16, false, T>(mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
   }
   return check_launch("fine_refl_kernel");
+}
+
+// Corrections of the first coarse level's elements with Dirichlet children (those along constrained
+// faces), in two passes without atomics:
+//  1. coarse1_dku_kernel, thread = (listed element w, row r): t[w][r] = (dK_w u_E)[r];
+//  2. coarse1_correct_kernel, thread = (listed element, corner): it owns the corner's node if no flagged
+//     element holding the node has a smaller index, and adds s(n) times the node's rows of t of every
+//     flagged element holding it, in index order (the result does not depend on the order of the
+//     list), with s = 1 (apply), -1 (residual) or -omega D^-1(n) (Jacobi and Chebyshev sweeps, which
+//     are affine in A u with that factor).
+template <typename T>
+__global__ void __launch_bounds__(256)
+coarse1_dku_kernel(GridDims g, const int* __restrict__ list, int nlist, const double* __restrict__ dK,
+                   const T* __restrict__ u, double* __restrict__ t) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 24LL * nlist) return;
+  const long long w = q / 24;
+  const int r = (int)(q % 24);
+  const long long E = list[w];
+  const int CI = (int)(E % g.ex);
+  const long long tt = E / g.ex;
+  const int CJ = (int)(tt % g.ey), CK = (int)(tt / g.ey);
+  const double* K = dK + w * 576 + r * 24;
+  double acc = 0.0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const long long nb = ((long long)(CK + (c >> 2)) * g.ny + CJ + ((c >> 1) & 1)) * g.nx + CI + (c & 1);
+#pragma unroll
+    for (int d2 = 0; d2 < 3; ++d2) acc = fma(K[c * 3 + d2], (double)u[nb * 3 + d2], acc);
+  }
+  t[q] = acc;
+}
+
+template <int MODE, typename T>
+__global__ void __launch_bounds__(256)
+coarse1_correct_kernel(GridDims g, const int* __restrict__ list, int nlist, const int* __restrict__ map,
+                       const double* __restrict__ t, const T* __restrict__ dinv, double omega, T* __restrict__ y) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 8LL * nlist) return;
+  const long long E = list[q >> 3];
+  const int a = (int)(q & 7);
+  const int CI = (int)(E % g.ex);
+  const long long tt = E / g.ex;
+  const int CJ = (int)(tt % g.ey), CK = (int)(tt / g.ey);
+  const int I = CI + (a & 1), J = CJ + ((a >> 1) & 1), K = CK + (a >> 2);
+  int me[8];
+  long long owner = -1;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {  // e ascending = element index ascending
+    const int ei = I - 1 + (e & 1), ej = J - 1 + ((e >> 1) & 1), ek = K - 1 + (e >> 2);
+    const bool ok = ei >= 0 && ei < g.ex && ej >= 0 && ej < g.ey && ek >= 0 && ek < g.ez;
+    const long long Ee = ((long long)ek * g.ey + ej) * g.ex + ei;
+    me[e] = ok ? map[Ee] : -1;
+    if (owner < 0 && me[e] >= 0) owner = Ee;
+  }
+  if (owner != E) return;
+  double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int A = (1 - (e & 1)) + 2 * (1 - ((e >> 1) & 1)) + 4 * (1 - (e >> 2));  // the node's corner in E_e
+    const double* te = t + (long long)(me[e] < 0 ? 0 : me[e]) * 24 + A * 3;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) acc[d] += me[e] < 0 ? 0.0 : te[d];
+  }
+  const long long n = ((long long)K * g.ny + J) * g.nx + I;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double f = acc[d];
+    if (MODE == MODE_RESIDUAL)
+      f = -f;
+    else if (MODE == MODE_JACOBI || MODE == MODE_CHEB)
+      f = -omega * (double)dinv[n * 3 + d] * f;
+    y[n * 3 + d] = (T)((double)y[n * 3 + d] + f);
+  }
+}
+
+template <typename T>
+int launch_coarse1(int mode, const GridDims& g, int pattern, const double* G0ext, const T* Xh, const int* list,
+                   int nlist, const int* map, const double* dK, double* dku, const T* u, const T* b, const T* dinv,
+                   double omega, T* y, hipStream_t s, const T* up, double beta) {
+  const T* tab = sizeof(T) == 8 ? (const T*)G0ext : (const T*)(G0ext + 576);
+  const T om = (T)omega, be = (T)beta;
+  if (pattern)
+    launch_refl_shape<1, 16, true, T>(mode, g, 0, tab, Xh, nullptr, (T)0, u, b, dinv, om, y, s, up, be);
+  else
+    launch_refl_shape<1, 16, false, T>(mode, g, 0, tab, Xh, nullptr, (T)0, u, b, dinv, om, y, s, up, be);
+  PMT_TRY(check_launch("fine_refl_kernel<coarse level 1>"));
+  if (nlist <= 0) return PMT_OK;
+  coarse1_dku_kernel<T><<<(unsigned)blocks_for(24LL * nlist, 256), 256, 0, s>>>(g, list, nlist, dK, u, dku);
+  PMT_TRY(check_launch("coarse1_dku_kernel"));
+  const unsigned blocks = (unsigned)blocks_for(8LL * nlist, 256);
+  if (mode == MODE_APPLY)
+    coarse1_correct_kernel<MODE_APPLY, T><<<blocks, 256, 0, s>>>(g, list, nlist, map, dku, dinv, omega, y);
+  else if (mode == MODE_RESIDUAL)
+    coarse1_correct_kernel<MODE_RESIDUAL, T><<<blocks, 256, 0, s>>>(g, list, nlist, map, dku, dinv, omega, y);
+  else
+    coarse1_correct_kernel<MODE_JACOBI, T><<<blocks, 256, 0, s>>>(g, list, nlist, map, dku, dinv, omega, y);
+  return check_launch("coarse1_correct_kernel");
+}
+
+// X^[m] = sum_c (-1)^(m.c) x_c over the unflagged children c of every first-coarse-level element (the
+// flagged ones are in dK), layout [layer][m][element of the layer], FP64 and (X32 set) FP32.
+__global__ void coarse1_xhat_kernel(GridDims gf, GridDims gc, const double* __restrict__ xe,
+                                    const uint8_t* __restrict__ cflag, double* __restrict__ X64, float* __restrict__ X32) {
+  const long long E = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (E >= gc.nel) return;
+  const int CI = (int)(E % gc.ex);
+  const long long tt = E / gc.ex;
+  const int CJ = (int)(tt % gc.ey), CK = (int)(tt / gc.ey);
+  const unsigned fl = cflag[E];
+  double x[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const long long f = ((long long)(2 * CK + (c >> 2)) * gf.ey + 2 * CJ + ((c >> 1) & 1)) * gf.ex + 2 * CI + (c & 1);
+    x[c] = ((fl >> c) & 1u) ? 0.0 : xe[f];
+  }
+#pragma unroll
+  for (int bit = 1; bit < 8; bit <<= 1)
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+      if (!(c & bit)) {
+        const double p = x[c], q = x[c | bit];
+        x[c] = p + q;
+        x[c | bit] = p - q;
+      }
+  const long long xplane = (long long)gc.ex * gc.ey, e2 = (long long)CJ * gc.ex + CI;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const long long o = ((long long)CK * 8 + m) * xplane + e2;
+    X64[o] = x[m];
+    if (X32) X32[o] = (float)x[m];
+  }
+}
+
+int launch_coarse1_xhat(const GridDims& gf, const GridDims& gc, const double* xe, const uint8_t* cflag, double* X64,
+                        float* X32, hipStream_t s) {
+  coarse1_xhat_kernel<<<blocks_for(gc.nel, 256), 256, 0, s>>>(gf, gc, xe, cflag, X64, X32);
+  return check_launch("coarse1_xhat_kernel");
 }
 
 template int launch_fine_refl<double>(int, const GridDims&, int, int, int, const double*, const double*, const uint8_t*,
@@ -440,6 +679,27 @@ template int launch_fine_refl<double>(int, const GridDims&, int, int, int, const
 template int launch_fine_refl<float>(int, const GridDims&, int, int, int, const double*, const float*, const uint8_t*,
                                      double, const float*, const float*, const float*, double, float*, hipStream_t,
                                      const float*, double);
+template int launch_coarse1<double>(int, const GridDims&, int, const double*, const double*, const int*, int,
+                                    const int*, const double*, double*, const double*, const double*, const double*,
+                                    double, double*, hipStream_t, const double*, double);
+template int launch_coarse1<float>(int, const GridDims&, int, const double*, const float*, const int*, int, const int*,
+                                   const double*, double*, const float*, const float*, const float*, double, float*,
+                                   hipStream_t, const float*, double);
+
+// G0tab: (1/64) H G_0 H of the child-0 matrix G_0 = P_0^T K0 P_0 (G0: host, 24 x 24), FP64 then FP32
+void coarse1_table(const double* G0, double* ext) {
+  auto sgn = [](int k, int a) { return (__builtin_popcount(k & a) & 1) ? -1.0 : 1.0; };
+  float* t32 = (float*)(ext + 576);
+  for (int al = 0; al < 24; ++al)
+    for (int be = 0; be < 24; ++be) {
+      const int k = al / 3, d = al % 3, l = be / 3, e = be % 3;
+      double acc = 0.0;
+      for (int a = 0; a < 8; ++a)
+        for (int b = 0; b < 8; ++b) acc += sgn(k, a) * sgn(l, b) * G0[(a * 3 + d) * 24 + b * 3 + e];
+      ext[al * 24 + be] = acc / 64.0;
+      t32[al * 24 + be] = (float)(acc / 64.0);
+    }
+}
 
 }  // namespace pmt
 

@@ -18,7 +18,7 @@ import torch
 
 from . import _lib
 
-FINE_KERNEL, STREAM_PLANES, STREAM_TILE, REFL_ROWS = 1, 2, 3, 4
+FINE_KERNEL, STREAM_PLANES, STREAM_TILE, REFL_ROWS, COARSE1 = 1, 2, 3, 4, 5
 DEFAULT = dict(fine_kernel=2, stream_tile=0, stream_planes=0, refl_rows=16)
 _cache = {}
 
@@ -35,6 +35,12 @@ def set_tuning(fine_kernel=None, stream_tile=None, stream_planes=None, refl_rows
                          (REFL_ROWS, refl_rows)):
         if value is not None:
             _lib.check(lib.pmt_set_tuning(param, int(value)), "set tuning")
+
+
+def set_coarse_level1(stencil_free=True):
+    """Operator of the first coarse level of hex8 hierarchies created from now on: element by element in
+    the reflection basis of K0 (default) or the stored half stencil."""
+    _lib.check(_lib.lib().pmt_set_tuning(COARSE1, 1 if stencil_free else 0), "set tuning")
 
 
 def candidates(nz, reflection=True):

@@ -285,6 +285,44 @@ def test_galerkin_levels_and_vcycle_vs_oracle(gpu_lib, nel, levels, bcs):
     assert relerr(z, z_ref) < 1e-9
 
 
+@pytest.mark.parametrize("nel,levels,bcs", [((16, 8, 8), 3, "std"), ((24, 12, 8), 3, "mixed"), ((8, 4, 4), 3, "std"),
+                                            ((32, 16, 16), 4, "std"), ((44, 20, 12), 3, "std")])
+def test_level1_without_stencil_matches_stencil(gpu_lib, nel, levels, bcs):
+    """The first coarse level applied element by element in the reflection basis (transformed child
+    densities, corrections of the elements with Dirichlet children) against its stored half stencil:
+    A_1 v (FP64), one V-cycle in FP64 (Jacobi and residual sweeps), one in FP32 and one with the
+    Chebyshev smoother; differences are rounding only."""
+    from pymoto_amd import tune
+    rng = np.random.default_rng(nel[0] + levels)
+    d = pm.VoxelDomain(*nel)
+    x = fe.simp(rng.uniform(0.05, 1.0, d.nel))
+    bc = _mixed_bc3d(nel) if bcs == "mixed" else problems.cantilever3d(*nel)[0]
+    K = pm.AssembleStiffness(d, bc=bc)(dev(x))
+    n1 = 3 * (nel[0] // 2 + 1) * (nel[1] // 2 + 1) * (nel[2] // 2 + 1)
+    v = dev(rng.standard_normal(n1))
+    r = dev(rng.standard_normal(K.shape[0]))
+    out = []
+    try:
+        for free in (False, True):
+            tune.set_coarse_level1(free)
+            res = []
+            for prec, smoother in (("fp64", "jacobi"), ("fp32", "jacobi"), ("fp64", "chebyshev")):
+                mg = pm.solvers.build_hierarchy(d, levels, omega=0.5 if smoother == "jacobi" else smoother,
+                                                smooth_steps=2, precision=prec)
+                mg.update(K)
+                if prec == "fp64" and smoother == "jacobi":
+                    res.append(host(mg.level_apply(1, v)))
+                res.append(host(mg.solve(r)))
+            out.append(res)
+    finally:
+        tune.set_coarse_level1(True)
+    a1, b1 = out[0][0], out[1][0]
+    npt.assert_allclose(b1, a1, rtol=0, atol=1e-12 * np.abs(a1).max())
+    assert relerr(out[1][1], out[0][1]) < 1e-11
+    assert relerr(out[1][2], out[0][2]) < 1e-5
+    assert relerr(out[1][3], out[0][3]) < 1e-10
+
+
 @pytest.mark.parametrize("levels", [3, 4])
 def test_pcg_iterations_and_history_vs_reference(gpu_lib, levels):
     g = golden(f"c2_mgcg_32x16x16_L{levels}.npz")
