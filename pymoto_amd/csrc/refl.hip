@@ -49,36 +49,6 @@ __device__ inline T from_upper(T v) {
   return dpp_move<0x101>(v);  // row_shl:1
 }
 
-// In-place unnormalised Walsh-Hadamard transform over the corner index, per component.
-template <typename T>
-__device__ inline void wht8(T (&v)[8][3]) {
-#pragma unroll
-  for (int bit = 1; bit < 8; bit <<= 1)
-#pragma unroll
-    for (int a = 0; a < 8; ++a)
-      if (!(a & bit)) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const T p = v[a][d], q = v[a | bit][d];
-          v[a][d] = p + q;
-          v[a | bit][d] = p - q;
-        }
-      }
-}
-
-// The x and y stages of the transform for the four corners (a = ax + 2 ay) of one node plane.
-template <typename T>
-__device__ inline void wht_plane(T (&p)[4][3]) {
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const T s0 = p[0][d] + p[1][d], d0 = p[0][d] - p[1][d], s1 = p[2][d] + p[3][d], d1 = p[2][d] - p[3][d];
-    p[0][d] = s0 + s1;
-    p[2][d] = s0 - s1;
-    p[1][d] = d0 + d1;
-    p[3][d] = d0 - d1;
-  }
-}
-
 // Entry (d, e) of block chi vanishes for every K0 with the box sign/class pattern: characters 1..6
 // split into a single dof (iso) and a coupled pair.
 __host__ __device__ constexpr int refl_iso(int chi) {
@@ -88,9 +58,10 @@ __host__ __device__ constexpr bool refl_zero(int chi, int d, int e) {
   return chi != 0 && chi != 7 && d != e && (d == refl_iso(chi) || e == refl_iso(chi));
 }
 
-// v[0..3] / v[4..7]: the lower / upper plane's masked corner displacements after wht_plane.
-// Returns x * (K0 u_e)[a][d] in v (corner order a = ax + 2 ay + 4 az).  PAT skips the entries that
-// the box pattern makes zero.
+// v[0..3] / v[4..7]: the lower / upper plane's masked corner displacements after the x and y stages.
+// Returns x * (K0 u_e) in v before the inverse transform (frequency index k = kx + 2 ky + 4 kz): the
+// caller interleaves its stages with the node sums.  PAT skips the entries that the box pattern makes
+// zero.
 template <bool PAT, typename T>
 __device__ inline void refl_element(const T* __restrict__ tab, T x, T (&v)[8][3]) {
 #pragma unroll
@@ -114,7 +85,6 @@ __device__ inline void refl_element(const T* __restrict__ tab, T x, T (&v)[8][3]
       w[chi ^ (1 << d)][d] = acc;
     }
   }
-  wht8(w);
 #pragma unroll
   for (int a = 0; a < 8; ++a)
 #pragma unroll
@@ -151,8 +121,8 @@ struct G0Pattern {
 };
 constexpr G0Pattern kG0Pattern{};
 
-// v: the element's corners after the plane stages (as for refl_element); X: its X^.  Returns
-// (K_E u_e)[a][d] in v, without the Dirichlet corrections.
+// v: the element's corners after the plane stages (as for refl_element); X: its X^.  Returns K_E u_e
+// before the inverse transform (as refl_element), without the Dirichlet corrections.
 template <bool PAT, typename T>
 __device__ inline void coarse1_element(const T* tab, const T (&X)[8], T (&v)[8][3]) {
 #pragma unroll
@@ -168,8 +138,9 @@ __device__ inline void coarse1_element(const T* tab, const T (&X)[8], T (&v)[8][
 #pragma unroll
   for (int ca = 0; ca < 8; ++ca) {
     // One output class at a time.  Its table entries are read through an offset the compiler cannot
-    // see is 0 and that depends on the previous class's outputs, so the reads of all ~400 entries are
-    // not hoisted to the start (they would not fit the register file).
+    // see is 0 and that exists only once the previous class's results do, so the reads of all ~400
+    // entries are not hoisted to the start (they would not fit the register file).  FP32: the offset
+    // is scalar and the entries come through the scalar cache (SGPR operands); FP64: LDS broadcasts.
     const T* tb = tab + zoff;
     T acc[3];
 #pragma unroll
@@ -192,11 +163,13 @@ __device__ inline void coarse1_element(const T* tab, const T (&X)[8], T (&v)[8][
       }
     }
     // the class's three results are complete before the next class's offset exists
-    asm volatile("v_mov_b32 %0, 0" : "=v"(zoff), "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]));
+    if constexpr (sizeof(T) == 4)
+      asm volatile("s_mov_b32 %0, 0" : "=s"(zoff) : "v"(acc[0]), "v"(acc[1]), "v"(acc[2]));
+    else
+      asm volatile("v_mov_b32 %0, 0" : "=v"(zoff), "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]));
 #pragma unroll
     for (int d = 0; d < 3; ++d) w[ca ^ (1 << d)][d] = acc[d];
   }
-  wht8(w);
 #pragma unroll
   for (int a = 0; a < 8; ++a)
 #pragma unroll
@@ -222,16 +195,17 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
   constexpr int XN = OP == 0 ? 1 : 8;  // element factors per element
   constexpr int TS = (int)sizeof(T);
   constexpr bool CARRY = sizeof(T) == 4;  // carry the transformed plane to the next layer
-  __shared__ T su[2][3][NT];  // masked displacements of the block's nodes, planes of both parities
-  __shared__ T so[6][NT];  // corner (., 1, .) results after the x combine, both planes of a layer
-  __shared__ T sG0[OP == 1 ? 576 : 1];  // OP 1: the 24 x 24 table (read as broadcasts; too large for SGPRs)
+  __shared__ T su[2][6][NT];  // x-stage sums / differences of the block's node planes, both parities
+  __shared__ T so[6][NT];  // y-stage differences (corners ay = 1) of the layer's results, both kz
+  constexpr bool LDS_TAB = OP == 1 && sizeof(T) == 8;
+  __shared__ T sG0[LDS_TAB ? 576 : 1];  // OP 1, FP64: the 24 x 24 table (read as broadcasts)
 
   const int work = (int)xcd_block();
   if (work >= nwork) return;  // whole block leaves together (grid padded for the XCD remap)
   const int txy = work % ntiles_xy;
   const int tid = threadIdx.x, tx = tid % RX, ty = tid / RX;
   if (OP == 1)
-    for (int q = tid; q < 576; q += NT) sG0[q] = tab[q];  // visible after the first barrier below
+    for (int q = tid; q < 576 && LDS_TAB; q += NT) sG0[q] = tab[q];  // visible after the first barrier below
   const int i = (txy % tiles_x) * (RX - 2) - 1 + tx;
   const int j = (txy / tiles_x) * (SY - 2) - 1 + ty;
   const int kbeg = (work / ntiles_xy) * nzc;
@@ -307,21 +281,28 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
     load_node(kbeg - 1 + r, raw[r], mk[r]);
     load_x(kbeg - 1 + r, xs[r]);
   }
-  // corners of a node plane after the x and y transform stages, from the own masked values c (written
-  // to LDS plane buffer pl before the barrier inside)
+  // corners of a node plane after the x and y transform stages, from the own masked values c: the x
+  // stage with the +x neighbour (DPP), its sum and difference to LDS plane buffer pl, and after the
+  // barrier inside the y stage with the +y row's
   auto plane_corners = [&](const T(&c)[3], T(*const pl)[NT], T(&P)[4][3]) {
+    T sx[3], dx[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) pl[d][tid] = c[d];
+    for (int d = 0; d < 3; ++d) {
+      const T cx = from_upper(c[d]);
+      sx[d] = c[d] + cx;
+      dx[d] = c[d] - cx;
+      pl[d][tid] = sx[d];
+      pl[3 + d][tid] = dx[d];
+    }
     __syncthreads();
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const T yv = pl[d][slot_up];
-      P[0][d] = c[d];
-      P[1][d] = from_upper(c[d]);
-      P[2][d] = yv;
-      P[3][d] = from_upper(yv);
+      const T sy = pl[d][slot_up], dy = pl[3 + d][slot_up];
+      P[0][d] = sx[d] + sy;
+      P[2][d] = sx[d] - sy;
+      P[1][d] = dx[d] + dy;
+      P[3][d] = dx[d] - dy;
     }
-    wht_plane(P);
   };
   T Wlo[4][3];  // plane k, transformed (FP32: carried from the previous layer's upper plane)
   {
@@ -347,13 +328,14 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
       T(*const pl)[NT] = su[k & 1];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        const T yv = pl[d][slot_up];
-        Wlo[0][d] = c0[d];
-        Wlo[1][d] = from_upper(c0[d]);
-        Wlo[2][d] = yv;
-        Wlo[3][d] = from_upper(yv);
+        const T cx = from_upper(c0[d]);
+        const T sx = c0[d] + cx, dx = c0[d] - cx;
+        const T sy = pl[d][slot_up], dy = pl[3 + d][slot_up];
+        Wlo[0][d] = sx + sy;
+        Wlo[2][d] = sx - sy;
+        Wlo[1][d] = dx + dy;
+        Wlo[3][d] = dx - dy;
       }
-      wht_plane(Wlo);
     }
     T v[8][3];
 #pragma unroll
@@ -369,30 +351,45 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
     if constexpr (OP == 0) {
       refl_element<PAT>(tab + zero_stride * k, xs[SA][0], v);
     } else {
-      coarse1_element<PAT>(sG0 + zero_stride * k, xs[SA], v);  // (re-read per layer: not hoisted)
+      coarse1_element<PAT>((LDS_TAB ? sG0 : tab) + zero_stride * k, xs[SA], v);  // (re-read per layer)
     }
 
-    // x combine: s[ay][az] = own corner (0, ay, az) + left neighbour's corner (1, ay, az)
-    T s[2][2][3];
+    // Inverse transform interleaved with the node sums, one axis at a time (all stages are linear):
+    // x stage, then the node takes its own corner ax = 0 plus the left neighbour's ax = 1 (DPP); y
+    // stage, its own ay = 0 plus the row below's ay = 1 (LDS); z stage: az = 0 completes plane k, az = 1
+    // is carried to plane k+1.  21 add/sub per component instead of 24 for the transform and 7 for
+    // the sums.
+    T t[2][2][3];  // [ky][kz]
 #pragma unroll
-    for (int ay = 0; ay < 2; ++ay)
+    for (int ky = 0; ky < 2; ++ky)
 #pragma unroll
-      for (int az = 0; az < 2; ++az)
+      for (int kz = 0; kz < 2; ++kz)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) s[ay][az][d] = v[2 * ay + 4 * az][d] + from_lower(v[1 + 2 * ay + 4 * az][d]);
-    // y combine through LDS: the row below contributes its corners (., 1, .)
+        for (int d = 0; d < 3; ++d) {
+          const int m = 2 * ky + 4 * kz;
+          const T p = v[m][d] + v[m + 1][d], q = v[m][d] - v[m + 1][d];
+          t[ky][kz][d] = p + from_lower(q);
+        }
+    T r[2][3];  // [kz]
 #pragma unroll
-    for (int az = 0; az < 2; ++az)
+    for (int kz = 0; kz < 2; ++kz)
 #pragma unroll
-      for (int d = 0; d < 3; ++d) so[3 * az + d][tid] = s[1][az][d];
+      for (int d = 0; d < 3; ++d) {
+        r[kz][d] = t[0][kz][d] + t[1][kz][d];
+        so[3 * kz + d][tid] = t[0][kz][d] - t[1][kz][d];
+      }
     __syncthreads();
+#pragma unroll
+    for (int kz = 0; kz < 2; ++kz)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) r[kz][d] += so[3 * kz + d][slot_dn];
 
     {
       // plane k is complete (stores of the prologue layer and of halo lanes are dropped)
       const __amdgpu_buffer_rsrc_t ry = plane_rsrc(y, 3 * (long long)k * uplane, 3 * uplane, k >= kbeg);
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        const T au = C[d] + s[0][0][d] + so[d][slot_dn];
+        const T au = C[d] + (r[0][d] + r[1][d]);
         const T uk = raw[SA][d];
         const T ku = ((mk[SA] >> d) & 1u) ? bcdiag * uk : au;
         T out;
@@ -408,7 +405,7 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
       }
     }
 #pragma unroll
-    for (int d = 0; d < 3; ++d) C[d] = s[0][1][d] + so[3 + d][slot_dn];
+    for (int d = 0; d < 3; ++d) C[d] = r[0][d] - r[1][d];
 
     __builtin_amdgcn_sched_barrier(0);  // keep the prefetch behind the stores
     load_node(k + 3, raw[SA], mk[SA]);
@@ -467,19 +464,21 @@ static int refl_resident_blocks() {
   return cache[dev];
 }
 
-// Planes per block.  A CU's share of the work is ceil(blocks / CUs) blocks of (planes + 1) layers
-// (one prologue layer each); minimise that, among the chunkings that still give every CU at least
-// two resident blocks (latency hiding) whenever the grid allows it.
+// Planes per block.  Every CU runs `resident` blocks at once; the launch takes
+// ceil(blocks / (CUs * resident)) rounds of (planes + 1) layers (one prologue layer per block).
+// Minimise that, among the chunkings that still give every CU at least two resident blocks (latency
+// hiding) whenever the grid allows it; ties go to the fewer blocks.
 static int refl_planes(int nxy, int nz, int resident) {
   const long long cus = device_cu_count();
+  const long long slots = cus * (resident > 0 ? resident : 1);
   const long long want = cus * (resident < 2 ? resident : 2);
   int best = 0;
   long long best_cost = -1;
   for (int pass = 0; pass < 2 && best == 0; ++pass)
-    for (int c = 2; c <= nz; ++c) {
+    for (int c = nz; c >= 2; --c) {
       const long long blocks = (long long)nxy * ((nz + c - 1) / c);
       if (pass == 0 && blocks < want) continue;
-      const long long cost = ((blocks + cus - 1) / cus) * (c + 1);
+      const long long cost = ((blocks + slots - 1) / slots) * (c + 1);
       if (best_cost < 0 || cost < best_cost) {
         best_cost = cost;
         best = c;
