@@ -197,9 +197,11 @@ def ku_driver(pm, torch, nel, reps=5):
     print(f"K.u x{reps}, copy x{reps}: vector of {u.numel()} FP64 = {u.numel() * 8 / 1024:.1f} KiB", flush=True)
 
 
-def batched_workload(pm, torch, nel=(128, 64, 64), beta=8.0, etas=(0.3, 0.5, 0.7), nloads=4, nuneven=16, seed=7,
-                     precision="fp32", omega=0.5, tol=1e-8):
-    """Robust multi-load batch: 3 Heaviside designs (eta 0.3/0.5/0.7, beta 8) x 4 seeded point loads on
+def batched_workload(pm, torch, nel=(128, 64, 64), beta=8.0, etas=(0.4, 0.5, 0.6), nloads=4, nuneven=16, seed=7,
+                     precision="fp32", omega=0.5, tol=1e-8, xmin=1e-5):
+    """Robust multi-load batch: 3 Heaviside designs (eta 0.4/0.5/0.6, beta 8, SIMP xmin 1e-5: the dilated /
+    nominal / eroded projections and stiffness floor of the reference's robust example,
+    examples/topology_optimization/ex_compliance_robust.py:27,137-149) x 4 seeded point loads on
     the x = max face, each design one 4-column block PCG (captured-graph iterations), timed against
     the same 12 solves one right-hand side at a time; plus 16 independent problems of seeded sizes
     from 64x32 Q4 to 96x48x48 hex8, solved concurrently on 8 streams and timed against one stream."""
@@ -214,15 +216,15 @@ def batched_workload(pm, torch, nel=(128, 64, 64), beta=8.0, etas=(0.3, 0.5, 0.7
         j, k = int(rng.integers(0, nel[1] + 1)), int(rng.integers(0, nel[2] + 1))
         F[3 * d.get_nodenumber(nel[0], j, k) + 1, c] = -1.0
     Fd = torch.as_tensor(F, device="cuda")
-    # densities in [0.35, 1]: after the r = 1.5 filter all three projections (eta 0.3/0.5/0.7) keep a
-    # connected structure (from uniform[0, 1] the eroded design is almost void and singular)
+    # densities in [0.35, 1]: after the r = 1.5 filter all three projections keep a connected structure
+    # (from uniform[0, 1] the eroded design is almost void)
     x = torch.as_tensor(rng.uniform(0.35, 1.0, d.nel), device="cuda")
     xf = pm.FilterConv(d, radius=1.5)(pm.Signal("x", x)).state
     hv = "(tanh({b} * {e}) + tanh({b} * (inp0 - {e}))) / (tanh({b} * {e}) + tanh({b} * (1 - {e})))"
     Ks = []
     for eta in etas:
         xp = pm.MathExpression(hv.format(b=beta, e=eta))(pm.Signal("xf", xf)).state
-        xs = pm.MathExpression(f"{1e-9} + {1.0 - 1e-9}*inp0^3")(pm.Signal("xp", xp)).state
+        xs = pm.MathExpression(f"{xmin} + {1.0 - xmin}*inp0^3")(pm.Signal("xp", xp)).state
         Ks.append(pm.AssembleStiffness(d, bc=bc)(xs))
     levels = pm.solvers.auto_levels(d)
 
@@ -285,8 +287,8 @@ def batched_workload(pm, torch, nel=(128, 64, 64), beta=8.0, etas=(0.3, 0.5, 0.7
     t_seq = _t.perf_counter() - t0
     ndof = 3 * d.nnodes
     return {
-        "workload": f"robust {nel[0]}x{nel[1]}x{nel[2]} hex8, Heaviside beta {beta} eta {list(etas)}, {nloads} point "
-                    f"loads per design; {precision} V-cycle (omega {omega}), FP64 CG, tol {tol:g}",
+        "workload": f"robust {nel[0]}x{nel[1]}x{nel[2]} hex8, Heaviside beta {beta} eta {list(etas)}, SIMP xmin {xmin:g}, "
+                    f"{nloads} point loads per design; {precision} V-cycle (omega {omega}), FP64 CG, tol {tol:g}",
         "rhs_solves": len(etas) * nloads, "ndof": ndof,
         "block_graph_s": t_block, "block_iterations": its_block,
         "single_rhs_s": t_single, "single_rhs_iterations": its_single,

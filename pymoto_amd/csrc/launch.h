@@ -69,13 +69,15 @@ int launch_fine_mfma(int mode, const GridDims& g, const double* K0, const float*
 
 // vector.hip
 int launch_dot(long long n, const double* x, const double* y, double* out, double* work, hipStream_t s);
+int launch_dot(long long n, const double* x, const float* y, double* out, double* work, hipStream_t s);
 int launch_dot2(long long n, const double* a0, const double* b0, const double* a1, const double* b1, double* out,
                 double* work, hipStream_t s);
 int launch_axpby(long long n, double a, const double* x, double b, double* y, hipStream_t s);
 int launch_cg_xr_update(long long n, const double* sc, const double* p, const double* q, double* x, double* r,
-                        double* work, double* out_rr, hipStream_t s);
+                        double* work, double* out_rr, hipStream_t s, float* r32 = nullptr);
 int launch_cg_x_update(long long n, const double* sc, const double* p, double* x, hipStream_t s);
 int launch_cg_p_update(long long n, const double* sc, const double* z, double* p, hipStream_t s);
+int launch_cg_p_update(long long n, const double* sc, const float* z, double* p, hipStream_t s);
 int launch_scale_rsqrt(long long n, const double* sc, int k, const double* z, double* p, hipStream_t s);
 int launch_fill(long long n, double v, double* y, hipStream_t s);
 int launch_to_double(long long n, const float* x, double* y, hipStream_t s);

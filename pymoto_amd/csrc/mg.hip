@@ -186,13 +186,19 @@ static int vcycle(pmt_mg* mg, int l, const T* b, T* out, hipStream_t s) {
   return PMT_OK;
 }
 
+// reduced-precision V-cycles: z32 = M r, from r rounded to FP32 (already in the fine level's FP32 right-hand
+// side when r32_ready: the CG update wrote it)
+static int precondition_f32(pmt_mg* mg, const double* r, bool r32_ready, hipStream_t s) {
+  Level& F = mg->L[0];
+  if (!r32_ready) PMT_TRY(launch_to_float(F.ndof, r, F.f.b, s));
+  return vcycle<float>(mg, 0, F.f.b, mg->z32, s);
+}
+
 // the preconditioner as CG sees it: FP64 in, FP64 out
 static int precondition(pmt_mg* mg, const double* r, double* z, hipStream_t s) {
   if (mg->prec == 64) return vcycle<double>(mg, 0, r, z, s);
-  Level& F = mg->L[0];
-  PMT_TRY(launch_to_float(F.ndof, r, F.f.b, s));
-  PMT_TRY(vcycle<float>(mg, 0, F.f.b, mg->z32, s));
-  return launch_to_double(F.ndof, mg->z32, z, s);
+  PMT_TRY(precondition_f32(mg, r, false, s));
+  return launch_to_double(mg->L[0].ndof, mg->z32, z, s);
 }
 
 static int read_scalar(pmt_mg* mg, int k, double* v, hipStream_t s) {
@@ -678,21 +684,32 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
   for (i = 0; i < maxit; ++i) {
     PMT_TRY(level_apply<double>(mg, 0, MODE_APPLY, mg->p, nullptr, mg->q, s));
     PMT_TRY(launch_dot2(n, mg->p, mg->q, mg->p, mg->r, sc, mg->work, s));  // sc[0] = p.q, sc[1] = p.r
+    // reduced-precision V-cycles: the update also writes r in FP32 (the V-cycle's input), and z stays
+    // FP32 (q.z and p = z + beta p widen it exactly: the same sums as through an FP64 copy)
+    const bool mixed = mg->prec != 64;
+    bool r32_ready = false;
     if (i % restart == 0) {
       PMT_TRY(launch_cg_x_update(n, sc, mg->p, x, s));
       PMT_TRY(level_apply<double>(mg, 0, MODE_RESIDUAL, x, b, mg->r, s));
       PMT_TRY(launch_dot(n, mg->r, mg->r, sc + 2, mg->work, s));
     } else {
-      PMT_TRY(launch_cg_xr_update(n, sc, mg->p, mg->q, x, mg->r, mg->work, sc + 2, s));
+      PMT_TRY(launch_cg_xr_update(n, sc, mg->p, mg->q, x, mg->r, mg->work, sc + 2, s, mixed ? F.f.b : nullptr));
+      r32_ready = mixed;
     }
     double rr = 0.0;
     PMT_TRY(read_scalar(mg, 2, &rr, s));
     tval = std::sqrt(rr) / bn;
     if (history) history[i + 1] = tval;
     if (tval <= tol) break;
-    PMT_TRY(precondition(mg, mg->r, mg->z, s));
-    PMT_TRY(launch_dot(n, mg->q, mg->z, sc + 3, mg->work, s));
-    PMT_TRY(launch_cg_p_update(n, sc, mg->z, mg->p, s));
+    if (mixed) {
+      PMT_TRY(precondition_f32(mg, mg->r, r32_ready, s));
+      PMT_TRY(launch_dot(n, mg->q, mg->z32, sc + 3, mg->work, s));
+      PMT_TRY(launch_cg_p_update(n, sc, mg->z32, mg->p, s));
+    } else {
+      PMT_TRY(precondition(mg, mg->r, mg->z, s));
+      PMT_TRY(launch_dot(n, mg->q, mg->z, sc + 3, mg->work, s));
+      PMT_TRY(launch_cg_p_update(n, sc, mg->z, mg->p, s));
+    }
   }
   if (iters) *iters = i < maxit ? i + 1 : maxit;
   if (relres) *relres = tval;

@@ -92,14 +92,16 @@ __device__ inline double block_sum(double v) {
   return r;  // valid in thread 0
 }
 
-// stage 1: NOUT dot products sharing the grid-stride loop; partial[o*kReduceBlocks + block]
-template <int NOUT>
+// stage 1: NOUT dot products sharing the grid-stride loop; partial[o*kReduceBlocks + block].  B0: type of
+// the first product's second operand (an FP32 vector is widened exactly: the sums are those of its
+// FP64 copy)
+template <int NOUT, typename B0 = double>
 __global__ void __launch_bounds__(kReduceThreads)
-dot_stage1(long long n, const double* __restrict__ a0, const double* __restrict__ b0, const double* __restrict__ a1,
+dot_stage1(long long n, const double* __restrict__ a0, const B0* __restrict__ b0, const double* __restrict__ a1,
            const double* __restrict__ b1, double* __restrict__ partial) {
   double s0 = 0.0, s1 = 0.0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    s0 = fma(a0[i], b0[i], s0);
+    s0 = fma(a0[i], (double)b0[i], s0);
     if (NOUT > 1) s1 = fma(a1[i], b1[i], s1);
   }
   double r0 = block_sum(s0);
@@ -143,6 +145,13 @@ int launch_dot(long long n, const double* x, const double* y, double* out, doubl
   return check_launch("dot_stage2");
 }
 
+int launch_dot(long long n, const double* x, const float* y, double* out, double* work, hipStream_t s) {
+  dot_stage1<1, float><<<kReduceBlocks, kReduceThreads, 0, s>>>(n, x, y, nullptr, nullptr, work);
+  PMT_TRY(check_launch("dot_stage1<float>"));
+  dot_stage2<1><<<1, kReduceThreads, 0, s>>>(work, out);
+  return check_launch("dot_stage2");
+}
+
 int launch_dot2(long long n, const double* a0, const double* b0, const double* a1, const double* b1, double* out,
                 double* work, hipStream_t s) {
   dot_stage1<2><<<kReduceBlocks, kReduceThreads, 0, s>>>(n, a0, b0, a1, b1, work);
@@ -151,16 +160,18 @@ int launch_dot2(long long n, const double* a0, const double* b0, const double* a
   return check_launch("dot_stage2<2>");
 }
 
-// CG: x += alpha p, r -= alpha q, partial ||r||^2; alpha = sc[1]/sc[0] read on device
+// CG: x += alpha p, r -= alpha q, partial ||r||^2; alpha = sc[1]/sc[0] read on device; r32 (optional):
+// the new r rounded to FP32, the input of a reduced-precision preconditioner
 __global__ void __launch_bounds__(kReduceThreads)
 cg_xr_update(long long n, const double* __restrict__ sc, const double* __restrict__ p, const double* __restrict__ q,
-             double* __restrict__ x, double* __restrict__ r, double* __restrict__ partial) {
+             double* __restrict__ x, double* __restrict__ r, double* __restrict__ partial, float* __restrict__ r32) {
   const double alpha = sc[1] / sc[0];
   double s = 0.0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     x[i] = fma(alpha, p[i], x[i]);
     const double rv = fma(-alpha, q[i], r[i]);
     r[i] = rv;
+    if (r32) r32[i] = (float)rv;
     s = fma(rv, rv, s);
   }
   double t = block_sum(s);
@@ -175,12 +186,14 @@ __global__ void cg_x_update(long long n, const double* __restrict__ sc, const do
   if (i < n) x[i] = fma(alpha, p[i], x[i]);
 }
 
-// CG: p = z + beta p with beta = -sc[3]/sc[0]   (iterative.py:394-395)
-__global__ void cg_p_update(long long n, const double* __restrict__ sc, const double* __restrict__ z,
+// CG: p = z + beta p with beta = -sc[3]/sc[0]   (iterative.py:394-395); z FP64 or the FP32 output of a
+// reduced-precision preconditioner (widened exactly)
+template <typename Z>
+__global__ void cg_p_update(long long n, const double* __restrict__ sc, const Z* __restrict__ z,
                             double* __restrict__ p) {
   const double beta = -sc[3] / sc[0];
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = fma(beta, p[i], z[i]);
+  if (i < n) p[i] = fma(beta, p[i], (double)z[i]);
 }
 
 // p = z / sqrt(sc[k])  (orth(z, normalize=True) for one column, iterative.py:286-290)
@@ -225,8 +238,8 @@ int launch_axpby(long long n, double a, const double* x, double b, double* y, hi
 }
 
 int launch_cg_xr_update(long long n, const double* sc, const double* p, const double* q, double* x, double* r,
-                        double* work, double* out_rr, hipStream_t s) {
-  cg_xr_update<<<kReduceBlocks, kReduceThreads, 0, s>>>(n, sc, p, q, x, r, work);
+                        double* work, double* out_rr, hipStream_t s, float* r32) {
+  cg_xr_update<<<kReduceBlocks, kReduceThreads, 0, s>>>(n, sc, p, q, x, r, work, r32);
   PMT_TRY(check_launch("cg_xr_update"));
   dot_stage2<1><<<1, kReduceThreads, 0, s>>>(work, out_rr);
   return check_launch("dot_stage2");
@@ -238,8 +251,13 @@ int launch_cg_x_update(long long n, const double* sc, const double* p, double* x
 }
 
 int launch_cg_p_update(long long n, const double* sc, const double* z, double* p, hipStream_t s) {
-  cg_p_update<<<blocks_for(n, 256), 256, 0, s>>>(n, sc, z, p);
+  cg_p_update<double><<<blocks_for(n, 256), 256, 0, s>>>(n, sc, z, p);
   return check_launch("cg_p_update");
+}
+
+int launch_cg_p_update(long long n, const double* sc, const float* z, double* p, hipStream_t s) {
+  cg_p_update<float><<<blocks_for(n, 256), 256, 0, s>>>(n, sc, z, p);
+  return check_launch("cg_p_update<float>");
 }
 
 int launch_scale_rsqrt(long long n, const double* sc, int k, const double* z, double* p, hipStream_t s) {
