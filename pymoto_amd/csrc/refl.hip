@@ -321,15 +321,12 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
     T Whi[4][3];
     plane_corners(c1, su[(k + 1) & 1], Whi);
     if (!CARRY) {
-      // FP64: rebuild the lower plane from its own values and the LDS plane the previous layer wrote
-      // (24 fewer live registers than carrying it)
-      T c0[3];
-      masked(raw[SA], mk[SA], c0);
+      // FP64: rebuild the lower plane from the x-stage values the previous layer left in LDS (its own
+      // and the +y row's; 24 fewer live registers than carrying it)
       T(*const pl)[NT] = su[k & 1];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        const T cx = from_upper(c0[d]);
-        const T sx = c0[d] + cx, dx = c0[d] - cx;
+        const T sx = pl[d][tid], dx = pl[3 + d][tid];
         const T sy = pl[d][slot_up], dy = pl[3 + d][slot_up];
         Wlo[0][d] = sx + sy;
         Wlo[2][d] = sx - sy;
