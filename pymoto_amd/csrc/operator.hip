@@ -643,7 +643,9 @@ __global__ void restrict_kernel(GridDims gf, GridDims gc, const T* __restrict__ 
   for (int d = 0; d < DIM; ++d) rc[N * DIM + d] = acc[d];
 }
 
-// uf += R uc : fine node interpolates from the 1, 2, 4 or 8 coarse nodes around it
+// uf += R uc : fine node interpolates from the 1, 2, 4 or 8 coarse nodes around it.  Branch free: every
+// lane reads 2^D corners, the second one along an axis of even fine index repeating the first with
+// weight 0 (the weights are powers of two, so the sum equals w * (sum of the nonzero corners) exactly).
 template <int DIM, typename T>
 __global__ void prolong_kernel(GridDims gf, GridDims gc, const T* __restrict__ uc, T* __restrict__ uf) {
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -652,21 +654,27 @@ __global__ void prolong_kernel(GridDims gf, GridDims gc, const T* __restrict__ u
   const long long t = n / gf.nx;
   const int j = (int)(t % gf.ny);
   const int k = (int)(t / gf.ny);
-  const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
-  const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (DIM == 3 && (k & 1)) ? 2 : 1;
-  const T w = (T)(((i & 1) ? 0.5 : 1.0) * ((j & 1) ? 0.5 : 1.0) * ((DIM == 3 && (k & 1)) ? 0.5 : 1.0));
+  const int oi = i & 1, oj = j & 1, ok = DIM == 3 ? (k & 1) : 0;
+  const T wx[2] = {(T)(oi ? 0.5 : 1.0), (T)(oi ? 0.5 : 0.0)};
+  const T wy[2] = {(T)(oj ? 0.5 : 1.0), (T)(oj ? 0.5 : 0.0)};
+  const T wz[2] = {(T)(ok ? 0.5 : 1.0), (T)(ok ? 0.5 : 0.0)};
   T acc[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) acc[d] = 0;
-  for (int c = 0; c < nk; ++c)
-    for (int bb = 0; bb < nj; ++bb)
-      for (int a = 0; a < ni; ++a) {
-        const long long N = ((long long)(k0 + c) * gc.ny + (j0 + bb)) * gc.nx + (i0 + a);
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) acc[d] += uc[N * DIM + d];
+  for (int c = 0; c < (DIM == 3 ? 2 : 1); ++c)
+#pragma unroll
+    for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const T w = wx[a] * wy[bb] * wz[c];
+        const long long N =
+            ((long long)((k >> 1) + c * ok) * gc.ny + ((j >> 1) + bb * oj)) * gc.nx + ((i >> 1) + a * oi);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) acc[d] = fma(w, uc[N * DIM + d], acc[d]);
       }
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) uf[n * DIM + d] += w * acc[d];
+  for (int d = 0; d < DIM; ++d) uf[n * DIM + d] += acc[d];
 }
 
 // dx_e (+)= alpha * u_e^T K0 v_e with Dirichlet entries zeroed (element-centric, no scatter)

@@ -196,7 +196,12 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
   constexpr int TS = (int)sizeof(T);
   constexpr bool CARRY = sizeof(T) == 4;  // carry the transformed plane to the next layer
   __shared__ T su[2][6][NT];  // x-stage sums / differences of the block's node planes, both parities
-  __shared__ T so[6][NT];  // y-stage differences (corners ay = 1) of the layer's results, both kz
+  // fine-grid sweeps take D^-1 from the densities instead of a stored vector: with K0 reflection
+  // symmetric every corner has the same diagonal k_d, so D(n, d) = k_d * (sum of the 8 element factors
+  // around n), bcdiag on Dirichlet dofs
+  constexpr bool DIAG = OP == 0 && (MODE == MODE_JACOBI || MODE == MODE_CHEB);
+  __shared__ T so[DIAG ? 7 : 6][NT];  // y-stage differences (corners ay = 1) of the layer's results, both kz;
+                                      // DIAG: then the x-summed element factors
   constexpr bool LDS_TAB = OP == 1 && sizeof(T) == 8;
   __shared__ T sG0[LDS_TAB ? 576 : 1];  // OP 1, FP64: the 24 x 24 table (read as broadcasts)
 
@@ -252,7 +257,7 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
 #pragma unroll
       for (int d = 0; d < 3; ++d) bo[d] = bload(rb, o_out + d * TS, T());
     }
-    if (MODE == MODE_JACOBI || MODE == MODE_CHEB) {
+    if ((MODE == MODE_JACOBI || MODE == MODE_CHEB) && !DIAG) {
       const __amdgpu_buffer_rsrc_t rd = plane_rsrc(dinv, first, cnt, pin);
 #pragma unroll
       for (int d = 0; d < 3; ++d) dv[d] = bload(rd, o_out + d * TS, T());
@@ -311,6 +316,19 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
     plane_corners(c, su[(kbeg - 1) & 1], Wlo);
   }
   T C[3] = {(T)0, (T)0, (T)0};  // own node, plane k: the sum over element layer k-1
+  // DIAG: 1 / k_d (k_d = sum over the classes of the blocks' (d, d) entries: K0 = H M H with H^2 = 8 I
+  // per axis and M holding the 1/64) and 1 / bcdiag, once per block; XC: the x sum of layer k-1
+  T ikd[3] = {(T)1, (T)1, (T)1}, ibc = (T)1, XC = (T)0;
+  if constexpr (DIAG) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      T kd = (T)0;
+#pragma unroll
+      for (int chi = 0; chi < 8; ++chi) kd += tab[chi * 9 + d * 4];
+      ikd[d] = (T)1 / kd;
+    }
+    ibc = (T)1 / bcdiag;
+  }
 
   auto layer = [&](int k, auto slot) {
     constexpr int SA = decltype(slot)::value, SB = (SA + 1) % 3;
@@ -375,11 +393,23 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
         r[kz][d] = t[0][kz][d] + t[1][kz][d];
         so[3 * kz + d][tid] = t[0][kz][d] - t[1][kz][d];
       }
+    T xsum = (T)0;  // DIAG: element factors of layer k around the node, x then y
+    if constexpr (DIAG) {
+      xsum = xs[SA][0] + from_lower(xs[SA][0]);
+      so[6][tid] = xsum;
+    }
     __syncthreads();
 #pragma unroll
     for (int kz = 0; kz < 2; ++kz)
 #pragma unroll
       for (int d = 0; d < 3; ++d) r[kz][d] += so[3 * kz + d][slot_dn];
+    if constexpr (DIAG) {
+      xsum += so[6][slot_dn];
+      const T inv = (T)1 / (XC + xsum);  // plane k: layers k-1 and k
+      XC = xsum;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) dv[d] = ((mk[SA] >> d) & 1u) ? ibc : inv * ikd[d];
+    }
 
     {
       // plane k is complete (stores of the prologue layer and of halo lanes are dropped)
