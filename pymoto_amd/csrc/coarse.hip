@@ -435,22 +435,35 @@ __global__ void gj_colcopy_kernel(long long n, long long k0, int kk, const doubl
   C[i * GJB + c] = A[i * n + k0 + c];
 }
 
-// row panel A[K][j] = P A[K][j] for j outside K (one thread per column)
+// row panel A[K][j] = P A[K][j] for j outside K.  Block = 16 columns, staged in LDS with P (the panel
+// is updated in place); thread = one column and 4 of the 64 rows (r, r + 16, r + 32, r + 48).
+constexpr int GJ_RP_COLS = 16;
 __global__ void __launch_bounds__(256) gj_rowpanel_kernel(long long n, long long k0, int kk, double* __restrict__ A,
                                                            const double* __restrict__ P) {
-  __shared__ double sP[GJB * GJB];
-  for (int q = threadIdx.x; q < GJB * GJB; q += blockDim.x) sP[q] = P[q];
+  __shared__ double sP[GJB][GJB + 1];
+  __shared__ double sA[GJB][GJ_RP_COLS + 1];
+  const int tid = threadIdx.x;
+  const long long j0 = (long long)blockIdx.x * GJ_RP_COLS;
+  for (int q = tid; q < GJB * GJB; q += blockDim.x) sP[q / GJB][q % GJB] = P[q];
+  for (int q = tid; q < GJB * GJ_RP_COLS; q += blockDim.x) {
+    const int r = q / GJ_RP_COLS, c = q % GJ_RP_COLS;
+    const long long j = j0 + c;
+    sA[r][c] = (r < kk && j < n) ? A[(k0 + r) * n + j] : 0.0;
+  }
   __syncthreads();
-  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n || (j >= k0 && j < k0 + kk)) return;
-  double col[GJB];
+  const int jj = tid % GJ_RP_COLS, rg = tid / GJ_RP_COLS;
+  const long long j = j0 + jj;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int c = 0; c < kk; ++c) {
+    const double a = sA[c][jj];
 #pragma unroll
-  for (int r = 0; r < GJB; ++r) col[r] = r < kk ? A[(k0 + r) * n + j] : 0.0;
-  for (int r = 0; r < kk; ++r) {
-    double s = 0.0;
+    for (int i = 0; i < 4; ++i) acc[i] = fma(sP[rg + 16 * i][c], a, acc[i]);
+  }
+  if (j >= n || (j >= k0 && j < k0 + kk)) return;  // the pivot columns are set by gj_update_kernel
 #pragma unroll
-    for (int c = 0; c < GJB; ++c) s = fma(sP[r * GJB + c], col[c], s);
-    A[(k0 + r) * n + j] = s;
+  for (int i = 0; i < 4; ++i) {
+    const int r = rg + 16 * i;
+    if (r < kk) A[(k0 + r) * n + j] = acc[i];
   }
 }
 
@@ -639,7 +652,7 @@ int gj_invert(long long n, double* A, double* wsP, double* wsC, hipStream_t s) {
     PMT_TRY(check_launch("gj_diag_kernel"));
     gj_colcopy_kernel<<<blocks_for(n * kk, 256), 256, 0, s>>>(n, k0, kk, A, wsC);
     PMT_TRY(check_launch("gj_colcopy_kernel"));
-    gj_rowpanel_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, k0, kk, A, wsP);
+    gj_rowpanel_kernel<<<(unsigned)((n + GJ_RP_COLS - 1) / GJ_RP_COLS), 256, 0, s>>>(n, k0, kk, A, wsP);
     PMT_TRY(check_launch("gj_rowpanel_kernel"));
     dim3 grid((unsigned)nb, (unsigned)nb);
     gj_update_kernel<<<grid, 256, 0, s>>>(n, k0, kk, A, wsC, wsP);
