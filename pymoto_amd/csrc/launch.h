@@ -7,7 +7,8 @@ namespace pmt {
 
 // MODE_CHEB: y = u + omega D^-1 (b - A u) + beta (u - up), the second-order (Chebyshev) sweep;
 // up may be NULL (= 0) and may alias y (each entry reads its own up before writing it).
-enum { MODE_APPLY = 0, MODE_RESIDUAL = 1, MODE_JACOBI = 2, MODE_CHEB = 3 };
+// MODE_APPLY_DOTS (fine reflection kernel, FP64): y = A u plus per-block partial sums of u.y and u.r.
+enum { MODE_APPLY = 0, MODE_RESIDUAL = 1, MODE_JACOBI = 2, MODE_CHEB = 3, MODE_APPLY_DOTS = 4 };
 
 // operator.hip (level kernels exist for T = double and T = float)
 template <typename T>
@@ -36,6 +37,14 @@ int launch_fine_refl(int mode, const GridDims& g, int planes /* 0 = from the occ
                      const T* xe,
                      const uint8_t* bcm, double bcdiag, const T* u, const T* b, const T* dinv, double omega, T* y,
                      hipStream_t s, const T* up, double beta);
+// q = A p on the fine grid with out[0] = p.q and out[1] = p.r (deterministic two-stage sums; ws holds
+// 2 * ws_parts doubles).  Returns PMT_ERR_STATE (nothing launched) when the reflection kernel is not the
+// fine operator or the launch needs more partial sums than ws holds; the caller then runs K.u and
+// the dot products separately.
+int launch_fine_apply_dots(const GridDims& g, const double* K0ext, int k0_pattern, const double* xe,
+                           const uint8_t* bcm, double bcdiag, const double* p, const double* r, double* q,
+                           double* ws, long long ws_parts, double* out, hipStream_t s);
+long long fine_apply_dots_parts(const GridDims& g);  // a sufficient ws_parts for the grid
 int set_tuning(int param, int value);
 int get_tuning(int param);
 int tuned_stream_planes();  // 0 = automatic

@@ -495,6 +495,10 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
   ALLOC(mg->r, n0);
   ALLOC(mg->sc, 16);
   ALLOC(mg->work, 2 * kReduceBlocks);
+  if (hex && (mg->k0pat & PMT_K0_REFLECTION)) {
+    mg->dotws_parts = fine_apply_dots_parts(mg->L[0].g);
+    ALLOC(mg->dotws, 2 * mg->dotws_parts);
+  }
 #undef ALLOC
   if (rc == PMT_OK) rc = hip_status(hipHostMalloc((void**)&mg->host_sc, 16 * sizeof(double), hipHostMallocDefault),
                                     "hipHostMalloc");
@@ -682,8 +686,17 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
   PMT_TRY(launch_scale_rsqrt(n, sc, 5, mg->z, mg->p, s));
   int i = 0;
   for (i = 0; i < maxit; ++i) {
-    PMT_TRY(level_apply<double>(mg, 0, MODE_APPLY, mg->p, nullptr, mg->q, s));
-    PMT_TRY(launch_dot2(n, mg->p, mg->q, mg->p, mg->r, sc, mg->work, s));  // sc[0] = p.q, sc[1] = p.r
+    // q = A p, sc[0] = p.q, sc[1] = p.r: fused into the reflection-basis operator when it is the fine
+    // operator, else K.u and one two-dot pass
+    const int frc = mg->dotws ? launch_fine_apply_dots(F.g, mg->K0, mg->k0pat, mg->xe, mg->bcm, mg->bcdiag, mg->p,
+                                                       mg->r, mg->q, mg->dotws, mg->dotws_parts, sc, s)
+                              : PMT_ERR_STATE;
+    if (frc == PMT_ERR_STATE) {
+      PMT_TRY(level_apply<double>(mg, 0, MODE_APPLY, mg->p, nullptr, mg->q, s));
+      PMT_TRY(launch_dot2(n, mg->p, mg->q, mg->p, mg->r, sc, mg->work, s));
+    } else {
+      PMT_TRY(frc);
+    }
     // reduced-precision V-cycles: the update also writes r in FP32 (the V-cycle's input), and z stays
     // FP32 (q.z and p = z + beta p widen it exactly: the same sums as through an FP64 copy)
     const bool mixed = mg->prec != 64;

@@ -88,6 +88,8 @@ struct pmt_mg {
   double* r = nullptr;
   double* sc = nullptr;    // device scalars
   double* work = nullptr;  // reduction partials
+  double* dotws = nullptr;  // partial sums of the fused fine K.u + dot products (2 x dotws_parts)
+  long long dotws_parts = 0;
   double* host_sc = nullptr;
   long long bytes = 0;
   std::vector<void*> allocs;

@@ -190,8 +190,12 @@ template <int OP, int SY, int MODE, bool PAT, typename T>
 __global__ void __launch_bounds__(kReflRow* SY)
 fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, const T* __restrict__ tab, int zero_stride,
                  const T* __restrict__ xe, const uint8_t* __restrict__ bcm, T bcdiag, const T* __restrict__ u,
-                 const T* __restrict__ b, const T* __restrict__ dinv, T omega, T* y, const T* up, T beta) {
+                 const T* __restrict__ b, const T* __restrict__ dinv, T omega, T* y, const T* up, T beta,
+                 double* __restrict__ partial) {
   constexpr int RX = kReflRow, NT = RX * SY;
+  // MODE_APPLY_DOTS: y = A u, and the block's sums of u.y and u.r over its output nodes to
+  // partial[work] and partial[nwork + work] (b holds r)
+  constexpr bool DOTS = MODE == MODE_APPLY_DOTS;
   constexpr int XN = OP == 0 ? 1 : 8;  // element factors per element
   constexpr int TS = (int)sizeof(T);
   constexpr bool CARRY = sizeof(T) == 4;  // carry the transformed plane to the next layer
@@ -252,7 +256,7 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
     const long long first = 3 * (long long)p * uplane, cnt = 3 * uplane;
 #pragma unroll
     for (int d = 0; d < 3; ++d) bo[d] = dv[d] = po[d] = (T)0;
-    if (MODE != MODE_APPLY) {
+    if (MODE != MODE_APPLY) {  // (DOTS: r)
       const __amdgpu_buffer_rsrc_t rb = plane_rsrc(b, first, cnt, pin);
 #pragma unroll
       for (int d = 0; d < 3; ++d) bo[d] = bload(rb, o_out + d * TS, T());
@@ -316,6 +320,7 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
     plane_corners(c, su[(kbeg - 1) & 1], Wlo);
   }
   T C[3] = {(T)0, (T)0, (T)0};  // own node, plane k: the sum over element layer k-1
+  double spq = 0.0, spr = 0.0;  // DOTS: this thread's output nodes
   // DIAG: 1 / k_d (k_d = sum over the classes of the blocks' (d, d) entries: K0 = H M H with H^2 = 8 I
   // per axis and M holding the 1/64) and 1 / bcdiag, once per block; XC: the x sum of layer k-1
   T ikd[3] = {(T)1, (T)1, (T)1}, ibc = (T)1, XC = (T)0;
@@ -420,7 +425,7 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
         const T uk = raw[SA][d];
         const T ku = ((mk[SA] >> d) & 1u) ? bcdiag * uk : au;
         T out;
-        if (MODE == MODE_APPLY)
+        if (MODE == MODE_APPLY || DOTS)
           out = ku;
         else if (MODE == MODE_RESIDUAL)
           out = bo[d] - ku;
@@ -429,6 +434,11 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
         else
           out = uk + omega * dv[d] * (bo[d] - ku) + beta * (uk - po[d]);
         bstore(ry, o_out + d * TS, out);
+        if constexpr (DOTS) {
+          const bool own = outp && k >= kbeg;
+          spq = fma((double)uk, own ? (double)out : 0.0, spq);
+          spr = fma((double)uk, own ? (double)bo[d] : 0.0, spr);
+        }
       }
     }
 #pragma unroll
@@ -446,6 +456,24 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
     if (k + 2 >= kend) break;
     layer(k + 2, std::integral_constant<int, 2>{});
     if (k + 3 >= kend) break;
+  }
+  if constexpr (DOTS) {  // fixed-order tree over the block (LDS rows 0 and 1 of so are free now)
+    __syncthreads();
+    so[0][tid] = spq;
+    so[1][tid] = spr;
+    __syncthreads();
+#pragma unroll
+    for (int st = NT / 2; st > 0; st >>= 1) {
+      if (tid < st) {
+        so[0][tid] += so[0][tid + st];
+        so[1][tid] += so[1][tid + st];
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      partial[work] = so[0][0];
+      partial[nwork + work] = so[1][0];
+    }
   }
 }
 
@@ -517,7 +545,7 @@ static int refl_planes(int nxy, int nz, int resident) {
 template <int OP, int SY, bool PAT, typename T>
 static void launch_refl_shape(int mode, const GridDims& g, int planes, const T* tab, const T* xe, const uint8_t* bcm,
                               T bcdiag, const T* u, const T* b, const T* dinv, T omega, T* y, hipStream_t s,
-                              const T* up, T beta) {
+                              const T* up, T beta, double* partial = nullptr, int* nwork_out = nullptr) {
   const int tx = (g.nx + kReflRow - 3) / (kReflRow - 2);
   const int ty = (g.ny + SY - 3) / (SY - 2);
   const int nxy = tx * ty;
@@ -526,10 +554,13 @@ static void launch_refl_shape(int mode, const GridDims& g, int planes, const T* 
     const int nzc = planes > 0 ? (planes < g.nz ? planes : g.nz)                                               \
                                : refl_planes(nxy, g.nz, refl_resident_blocks<OP, SY, M, PAT, T>());             \
     const int nwork = nxy * ((g.nz + nzc - 1) / nzc);                                                          \
+    if (nwork_out) *nwork_out = nwork;                                                                          \
     fine_refl_kernel<OP, SY, M, PAT, T><<<(unsigned)((nwork + 7) & ~7), kReflRow * SY, 0, s>>>(                \
-        g, tx, nxy, nzc, nwork, tab, 0, xe, bcm, bcdiag, u, b, dinv, omega, y, up, beta);                        \
+        g, tx, nxy, nzc, nwork, tab, 0, xe, bcm, bcdiag, u, b, dinv, omega, y, up, beta, partial);               \
   } while (0)
-  if (mode == MODE_APPLY)
+  if (mode == MODE_APPLY_DOTS) {
+    if constexpr (OP == 0 && sizeof(T) == 8) PMT_REFL(MODE_APPLY_DOTS);
+  } else if (mode == MODE_APPLY)
     PMT_REFL(MODE_APPLY);
   else if (mode == MODE_RESIDUAL)
     PMT_REFL(MODE_RESIDUAL);
@@ -559,6 +590,48 @@ int launch_fine_refl(int mode, const GridDims& g, int planes, int rows, int patt
       launch_refl_shape<0, 16, false, T>(mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
   }
   return check_launch("fine_refl_kernel");
+}
+
+long long fine_apply_dots_parts(const GridDims& g) {
+  const long long tx = (g.nx + kReflRow - 3) / (kReflRow - 2), ty = (g.ny + 16 - 3) / (16 - 2);
+  return tx * ty * (long long)g.nz;  // blocks of the 16-row shape with one plane each: an upper bound
+}
+
+// out[o] = sum over i of partial[o * n + i], o = 0, 1, in a fixed order (one block)
+__global__ void __launch_bounds__(256) sum_parts_kernel(const double* __restrict__ partial, int n,
+                                                          double* __restrict__ out) {
+  __shared__ double sh[256];
+  for (int o = 0; o < 2; ++o) {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) acc += partial[(long long)o * n + i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[o] = sh[0];
+    __syncthreads();
+  }
+}
+
+int launch_fine_apply_dots(const GridDims& g, const double* K0ext, int k0_pattern, const double* xe,
+                           const uint8_t* bcm, double bcdiag, const double* p, const double* r, double* q,
+                           double* ws, long long ws_parts, double* out, hipStream_t s) {
+  if (g.dim != 3 || get_tuning(PMT_TUNE_FINE_KERNEL) != 2 || !(k0_pattern & PMT_K0_REFLECTION) ||
+      ws_parts < fine_apply_dots_parts(g))
+    return PMT_ERR_STATE;
+  const double* tab = K0ext + kK0ReflOffset64;
+  int nwork = 0;
+  if (get_tuning(PMT_TUNE_REFL_ROWS) == 32)
+    launch_refl_shape<0, 32, false, double>(MODE_APPLY_DOTS, g, tuned_stream_planes(), tab, xe, bcm, bcdiag, p, r,
+                                            nullptr, 0.0, q, s, nullptr, 0.0, ws, &nwork);
+  else
+    launch_refl_shape<0, 16, false, double>(MODE_APPLY_DOTS, g, tuned_stream_planes(), tab, xe, bcm, bcdiag, p, r,
+                                            nullptr, 0.0, q, s, nullptr, 0.0, ws, &nwork);
+  PMT_TRY(check_launch("fine_refl_kernel<apply + dots>"));
+  sum_parts_kernel<<<1, 256, 0, s>>>(ws, nwork, out);
+  return check_launch("sum_parts_kernel");
 }
 
 // Corrections of the first coarse level's elements with Dirichlet children (those along constrained
