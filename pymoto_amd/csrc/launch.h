@@ -22,15 +22,19 @@ constexpr int kK0ReflOffset32 = 648;  // 72 FP32 block entries (36 doubles of st
 bool k0_reflection_table(const double* K0_host, double* tab);
 // first coarse level of a hex8 hierarchy in the same basis (K0 reflection symmetric), without a stencil:
 // G0ext = coarse1_table (864 doubles), Xh[layer][8][element] the transformed densities of the unflagged
-// children (launch_coarse1_xhat), list (nlist entries) / map / dK the elements with Dirichlet children and
-// their corrections, dku workspace of 24 * nlist doubles
+// children (launch_coarse1_xhat); the elements with Dirichlet children enter through the correction
+// stencils TS[243][cap] of the nt touched nodes tnode (launch_coarse1_touched, then _tstencil)
 void coarse1_table(const double* G0_host, double* ext_host);
 int launch_coarse1_xhat(const GridDims& gf, const GridDims& gc, const double* xe, const uint8_t* cflag, double* X64,
                         float* X32, hipStream_t s);
+int launch_coarse1_touched(const GridDims& g, const int* list, const int* map, int* tnode, int* tcount,
+                           hipStream_t s);
+int launch_coarse1_tstencil(const GridDims& g, const int* tnode, int nt, long long cap, const int* map,
+                            const double* dK, double* TS, hipStream_t s);
 template <typename T>
-int launch_coarse1(int mode, const GridDims& g, int pattern, const double* G0ext, const T* Xh, const int* list,
-                   int nlist, const int* map, const double* dK, double* dku, const T* u, const T* b, const T* dinv,
-                   double omega, T* y, hipStream_t s, const T* up = nullptr, double beta = 0.0);
+int launch_coarse1(int mode, const GridDims& g, int pattern, const double* G0ext, const T* Xh, const int* tnode, int nt,
+                   long long cap, const double* TS, const T* u, const T* b, const T* dinv, double omega, T* y,
+                   hipStream_t s, const T* up = nullptr, double beta = 0.0);
 template <typename T>
 int launch_fine_refl(int mode, const GridDims& g, int planes /* 0 = from the occupancy */, int rows,
                      int pattern /* K0 has the box class pattern: skip its structural zeros */, const double* K0ext,

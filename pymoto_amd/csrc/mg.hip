@@ -107,8 +107,8 @@ static int level_apply(pmt_mg* mg, int l, int mode, const T* u, const T* b, T* y
     return arg_error("level_apply: the coarsest level is solved directly");
   }
   if (l == 1 && mg->l1free)
-    return launch_coarse1<T>(mode, L.g, mg->k0pat & PMT_K0_CLASS_PATTERN, mg->G0ext, B.Xh, L.list, mg->host_counts[1],
-                             L.map, L.dK, L.dku, u, b, B.dinv, omega, y, s, up, beta);
+    return launch_coarse1<T>(mode, L.g, mg->k0pat & PMT_K0_CLASS_PATTERN, mg->G0ext, B.Xh, L.tnode, mg->host_counts[0],
+                             L.t_cap, L.TS, u, b, B.dinv, omega, y, s, up, beta);
   return launch_stencil_half<T>(mode, L.g, B.H, u, b, B.dinv, omega, y, s, up, beta);
 }
 
@@ -333,6 +333,11 @@ static int hex_coarse_stencils(pmt_mg* mg, hipStream_t s) {
   PMT_TRY(launch_level1_flags(mg->L[0].g, mg->bcm, mg->L[1].cflag, mg->L[1].list, mg->L[1].map, s));
   for (int l = 2; l < nl; ++l)
     PMT_TRY(launch_flags_up(mg->L[l - 1].g, mg->L[l - 1].cflag, mg->L[l].cflag, mg->L[l].list, mg->L[l].map, s));
+  Level& L1 = mg->L[1];
+  if (mg->l1free) {
+    PMT_TRY(launch_coarse1_touched(L1.g, L1.list, L1.map, L1.tnode, L1.tnode + L1.g.nnodes, s));
+    PMT_HIP(hipMemcpyAsync(mg->host_counts, L1.tnode + L1.g.nnodes, sizeof(int), hipMemcpyDeviceToHost, s));
+  }
   for (int l = 1; l < nl; ++l)
     PMT_HIP(hipMemcpyAsync(mg->host_counts + l, mg->L[l].list + mg->L[l].g.nel, sizeof(int), hipMemcpyDeviceToHost,
                            s));
@@ -343,16 +348,22 @@ static int hex_coarse_stencils(pmt_mg* mg, hipStream_t s) {
     if (need > L.dk_cap) {
       PMT_TRY(mg_free(mg, L.dK, L.dk_cap * 576 * (long long)sizeof(double)));
       L.dK = nullptr;
-      if (L.dku) PMT_TRY(mg_free(mg, L.dku, L.dk_cap * 24 * (long long)sizeof(double)));
-      L.dku = nullptr;
       L.dk_cap = 0;
       PMT_TRY(mg_alloc(mg, (void**)&L.dK, need * 576 * (long long)sizeof(double)));
-      if (l == 1 && mg->l1free) PMT_TRY(mg_alloc(mg, (void**)&L.dku, need * 24 * (long long)sizeof(double)));
       L.dk_cap = need;
     }
   }
+  if (mg->l1free && mg->host_counts[0] > L1.t_cap) {
+    if (L1.TS) PMT_TRY(mg_free(mg, L1.TS, L1.t_cap * 243 * (long long)sizeof(double)));
+    L1.TS = nullptr;
+    L1.t_cap = 0;
+    PMT_TRY(mg_alloc(mg, (void**)&L1.TS, (long long)mg->host_counts[0] * 243 * (long long)sizeof(double)));
+    L1.t_cap = mg->host_counts[0];
+  }
   PMT_TRY(launch_level1_dk(mg->L[0].g, mg->K0, mg->xe, mg->bcm, mg->bcdiag, mg->L[1].cflag, mg->L[1].list,
                            mg->L[1].dK, s));
+  if (mg->l1free)
+    PMT_TRY(launch_coarse1_tstencil(L1.g, L1.tnode, mg->host_counts[0], L1.t_cap, L1.map, L1.dK, L1.TS, s));
   for (int l = 2; l < nl; ++l)
     PMT_TRY(launch_dk_up(mg->L[l - 1].g, mg->L[l].cflag, mg->L[l].list, mg->L[l - 1].map, mg->L[l - 1].dK,
                          mg->L[l].dK, s));
@@ -454,7 +465,12 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
     const long long face = std::max({(long long)L.g.ex * L.g.ey, (long long)L.g.ey * L.g.ez, (long long)L.g.ex * L.g.ez});
     L.dk_cap = std::max(1LL, std::min(L.g.nel, face));
     ALLOC(L.dK, L.dk_cap * NE * NE);
-    if (l == 1 && mg->l1free) ALLOC(L.dku, L.dk_cap * NE);
+    if (l == 1 && mg->l1free) {
+      if (rc == PMT_OK) rc = dalloc(mg, &L.tnode, L.g.nnodes + 1);
+      // correction stencils for the nodes along the largest face; grown at update if needed
+      L.t_cap = std::max({(long long)L.g.nx * L.g.ny, (long long)L.g.ny * L.g.nz, (long long)L.g.nx * L.g.nz});
+      ALLOC(L.TS, L.t_cap * 243);
+    }
   }
   if (rc == PMT_OK)
     rc = hip_status(hipHostMalloc((void**)&mg->host_counts, kMaxLevels * sizeof(int), hipHostMallocDefault),

@@ -36,7 +36,11 @@ struct Level {
   int* list = nullptr;
   int* map = nullptr;
   double* dK = nullptr;
-  double* dku = nullptr;  // level 1 without a stencil: dK_E u_E per listed element (dk_cap x 24)
+  // level 1 without a stencil: the nodes of elements with Dirichlet children (nnodes + 1 ints, count
+  // last) and their correction stencils TS[243][t_cap]
+  int* tnode = nullptr;
+  double* TS = nullptr;
+  long long t_cap = 0;
   long long dk_cap = 0;
   double* M = nullptr;
   double* C = nullptr;
@@ -70,7 +74,8 @@ struct pmt_mg {
   double* G = nullptr;  // P_c^T K0 P_c per child position (level-1 stencil of hex8 grids)
   bool l1free = false;     // hex8 level 1 applied element by element in the reflection basis (no stencil)
   double* G0ext = nullptr;  // its 24 x 24 table (coarse1_table), FP64 then FP32
-  int* host_counts = nullptr;  // pinned: flagged elements per level (read once per update)
+  int* host_counts = nullptr;  // pinned: [0] touched nodes of level 1 (without a stencil), [l] flagged elements
+                               // of level l (read once per update)
   int k0pat = 0;
   const double* xe = nullptr;
   float* xe32 = nullptr;

@@ -635,83 +635,113 @@ int launch_fine_apply_dots(const GridDims& g, const double* K0ext, int k0_patter
 }
 
 // Corrections of the first coarse level's elements with Dirichlet children (those along constrained
-// faces), in two passes without atomics:
-//  1. coarse1_dku_kernel, thread = (listed element w, row r): t[w][r] = (dK_w u_E)[r];
-//  2. coarse1_correct_kernel, thread = (listed element, corner): it owns the corner's node if no flagged
-//     element holding the node has a smaller index, and adds s(n) times the node's rows of t of every
-//     flagged element holding it, in index order (the result does not depend on the order of the
-//     list), with s = 1 (apply), -1 (residual) or -omega D^-1(n) (Jacobi and Chebyshev sweeps, which
-//     are affine in A u with that factor).
-template <typename T>
+// faces).  At update, the nodes of the flagged elements ("touched" nodes) get a 27-block correction
+// stencil T(n, n+o) = sum over the flagged elements E holding n and n+o of dK_E(loc n, loc n+o); every
+// application then adds y(n, d) += s(n, d) * (T u)(n, d) in one launch, with s = 1 (apply), -1
+// (residual) or -omega D^-1(n, d) (Jacobi and Chebyshev sweeps, which are affine in A u with that
+// factor).  The sums run over the elements in index order: no atomics in the values, results
+// independent of the order of the lists.
+//  1. coarse1_touched_kernel: thread = (listed element, corner); it lists the corner's node if no
+//     flagged element holding the node has a smaller index (the list's order is irrelevant);
+//  2. coarse1_tstencil_kernel: thread = (touched node, coefficient), T in [243][cap] layout;
+//  3. coarse1_tapply_kernel: thread = (touched node, component), 81 multiply-adds.
 __global__ void __launch_bounds__(256)
-coarse1_dku_kernel(GridDims g, const int* __restrict__ list, int nlist, const double* __restrict__ dK,
-                   const T* __restrict__ u, double* __restrict__ t) {
-  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= 24LL * nlist) return;
-  const long long w = q / 24;
-  const int r = (int)(q % 24);
-  const long long E = list[w];
-  const int CI = (int)(E % g.ex);
-  const long long tt = E / g.ex;
-  const int CJ = (int)(tt % g.ey), CK = (int)(tt / g.ey);
-  const double* K = dK + w * 576 + r * 24;
-  double acc = 0.0;
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const long long nb = ((long long)(CK + (c >> 2)) * g.ny + CJ + ((c >> 1) & 1)) * g.nx + CI + (c & 1);
-#pragma unroll
-    for (int d2 = 0; d2 < 3; ++d2) acc = fma(K[c * 3 + d2], (double)u[nb * 3 + d2], acc);
+coarse1_touched_kernel(GridDims g, const int* __restrict__ list, const int* __restrict__ count,
+                       const int* __restrict__ map, int* __restrict__ tnode, int* __restrict__ tcount) {
+  const long long nq = 8LL * (*count);
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += (long long)gridDim.x * blockDim.x) {
+    const long long E = list[q >> 3];
+    const int a = (int)(q & 7);
+    const int CI = (int)(E % g.ex);
+    const long long tt = E / g.ex;
+    const int CJ = (int)(tt % g.ey), CK = (int)(tt / g.ey);
+    const int I = CI + (a & 1), J = CJ + ((a >> 1) & 1), K = CK + (a >> 2);
+    long long owner = -1;
+    for (int e = 0; e < 8 && owner < 0; ++e) {  // e ascending = element index ascending
+      const int ei = I - 1 + (e & 1), ej = J - 1 + ((e >> 1) & 1), ek = K - 1 + (e >> 2);
+      if (ei < 0 || ei >= g.ex || ej < 0 || ej >= g.ey || ek < 0 || ek >= g.ez) continue;
+      const long long Ee = ((long long)ek * g.ey + ej) * g.ex + ei;
+      if (map[Ee] >= 0) owner = Ee;
+    }
+    if (owner == E) tnode[atomicAdd(tcount, 1)] = (int)(((long long)K * g.ny + J) * g.nx + I);
   }
-  t[q] = acc;
+}
+
+__global__ void __launch_bounds__(256)
+coarse1_tstencil_kernel(GridDims g, const int* __restrict__ tnode, int nt, long long cap, const int* __restrict__ map,
+                        const double* __restrict__ dK, double* __restrict__ T) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 243LL * nt) return;
+  const long long t = q / 243;
+  const int c = (int)(q % 243), o = c / 9, d = (c / 3) % 3, d2 = c % 3;
+  const long long n = tnode[t];
+  const int I = (int)(n % g.nx);
+  const long long tt = n / g.nx;
+  const int J = (int)(tt % g.ny), K = (int)(tt / g.ny);
+  const int ox = o % 3 - 1, oy = (o / 3) % 3 - 1, oz = o / 9 - 1;
+  double v = 0.0;
+  for (int e = 0; e < 8; ++e) {  // elements holding n, index order
+    const int ex = e & 1, ey = (e >> 1) & 1, ez = e >> 2;
+    const int ei = I - 1 + ex, ej = J - 1 + ey, ek = K - 1 + ez;
+    if (ei < 0 || ei >= g.ex || ej < 0 || ej >= g.ey || ek < 0 || ek >= g.ez) continue;
+    const int bx = 1 - ex + ox, by = 1 - ey + oy, bz = 1 - ez + oz;  // n + o's corner in the element
+    if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
+    const int m = map[((long long)ek * g.ey + ej) * g.ex + ei];
+    if (m < 0) continue;
+    const int A = (1 - ex) + 2 * (1 - ey) + 4 * (1 - ez), B = bx + 2 * by + 4 * bz;
+    v += dK[(long long)m * 576 + (A * 3 + d) * 24 + B * 3 + d2];
+  }
+  T[(long long)c * cap + t] = v;
 }
 
 template <int MODE, typename T>
 __global__ void __launch_bounds__(256)
-coarse1_correct_kernel(GridDims g, const int* __restrict__ list, int nlist, const int* __restrict__ map,
-                       const double* __restrict__ t, const T* __restrict__ dinv, double omega, T* __restrict__ y) {
+coarse1_tapply_kernel(GridDims g, const int* __restrict__ tnode, int nt, long long cap, const double* __restrict__ TS,
+                      const T* __restrict__ u, const T* __restrict__ dinv, double omega, T* __restrict__ y) {
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= 8LL * nlist) return;
-  const long long E = list[q >> 3];
-  const int a = (int)(q & 7);
-  const int CI = (int)(E % g.ex);
-  const long long tt = E / g.ex;
-  const int CJ = (int)(tt % g.ey), CK = (int)(tt / g.ey);
-  const int I = CI + (a & 1), J = CJ + ((a >> 1) & 1), K = CK + (a >> 2);
-  int me[8];
-  long long owner = -1;
+  if (q >= 3LL * nt) return;
+  const long long t = q / 3;
+  const int d = (int)(q % 3);
+  const long long n = tnode[t];
+  const int I = (int)(n % g.nx);
+  const long long tt = n / g.nx;
+  const int J = (int)(tt % g.ny), K = (int)(tt / g.ny);
+  double acc = 0.0;
 #pragma unroll
-  for (int e = 0; e < 8; ++e) {  // e ascending = element index ascending
-    const int ei = I - 1 + (e & 1), ej = J - 1 + ((e >> 1) & 1), ek = K - 1 + (e >> 2);
-    const bool ok = ei >= 0 && ei < g.ex && ej >= 0 && ej < g.ey && ek >= 0 && ek < g.ez;
-    const long long Ee = ((long long)ek * g.ey + ej) * g.ex + ei;
-    me[e] = ok ? map[Ee] : -1;
-    if (owner < 0 && me[e] >= 0) owner = Ee;
+  for (int o = 0; o < 27; ++o) {
+    // neighbours outside the grid have zero coefficients: read a clamped (valid) node instead
+    const int ni = min(max(I + o % 3 - 1, 0), g.nx - 1), nj = min(max(J + (o / 3) % 3 - 1, 0), g.ny - 1),
+              nk = min(max(K + o / 9 - 1, 0), g.nz - 1);
+    const long long m = ((long long)nk * g.ny + nj) * g.nx + ni;
+#pragma unroll
+    for (int d2 = 0; d2 < 3; ++d2) acc = fma(TS[(long long)((o * 3 + d) * 3 + d2) * cap + t], (double)u[m * 3 + d2], acc);
   }
-  if (owner != E) return;
-  double acc[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-  for (int e = 0; e < 8; ++e) {
-    const int A = (1 - (e & 1)) + 2 * (1 - ((e >> 1) & 1)) + 4 * (1 - (e >> 2));  // the node's corner in E_e
-    const double* te = t + (long long)(me[e] < 0 ? 0 : me[e]) * 24 + A * 3;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) acc[d] += me[e] < 0 ? 0.0 : te[d];
-  }
-  const long long n = ((long long)K * g.ny + J) * g.nx + I;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    double f = acc[d];
-    if (MODE == MODE_RESIDUAL)
-      f = -f;
-    else if (MODE == MODE_JACOBI || MODE == MODE_CHEB)
-      f = -omega * (double)dinv[n * 3 + d] * f;
-    y[n * 3 + d] = (T)((double)y[n * 3 + d] + f);
-  }
+  double f = acc;
+  if (MODE == MODE_RESIDUAL)
+    f = -f;
+  else if (MODE == MODE_JACOBI || MODE == MODE_CHEB)
+    f = -omega * (double)dinv[n * 3 + d] * f;
+  y[n * 3 + d] = (T)((double)y[n * 3 + d] + f);
+}
+
+int launch_coarse1_touched(const GridDims& g, const int* list, const int* map, int* tnode, int* tcount,
+                           hipStream_t s) {
+  zero_int(tcount, s);
+  coarse1_touched_kernel<<<1024, 256, 0, s>>>(g, list, list + g.nel, map, tnode, tcount);
+  return check_launch("coarse1_touched_kernel");
+}
+
+int launch_coarse1_tstencil(const GridDims& g, const int* tnode, int nt, long long cap, const int* map,
+                            const double* dK, double* TS, hipStream_t s) {
+  if (nt <= 0) return PMT_OK;
+  coarse1_tstencil_kernel<<<(unsigned)blocks_for(243LL * nt, 256), 256, 0, s>>>(g, tnode, nt, cap, map, dK, TS);
+  return check_launch("coarse1_tstencil_kernel");
 }
 
 template <typename T>
-int launch_coarse1(int mode, const GridDims& g, int pattern, const double* G0ext, const T* Xh, const int* list,
-                   int nlist, const int* map, const double* dK, double* dku, const T* u, const T* b, const T* dinv,
-                   double omega, T* y, hipStream_t s, const T* up, double beta) {
+int launch_coarse1(int mode, const GridDims& g, int pattern, const double* G0ext, const T* Xh, const int* tnode, int nt,
+                   long long cap, const double* TS, const T* u, const T* b, const T* dinv, double omega, T* y,
+                   hipStream_t s, const T* up, double beta) {
   const T* tab = sizeof(T) == 8 ? (const T*)G0ext : (const T*)(G0ext + 576);
   const T om = (T)omega, be = (T)beta;
   if (pattern)
@@ -719,17 +749,15 @@ int launch_coarse1(int mode, const GridDims& g, int pattern, const double* G0ext
   else
     launch_refl_shape<1, 16, false, T>(mode, g, 0, tab, Xh, nullptr, (T)0, u, b, dinv, om, y, s, up, be);
   PMT_TRY(check_launch("fine_refl_kernel<coarse level 1>"));
-  if (nlist <= 0) return PMT_OK;
-  coarse1_dku_kernel<T><<<(unsigned)blocks_for(24LL * nlist, 256), 256, 0, s>>>(g, list, nlist, dK, u, dku);
-  PMT_TRY(check_launch("coarse1_dku_kernel"));
-  const unsigned blocks = (unsigned)blocks_for(8LL * nlist, 256);
+  if (nt <= 0) return PMT_OK;
+  const unsigned blocks = (unsigned)blocks_for(3LL * nt, 256);
   if (mode == MODE_APPLY)
-    coarse1_correct_kernel<MODE_APPLY, T><<<blocks, 256, 0, s>>>(g, list, nlist, map, dku, dinv, omega, y);
+    coarse1_tapply_kernel<MODE_APPLY, T><<<blocks, 256, 0, s>>>(g, tnode, nt, cap, TS, u, dinv, omega, y);
   else if (mode == MODE_RESIDUAL)
-    coarse1_correct_kernel<MODE_RESIDUAL, T><<<blocks, 256, 0, s>>>(g, list, nlist, map, dku, dinv, omega, y);
+    coarse1_tapply_kernel<MODE_RESIDUAL, T><<<blocks, 256, 0, s>>>(g, tnode, nt, cap, TS, u, dinv, omega, y);
   else
-    coarse1_correct_kernel<MODE_JACOBI, T><<<blocks, 256, 0, s>>>(g, list, nlist, map, dku, dinv, omega, y);
-  return check_launch("coarse1_correct_kernel");
+    coarse1_tapply_kernel<MODE_JACOBI, T><<<blocks, 256, 0, s>>>(g, tnode, nt, cap, TS, u, dinv, omega, y);
+  return check_launch("coarse1_tapply_kernel");
 }
 
 // X^[m] = sum_c (-1)^(m.c) x_c over the unflagged children c of every first-coarse-level element (the
@@ -779,11 +807,11 @@ template int launch_fine_refl<float>(int, const GridDims&, int, int, int, const 
                                      double, const float*, const float*, const float*, double, float*, hipStream_t,
                                      const float*, double);
 template int launch_coarse1<double>(int, const GridDims&, int, const double*, const double*, const int*, int,
-                                    const int*, const double*, double*, const double*, const double*, const double*,
-                                    double, double*, hipStream_t, const double*, double);
-template int launch_coarse1<float>(int, const GridDims&, int, const double*, const float*, const int*, int, const int*,
-                                   const double*, double*, const float*, const float*, const float*, double, float*,
-                                   hipStream_t, const float*, double);
+                                    long long, const double*, const double*, const double*, const double*, double,
+                                    double*, hipStream_t, const double*, double);
+template int launch_coarse1<float>(int, const GridDims&, int, const double*, const float*, const int*, int, long long,
+                                   const double*, const float*, const float*, const float*, double, float*, hipStream_t,
+                                   const float*, double);
 
 // G0tab: (1/64) H G_0 H of the child-0 matrix G_0 = P_0^T K0 P_0 (G0: host, 24 x 24), FP64 then FP32
 void coarse1_table(const double* G0, double* ext) {
