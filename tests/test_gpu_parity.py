@@ -518,18 +518,18 @@ def _block_problem(k, seed=0, nel=(16, 8, 8)):
 @pytest.mark.parametrize("graph", [True, False])
 def test_block_pcg_vs_oracle(gpu_lib, graph):
     nel, d, K, Kref, B = _block_problem(4)
-    ocg = OCG(preconditioner=build_mg(nel, 3, omega=0.6, smooth_steps=2), tol=1e-10)
+    # omega 0.5: with the reference example's 0.6 (above the Jacobi stability bound of these operators)
+    # block CG's iteration count is chaotic in the rounding in the reference's own algorithm -
+    # perturbing the oracle's K by 1e-15 (relative, symmetric) gave 53 to 121 iterations - while with
+    # 0.5 the oracle takes 23 iterations under every such perturbation
+    ocg = OCG(preconditioner=build_mg(nel, 3, omega=0.5, smooth_steps=2), tol=1e-10)
     ocg.update(Kref)
     Xref = ocg.solve(B)
-    cg = pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(d, 3, omega=0.6, smooth_steps=2), tol=1e-10,
+    cg = pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(d, 3, omega=0.5, smooth_steps=2), tol=1e-10,
                        graph=graph)
     cg.update(K)
     X = host(cg.solve(dev(B)))
-    # Block CG's iteration count on this problem is chaotic in the rounding, in the reference's own
-    # algorithm: perturbing the oracle's K by 1e-15 (relative, symmetric) gives 53, 121, 54, 53, 57,
-    # 56, 55, 55 iterations for 8 seeds.  So the count is only bounded; the solution and residual are
-    # compared tightly.
-    assert cg.iterations <= 3 * ocg.iterations
+    assert abs(cg.iterations - ocg.iterations) <= 1
     assert relerr(X, Xref) < 1e-8
     R = B - Kref @ X
     assert (np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)).max() <= 1e-10
