@@ -120,8 +120,8 @@ void moment_tables(int level, const double* K0_host, double* C_host);  // 64 * 2
 int launch_moments1(const GridDims& gf, const double* xe, const uint8_t* cflag, double* M, hipStream_t s);
 int launch_moments_up(const GridDims& gf, const double* Mf, double* M, hipStream_t s);
 int launch_flags_up(const GridDims& gf, const uint8_t* cflag_f, uint8_t* cflag, int* list, int* map, hipStream_t s);
-int launch_dk_up(const GridDims& gf, const uint8_t* cflag, const int* list, const int* map_f, const double* dKf,
-                 double* dK, hipStream_t s);
+int launch_dk_up(const GridDims& gf, const uint8_t* cflag, const int* list, int nlist, const int* map_f,
+                 const double* dKf, double* dK, hipStream_t s);
 // element matrices (Ke: 576 x nel, structure of arrays) from the moments plus the corrections, then S
 int launch_stencil_from_moments(const GridDims& g, const double* C, const double* M, const int* list,
                                 const double* dK, double* Ke, const StencilOut& out, hipStream_t s);

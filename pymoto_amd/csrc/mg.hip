@@ -365,8 +365,8 @@ static int hex_coarse_stencils(pmt_mg* mg, hipStream_t s) {
   if (mg->l1free)
     PMT_TRY(launch_coarse1_tstencil(L1.g, L1.tnode, mg->host_counts[0], L1.t_cap, L1.map, L1.dK, L1.TS, s));
   for (int l = 2; l < nl; ++l)
-    PMT_TRY(launch_dk_up(mg->L[l - 1].g, mg->L[l].cflag, mg->L[l].list, mg->L[l - 1].map, mg->L[l - 1].dK,
-                         mg->L[l].dK, s));
+    PMT_TRY(launch_dk_up(mg->L[l - 1].g, mg->L[l].cflag, mg->L[l].list, mg->host_counts[l], mg->L[l - 1].map,
+                         mg->L[l - 1].dK, mg->L[l].dK, s));
   PMT_TRY(launch_moments1(mg->L[0].g, mg->xe, mg->L[1].cflag, mg->L[1].M, s));
   for (int l = 2; l < nl; ++l) PMT_TRY(launch_moments_up(mg->L[l - 1].g, mg->L[l - 1].M, mg->L[l].M, s));
   // smoothing levels get the half stencil and D^-1 (FP64, and FP32 for reduced-precision cycles); the

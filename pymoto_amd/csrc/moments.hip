@@ -133,61 +133,71 @@ __global__ void flags_up_kernel(GridDims gf, GridDims gc, const uint8_t* __restr
   map[E] = idx;
 }
 
-// dK of a listed level-l element: sum over flagged children c of P_c^T dK_c P_c (lane = (A, B))
-__global__ void __launch_bounds__(256)
+// dK of a listed level-l element: sum over flagged children c of P_c^T dK_c P_c.  One 64-lane block per
+// listed element, lane = (A, B) 3x3 block; per child in two passes through LDS: T[a][B] = sum_b
+// w(c, b, B) dK_c[a][b] (lane a = A), then dK[A][B] += sum_a w(c, a, A) T[a][B] - 144 multiply-adds per
+// lane and child, no divergent loops.
+__global__ void __launch_bounds__(64)
 dk_up_kernel(GridDims gf, GridDims gc, const uint8_t* __restrict__ cflag, const int* __restrict__ list,
-             const int* __restrict__ count, const int* __restrict__ map_f, const double* __restrict__ dKf,
-             double* __restrict__ dK) {
-  const int n = *count;
-  if (blockIdx.x * (blockDim.x >> 6) >= n) return;  // whole block leaves together
-  const int lane = threadIdx.x & 63;
-  const int A = lane >> 3, B = lane & 7;
-  const int nwaves = gridDim.x * (blockDim.x >> 6);
-  for (int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < n; w += nwaves) {
-    const long long E = list[w];
-    const unsigned bits = cflag[E];
-    double acc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    for (int c = 0; c < 8; ++c) {
-      if (!((bits >> c) & 1u)) continue;
-      const double* K = dKf + (long long)map_f[child_index(gf, E, gc, c)] * 576;
-#pragma unroll 1
-      for (int a = 0; a < 8; ++a) {
-        const double wa = pweight3(c, a, A);
-        if (wa == 0.0) continue;
-#pragma unroll 1
-        for (int b = 0; b < 8; ++b) {
-          const double wb = pweight3(c, b, B);
-          if (wb == 0.0) continue;
+             const int* __restrict__ map_f, const double* __restrict__ dKf, double* __restrict__ dK) {
+  __shared__ double sT[8][8][9];  // [a][B][d * 3 + d2]
+  const long long w = blockIdx.x;
+  const int lane = threadIdx.x, A = lane >> 3, B = lane & 7;
+  const long long E = list[w];
+  const unsigned bits = cflag[E];
+  double acc[9];
 #pragma unroll
-          for (int d = 0; d < 3; ++d)
+  for (int q = 0; q < 9; ++q) acc[q] = 0.0;
+  for (int c = 0; c < 8; ++c) {
+    if (!((bits >> c) & 1u)) continue;  // uniform over the block
+    const double* K = dKf + (long long)map_f[child_index(gf, E, gc, c)] * 576;
+    double t[9];
 #pragma unroll
-            for (int d2 = 0; d2 < 3; ++d2) acc[d][d2] = fma(wa * wb, K[(a * 3 + d) * 24 + b * 3 + d2], acc[d][d2]);
-        }
-      }
+    for (int q = 0; q < 9; ++q) t[q] = 0.0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const double wb = pweight3(c, b, B);
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int d2 = 0; d2 < 3; ++d2) t[d * 3 + d2] = fma(wb, K[(A * 3 + d) * 24 + b * 3 + d2], t[d * 3 + d2]);
     }
+    __syncthreads();  // the previous child's reads of sT are done
 #pragma unroll
-    for (int d = 0; d < 3; ++d)
+    for (int q = 0; q < 9; ++q) sT[A][B][q] = t[q];
+    __syncthreads();
 #pragma unroll
-      for (int d2 = 0; d2 < 3; ++d2) dK[(long long)w * 576 + (A * 3 + d) * 24 + B * 3 + d2] = acc[d][d2];
+    for (int a = 0; a < 8; ++a) {
+      const double wa = pweight3(c, a, A);
+#pragma unroll
+      for (int q = 0; q < 9; ++q) acc[q] = fma(wa, sT[a][B][q], acc[q]);
+    }
   }
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int d2 = 0; d2 < 3; ++d2) dK[w * 576 + (A * 3 + d) * 24 + B * 3 + d2] = acc[d * 3 + d2];
 }
 
 // Element matrices of level l >= 2 in structure-of-arrays layout Ke[q * nel + E] (q = row * 24 + col):
-// K_E = sum_m M_m(E) C_m.  Thread = (element, node row block A): coalesced moment loads and stores; the
-// 27 x 9 entries of each (A, B) block of C are uniform across a wave (scalar loads).
+// K_E = sum_m M_m(E) C_m.  Block = 256 elements of one node row block A (blockIdx.y): the block's 8 x 27
+// x 9 entries of C are staged in LDS and read as broadcasts (through the scalar cache every wave
+// re-read all of them: ~1 GB of scalar loads at 256x128x128, 6% of the FP64 rate); coalesced moment
+// loads and stores.
 __global__ void __launch_bounds__(256)
 elem_from_moments_kernel(GridDims g, const double* __restrict__ C, const double* __restrict__ M,
                          double* __restrict__ Ke) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= g.nel * 8) return;
-  const long long E = t % g.nel;
-  const int A = (int)(t / g.nel);  // uniform across a wave unless the wave straddles two row blocks
+  __shared__ double sC[8 * kNM * 9];
+  const int A = blockIdx.y;
+  for (int q = threadIdx.x; q < 8 * kNM * 9; q += blockDim.x) sC[q] = C[A * 8 * kNM * 9 + q];
+  __syncthreads();
+  const long long E = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (E >= g.nel) return;
   double m[kNM];
 #pragma unroll
   for (int q = 0; q < kNM; ++q) m[q] = M[q * g.nel + E];
   for (int B = 0; B < 8; ++B) {
-    const int AB = A * 8 + B;
-    const double* Cb = C + AB * kNM * 9;
+    const double* Cb = sC + B * kNM * 9;
     double blk[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) blk[q] = 0.0;
@@ -275,17 +285,17 @@ int launch_flags_up(const GridDims& gf, const uint8_t* cflag_f, uint8_t* cflag, 
   return check_launch("flags_up_kernel");
 }
 
-int launch_dk_up(const GridDims& gf, const uint8_t* cflag, const int* list, const int* map_f, const double* dKf,
-                 double* dK, hipStream_t s) {
+int launch_dk_up(const GridDims& gf, const uint8_t* cflag, const int* list, int nlist, const int* map_f,
+                 const double* dKf, double* dK, hipStream_t s) {
+  if (nlist <= 0) return PMT_OK;
   GridDims gc = make_grid(gf.ex / 2, gf.ey / 2, gf.ez / 2);
-  const long long blocks = (gc.nel + 3) / 4 < 8192 ? (gc.nel + 3) / 4 : 8192;
-  dk_up_kernel<<<(unsigned)blocks, 256, 0, s>>>(gf, gc, cflag, list, list + gc.nel, map_f, dKf, dK);
+  dk_up_kernel<<<(unsigned)nlist, 64, 0, s>>>(gf, gc, cflag, list, map_f, dKf, dK);
   return check_launch("dk_up_kernel");
 }
 
 int launch_stencil_from_moments(const GridDims& g, const double* C, const double* M, const int* list,
                                 const double* dK, double* Ke, const StencilOut& out, hipStream_t s) {
-  elem_from_moments_kernel<<<blocks_for(g.nel * 8, 256), 256, 0, s>>>(g, C, M, Ke);
+  elem_from_moments_kernel<<<dim3((unsigned)blocks_for(g.nel, 256), 8), 256, 0, s>>>(g, C, M, Ke);
   PMT_TRY(check_launch("elem_from_moments_kernel"));
   dk_add_kernel<<<blocks_for(g.nel * 576, 256), 256, 0, s>>>(g, list, list + g.nel, dK, Ke);
   PMT_TRY(check_launch("dk_add_kernel"));

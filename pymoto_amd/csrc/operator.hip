@@ -643,48 +643,6 @@ __global__ void restrict_kernel(GridDims gf, GridDims gc, const T* __restrict__ 
   for (int d = 0; d < DIM; ++d) rc[N * DIM + d] = acc[d];
 }
 
-// Hex8 rc = R^T rf by axes through LDS: a block owns 32 x 8 coarse nodes of one coarse plane K; it
-// sums the fine planes 2K-1, 2K, 2K+1 (weights 1/2, 1, 1/2) of its 65 x 17 fine rows while loading them
-// (coalesced rows), then the fine rows 2J-1..2J+1, then the columns.  Fine nodes outside the grid count 0.
-constexpr int kRestrictCX = 32, kRestrictCY = 8;
-template <typename T>
-__global__ void __launch_bounds__(256) restrict3_kernel(GridDims gf, GridDims gc, const T* __restrict__ rf,
-                                                        T* __restrict__ rc) {
-  constexpr int FX = 2 * kRestrictCX + 1, FY = 2 * kRestrictCY + 1, W = FX * 3;
-  __shared__ T sz[FY][W + 1];
-  __shared__ T sy[kRestrictCY][W + 1];
-  const int I0 = blockIdx.x * kRestrictCX, J0 = blockIdx.y * kRestrictCY, K = blockIdx.z;
-  const int fi0 = 2 * I0 - 1, fj0 = 2 * J0 - 1;
-  const int tid = threadIdx.x;
-  for (int q = tid; q < FY * W; q += 256) {
-    const int fy = q / W, rem = q - fy * W, fx = rem / 3, d = rem - fx * 3;
-    const int fi = fi0 + fx, fj = fj0 + fy;
-    T v = (T)0;
-    if (fi >= 0 && fi < gf.nx && fj >= 0 && fj < gf.ny) {
-#pragma unroll
-      for (int dz = -1; dz <= 1; ++dz) {
-        const int fk = 2 * K + dz;
-        if (fk >= 0 && fk < gf.nz) v += (T)(dz ? 0.5 : 1.0) * rf[(((long long)fk * gf.ny + fj) * gf.nx + fi) * 3 + d];
-      }
-    }
-    sz[fy][rem] = v;
-  }
-  __syncthreads();
-  for (int q = tid; q < kRestrictCY * W; q += 256) {
-    const int cy = q / W, rem = q - cy * W;
-    sy[cy][rem] = (T)0.5 * sz[2 * cy][rem] + sz[2 * cy + 1][rem] + (T)0.5 * sz[2 * cy + 2][rem];
-  }
-  __syncthreads();
-  const int cx = tid % kRestrictCX, cy = tid / kRestrictCX;
-  const int I = I0 + cx, J = J0 + cy;
-  if (I < gc.nx && J < gc.ny) {
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-      rc[(((long long)K * gc.ny + J) * gc.nx + I) * 3 + d] =
-          (T)0.5 * sy[cy][6 * cx + d] + sy[cy][6 * cx + 3 + d] + (T)0.5 * sy[cy][6 * cx + 6 + d];
-  }
-}
-
 // uf += R uc : fine node interpolates from the 1, 2, 4 or 8 coarse nodes around it.  Branch free: every
 // lane reads 2^D corners, the second one along an axis of even fine index repeating the first with
 // weight 0 (the weights are powers of two, so the sum equals w * (sum of the nonzero corners) exactly).
@@ -1034,13 +992,11 @@ int launch_jacobi0(long long n, const T* dinv, const T* b, double omega, T* u, h
 
 template <typename T>
 int launch_restrict(const GridDims& gf, const GridDims& gc, const T* rf, T* rc, hipStream_t s) {
-  if (gf.dim == 3) {
-    dim3 grid((unsigned)((gc.nx + kRestrictCX - 1) / kRestrictCX), (unsigned)((gc.ny + kRestrictCY - 1) / kRestrictCY),
-              (unsigned)gc.nz);
-    restrict3_kernel<T><<<grid, 256, 0, s>>>(gf, gc, rf, rc);
-  } else {
-    restrict_kernel<2, T><<<blocks_for(gc.nnodes, 256), 256, 0, s>>>(gf, gc, rf, rc);
-  }
+  dim3 grid(blocks_for(gc.nnodes, 256));
+  if (gf.dim == 3)
+    restrict_kernel<3, T><<<grid, 256, 0, s>>>(gf, gc, rf, rc);
+  else
+    restrict_kernel<2, T><<<grid, 256, 0, s>>>(gf, gc, rf, rc);
   return check_launch("restrict_kernel");
 }
 
