@@ -106,7 +106,7 @@ void zero_int(int* p, hipStream_t s);
 int launch_level1_flags(const GridDims& gf, const uint8_t* bcm, uint8_t* cflag, int* list, int* map, hipStream_t s);
 // corrections dK (576 per listed element) of the flagged level-1 elements
 int launch_level1_dk(const GridDims& gf, const double* K0, const double* xe, const uint8_t* bcm, double bcdiag,
-                     const uint8_t* cflag, const int* list, double* dK, hipStream_t s);
+                     const uint8_t* cflag, const int* list, int nlist, double* dK, hipStream_t s);
 int launch_galerkin_coarse(const GridDims& gf, const double* Kf, double* Kc, hipStream_t s);
 int launch_stencil_from_elements(const GridDims& g, const double* Ke, double* S, hipStream_t s);
 // level-1 stencil of a hex8 grid from the fine densities, G and the corrections of flagged elements

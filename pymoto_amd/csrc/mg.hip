@@ -361,7 +361,7 @@ static int hex_coarse_stencils(pmt_mg* mg, hipStream_t s) {
     L1.t_cap = mg->host_counts[0];
   }
   PMT_TRY(launch_level1_dk(mg->L[0].g, mg->K0, mg->xe, mg->bcm, mg->bcdiag, mg->L[1].cflag, mg->L[1].list,
-                           mg->L[1].dK, s));
+                           mg->host_counts[1], mg->L[1].dK, s));
   if (mg->l1free)
     PMT_TRY(launch_coarse1_tstencil(L1.g, L1.tnode, mg->host_counts[0], L1.t_cap, L1.map, L1.dK, L1.TS, s));
   for (int l = 2; l < nl; ++l)
