@@ -97,6 +97,92 @@ filter_bwd_kernel(int nx, int ny, int nz, int kx, int ky, int kz, const double* 
   dx[e] = acc;
 }
 
+// The same two kernels with the kernel extents fixed at compile time (KXY x KXY x KZ, the shapes that
+// FilterConv builds from a radius): the per-axis maps are read once per element and the KXY^2 KZ taps
+// are unrolled, so every load of a thread is independent and in flight at once (the general kernels
+// chain them through their loops).  Same summation order as the general kernels.
+template <int KXY, int KZ>
+__global__ void __launch_bounds__(256)
+filter_fwd_fixed_kernel(int nx, int ny, int nz, const double* __restrict__ w, const int32_t* __restrict__ map,
+                        const double* __restrict__ cval, const double* __restrict__ x, double* __restrict__ y) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nel = (long long)nx * ny * nz;
+  if (e >= nel) return;
+  const int i = (int)(e % nx);
+  const long long t = e / nx;
+  const int j = (int)(t % ny);
+  const int k = (int)(t / ny);
+  const int Lx = nx + KXY - 1, Ly = ny + KXY - 1;
+  int ma[KXY], mb[KXY], mc[KZ];
+#pragma unroll
+  for (int p = 0; p < KXY; ++p) ma[p] = map[i + KXY - 1 - p];
+#pragma unroll
+  for (int q = 0; q < KXY; ++q) mb[q] = map[Lx + j + KXY - 1 - q];
+#pragma unroll
+  for (int r = 0; r < KZ; ++r) mc[r] = map[Lx + Ly + k + KZ - 1 - r];
+  double acc = 0.0;
+#pragma unroll
+  for (int r = 0; r < KZ; ++r)
+#pragma unroll
+    for (int q = 0; q < KXY; ++q)
+#pragma unroll
+      for (int p = 0; p < KXY; ++p) {
+        double v;
+        if (mc[r] < 0)
+          v = cval[Lx + Ly + k + KZ - 1 - r];
+        else if (mb[q] < 0)
+          v = cval[Lx + j + KXY - 1 - q];
+        else if (ma[p] < 0)
+          v = cval[i + KXY - 1 - p];
+        else
+          v = x[((long long)mc[r] * ny + mb[q]) * nx + ma[p]];
+        acc = fma(w[(p * KXY + q) * KZ + r], v, acc);
+      }
+  y[e] = acc;
+}
+
+template <int KXY, int KZ>
+__global__ void __launch_bounds__(256)
+filter_bwd_fixed_kernel(int nx, int ny, int nz, const double* __restrict__ w, const int32_t* __restrict__ inv_ptr,
+                        const int32_t* __restrict__ inv_idx, const double* __restrict__ dy, double* __restrict__ dx) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nel = (long long)nx * ny * nz;
+  if (e >= nel) return;
+  const int i = (int)(e % nx);
+  const long long t = e / nx;
+  const int j = (int)(t % ny);
+  const int k = (int)(t / ny);
+  const int32_t* px = inv_ptr;
+  const int32_t* py = inv_ptr + (nx + 1);
+  const int32_t* pz = inv_ptr + (nx + 1) + (ny + 1);
+  double acc = 0.0;
+  for (int ic = pz[k]; ic < pz[k + 1]; ++ic) {
+    const int c = inv_idx[ic];
+    for (int ib = py[j]; ib < py[j + 1]; ++ib) {
+      const int b = inv_idx[ib];
+      for (int ia = px[i]; ia < px[i + 1]; ++ia) {
+        const int a = inv_idx[ia];
+#pragma unroll
+        for (int r = 0; r < KZ; ++r) {
+          const int cc = c - (KZ - 1) + r;
+#pragma unroll
+          for (int q = 0; q < KXY; ++q) {
+            const int bb = b - (KXY - 1) + q;
+#pragma unroll
+            for (int p = 0; p < KXY; ++p) {
+              const int aa = a - (KXY - 1) + p;
+              const bool in = cc >= 0 && cc < nz && bb >= 0 && bb < ny && aa >= 0 && aa < nx;
+              const long long m = in ? ((long long)cc * ny + bb) * nx + aa : 0;
+              if (in) acc = fma(w[(p * KXY + q) * KZ + r], dy[m], acc);
+            }
+          }
+        }
+      }
+    }
+  }
+  dx[e] = acc;
+}
+
 // DensityFilter (pymoto/modules/filter.py:223-378): y_i = so_i * sum_j H_ij (si_j * x_j) over the
 // clipped window |j - i|_inf <= int(radius) with H_ij = max(0, radius - |i - j|) in element units.
 // Forward: si = 1, so = 1/s; adjoint: si = 1/s, so = 1 (H symmetric); s = H 1 with si = so = 1.
@@ -133,8 +219,18 @@ int pmt_filter_fwd(int nelx, int nely, int nelz, int kx, int ky, int kz, const d
   const int nz = nelz > 0 ? nelz : 1;
   if (nelz == 0 && kz != 1) return arg_error("pmt_filter_fwd: 2D grid needs kz == 1");
   const long long nel = (long long)nelx * nely * nz;
-  filter_fwd_kernel<<<blocks_for(nel, 256), 256, 0, (hipStream_t)stream>>>(nelx, nely, nz, kx, ky, kz, w, map, cval,
-                                                                          x, y);
+  const unsigned nb = blocks_for(nel, 256);
+  hipStream_t s = (hipStream_t)stream;
+  if (kx == 3 && ky == 3 && kz == 3)
+    filter_fwd_fixed_kernel<3, 3><<<nb, 256, 0, s>>>(nelx, nely, nz, w, map, cval, x, y);
+  else if (kx == 3 && ky == 3 && kz == 1)
+    filter_fwd_fixed_kernel<3, 1><<<nb, 256, 0, s>>>(nelx, nely, nz, w, map, cval, x, y);
+  else if (kx == 5 && ky == 5 && kz == 5)
+    filter_fwd_fixed_kernel<5, 5><<<nb, 256, 0, s>>>(nelx, nely, nz, w, map, cval, x, y);
+  else if (kx == 5 && ky == 5 && kz == 1)
+    filter_fwd_fixed_kernel<5, 1><<<nb, 256, 0, s>>>(nelx, nely, nz, w, map, cval, x, y);
+  else
+    filter_fwd_kernel<<<nb, 256, 0, s>>>(nelx, nely, nz, kx, ky, kz, w, map, cval, x, y);
   return check_launch("filter_fwd_kernel");
 }
 
@@ -158,8 +254,18 @@ int pmt_filter_bwd(int nelx, int nely, int nelz, int kx, int ky, int kz, const d
   const int nz = nelz > 0 ? nelz : 1;
   if (nelz == 0 && kz != 1) return arg_error("pmt_filter_bwd: 2D grid needs kz == 1");
   const long long nel = (long long)nelx * nely * nz;
-  filter_bwd_kernel<<<blocks_for(nel, 256), 256, 0, (hipStream_t)stream>>>(nelx, nely, nz, kx, ky, kz, w, inv_ptr,
-                                                                          inv_idx, dy, dx);
+  const unsigned nb = blocks_for(nel, 256);
+  hipStream_t s = (hipStream_t)stream;
+  if (kx == 3 && ky == 3 && kz == 3)
+    filter_bwd_fixed_kernel<3, 3><<<nb, 256, 0, s>>>(nelx, nely, nz, w, inv_ptr, inv_idx, dy, dx);
+  else if (kx == 3 && ky == 3 && kz == 1)
+    filter_bwd_fixed_kernel<3, 1><<<nb, 256, 0, s>>>(nelx, nely, nz, w, inv_ptr, inv_idx, dy, dx);
+  else if (kx == 5 && ky == 5 && kz == 5)
+    filter_bwd_fixed_kernel<5, 5><<<nb, 256, 0, s>>>(nelx, nely, nz, w, inv_ptr, inv_idx, dy, dx);
+  else if (kx == 5 && ky == 5 && kz == 1)
+    filter_bwd_fixed_kernel<5, 1><<<nb, 256, 0, s>>>(nelx, nely, nz, w, inv_ptr, inv_idx, dy, dx);
+  else
+    filter_bwd_kernel<<<nb, 256, 0, s>>>(nelx, nely, nz, kx, ky, kz, w, inv_ptr, inv_idx, dy, dx);
   return check_launch("filter_bwd_kernel");
 }
 

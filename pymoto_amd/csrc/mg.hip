@@ -45,6 +45,11 @@ static void release(pmt_mg* mg) {
   mg->host_sc = nullptr;
   if (mg->host_counts) (void)hipHostFree(mg->host_counts);
   mg->host_counts = nullptr;
+  if (mg->ev_fork) (void)hipEventDestroy(mg->ev_fork);
+  if (mg->ev_join) (void)hipEventDestroy(mg->ev_join);
+  if (mg->s2) (void)hipStreamDestroy(mg->s2);
+  mg->ev_fork = mg->ev_join = nullptr;
+  mg->s2 = nullptr;
 }
 
 // frees one allocation of the handle (bytes: its size, for the accounting)
@@ -386,15 +391,28 @@ static int hex_coarse_stencils(pmt_mg* mg, hipStream_t s) {
     }
     return o;
   };
+  // the coarsest level first; its dense inverse (a chain of small, latency-bound launches) then runs on
+  // the handle's second stream while this one builds the other levels
+  const int lc = nl - 1;
+  Level& Lc = mg->L[lc];
+  if (lc == 1)
+    PMT_TRY(launch_stencil_from_fine(mg->L[0].g, mg->G, mg->xe, L1.cflag, L1.map, L1.dK, target(1), s));
+  else
+    PMT_TRY(launch_stencil_from_moments(Lc.g, Lc.C, Lc.M, Lc.list, Lc.dK, Lc.Ke, target(lc), s));
+  PMT_HIP(hipEventRecord(mg->ev_fork, s));
+  PMT_HIP(hipStreamWaitEvent(mg->s2, mg->ev_fork, 0));
+  PMT_TRY(launch_dense_from_stencil(Lc.g, Lc.d.S, mg->Ainv, mg->s2));
+  PMT_TRY(gj_invert(mg->nc, mg->Ainv, mg->gjP, mg->gjC, mg->s2));
+  PMT_HIP(hipEventRecord(mg->ev_join, mg->s2));
   if (mg->l1free)
-    PMT_TRY(launch_coarse1_xhat(mg->L[0].g, mg->L[1].g, mg->xe, mg->L[1].cflag, mg->L[1].d.Xh,
-                                mg->prec != 64 ? mg->L[1].f.Xh : nullptr, s));
-  PMT_TRY(launch_stencil_from_fine(mg->L[0].g, mg->G, mg->xe, mg->L[1].cflag, mg->L[1].map, mg->L[1].dK, target(1),
-                                   s));
-  for (int l = 2; l < nl; ++l) {
+    PMT_TRY(launch_coarse1_xhat(mg->L[0].g, L1.g, mg->xe, L1.cflag, L1.d.Xh, mg->prec != 64 ? L1.f.Xh : nullptr, s));
+  if (lc > 1)
+    PMT_TRY(launch_stencil_from_fine(mg->L[0].g, mg->G, mg->xe, L1.cflag, L1.map, L1.dK, target(1), s));
+  for (int l = 2; l < lc; ++l) {
     Level& L = mg->L[l];
     PMT_TRY(launch_stencil_from_moments(L.g, L.C, L.M, L.list, L.dK, L.Ke, target(l), s));
   }
+  PMT_HIP(hipStreamWaitEvent(s, mg->ev_join, 0));
   return PMT_OK;
 }
 
@@ -475,6 +493,9 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
   if (rc == PMT_OK)
     rc = hip_status(hipHostMalloc((void**)&mg->host_counts, kMaxLevels * sizeof(int), hipHostMallocDefault),
                     "hipHostMalloc");
+  if (rc == PMT_OK && hex) rc = hip_status(hipStreamCreateWithFlags(&mg->s2, hipStreamNonBlocking), "hipStreamCreate");
+  if (rc == PMT_OK && hex) rc = hip_status(hipEventCreateWithFlags(&mg->ev_fork, hipEventDisableTiming), "hipEventCreate");
+  if (rc == PMT_OK && hex) rc = hip_status(hipEventCreateWithFlags(&mg->ev_join, hipEventDisableTiming), "hipEventCreate");
   for (int l = 0; l < nlevels; ++l) {
     Level& L = mg->L[l];
     if (l == nlevels - 1) {
@@ -615,9 +636,11 @@ int pmt_mg_update(pmt_mg* mg, const double* xe, const uint8_t* bcmask, double bc
       PMT_TRY(launch_stencil_compress<double>(L.g, L.d.S, L.d.H, s));
     }
   }
-  Level& Cl = mg->L[mg->nlev - 1];
-  PMT_TRY(launch_dense_from_stencil(Cl.g, Cl.d.S, mg->Ainv, s));
-  PMT_TRY(gj_invert(mg->nc, mg->Ainv, mg->gjP, mg->gjC, s));
+  if (mg->dim != 3) {  // hex8: inverted on the second stream inside hex_coarse_stencils
+    Level& Cl = mg->L[mg->nlev - 1];
+    PMT_TRY(launch_dense_from_stencil(Cl.g, Cl.d.S, mg->Ainv, s));
+    PMT_TRY(gj_invert(mg->nc, mg->Ainv, mg->gjP, mg->gjC, s));
+  }
   if (mg->smoother == 1) {
     for (int l = 0; l < mg->nlev - 1; ++l) {
       double lam = mg->lmax_elem;  // the fine grid: lambda_max(D^-1 K) <= lambda_max(D0^-1 K0)

@@ -84,6 +84,10 @@ struct pmt_mg {
   bool ready = false;
   long long nc = 0;
   double* Ainv = nullptr;
+  // hex8 update: the coarsest inverse (a chain of small launches) runs on a second stream of the
+  // handle, forked after the coarsest stencil and joined before update returns
+  hipStream_t s2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   double* gjP = nullptr;
   double* gjC = nullptr;
   float* z32 = nullptr;  // FP32 V-cycle output on the fine grid
