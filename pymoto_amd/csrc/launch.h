@@ -25,6 +25,7 @@ bool k0_reflection_table(const double* K0_host, double* tab);
 // children (launch_coarse1_xhat); the elements with Dirichlet children enter through the correction
 // stencils TS[243][cap] of the nt touched nodes tnode (launch_coarse1_touched, then _tstencil)
 void coarse1_table(const double* G0_host, double* ext_host);
+int coarse1_table_pattern(const double* ext_host, int k0_pattern);  // the launch_coarse1 pattern (0, 1, 2)
 int launch_coarse1_xhat(const GridDims& gf, const GridDims& gc, const double* xe, const uint8_t* cflag, double* X64,
                         float* X32, hipStream_t s);
 int launch_coarse1_touched(const GridDims& g, const int* list, const int* map, int* tnode, int* tcount,
@@ -87,7 +88,8 @@ int launch_dot2(long long n, const double* a0, const double* b0, const double* a
                 double* work, hipStream_t s);
 int launch_axpby(long long n, double a, const double* x, double b, double* y, hipStream_t s);
 int launch_cg_xr_update(long long n, const double* sc, const double* p, const double* q, double* x, double* r,
-                        double* work, double* out_rr, hipStream_t s, float* r32 = nullptr);
+                        double* work, double* out_rr, hipStream_t s, float* r32 = nullptr,
+                        const float* dinv32 = nullptr, float omega32 = 0.0f, float* u1 = nullptr);
 int launch_cg_x_update(long long n, const double* sc, const double* p, double* x, hipStream_t s);
 int launch_cg_p_update(long long n, const double* sc, const double* z, double* p, hipStream_t s);
 int launch_cg_p_update(long long n, const double* sc, const float* z, double* p, hipStream_t s);

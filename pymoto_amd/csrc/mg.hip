@@ -112,7 +112,7 @@ static int level_apply(pmt_mg* mg, int l, int mode, const T* u, const T* b, T* y
     return arg_error("level_apply: the coarsest level is solved directly");
   }
   if (l == 1 && mg->l1free)
-    return launch_coarse1<T>(mode, L.g, mg->k0pat & PMT_K0_CLASS_PATTERN, mg->G0ext, B.Xh, L.tnode, mg->host_counts[0],
+    return launch_coarse1<T>(mode, L.g, mg->g0pat, mg->G0ext, B.Xh, L.tnode, mg->host_counts[0],
                              L.t_cap, L.TS, u, b, B.dinv, omega, y, s, up, beta);
   return launch_stencil_half<T>(mode, L.g, B.H, u, b, B.dinv, omega, y, s, up, beta);
 }
@@ -130,13 +130,15 @@ static int level_apply(pmt_mg* mg, int l, int mode, const T* u, const T* b, T* y
 // rho_k = 1/(2 sigma - rho_{k-1}), alpha_k = 2 rho_k/delta, beta_k = rho_k rho_{k-1}.
 // The ping-pong buffers cur/oth swap after every sweep; a Chebyshev sweep overwrites u_{k-1} in place.
 template <typename T>
-static int smooth(pmt_mg* mg, int l, const T* b, T*& cur, T*& oth, bool from_zero, hipStream_t s) {
+static int smooth(pmt_mg* mg, int l, const T* b, T*& cur, T*& oth, bool from_zero, hipStream_t s,
+                  bool first_done = false) {
   Level& L = mg->L[l];
   LevelBuf<T>& B = L.buf<T>();
   if (mg->smoother == 0) {
     int it0 = 0;
     if (from_zero) {
-      PMT_TRY(launch_jacobi0<T>(L.ndof, B.dinv, b, L.omega, cur, s));
+      // first_done: cur already holds omega D^-1 b (written by the CG update)
+      if (!first_done) PMT_TRY(launch_jacobi0<T>(L.ndof, B.dinv, b, L.omega, cur, s));
       it0 = 1;
     }
     for (int it = it0; it < L.steps; ++it) {
@@ -172,7 +174,7 @@ static int smooth(pmt_mg* mg, int l, const T* b, T*& cur, T*& oth, bool from_zer
 
 // out = V-cycle(b) on level l (GeometricMultigrid.solve with x0 = None, iterative.py:222-256)
 template <typename T>
-static int vcycle(pmt_mg* mg, int l, const T* b, T* out, hipStream_t s) {
+static int vcycle(pmt_mg* mg, int l, const T* b, T* out, hipStream_t s, bool first_done = false) {
   if (l == mg->nlev - 1) return coarse_solve<T>(mg, b, out, s);
   Level& L = mg->L[l];
   Level& C = mg->L[l + 1];
@@ -181,7 +183,7 @@ static int vcycle(pmt_mg* mg, int l, const T* b, T* out, hipStream_t s) {
   // 2*steps-1 out-of-place sweeps follow the first one: start in the buffer that ends in `out`
   T* cur = B.t;
   T* oth = out;
-  PMT_TRY(smooth<T>(mg, l, b, cur, oth, true, s));
+  PMT_TRY(smooth<T>(mg, l, b, cur, oth, true, s, first_done));
   PMT_TRY(level_apply<T>(mg, l, MODE_RESIDUAL, cur, b, B.r, s));
   PMT_TRY(launch_restrict<T>(L.g, C.g, B.r, CB.b, s));
   PMT_TRY(vcycle<T>(mg, l + 1, CB.b, CB.u, s));
@@ -193,10 +195,10 @@ static int vcycle(pmt_mg* mg, int l, const T* b, T* out, hipStream_t s) {
 
 // reduced-precision V-cycles: z32 = M r, from r rounded to FP32 (already in the fine level's FP32 right-hand
 // side when r32_ready: the CG update wrote it)
-static int precondition_f32(pmt_mg* mg, const double* r, bool r32_ready, hipStream_t s) {
+static int precondition_f32(pmt_mg* mg, const double* r, bool r32_ready, hipStream_t s, bool first_done = false) {
   Level& F = mg->L[0];
   if (!r32_ready) PMT_TRY(launch_to_float(F.ndof, r, F.f.b, s));
-  return vcycle<float>(mg, 0, F.f.b, mg->z32, s);
+  return vcycle<float>(mg, 0, F.f.b, mg->z32, s, first_done);
 }
 
 // the preconditioner as CG sees it: FP64 in, FP64 out
@@ -549,6 +551,7 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
     if (rc == PMT_OK && mg->l1free) {
       std::vector<double> ext(864);
       coarse1_table(G.data(), ext.data());  // from G_0, the child at the element's corner 0
+      mg->g0pat = coarse1_table_pattern(ext.data(), mg->k0pat);
       rc = hip_status(hipMemcpy(mg->G0ext, ext.data(), ext.size() * sizeof(double), hipMemcpyHostToDevice),
                       "level-1 table upload");
     }
@@ -739,13 +742,17 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
     // reduced-precision V-cycles: the update also writes r in FP32 (the V-cycle's input), and z stays
     // FP32 (q.z and p = z + beta p widen it exactly: the same sums as through an FP64 copy)
     const bool mixed = mg->prec != 64;
+    // damped Jacobi: the update also writes the V-cycle's first fine sweep from zero, omega D^-1 r32, into
+    // the fine level's work vector (the same FP32 expression as the sweep kernel)
+    const bool first = mixed && mg->smoother == 0;
     bool r32_ready = false;
     if (i % restart == 0) {
       PMT_TRY(launch_cg_x_update(n, sc, mg->p, x, s));
       PMT_TRY(level_apply<double>(mg, 0, MODE_RESIDUAL, x, b, mg->r, s));
       PMT_TRY(launch_dot(n, mg->r, mg->r, sc + 2, mg->work, s));
     } else {
-      PMT_TRY(launch_cg_xr_update(n, sc, mg->p, mg->q, x, mg->r, mg->work, sc + 2, s, mixed ? F.f.b : nullptr));
+      PMT_TRY(launch_cg_xr_update(n, sc, mg->p, mg->q, x, mg->r, mg->work, sc + 2, s, mixed ? F.f.b : nullptr,
+                                  first ? F.f.dinv : nullptr, (float)F.omega, first ? F.f.t : nullptr));
       r32_ready = mixed;
     }
     double rr = 0.0;
@@ -754,7 +761,7 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
     if (history) history[i + 1] = tval;
     if (tval <= tol) break;
     if (mixed) {
-      PMT_TRY(precondition_f32(mg, mg->r, r32_ready, s));
+      PMT_TRY(precondition_f32(mg, mg->r, r32_ready, s, r32_ready && first));
       PMT_TRY(launch_dot(n, mg->q, mg->z32, sc + 3, mg->work, s));
       PMT_TRY(launch_cg_p_update(n, sc, mg->z32, mg->p, s));
     } else {

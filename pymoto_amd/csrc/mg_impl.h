@@ -74,6 +74,7 @@ struct pmt_mg {
   double* G = nullptr;  // P_c^T K0 P_c per child position (level-1 stencil of hex8 grids)
   bool l1free = false;     // hex8 level 1 applied element by element in the reflection basis (no stencil)
   double* G0ext = nullptr;  // its 24 x 24 table (coarse1_table), FP64 then FP32
+  int g0pat = 0;            // zero pattern of the table (coarse1_table_pattern)
   int* host_counts = nullptr;  // pinned: [0] touched nodes of level 1 (without a stencil), [l] flagged elements
                                // of level l (read once per update)
   int k0pat = 0;

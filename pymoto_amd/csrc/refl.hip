@@ -112,18 +112,29 @@ __host__ __device__ constexpr bool g0_nonzero(int a, int b, bool pat) {
   return false;
 }
 
+// Rows of the entries of the table that can be nonzero for box elements (279 of the 360 above): the
+// rows and columns of the zero frequency (rigid translations, which K0 annihilates) and further
+// entries vanish.  Measured on the isotropic unit cube (Poisson ratios 0.2 to 0.45) and a 1 x 1 x 2
+// box; coarse1_table_pattern checks a given table against it on the host, so the pattern is only used
+// where it holds.
+constexpr unsigned kG0NumericRows[24] = {
+    0x000000, 0x000000, 0x000000, 0xfac688, 0x650650, 0xa69820, 0x650650, 0xfac688, 0xd92900, 0xffc6d8, 0xffc6d8, 0xffb920,
+    0xa69820, 0xd92900, 0xfac688, 0xfedea8, 0xfd2f50, 0xfedea8, 0xe79e70, 0xfbef88, 0xfbef88, 0xffdef8, 0xffefd8, 0xffffa8};
+
+// nz[p]: p = 0 structural, 1 with the box class pattern, 2 numerical (kG0NumericRows)
 struct G0Pattern {
-  bool nz[2][576];
+  bool nz[3][576];
   constexpr G0Pattern() : nz() {
     for (int p = 0; p < 2; ++p)
       for (int q = 0; q < 576; ++q) nz[p][q] = g0_nonzero(q / 24, q % 24, p == 1);
+    for (int q = 0; q < 576; ++q) nz[2][q] = nz[1][q] && ((kG0NumericRows[q / 24] >> (q % 24)) & 1u);
   }
 };
 constexpr G0Pattern kG0Pattern{};
 
 // v: the element's corners after the plane stages (as for refl_element); X: its X^.  Returns K_E u_e
 // before the inverse transform (as refl_element), without the Dirichlet corrections.
-template <bool PAT, typename T>
+template <int PAT, typename T>
 __device__ inline void coarse1_element(const T* tab, const T (&X)[8], T (&v)[8][3]) {
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -154,7 +165,7 @@ __device__ inline void coarse1_element(const T* tab, const T (&X)[8], T (&v)[8][
 #pragma unroll
         for (int d2 = 0; d2 < 3; ++d2) {
           const int be = (cb ^ (1 << d2)) * 3 + d2;  // the dof of class cb with component d2
-          if (kG0Pattern.nz[PAT ? 1 : 0][al * 24 + be]) {
+          if (kG0Pattern.nz[PAT][al * 24 + be]) {
             t = any ? fma(tb[al * 24 + be], v[be / 3][d2], t) : tb[al * 24 + be] * v[be / 3][d2];
             any = true;
           }
@@ -186,7 +197,7 @@ __device__ inline void coarse1_element(const T* tab, const T (&X)[8], T (&v)[8][
 // OP 0: the fine grid (element factor x_e, Dirichlet-masked nodes, bcdiag rows).  OP 1: the first
 // coarse level of a hex8 hierarchy (element factors X^[8], layout [layer][m][element]; no mask: the
 // elements with Dirichlet children get their corrections from coarse1_correct_kernel afterwards).
-template <int OP, int SY, int MODE, bool PAT, typename T>
+template <int OP, int SY, int MODE, int PAT, typename T>
 __global__ void __launch_bounds__(kReflRow* SY)
 fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, const T* __restrict__ tab, int zero_stride,
                  const T* __restrict__ xe, const uint8_t* __restrict__ bcm, T bcdiag, const T* __restrict__ u,
@@ -369,7 +380,7 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
     // the table entries are re-read through the scalar cache each layer (zero_stride is 0; the
     // compiler cannot hoist the loads, which would exceed the scalar register file)
     if constexpr (OP == 0) {
-      refl_element<PAT>(tab + zero_stride * k, xs[SA][0], v);
+      refl_element<PAT != 0>(tab + zero_stride * k, xs[SA][0], v);
     } else {
       coarse1_element<PAT>((LDS_TAB ? sG0 : tab) + zero_stride * k, xs[SA], v);  // (re-read per layer)
     }
@@ -503,7 +514,7 @@ bool k0_reflection_table(const double* K0, double* tab) {
 }
 
 // Resident blocks per CU of one kernel instance (cached per device).
-template <int OP, int SY, int MODE, bool PAT, typename T>
+template <int OP, int SY, int MODE, int PAT, typename T>
 static int refl_resident_blocks() {
   static int cache[64] = {0};
   int dev = 0;
@@ -542,7 +553,7 @@ static int refl_planes(int nxy, int nz, int resident) {
   return best > 0 ? best : nz;
 }
 
-template <int OP, int SY, bool PAT, typename T>
+template <int OP, int SY, int PAT, typename T>
 static void launch_refl_shape(int mode, const GridDims& g, int planes, const T* tab, const T* xe, const uint8_t* bcm,
                               T bcdiag, const T* u, const T* b, const T* dinv, T omega, T* y, hipStream_t s,
                               const T* up, T beta, double* partial = nullptr, int* nwork_out = nullptr) {
@@ -744,10 +755,12 @@ int launch_coarse1(int mode, const GridDims& g, int pattern, const double* G0ext
                    hipStream_t s, const T* up, double beta) {
   const T* tab = sizeof(T) == 8 ? (const T*)G0ext : (const T*)(G0ext + 576);
   const T om = (T)omega, be = (T)beta;
-  if (pattern)
-    launch_refl_shape<1, 16, true, T>(mode, g, 0, tab, Xh, nullptr, (T)0, u, b, dinv, om, y, s, up, be);
+  if (pattern == 2)
+    launch_refl_shape<1, 16, 2, T>(mode, g, 0, tab, Xh, nullptr, (T)0, u, b, dinv, om, y, s, up, be);
+  else if (pattern == 1)
+    launch_refl_shape<1, 16, 1, T>(mode, g, 0, tab, Xh, nullptr, (T)0, u, b, dinv, om, y, s, up, be);
   else
-    launch_refl_shape<1, 16, false, T>(mode, g, 0, tab, Xh, nullptr, (T)0, u, b, dinv, om, y, s, up, be);
+    launch_refl_shape<1, 16, 0, T>(mode, g, 0, tab, Xh, nullptr, (T)0, u, b, dinv, om, y, s, up, be);
   PMT_TRY(check_launch("fine_refl_kernel<coarse level 1>"));
   if (nt <= 0) return PMT_OK;
   const unsigned blocks = (unsigned)blocks_for(3LL * nt, 256);
@@ -812,6 +825,18 @@ template int launch_coarse1<double>(int, const GridDims&, int, const double*, co
 template int launch_coarse1<float>(int, const GridDims&, int, const double*, const float*, const int*, int, long long,
                                    const double*, const float*, const float*, const float*, double, float*, hipStream_t,
                                    const float*, double);
+
+// Zero pattern the level-1 kernel may skip for a table (coarse1_table layout): 2 when K0 has the box class
+// pattern and every entry outside the numerical pattern is below 1e-14 of the largest, 1 with the class
+// pattern only, else 0.
+int coarse1_table_pattern(const double* ext, int k0_pattern) {
+  if (!(k0_pattern & PMT_K0_CLASS_PATTERN)) return 0;
+  double scale = 0.0;
+  for (int q = 0; q < 576; ++q) scale = fmax(scale, fabs(ext[q]));
+  for (int q = 0; q < 576; ++q)
+    if (!kG0Pattern.nz[2][q] && fabs(ext[q]) > 1e-14 * scale) return 1;
+  return 2;
+}
 
 // G0tab: (1/64) H G_0 H of the child-0 matrix G_0 = P_0^T K0 P_0 (G0: host, 24 x 24), FP64 then FP32
 void coarse1_table(const double* G0, double* ext) {

@@ -161,17 +161,24 @@ int launch_dot2(long long n, const double* a0, const double* b0, const double* a
 }
 
 // CG: x += alpha p, r -= alpha q, partial ||r||^2; alpha = sc[1]/sc[0] read on device; r32 (optional):
-// the new r rounded to FP32, the input of a reduced-precision preconditioner
+// the new r rounded to FP32, the input of a reduced-precision preconditioner; u1 (optional, with r32):
+// omega * D^-1 * r32, the preconditioner's first damped-Jacobi sweep from zero (same FP32 expression as
+// jacobi0_kernel)
 __global__ void __launch_bounds__(kReduceThreads)
 cg_xr_update(long long n, const double* __restrict__ sc, const double* __restrict__ p, const double* __restrict__ q,
-             double* __restrict__ x, double* __restrict__ r, double* __restrict__ partial, float* __restrict__ r32) {
+             double* __restrict__ x, double* __restrict__ r, double* __restrict__ partial, float* __restrict__ r32,
+             const float* __restrict__ dinv32, float omega32, float* __restrict__ u1) {
   const double alpha = sc[1] / sc[0];
   double s = 0.0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     x[i] = fma(alpha, p[i], x[i]);
     const double rv = fma(-alpha, q[i], r[i]);
     r[i] = rv;
-    if (r32) r32[i] = (float)rv;
+    if (r32) {
+      const float rf = (float)rv;
+      r32[i] = rf;
+      if (u1) u1[i] = omega32 * dinv32[i] * rf;
+    }
     s = fma(rv, rv, s);
   }
   double t = block_sum(s);
@@ -238,8 +245,9 @@ int launch_axpby(long long n, double a, const double* x, double b, double* y, hi
 }
 
 int launch_cg_xr_update(long long n, const double* sc, const double* p, const double* q, double* x, double* r,
-                        double* work, double* out_rr, hipStream_t s, float* r32) {
-  cg_xr_update<<<kReduceBlocks, kReduceThreads, 0, s>>>(n, sc, p, q, x, r, work, r32);
+                        double* work, double* out_rr, hipStream_t s, float* r32, const float* dinv32, float omega32,
+                        float* u1) {
+  cg_xr_update<<<kReduceBlocks, kReduceThreads, 0, s>>>(n, sc, p, q, x, r, work, r32, dinv32, omega32, u1);
   PMT_TRY(check_launch("cg_xr_update"));
   dot_stage2<1><<<1, kReduceThreads, 0, s>>>(work, out_rr);
   return check_launch("dot_stage2");
