@@ -17,7 +17,9 @@
 // with BF16 operands and FP32 accumulation (hex8 only); the fine residual that is restricted to the
 // coarse grid stays FP32, so the coarse correction sees no BF16 rounding.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <limits>
 #include <new>
 #include <type_traits>
 #include <vector>
@@ -43,6 +45,8 @@ static void release(pmt_mg* mg) {
   mg->allocs.clear();
   if (mg->host_sc) (void)hipHostFree(mg->host_sc);
   mg->host_sc = nullptr;
+  if (mg->host_rr) (void)hipHostFree(mg->host_rr);
+  mg->host_rr = nullptr;
   if (mg->host_counts) (void)hipHostFree(mg->host_counts);
   mg->host_counts = nullptr;
   if (mg->ev_fork) (void)hipEventDestroy(mg->ev_fork);
@@ -213,6 +217,25 @@ static int read_scalar(pmt_mg* mg, int k, double* v, hipStream_t s) {
   PMT_HIP(hipStreamSynchronize(s));
   *v = mg->host_sc[k];
   return PMT_OK;
+}
+
+// The same for the CG stopping test without a stream synchronisation (whose wake-up left the GPU idle
+// for ~60 us per iteration): a one-thread kernel stores the value into pinned memory that the host
+// polls; a NaN sentinel marks "not yet written".  After 1 s without it the host falls back to the
+// synchronising read, so a value that never arrives cannot hang the solve.
+static int read_scalar_polled(pmt_mg* mg, int k, double* v, hipStream_t s) {
+  volatile double* h = mg->host_rr;
+  h[0] = std::numeric_limits<double>::quiet_NaN();
+  PMT_TRY(launch_publish(mg->sc + k, mg->host_rr, s));
+  const auto t0 = std::chrono::steady_clock::now();
+  for (;;) {
+    const double x = h[0];
+    if (!std::isnan(x)) {
+      *v = x;
+      return PMT_OK;
+    }
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(1)) return read_scalar(mg, k, v, s);
+  }
 }
 
 static int alloc_fp32(pmt_mg* mg) {
@@ -402,10 +425,8 @@ static int hex_coarse_stencils(pmt_mg* mg, hipStream_t s) {
   else
     PMT_TRY(launch_stencil_from_moments(Lc.g, Lc.C, Lc.M, Lc.list, Lc.dK, Lc.Ke, target(lc), s));
   PMT_HIP(hipEventRecord(mg->ev_fork, s));
-  PMT_HIP(hipStreamWaitEvent(mg->s2, mg->ev_fork, 0));
-  PMT_TRY(launch_dense_from_stencil(Lc.g, Lc.d.S, mg->Ainv, mg->s2));
-  PMT_TRY(gj_invert(mg->nc, mg->Ainv, mg->gjP, mg->gjC, mg->s2));
-  PMT_HIP(hipEventRecord(mg->ev_join, mg->s2));
+  // this stream's remaining work is enqueued first (a handful of launches), so it runs while the host
+  // enqueues the ~45 launches of the inversion
   if (mg->l1free)
     PMT_TRY(launch_coarse1_xhat(mg->L[0].g, L1.g, mg->xe, L1.cflag, L1.d.Xh, mg->prec != 64 ? L1.f.Xh : nullptr, s));
   if (lc > 1)
@@ -414,6 +435,10 @@ static int hex_coarse_stencils(pmt_mg* mg, hipStream_t s) {
     Level& L = mg->L[l];
     PMT_TRY(launch_stencil_from_moments(L.g, L.C, L.M, L.list, L.dK, L.Ke, target(l), s));
   }
+  PMT_HIP(hipStreamWaitEvent(mg->s2, mg->ev_fork, 0));
+  PMT_TRY(launch_dense_from_stencil(Lc.g, Lc.d.S, mg->Ainv, mg->s2));
+  PMT_TRY(gj_invert(mg->nc, mg->Ainv, mg->gjP, mg->gjC, mg->s2));
+  PMT_HIP(hipEventRecord(mg->ev_join, mg->s2));
   PMT_HIP(hipStreamWaitEvent(s, mg->ev_join, 0));
   return PMT_OK;
 }
@@ -539,6 +564,8 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
     ALLOC(mg->dotws, 2 * mg->dotws_parts);
   }
 #undef ALLOC
+  if (rc == PMT_OK)
+    rc = hip_status(hipHostMalloc((void**)&mg->host_rr, 2 * sizeof(double), hipHostMallocCoherent), "hipHostMalloc");
   if (rc == PMT_OK) rc = hip_status(hipHostMalloc((void**)&mg->host_sc, 16 * sizeof(double), hipHostMallocDefault),
                                     "hipHostMalloc");
   if (rc == PMT_OK)
@@ -756,7 +783,7 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
       r32_ready = mixed;
     }
     double rr = 0.0;
-    PMT_TRY(read_scalar(mg, 2, &rr, s));
+    PMT_TRY(read_scalar_polled(mg, 2, &rr, s));
     tval = std::sqrt(rr) / bn;
     if (history) history[i + 1] = tval;
     if (tval <= tol) break;

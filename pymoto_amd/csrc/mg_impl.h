@@ -101,6 +101,7 @@ struct pmt_mg {
   double* dotws = nullptr;  // partial sums of the fused fine K.u + dot products (2 x dotws_parts)
   long long dotws_parts = 0;
   double* host_sc = nullptr;
+  double* host_rr = nullptr;  // pinned, written by publish_kernel: the CG stopping test polls it
   long long bytes = 0;
   std::vector<void*> allocs;
   // multi-right-hand-side CG (blockcg.hip): workspace for kcap columns and captured iterations

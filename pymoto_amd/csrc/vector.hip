@@ -253,6 +253,20 @@ int launch_cg_xr_update(long long n, const double* sc, const double* p, const do
   return check_launch("dot_stage2");
 }
 
+// v -> host-visible memory (pinned): a normal store followed by a system-scope fence, so a host thread
+// polling the value sees it without a stream synchronisation
+__global__ void publish_kernel(const double* __restrict__ v, double* host) {
+  if (threadIdx.x == 0) {
+    host[0] = v[0];
+    __threadfence_system();
+  }
+}
+
+int launch_publish(const double* v, double* host, hipStream_t s) {
+  publish_kernel<<<1, 64, 0, s>>>(v, host);
+  return check_launch("publish_kernel");
+}
+
 int launch_cg_x_update(long long n, const double* sc, const double* p, double* x, hipStream_t s) {
   cg_x_update<<<blocks_for(n, 256), 256, 0, s>>>(n, sc, p, x);
   return check_launch("cg_x_update");
