@@ -508,6 +508,7 @@ int pmt_block_pcg_solve(pmt_mg* mg, int k, const double* B, double* X, int x0_is
     return PMT_ERR_STATE;
   }
   hipStream_t caller = (hipStream_t)stream;
+  PMT_TRY(mg_join(mg, caller));  // the coarsest inverse of the last update, before anything is captured
   PMT_TRY(ensure_workspace(mg, k, caller));
   BlockCG* w = mg->bcg;
   // all work on the handle's stream, after what the caller queued before this call

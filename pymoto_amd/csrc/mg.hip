@@ -40,6 +40,7 @@ static int dalloc(pmt_mg* mg, T** p, long long n) {
 }
 
 static void release(pmt_mg* mg) {
+  if (mg->s2) (void)hipStreamSynchronize(mg->s2);  // an inverse that no solve has joined yet
   block_cg_release(mg);
   for (void* p : mg->allocs) (void)hipFree(p);
   mg->allocs.clear();
@@ -86,10 +87,12 @@ template <typename T>
 static int coarse_solve(pmt_mg* mg, const T* b, T* out, hipStream_t s);
 template <>
 int coarse_solve<double>(pmt_mg* mg, const double* b, double* out, hipStream_t s) {
+  PMT_TRY(mg_join(mg, s));
   return launch_gemv<double>(mg->nc, mg->Ainv, b, out, s);
 }
 template <>
 int coarse_solve<float>(pmt_mg* mg, const float* b, float* out, hipStream_t s) {
+  PMT_TRY(mg_join(mg, s));
   Level& C = mg->L[mg->nlev - 1];
   PMT_TRY(launch_to_double(mg->nc, b, C.d.b, s));
   PMT_TRY(launch_gemv<double>(mg->nc, mg->Ainv, C.d.b, C.d.u, s));
@@ -348,6 +351,13 @@ int mg_level_apply(pmt_mg* mg, int l, int mode, const double* u, const double* b
 
 int mg_precondition(pmt_mg* mg, const double* r, double* z, hipStream_t s) { return precondition(mg, r, z, s); }
 
+int mg_join(pmt_mg* mg, hipStream_t s) {
+  if (!mg->pending_join) return PMT_OK;
+  PMT_HIP(hipStreamWaitEvent(s, mg->ev_join, 0));
+  mg->pending_join = false;
+  return PMT_OK;
+}
+
 int mg_alloc(pmt_mg* mg, void** p, long long bytes) {
   char* q = nullptr;
   PMT_TRY(dalloc(mg, &q, bytes));
@@ -360,6 +370,7 @@ int mg_alloc(pmt_mg* mg, void** p, long long bytes) {
 // corrections, moments, then each level's stencil.
 static int hex_coarse_stencils(pmt_mg* mg, hipStream_t s) {
   const int nl = mg->nlev;
+  PMT_TRY(mg_join(mg, s));  // the previous inversion still reads the coarsest stencil rewritten below
   PMT_TRY(launch_level1_flags(mg->L[0].g, mg->bcm, mg->L[1].cflag, mg->L[1].list, mg->L[1].map, s));
   for (int l = 2; l < nl; ++l)
     PMT_TRY(launch_flags_up(mg->L[l - 1].g, mg->L[l - 1].cflag, mg->L[l].cflag, mg->L[l].list, mg->L[l].map, s));
@@ -417,7 +428,8 @@ static int hex_coarse_stencils(pmt_mg* mg, hipStream_t s) {
     return o;
   };
   // the coarsest level first; its dense inverse (a chain of small, latency-bound launches) then runs on
-  // the handle's second stream while this one builds the other levels
+  // the handle's second stream while this one builds the other levels and, after update, starts the
+  // solve (coarse_solve joins it)
   const int lc = nl - 1;
   Level& Lc = mg->L[lc];
   if (lc == 1)
@@ -439,7 +451,8 @@ static int hex_coarse_stencils(pmt_mg* mg, hipStream_t s) {
   PMT_TRY(launch_dense_from_stencil(Lc.g, Lc.d.S, mg->Ainv, mg->s2));
   PMT_TRY(gj_invert(mg->nc, mg->Ainv, mg->gjP, mg->gjC, mg->s2));
   PMT_HIP(hipEventRecord(mg->ev_join, mg->s2));
-  PMT_HIP(hipStreamWaitEvent(s, mg->ev_join, 0));
+  // joined by the first coarsest solve: the V-cycle descent before it hides the inversion
+  mg->pending_join = true;
   return PMT_OK;
 }
 

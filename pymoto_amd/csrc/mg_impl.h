@@ -89,6 +89,7 @@ struct pmt_mg {
   // handle, forked after the coarsest stencil and joined before update returns
   hipStream_t s2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool pending_join = false;  // the inverse is joined by the first coarsest solve (or mg_join) after update
   double* gjP = nullptr;
   double* gjC = nullptr;
   float* z32 = nullptr;  // FP32 V-cycle output on the fine grid
@@ -121,5 +122,8 @@ int mg_level_apply(pmt_mg* mg, int l, int mode, const double* u, const double* b
 int mg_precondition(pmt_mg* mg, const double* r, double* z, hipStream_t s);
 // device allocation owned (and freed) by the handle
 int mg_alloc(pmt_mg* mg, void** p, long long bytes);
+// make stream s wait for the coarsest inverse of the last update (no-op once joined); callers that
+// capture graphs call it before capturing
+int mg_join(pmt_mg* mg, hipStream_t s);
 
 }  // namespace pmt
