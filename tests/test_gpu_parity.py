@@ -323,6 +323,32 @@ def test_level1_without_stencil_matches_stencil(gpu_lib, nel, levels, bcs):
     assert relerr(out[1][3], out[0][3]) < 1e-10
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_coarsest_inverse_join_across_updates_and_streams(gpu_lib, prec):
+    """The coarsest inverse is built on the handle's second stream and joined by the first coarse
+    solve: two updates in a row with no solve between them, then a solve on another stream, give the
+    bits of a hierarchy updated once with the second operator and solved on the update stream."""
+    rng = np.random.default_rng(7)
+    nel = (32, 16, 16)
+    d = pm.VoxelDomain(*nel)
+    bc = problems.cantilever3d(*nel)[0]
+    Ka = pm.AssembleStiffness(d, bc=bc)(dev(fe.simp(rng.uniform(0.05, 1.0, d.nel))))
+    Kb = pm.AssembleStiffness(d, bc=bc)(dev(fe.simp(rng.uniform(0.05, 1.0, d.nel))))
+    r = dev(rng.standard_normal(Kb.shape[0]))
+    ref = pm.solvers.build_hierarchy(d, 4, omega=0.5, smooth_steps=2, precision=prec)
+    ref.update(Kb)
+    want = host(ref.solve(r))
+    mg = pm.solvers.build_hierarchy(d, 4, omega=0.5, smooth_steps=2, precision=prec)
+    mg.update(Ka)
+    mg.update(Kb)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        got = mg.solve(r)
+    torch.cuda.current_stream().wait_stream(side)
+    npt.assert_array_equal(host(got), want)
+
+
 @pytest.mark.parametrize("levels", [3, 4])
 def test_pcg_iterations_and_history_vs_reference(gpu_lib, levels):
     g = golden(f"c2_mgcg_32x16x16_L{levels}.npz")
