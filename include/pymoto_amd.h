@@ -208,7 +208,9 @@ int pmt_mg_set_smoother(pmt_mg* mg, int type, double lower, double upper);
 int pmt_mg_smoother_interval(pmt_mg* mg, int level, double* lo, double* hi);
 /* Rebuild diagonals, Galerkin coarse operators and the coarsest inverse for new densities xe
  * (GeometricMultigrid.update, iterative.py:168-176).  xe and bcmask must stay alive until the
- * next update. */
+ * next update.  Hex8 grids: the coarsest dense inverse may still be in progress on the handle's own
+ * stream when this returns; pmt_mg_vcycle, pmt_pcg_solve, pmt_block_pcg_solve and the next update
+ * make their stream wait for it. */
 int pmt_mg_update(pmt_mg* mg, const double* xe, const uint8_t* bcmask, double bcdiag, void* stream);
 /* z = M r, one V-cycle (GeometricMultigrid.solve with x0=None, iterative.py:222-256). */
 int pmt_mg_vcycle(pmt_mg* mg, const double* r, double* z, void* stream);
