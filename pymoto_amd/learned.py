@@ -17,7 +17,7 @@ about 0.66 M parameters.  The input channels are: density, force x/y/z, coordina
 ``WarmStartCG`` wraps a ``CG`` solver: it starts from the better of the network's guess and the
 caller's x0 (smaller residual), and falls back to the plain solve from the caller's x0 when the
 guarded first iterations do not reduce the residual at least as fast as the plain start would
-(the final FP64 residual tolerance is the judge either way).
+(either way the solve ends at the same FP64 residual tolerance).
 """
 import ctypes
 
