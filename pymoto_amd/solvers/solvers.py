@@ -112,7 +112,7 @@ class LDAWrapper(LinearSolver):
             if x0s is not None:
                 x0l = []
                 for k in todo:
-                    v, _ = self._split(_lib.as_device(x0s[k]).clone())
+                    v, _ = self._split(_lib.as_device(x0s[k]))  # new tensors: x0 is not modified
                     for x in self.x_stored:
                         _axpy(-_dot(v, x) / _dot(x, x), x, v)
                     x0l.append(v)
