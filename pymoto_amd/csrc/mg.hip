@@ -226,10 +226,16 @@ static int read_scalar(pmt_mg* mg, int k, double* v, hipStream_t s) {
 // for ~60 us per iteration): a one-thread kernel stores the value into pinned memory that the host
 // polls; a NaN sentinel marks "not yet written".  After 1 s without it the host falls back to the
 // synchronising read, so a value that never arrives cannot hang the solve.
-static int read_scalar_polled(pmt_mg* mg, int k, double* v, hipStream_t s) {
+// sc[k] published to pinned host memory by a one-thread kernel (publish_scalar), then read by polling
+// (wait_published) instead of synchronising the stream: work enqueued between the two keeps the GPU busy
+static int publish_scalar(pmt_mg* mg, int k, hipStream_t s) {
   volatile double* h = mg->host_rr;
   h[0] = std::numeric_limits<double>::quiet_NaN();
-  PMT_TRY(launch_publish(mg->sc + k, mg->host_rr, s));
+  return launch_publish(mg->sc + k, mg->host_rr, s);
+}
+
+static int wait_published(pmt_mg* mg, int k, double* v, hipStream_t s) {
+  volatile double* h = mg->host_rr;
   const auto t0 = std::chrono::steady_clock::now();
   for (;;) {
     const double x = h[0];
@@ -767,6 +773,7 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
   PMT_TRY(launch_dot(n, mg->z, mg->z, sc + 5, mg->work, s));
   PMT_TRY(launch_scale_rsqrt(n, sc, 5, mg->z, mg->p, s));
   int i = 0;
+  double tprev = 0.0;  // the relative residual one iteration before tval (0: none yet)
   for (i = 0; i < maxit; ++i) {
     // q = A p, sc[0] = p.q, sc[1] = p.r: fused into the reflection-basis operator when it is the fine
     // operator, else K.u and one two-dot pass
@@ -795,20 +802,31 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
                                   first ? F.f.dinv : nullptr, (float)F.omega, first ? F.f.t : nullptr));
       r32_ready = mixed;
     }
+    // z = M r, sc[3] = q.z, p = z + beta p: reads r and q, writes only z, sc[3] and p (no result)
+    auto next_direction = [&]() -> int {
+      if (mixed) {
+        PMT_TRY(precondition_f32(mg, mg->r, r32_ready, s, r32_ready && first));
+        PMT_TRY(launch_dot(n, mg->q, mg->z32, sc + 3, mg->work, s));
+        return launch_cg_p_update(n, sc, mg->z32, mg->p, s);
+      }
+      PMT_TRY(precondition(mg, mg->r, mg->z, s));
+      PMT_TRY(launch_dot(n, mg->q, mg->z, sc + 3, mg->work, s));
+      return launch_cg_p_update(n, sc, mg->z, mg->p, s);
+    };
+    PMT_TRY(publish_scalar(mg, 2, s));
+    // when the residual's rate of decrease says this iteration will not stop, the next direction is
+    // enqueued before the host waits for r.r, so the GPU stays busy meanwhile; if it does stop after all,
+    // that work is discarded (the iterates and the iteration count are the same either way)
+    const double rate = tprev > 0.0 ? tval / tprev : 0.3;
+    const bool ahead = tval * rate > 2.0 * tol;
+    if (ahead) PMT_TRY(next_direction());
     double rr = 0.0;
-    PMT_TRY(read_scalar_polled(mg, 2, &rr, s));
+    PMT_TRY(wait_published(mg, 2, &rr, s));
+    tprev = tval;
     tval = std::sqrt(rr) / bn;
     if (history) history[i + 1] = tval;
     if (tval <= tol) break;
-    if (mixed) {
-      PMT_TRY(precondition_f32(mg, mg->r, r32_ready, s, r32_ready && first));
-      PMT_TRY(launch_dot(n, mg->q, mg->z32, sc + 3, mg->work, s));
-      PMT_TRY(launch_cg_p_update(n, sc, mg->z32, mg->p, s));
-    } else {
-      PMT_TRY(precondition(mg, mg->r, mg->z, s));
-      PMT_TRY(launch_dot(n, mg->q, mg->z, sc + 3, mg->work, s));
-      PMT_TRY(launch_cg_p_update(n, sc, mg->z, mg->p, s));
-    }
+    if (!ahead) PMT_TRY(next_direction());
   }
   if (iters) *iters = i < maxit ? i + 1 : maxit;
   if (relres) *relres = tval;
