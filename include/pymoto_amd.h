@@ -26,6 +26,8 @@ extern "C" {
 #define PMT_ERR_ARG (-1)
 #define PMT_ERR_HIP (-2)
 #define PMT_ERR_STATE (-3)
+/* a non-finite right-hand side, start vector or residual (the solve stops at once) */
+#define PMT_ERR_NUMERIC (-4)
 
 /* Message of the last failing call on this thread. */
 const char* pmt_last_error(void);

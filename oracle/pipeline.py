@@ -49,3 +49,38 @@ def design_iteration(nel, x, bc, f, radius, xmin=1e-9, penal=3.0, plane="strain"
     if return_all:
         out.update(K=K, K0=K0, asm=asm, filt=filt)
     return out
+
+
+def heaviside(x, beta, eta):
+    """Heaviside projection of ``examples/topology_optimization/ex_compliance_robust.py:139`` (the
+    MathExpression ``(tanh(b*eta) + tanh(b*(x-eta))) / (tanh(b*eta) + tanh(b*(1-eta)))``) and its
+    derivative d/dx = b (1 - tanh^2(b (x - eta))) / (tanh(b*eta) + tanh(b*(1-eta)))."""
+    den = np.tanh(beta * eta) + np.tanh(beta * (1.0 - eta))
+    t = np.tanh(beta * (x - eta))
+    return (np.tanh(beta * eta) + t) / den, beta * (1.0 - t * t) / den
+
+
+def robust_iteration(nel, x, bc, F, radius, etas=(0.3, 0.5, 0.7), beta=8.0, xmin=1e-9, penal=3.0, E=1.0, nu=0.3):
+    """Robust multi-load design iteration: filter -> Heaviside at each eta -> SIMP -> K -> solve the k load
+    cases F (ndof, k) with the direct solve -> c = sum_j f_j.u_j (EinSum 'ij,ij->j' then 'i->') -> dc/dx
+    through the adjoint chain (self-adjoint: lambda = u, ``linalg.py:199-219``).  One entry per eta:
+    xp (projected), U, c and dcdx.  Chain of ``ex_compliance_robust.py:132-149`` with the multi-load
+    compliance of ``ex_compliant_mechanism.py:83``."""
+    nelx, nely, nelz = nel
+    dim = fe.grid_dim(nelx, nely, nelz)
+    filt = ConvFilter(nel, filter_weights(radius, nel), (("symmetric",) * 2,) * 3)
+    xf = filt.forward(x)
+    K0 = fe.element_stiffness(dim, E=E, nu=nu)
+    asm = fe.Assembly(nelx, nely, nelz, K0, bc=bc)
+    out = []
+    for eta in etas:
+        xp, dxp = heaviside(xf, beta, eta)
+        xs = fe.simp(xp, xmin, penal)
+        lu = DirectLU()
+        lu.update(asm.matrix(xs))
+        U = lu.solve(F)
+        c = float(np.sum(F * U))
+        dxs = asm.sens_contract([-U[:, j] for j in range(F.shape[1])], [U[:, j] for j in range(F.shape[1])])
+        dx = filt.adjoint(dxs * fe.dsimp(xp, xmin, penal) * dxp)
+        out.append(dict(eta=eta, xf=xf, xp=xp, xsimp=xs, U=U, c=c, dcdx=dx))
+    return out

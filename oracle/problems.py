@@ -38,3 +38,15 @@ def cantilever3d(nelx, nely, nelz):
     w[-1] *= 0.5
     f[node(nelx, nely, nelx, 0, kload) * 3 + 1] = -w
     return bc, f
+
+
+def face_point_loads(nelx, nely, nelz, nloads, seed):
+    """k unit downward (-y) point loads at seeded nodes (j, k) of the x = nelx face, one per column of
+    F (ndof, k): the load cases of the batched robust workload."""
+    rng = np.random.default_rng(seed)
+    nn = (nelx + 1) * (nely + 1) * (nelz + 1)
+    F = np.zeros((3 * nn, nloads))
+    for c in range(nloads):
+        j, k = int(rng.integers(0, nely + 1)), int(rng.integers(0, nelz + 1))
+        F[3 * node(nelx, nely, nelx, j, k) + 1, c] = -1.0
+    return F

@@ -85,6 +85,13 @@ class HIPError(RuntimeError):
     pass
 
 
+class NumericalError(HIPError, FloatingPointError):
+    """A solve met a non-finite right-hand side, start vector or residual (status PMT_ERR_NUMERIC)."""
+
+
+PMT_ERR_NUMERIC = -4
+
+
 def load(path: str = _LIBPATH):
     """Load the shared library and declare all signatures (no GPU needed)."""
     global _lib
@@ -115,7 +122,8 @@ def exported_symbols():
 def check(rc: int, what: str = ""):
     if rc != 0:
         msg = load().pmt_last_error().decode(errors="replace")
-        raise HIPError(f"{what or 'HIP call'} failed with status {rc}: {msg}")
+        cls = NumericalError if rc == PMT_ERR_NUMERIC else HIPError
+        raise cls(f"{what or 'HIP call'} failed with status {rc}: {msg}")
 
 
 def stream():

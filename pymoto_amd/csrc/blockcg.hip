@@ -257,11 +257,13 @@ __global__ void bcg_check_kernel(int k, int init, double tol, double* sm) {
     const double* BB = sm + Small::G0 + 2 * kMaxRhs * kMaxRhs;
     for (int j = 0; j < k; ++j) bn[j] = sqrt(BB[j * kMaxRhs + j]);
   }
+  // a non-finite column (NaN/inf in b, x0 or the iterates) makes the maximum NaN, which the host turns into
+  // an error; a zero column counts as converged
   double mx = 0.0;
   for (int j = 0; j < k; ++j) {
-    const double t = bn[j] > 0.0 ? sqrt(RR[j * kMaxRhs + j]) / bn[j] : 0.0;
+    const double t = bn[j] == 0.0 ? 0.0 : sqrt(RR[j * kMaxRhs + j]) / bn[j];
     sm[Small::TVAL + j] = t;
-    mx = fmax(mx, t);
+    mx = (t != t || mx != mx) ? t + mx : fmax(mx, t);
   }
   sm[Small::DONE] = mx <= tol ? 1.0 : 0.0;
   sm[Small::DONE + 1] = mx;
@@ -414,6 +416,7 @@ static int ensure_workspace(pmt_mg* mg, int k, hipStream_t s) {
   const int kc = kround(k);
   if (w) {
     PMT_HIP(hipStreamSynchronize(s));
+    mg_forget_stream(mg, w->cs);
     free_workspace(w);
     mg->bcg = nullptr;
   }
@@ -485,6 +488,7 @@ static int read_done(BlockCG* w, hipStream_t s, double* mx) {
 
 void block_cg_release(pmt_mg* mg) {
   if (!mg->bcg) return;
+  mg_forget_stream(mg, mg->bcg->cs);
   free_workspace(mg->bcg);
   mg->bcg = nullptr;
 }
@@ -508,13 +512,13 @@ int pmt_block_pcg_solve(pmt_mg* mg, int k, const double* B, double* X, int x0_is
     return PMT_ERR_STATE;
   }
   hipStream_t caller = (hipStream_t)stream;
-  PMT_TRY(mg_join(mg, caller));  // the coarsest inverse of the last update, before anything is captured
   PMT_TRY(ensure_workspace(mg, k, caller));
   BlockCG* w = mg->bcg;
   // all work on the handle's stream, after what the caller queued before this call
   hipStream_t s = w->cs;
   PMT_HIP(hipEventRecord(w->ev_in, caller));
   PMT_HIP(hipStreamWaitEvent(s, w->ev_in, 0));
+  PMT_TRY(mg_join(mg, s));  // the coarsest inverse of the last update, before anything is captured on s
   const long long n = w->n;
   const size_t bytes = (size_t)k * n * sizeof(double);
   PMT_HIP(hipMemcpyAsync(w->B, B, bytes, hipMemcpyDeviceToDevice, s));
@@ -533,6 +537,17 @@ int pmt_block_pcg_solve(pmt_mg* mg, int k, const double* B, double* X, int x0_is
   int done = read_done(w, s, &mx);
   if (done < 0) return PMT_ERR_HIP;
   if (history) history[0] = mx;
+  auto non_finite = [&](int at) -> int {
+    PMT_HIP(hipEventRecord(w->ev_out, s));
+    PMT_HIP(hipStreamWaitEvent(caller, w->ev_out, 0));
+    if (iters) *iters = at;
+    if (relres)
+      for (int j = 0; j < k; ++j) relres[j] = mx;
+    set_error("pmt_block_pcg_solve: non-finite residual at iteration " + std::to_string(at) +
+              " (non-finite right-hand side, start vector or operator)");
+    return PMT_ERR_NUMERIC;
+  };
+  if (!std::isfinite(mx)) return non_finite(0);
   int it = 0;
   if (!done) {
     // P = orth(Z, normalize=True), Z = M R
@@ -555,6 +570,7 @@ int pmt_block_pcg_solve(pmt_mg* mg, int k, const double* B, double* X, int x0_is
         PMT_TRY(iter_part1(mg, w, k, rs, tol, s));
       done = read_done(w, s, &mx);
       if (history) history[it + 1] = mx;
+      if (!std::isfinite(mx)) return non_finite(it + 1);
       if (done) break;
       if (graphs)
         PMT_HIP(hipGraphLaunch(w->exec[2], s));

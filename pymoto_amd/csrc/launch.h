@@ -95,7 +95,8 @@ int launch_cg_p_update(long long n, const double* sc, const double* z, double* p
 int launch_cg_p_update(long long n, const double* sc, const float* z, double* p, hipStream_t s);
 int launch_scale_rsqrt(long long n, const double* sc, int k, const double* z, double* p, hipStream_t s);
 int launch_fill(long long n, double v, double* y, hipStream_t s);
-int launch_publish(const double* v, double* host, hipStream_t s);  // *host = *v (pinned host memory)
+// host[0] = *v, then host[1] = seq (pinned host memory; the host polls host[1])
+int launch_publish(const double* v, double seq, double* host, hipStream_t s);
 int launch_to_double(long long n, const float* x, double* y, hipStream_t s);
 
 // coarse.hip

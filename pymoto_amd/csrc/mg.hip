@@ -222,25 +222,23 @@ static int read_scalar(pmt_mg* mg, int k, double* v, hipStream_t s) {
   return PMT_OK;
 }
 
-// The same for the CG stopping test without a stream synchronisation (whose wake-up left the GPU idle
-// for ~60 us per iteration): a one-thread kernel stores the value into pinned memory that the host
-// polls; a NaN sentinel marks "not yet written".  After 1 s without it the host falls back to the
-// synchronising read, so a value that never arrives cannot hang the solve.
-// sc[k] published to pinned host memory by a one-thread kernel (publish_scalar), then read by polling
-// (wait_published) instead of synchronising the stream: work enqueued between the two keeps the GPU busy
+// The CG stopping test without a stream synchronisation (whose wake-up left the GPU idle for ~60 us per
+// iteration): a one-thread kernel (publish_scalar) stores sc[k] and then a fresh sequence number into
+// pinned memory, and the host polls for that sequence number (wait_published).  Work enqueued between
+// the two keeps the GPU busy.  After 1 s without it the host falls back to the synchronising read, so a
+// value that never arrives cannot hang the solve.
 static int publish_scalar(pmt_mg* mg, int k, hipStream_t s) {
-  volatile double* h = mg->host_rr;
-  h[0] = std::numeric_limits<double>::quiet_NaN();
-  return launch_publish(mg->sc + k, mg->host_rr, s);
+  mg->pub_seq += 1.0;
+  return launch_publish(mg->sc + k, mg->pub_seq, mg->host_rr, s);
 }
 
 static int wait_published(pmt_mg* mg, int k, double* v, hipStream_t s) {
   volatile double* h = mg->host_rr;
+  const double want = mg->pub_seq;
   const auto t0 = std::chrono::steady_clock::now();
   for (;;) {
-    const double x = h[0];
-    if (!std::isnan(x)) {
-      *v = x;
+    if (h[1] == want) {
+      *v = h[0];
       return PMT_OK;
     }
     if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(1)) return read_scalar(mg, k, v, s);
@@ -357,11 +355,32 @@ int mg_level_apply(pmt_mg* mg, int l, int mode, const double* u, const double* b
 
 int mg_precondition(pmt_mg* mg, const double* r, double* z, hipStream_t s) { return precondition(mg, r, z, s); }
 
+// Every stream that reads the coarsest inverse after an update waits for its inversion once: the handle
+// lists the streams that have waited since the update (a stream ordered after a joined stream is not
+// thereby joined itself).  Once the inversion has finished nobody needs to wait and the list is dropped.
+// A stream under graph capture must already be listed (block CG joins its stream before capturing).
 int mg_join(pmt_mg* mg, hipStream_t s) {
   if (!mg->pending_join) return PMT_OK;
+  for (hipStream_t j : mg->joined)
+    if (j == s) return PMT_OK;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  PMT_HIP(hipStreamIsCapturing(s, &cap));
+  if (cap == hipStreamCaptureStatusNone && hipEventQuery(mg->ev_join) == hipSuccess) {
+    mg->pending_join = false;
+    mg->joined.clear();
+    return PMT_OK;
+  }
   PMT_HIP(hipStreamWaitEvent(s, mg->ev_join, 0));
-  mg->pending_join = false;
+  mg->joined.push_back(s);
   return PMT_OK;
+}
+
+void mg_forget_stream(pmt_mg* mg, hipStream_t s) {
+  for (size_t q = 0; q < mg->joined.size(); ++q)
+    if (mg->joined[q] == s) {
+      mg->joined.erase(mg->joined.begin() + q);
+      return;
+    }
 }
 
 int mg_alloc(pmt_mg* mg, void** p, long long bytes) {
@@ -457,8 +476,9 @@ static int hex_coarse_stencils(pmt_mg* mg, hipStream_t s) {
   PMT_TRY(launch_dense_from_stencil(Lc.g, Lc.d.S, mg->Ainv, mg->s2));
   PMT_TRY(gj_invert(mg->nc, mg->Ainv, mg->gjP, mg->gjC, mg->s2));
   PMT_HIP(hipEventRecord(mg->ev_join, mg->s2));
-  // joined by the first coarsest solve: the V-cycle descent before it hides the inversion
+  // joined by the first coarsest solve on each stream: the V-cycle descent before it hides the inversion
   mg->pending_join = true;
+  mg->joined.clear();
   return PMT_OK;
 }
 
@@ -759,14 +779,24 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
   PMT_HIP(hipMemcpyAsync(mg->host_sc + 4, sc + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
   PMT_HIP(hipStreamSynchronize(s));
   const double bn = std::sqrt(mg->host_sc[4]);
-  double tval = std::sqrt(mg->host_sc[5]) / bn;
-  if (history) history[0] = tval;
   if (iters) *iters = 0;
-  if (relres) *relres = tval;
-  if (!(bn > 0.0)) {  // zero right-hand side: x = 0 is the solution
+  if (!std::isfinite(bn)) {
+    if (relres) *relres = bn;
+    set_error("pmt_pcg_solve: the right-hand side has non-finite entries");
+    return PMT_ERR_NUMERIC;
+  }
+  if (bn == 0.0) {  // zero right-hand side: x = 0 is the solution
     PMT_TRY(launch_fill(n, 0.0, x, s));
+    if (history) history[0] = 0.0;
     if (relres) *relres = 0.0;
     return PMT_OK;
+  }
+  double tval = std::sqrt(mg->host_sc[5]) / bn;
+  if (history) history[0] = tval;
+  if (relres) *relres = tval;
+  if (!std::isfinite(tval)) {
+    set_error("pmt_pcg_solve: the initial residual is not finite (non-finite start vector or operator)");
+    return PMT_ERR_NUMERIC;
   }
   if (tval <= tol) return PMT_OK;
   PMT_TRY(precondition(mg, mg->r, mg->z, s));
@@ -825,6 +855,14 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
     tprev = tval;
     tval = std::sqrt(rr) / bn;
     if (history) history[i + 1] = tval;
+    if (!std::isfinite(tval)) {
+      PMT_HIP(hipStreamSynchronize(s));  // nothing enqueued ahead is left running on the caller's buffers
+      if (iters) *iters = i + 1;
+      if (relres) *relres = tval;
+      set_error("pmt_pcg_solve: the residual became non-finite at iteration " + std::to_string(i + 1) +
+                " (non-finite operator, preconditioner or breakdown)");
+      return PMT_ERR_NUMERIC;
+    }
     if (tval <= tol) break;
     if (!ahead) PMT_TRY(next_direction());
   }

@@ -89,7 +89,10 @@ struct pmt_mg {
   // handle, forked after the coarsest stencil and joined before update returns
   hipStream_t s2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  bool pending_join = false;  // the inverse is joined by the first coarsest solve (or mg_join) after update
+  // the inverse is waited for by every stream that runs a coarsest solve (or mg_join) after an update: the
+  // streams that have waited since the last update are listed, so each waits once and none is skipped
+  bool pending_join = false;
+  std::vector<hipStream_t> joined;
   double* gjP = nullptr;
   double* gjC = nullptr;
   float* z32 = nullptr;  // FP32 V-cycle output on the fine grid
@@ -102,7 +105,9 @@ struct pmt_mg {
   double* dotws = nullptr;  // partial sums of the fused fine K.u + dot products (2 x dotws_parts)
   long long dotws_parts = 0;
   double* host_sc = nullptr;
-  double* host_rr = nullptr;  // pinned, written by publish_kernel: the CG stopping test polls it
+  double* host_rr = nullptr;  // pinned [value, sequence number], written by publish_kernel: the CG stopping
+                              // test polls the sequence number
+  double pub_seq = 0.0;       // sequence number of the last publish
   long long bytes = 0;
   std::vector<void*> allocs;
   // multi-right-hand-side CG (blockcg.hip): workspace for kcap columns and captured iterations
@@ -125,5 +130,7 @@ int mg_alloc(pmt_mg* mg, void** p, long long bytes);
 // make stream s wait for the coarsest inverse of the last update (no-op once joined); callers that
 // capture graphs call it before capturing
 int mg_join(pmt_mg* mg, hipStream_t s);
+// drop a stream that is about to be destroyed from the list of joined streams
+void mg_forget_stream(pmt_mg* mg, hipStream_t s);
 
 }  // namespace pmt

@@ -253,17 +253,21 @@ int launch_cg_xr_update(long long n, const double* sc, const double* p, const do
   return check_launch("dot_stage2");
 }
 
-// v -> host-visible memory (pinned): a normal store followed by a system-scope fence, so a host thread
-// polling the value sees it without a stream synchronisation
-__global__ void publish_kernel(const double* __restrict__ v, double* host) {
+// v -> host-visible memory (pinned): host[0] = value, then (after a system-scope fence) host[1] = seq, so a
+// host thread that polls host[1] for the sequence number it expects reads a complete value in host[0]
+// without a stream synchronisation, whatever the value is (NaN included)
+__global__ void publish_kernel(const double* __restrict__ v, double seq, double* host) {
   if (threadIdx.x == 0) {
-    host[0] = v[0];
+    volatile double* h = host;
+    h[0] = v[0];
+    __threadfence_system();
+    h[1] = seq;
     __threadfence_system();
   }
 }
 
-int launch_publish(const double* v, double* host, hipStream_t s) {
-  publish_kernel<<<1, 64, 0, s>>>(v, host);
+int launch_publish(const double* v, double seq, double* host, hipStream_t s) {
+  publish_kernel<<<1, 64, 0, s>>>(v, seq, host);
   return check_launch("publish_kernel");
 }
 

@@ -165,32 +165,73 @@ class MathExpression(Module):
 
 
 class EinSum(Module):
-    """Dot product ``'i,i->'`` on the device (``generic.py:143-226``)."""
+    """The contractions of ``generic.py:143-226`` that the compliance workloads use, on the device:
+
+    * ``'i,i->'``   dot product (compliance ``f.u``, ``ex_compliance.py``);
+    * ``'ij,ij->j'`` column-wise dot products (compliance per load case, ``ex_compliant_mechanism.py:83``);
+    * ``'ij,ij->'``  sum over all columns;
+    * ``'i->'``      vector sum (volume, ``ex_compliance_robust.py``).
+
+    Sensitivities follow the adjoint einsum of ``generic.py:213-226``: for a dot product each input
+    receives the other times the output sensitivity (per column for ``'ij,ij->j'``), for a sum every
+    entry receives the output sensitivity.  Other expressions raise ``NotImplementedError``."""
+
+    _supported = {(("i", "i"), ""), (("ij", "ij"), "j"), (("ij", "ij"), ""), (("i",), "")}
 
     def __init__(self, expression: str):
         self.expr = expression
         cmd = expression.replace(" ", "").split("->")
         self.indices_in = cmd[0].split(",")
         self.indices_out = cmd[1] if "->" in expression else ""
-        if self.indices_in != ["i", "i"] or self.indices_out != "":
-            raise NotImplementedError(f"device EinSum supports 'i,i->' only, not '{expression}'")
+        if (tuple(self.indices_in), self.indices_out) not in self._supported:
+            raise NotImplementedError(f"device EinSum supports 'i,i->', 'ij,ij->j', 'ij,ij->' and 'i->', "
+                                      f"not '{expression}'")
 
-    def __call__(self, a, b):
-        u, v = _lib.as_device(a), _lib.as_device(b)
-        if u.shape != v.shape or u.ndim != 1:
-            raise ValueError("EinSum 'i,i->' needs two vectors of equal length")
+    @staticmethod
+    def _dot(u, v):
         out = torch.empty(1, dtype=torch.float64, device=u.device)
         _lib.check(_lib.lib().pmt_dot(u.numel(), _lib.ptr(u), _lib.ptr(v), _lib.ptr(out),
                                       _lib.ptr(_lib.reduce_work()), _lib.stream()), "dot")
-        return np.float64(out.item())
+        return out
+
+    @staticmethod
+    def _scaled(v, a):
+        d = torch.empty_like(v)
+        _lib.check(_lib.lib().pmt_axpby(v.numel(), float(a), _lib.ptr(v), 0.0, _lib.ptr(d), _lib.stream()), "scale")
+        return d
+
+    def __call__(self, *args):
+        ops = [_lib.as_device(a) for a in args]
+        if len(ops) == 1:  # 'i->'
+            u = ops[0]
+            if u.ndim != 1:
+                raise ValueError("EinSum 'i->' needs a vector")
+            return np.float64(self._dot(u, torch.ones_like(u)).item())
+        u, v = ops
+        if u.shape != v.shape or u.ndim != len(self.indices_in[0]):
+            raise ValueError(f"EinSum '{self.expr}' needs two operands of equal shape and rank "
+                             f"{len(self.indices_in[0])}")
+        if self.indices_out == "":
+            return np.float64(self._dot(u, v).item())
+        ut, vt = u.t().contiguous(), v.t().contiguous()  # column j contiguous
+        return torch.cat([self._dot(ut[j], vt[j]) for j in range(ut.shape[0])])
 
     def _sensitivity(self, df_in):
-        a, b = self.get_input_states(as_list=True)
-        lib = _lib.lib()
+        inps = self.get_input_states(as_list=True)
+        if len(inps) == 1:
+            u = _lib.as_device(inps[0])
+            d = self._scaled(torch.ones_like(u), float(df_in))
+            return d.cpu().numpy() if isinstance(inps[0], np.ndarray) else d
+        a, b = inps
         res = []
         for other, mine in ((b, a), (a, b)):
             o = _lib.as_device(other)
-            d = torch.empty_like(o)
-            _lib.check(lib.pmt_axpby(o.numel(), float(df_in), _lib.ptr(o), 0.0, _lib.ptr(d), _lib.stream()), "scale")
+            if self.indices_out == "j":
+                dfj = np.asarray(df_in.cpu().numpy() if isinstance(df_in, torch.Tensor) else df_in,
+                                 dtype=np.float64).reshape(-1)
+                ot = o.t().contiguous()
+                d = torch.stack([self._scaled(ot[j], dfj[j]) for j in range(ot.shape[0])], dim=1)
+            else:
+                d = self._scaled(o, float(df_in))
             res.append(d.cpu().numpy() if isinstance(mine, np.ndarray) else d)
         return res

@@ -31,3 +31,32 @@ def cantilever3d(domain: VoxelDomain):
     w[[0, -1]] *= 0.5
     f[domain.get_nodenumber(nelx, 0, k) * 3 + 1] = -w
     return bc, f
+
+
+def face_point_loads(domain: VoxelDomain, nloads: int, seed: int):
+    """``nloads`` unit downward (-y) point loads at seeded nodes (j, k) of the x = nelx face, one per
+    column of F (ndof, nloads): the load cases of the batched robust workload."""
+    rng = np.random.default_rng(seed)
+    F = np.zeros((3 * domain.nnodes, nloads))
+    for c in range(nloads):
+        j, k = int(rng.integers(0, domain.nely + 1)), int(rng.integers(0, domain.nelz + 1))
+        F[3 * domain.get_nodenumber(domain.nelx, j, k) + 1, c] = -1.0
+    return F
+
+
+def uneven_batch_shapes(nproblems: int, seed: int, reduced: bool = False):
+    """Grid shapes of the uneven batch of independent problems, drawn from ``seed``: half-and-half Q4 beams
+    (nx x nx/2) and hex8 cantilevers (nx x nx/2 x nx/2), from 64x32 Q4 to 96x48x48 hex8 (``reduced``: from
+    16x8 Q4 to 24x12x12 hex8, the sizes the parity tests solve directly on the host)."""
+    rng = np.random.default_rng(seed)
+    q4 = [16, 24, 32, 48, 64] if reduced else [64, 96, 128, 160, 192, 256]
+    hx = [8, 12, 16, 20, 24] if reduced else [32, 48, 64, 80, 96]
+    out = []
+    for _ in range(nproblems):
+        if rng.random() < 0.5:
+            nx = int(rng.choice(q4))
+            out.append((nx, nx // 2, 0))
+        else:
+            nx = int(rng.choice(hx))
+            out.append((nx, nx // 2, nx // 2))
+    return out

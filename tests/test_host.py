@@ -273,3 +273,11 @@ def test_vti_writer_matches_oracle_bytes_and_decodes(tmp_path):
         npt.assert_allclose(u[:, : d.dim].ravel(), vecs["u"].astype(np.float32))
         npt.assert_allclose(arrays["U(1)"][1].reshape(-1, 3)[:, : d.dim].ravel(), vecs["U"][:, 1].astype(np.float32))
         npt.assert_allclose(arrays["S(1)"][1], vecs["S"][1].astype(np.float32))
+
+
+def test_face_point_loads_match_oracle():
+    from pymoto_amd import problems as pp
+    d = pm.VoxelDomain(12, 6, 4)
+    npt.assert_array_equal(pp.face_point_loads(d, 4, 7), problems.face_point_loads(12, 6, 4, 4, 7))
+    F = pp.face_point_loads(d, 4, 7)
+    assert np.all(np.sum(F, axis=0) == -1.0) and np.count_nonzero(F) == 4

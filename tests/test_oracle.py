@@ -305,3 +305,43 @@ def test_density_filter_oracle_interior_matches_convolution_and_is_self_adjoint(
         assert abs(F.H - F.H.T).max() == 0.0
         dy = rng.random(x.size)
         npt.assert_allclose(np.dot(F.forward(x), dy), np.dot(x, F.adjoint(dy)), rtol=1e-12)
+
+
+def test_heaviside_oracle_known_values_and_derivative():
+    from oracle.pipeline import heaviside
+    x = np.linspace(0.0, 1.0, 41)
+    for eta in (0.3, 0.5, 0.7):
+        y, dy = heaviside(x, 8.0, eta)
+        npt.assert_allclose([y[0], y[-1]], [0.0, 1.0], atol=1e-15)  # end points are fixed for every beta, eta
+        assert np.all(np.diff(y) > 0)
+        e = 1e-6
+        npt.assert_allclose(dy, (heaviside(x + e, 8.0, eta)[0] - heaviside(x - e, 8.0, eta)[0]) / (2 * e), rtol=1e-7)
+    # eta = 0.5 is symmetric about (0.5, 0.5); beta -> 0 tends to the identity
+    y, _ = heaviside(x, 8.0, 0.5)
+    npt.assert_allclose(y + y[::-1], 1.0, atol=1e-14)
+    npt.assert_allclose(heaviside(x, 1e-4, 0.3)[0], x, atol=1e-7)
+
+
+def test_robust_iteration_oracle_vs_finite_differences():
+    """c = sum_j f_j.u_j and its gradient through filter, Heaviside, SIMP and the multi-load solve,
+    against central differences of c on a 4x2x2 grid (pins every factor of the adjoint chain)."""
+    from oracle.pipeline import robust_iteration
+    nel = (4, 2, 2)
+    bc = problems.cantilever3d(*nel)[0]
+    F = problems.face_point_loads(*nel, 3, 5)
+    x = np.random.default_rng(2).uniform(0.2, 0.9, 16)
+    out = robust_iteration(nel, x, bc, F, radius=1.5, etas=(0.3, 0.7), beta=4.0)
+    for o in out:
+        U = o["U"]
+        K = fe.Assembly(*nel, fe.element_stiffness(3), bc).matrix(o["xsimp"])
+        npt.assert_allclose(np.sum(U * (K @ U)), o["c"], rtol=1e-12)  # c = sum_j u_j^T K u_j
+    e = 1e-6
+    for i in (0, 5, 11, 15):
+        xp, xm = x.copy(), x.copy()
+        xp[i] += e
+        xm[i] -= e
+        cp = robust_iteration(nel, xp, bc, F, radius=1.5, etas=(0.3, 0.7), beta=4.0)
+        cm = robust_iteration(nel, xm, bc, F, radius=1.5, etas=(0.3, 0.7), beta=4.0)
+        for k, o in enumerate(out):
+            fd = (cp[k]["c"] - cm[k]["c"]) / (2 * e)
+            npt.assert_allclose(o["dcdx"][i], fd, rtol=2e-6)
