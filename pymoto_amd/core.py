@@ -1,54 +1,132 @@
-"""Dataflow graph: Signal, Module, Network.
+"""Dataflow graph of the hot path: ``Signal``, ``SignalSlice``, ``Module``, ``Network``.
 
-Host-side mirror of pyMOTO's core contract (``pymoto/core_objects.py:71-998``) so that
-the device modules of this package can be composed exactly like pyMOTO modules:
+The contract is pyMOTO's Module API (``pymoto/core_objects.py:71-998``); the mechanics here are
+this package's own:
 
-* ``Signal`` carries ``state`` and ``sensitivity``; sensitivities accumulate through
-  ``add_sensitivity`` (deep copy on first use, ``+=`` afterwards), ``reset`` zeroes or
-  clears them (``core_objects.py:107-165``).
-* ``Module.__call__(*signals)`` connects the module, evaluates the response once, creates
-  ``"{Class}_output{i}"`` signals and appends the module to every active ``Network``
-  (``core_objects.py:481-548``). ``response()`` re-evaluates, ``sensitivity()`` back-propagates
-  and is skipped when every output sensitivity is ``None`` (``core_objects.py:625-674``).
-* ``Network`` runs responses in order and sensitivities in reverse (``core_objects.py:803-829``).
+* A module class writes its response as ``__call__(self, *inputs)`` and its adjoint as
+  ``_sensitivity(self, *d_outputs)``, positional parameters only.  The metaclass ``_ModuleMeta``
+  moves the class's ``__call__`` to ``_evaluate`` when the class is created (after checking the
+  signature), and the single ``Module.__call__`` below wires signals around it: calling a module
+  with at least one signal (or with nothing: a source) records the inputs, evaluates once, creates
+  ``"{Class}_output{i}"`` signals and appends the module to every open ``Network``; calling it with
+  plain values only just evaluates.
+* A module object is wired once.  Wired modules are kept in a weak set (identity, not equality), so
+  a second call with signals raises ``RuntimeError`` while a deep copy is a new, unwired module.
+* ``sensitivity()`` is skipped when every output sensitivity is ``None``; results are accumulated
+  into the input signals by ``add_sensitivity``, which stores a private copy of the first term so
+  that two inputs never share one array.
+* Anything with ``state/sensitivity/add_sensitivity/reset`` is a signal and anything with
+  ``response/sensitivity/reset`` is a module, so reference pyMOTO objects mix with these.
 
-Device data (``torch`` tensors on the GPU) lives in ``Signal.state`` unchanged; nothing in this
-file touches a device.  Duck typing is kept: any object with ``state/sensitivity/add_sensitivity/reset``
-is a signal and any object with ``response/sensitivity/reset`` is a module, so modules of this
-package can be appended to a pyMOTO ``Network`` and vice versa (``core_objects.py:360-381``).
+Device tensors (``torch`` on the GPU) pass through unchanged; nothing here touches a device.
 """
 import copy
-import inspect
 import sys
 import time
 import warnings
-from abc import ABC, abstractmethod
-from typing import Any, List
+import weakref
+from abc import ABCMeta, abstractmethod
+from inspect import Parameter, signature
+from typing import Any, List, Optional
+
+import numpy as np
+
+__all__ = ["Signal", "SignalSlice", "Module", "Network", "make_signals"]
 
 
-def _parse_to_list(*args: Any) -> list:
-    """Normalise ``None``/scalar/tuple/set/list arguments to a list (``pymoto/utils.py:5-20``)."""
-    if len(args) == 0:
+# --------------------------------------------------------------------------------------------
+# helpers
+
+def _as_list(value: Any) -> list:
+    """``None`` -> [], list -> itself, tuple/set -> list, anything else -> [value]."""
+    if value is None:
         return []
-    var_in = args[0] if len(args) == 1 else args
-    if var_in is None:
-        return []
-    if isinstance(var_in, list):
-        return var_in
-    if isinstance(var_in, (tuple, set)):
-        return list(var_in)
-    return [var_in]
+    if isinstance(value, list):
+        return value
+    if isinstance(value, (tuple, set)):
+        return list(value)
+    return [value]
 
 
-def _caller_location() -> str:
-    for fr in inspect.stack():
-        if fr.filename != __file__:
-            return f'File "{fr.filename}", line {fr.lineno}, in {fr.function}'
-    return "N/A"
+def _is_signal(obj: Any) -> bool:
+    return isinstance(obj, Signal) or all(hasattr(obj, a) for a in ("state", "sensitivity", "add_sensitivity", "reset"))
 
+
+def _is_module(obj: Any) -> bool:
+    return isinstance(obj, Module) or all(hasattr(obj, a) for a in ("response", "sensitivity", "reset"))
+
+
+def _root(sig: Any) -> Any:
+    """The signal that owns the data of ``sig`` (a slice points at its base)."""
+    while isinstance(sig, SignalSlice):
+        sig = sig.base
+    return sig
+
+
+def _signal_set(sigs: Any) -> set:
+    return {_root(s) for s in _as_list(sigs) if _is_signal(s)}
+
+
+def _where() -> str:
+    """File and line of the first caller outside this file (for error messages)."""
+    frame = sys._getframe(1)
+    while frame is not None and frame.f_code.co_filename == __file__:
+        frame = frame.f_back
+    return "unknown location" if frame is None else f"{frame.f_code.co_filename}:{frame.f_lineno}"
+
+
+def _rethrow(exc: BaseException, context: str):
+    """Raise an exception of the same type with ``context`` appended to its message."""
+    try:
+        new = type(exc)(f"{exc}{context}")
+    except Exception:  # exception types whose constructor takes other arguments
+        raise exc
+    raise new.with_traceback(exc.__traceback__) from None
+
+
+def _arity(fn) -> tuple:
+    """(minimum, maximum or None) positional arguments of ``fn``; keyword-only parameters and
+    ``**kwargs`` are refused (``core_objects.py:408-448``)."""
+    lo, hi = 0, 0
+    for p in signature(fn).parameters.values():
+        if p.kind in (Parameter.POSITIONAL_ONLY, Parameter.POSITIONAL_OR_KEYWORD):
+            hi = None if hi is None else hi + 1
+            lo += p.default is Parameter.empty
+        elif p.kind == Parameter.VAR_POSITIONAL:
+            hi = None
+        else:
+            raise SyntaxError(f"{getattr(fn, '__qualname__', fn)}: only positional parameters are allowed, "
+                              f"'{p}' is not")
+    return lo, hi
+
+
+def _check_count(fn, n: int, what: str):
+    lo, hi = _arity(fn)
+    if n < lo or (hi is not None and n > hi):
+        want = f"{lo}" if hi == lo else f"{lo}..{'any' if hi is None else hi}"
+        raise TypeError(f"{getattr(fn, '__qualname__', fn)} takes {want} {what}, {n} would be passed")
+
+
+def _zeros_like(state: Any) -> Any:
+    if state is None:
+        raise TypeError("a signal without a state has no shape to hold a sensitivity")
+    if hasattr(state, "new_zeros"):  # torch tensor
+        return state.new_zeros(state.shape)
+    try:
+        return np.zeros_like(state) if isinstance(state, np.ndarray) else state * 0
+    except TypeError:
+        raise TypeError(f"cannot create a zero sensitivity for a state of type {type(state).__name__}") from None
+
+
+# --------------------------------------------------------------------------------------------
+# signals
 
 class Signal:
-    """State/sensitivity carrier (``core_objects.py:71-195``)."""
+    """Carries a ``state`` and its ``sensitivity`` between modules (``core_objects.py:71-195``).
+
+    ``keep_alloc`` (default: whether a sensitivity was given at construction) makes ``reset()``
+    zero the sensitivity in place instead of dropping it.
+    """
 
     def __init__(self, tag: str = "", state: Any = None, sensitivity: Any = None, min: Any = None, max: Any = None):
         self.tag = tag
@@ -57,212 +135,186 @@ class Signal:
         self.min = min
         self.max = max
         self.keep_alloc = sensitivity is not None
-        self._init_loc = _caller_location()
-
-    def _err_str(self):
-        return f"\n\t| Signal '{self.tag}', initialized in {self._init_loc}"
+        self._origin = _where()
 
     def add_sensitivity(self, ds: Any):
-        """Accumulate a sensitivity term: deep copy on first use, in-place add afterwards."""
+        """Accumulate ``ds``: a private copy of the first term, in-place addition afterwards."""
+        if ds is None:
+            return self
         try:
-            if ds is None:
-                return
             if self.sensitivity is None:
                 self.sensitivity = copy.deepcopy(ds)
-            elif hasattr(self.sensitivity, "add_sensitivity"):
+            elif hasattr(self.sensitivity, "add_sensitivity"):  # e.g. a dyadic carrier
                 self.sensitivity.add_sensitivity(ds)
             else:
                 self.sensitivity += ds
-            return self
-        except TypeError:
-            if isinstance(ds, type(self.sensitivity)):
-                raise TypeError(f"Cannot add to the sensitivity with type '{type(self.sensitivity).__name__}'"
-                                + self._err_str())
-            raise TypeError(f"Adding wrong type '{type(ds).__name__}' to the sensitivity "
-                            f"'{type(self.sensitivity).__name__}'" + self._err_str())
-        except ValueError:
-            s1 = getattr(self.sensitivity, "shape", ())
-            s2 = getattr(ds, "shape", ())
-            raise ValueError(f"Cannot add argument of shape {s2} to the sensitivity of shape {s1}"
-                             + self._err_str()) from None
-
-    def reset(self, keep_alloc: bool = None):
-        """Zero the sensitivity (keeping its allocation) or drop it."""
-        if self.sensitivity is None:
-            return self
-        if keep_alloc is None:
-            keep_alloc = self.keep_alloc
-        if keep_alloc:
-            try:
-                self.sensitivity[...] = 0
-            except TypeError:
-                try:
-                    self.sensitivity *= 0
-                except TypeError:
-                    self.sensitivity = None
-        else:
-            self.sensitivity = None
+        except (TypeError, ValueError) as e:
+            _rethrow(e, f"\n  while adding a {type(ds).__name__} to the sensitivity "
+                        f"({type(self.sensitivity).__name__}) of signal '{self.tag}' created at {self._origin}")
         return self
 
-    def __getitem__(self, item):
-        return SignalSlice(self, item)
+    def reset(self, keep_alloc: Optional[bool] = None):
+        """Zero the sensitivity in place (``keep_alloc``) or drop it."""
+        if self.sensitivity is None:
+            return self
+        if not (self.keep_alloc if keep_alloc is None else keep_alloc):
+            self.sensitivity = None
+            return self
+        try:
+            self.sensitivity[...] = 0  # arrays and tensors keep their memory
+            return self
+        except (TypeError, AttributeError, IndexError):
+            pass
+        try:
+            value = self.sensitivity
+            value *= 0  # numbers (rebinds) and objects with in-place scaling
+            self.sensitivity = value
+            return self
+        except TypeError:
+            pass
+        warnings.warn(f"signal '{self.tag}': a sensitivity of type {type(self.sensitivity).__name__} cannot be "
+                      "zeroed in place; it is dropped instead", RuntimeWarning)
+        self.sensitivity = None
+        return self
+
+    def __getitem__(self, index):
+        return SignalSlice(self, index)
 
     def __repr__(self):
-        st = "empty state" if self.state is None else f"state of type {type(self.state).__name__}"
-        return f'Signal "{self.tag}" with {st}'
+        what = "no state" if self.state is None else f"a {type(self.state).__name__} state"
+        return f"Signal '{self.tag}' with {what}"
 
 
 class SignalSlice(Signal):
-    """A view into part of a base signal (``core_objects.py:198-315``)."""
+    """View of part of a signal: reads and writes go to the base signal's arrays
+    (``core_objects.py:198-315``)."""
 
-    def __init__(self, base, sl, tag=None):
+    def __init__(self, base: Signal, index, tag: Optional[str] = None):
         self.base = base
-        self.slice = sl
+        self.slice = index
         self.keep_alloc = False
-        self.tag = f"{base.tag}[{sl}]" if tag is None else tag
-        self._init_loc = _caller_location()
+        self.tag = f"{base.tag}[{index}]" if tag is None else tag
+        self.min = self.max = None
+        self._origin = _where()
 
     @property
     def state(self):
         return None if self.base.state is None else self.base.state[self.slice]
 
     @state.setter
-    def state(self, new_state):
-        self.base.state[self.slice] = new_state
+    def state(self, value):
+        self.base.state[self.slice] = value
 
     @property
     def sensitivity(self):
         return None if self.base.sensitivity is None else self.base.sensitivity[self.slice]
 
     @sensitivity.setter
-    def sensitivity(self, new_sens):
+    def sensitivity(self, value):
         if self.base.sensitivity is None:
-            if new_sens is None:
+            if value is None:
                 return
-            self.base.sensitivity = self.base.state * 0
-        self.base.sensitivity[self.slice] = 0 if new_sens is None else new_sens
+            self.base.sensitivity = _zeros_like(self.base.state)
+        self.base.sensitivity[self.slice] = 0 if value is None else value
 
     def add_sensitivity(self, ds: Any):
         if ds is None:
-            return
+            return self
         if self.base.sensitivity is None:
-            self.base.sensitivity = self.base.state * 0
+            self.base.sensitivity = _zeros_like(self.base.state)
         self.base.sensitivity[self.slice] += ds
         return self
 
-    def reset(self, keep_alloc: bool = None):
-        if self.sensitivity is not None:
-            self.sensitivity = None
+    def reset(self, keep_alloc: Optional[bool] = None):
+        """Zero this part of the base sensitivity (the base keeps its array)."""
+        if self.base.sensitivity is not None:
+            self.base.sensitivity[self.slice] = 0
         return self
 
 
-def make_signals(*args: str, **kwargs: Any):
-    """Batch-create signals by tag (``core_objects.py:318-357``)."""
-    ret_args = tuple(Signal(tag) for tag in args)
-    ret_kwargs = {tag: Signal(tag, val) for tag, val in kwargs.items()}
-    if args and kwargs:
-        return ret_args, ret_kwargs
-    if args:
-        return ret_args[0] if len(ret_args) == 1 else ret_args
-    return ret_kwargs
+def make_signals(*tags: str, **tagged_states: Any):
+    """``make_signals('a', 'b')`` -> two signals; ``make_signals(a=1.0)`` -> {tag: signal with state};
+    both forms together -> (tuple, dict) (``core_objects.py:318-357``)."""
+    plain = tuple(Signal(t) for t in tags)
+    named = {t: Signal(t, v) for t, v in tagged_states.items()}
+    if tags and tagged_states:
+        return plain, named
+    if tagged_states:
+        return named
+    return plain[0] if len(plain) == 1 else plain
 
 
-def _is_valid_signal(sig: Any) -> bool:
-    return isinstance(sig, Signal) or all(hasattr(sig, f) for f in ("state", "sensitivity", "add_sensitivity", "reset"))
+# --------------------------------------------------------------------------------------------
+# modules
+
+_WIRED = weakref.WeakSet()  # module objects that have been wired to signals (identity)
 
 
-def _is_valid_module(mod: Any) -> bool:
-    return isinstance(mod, Module) or all(hasattr(mod, f) for f in ("response", "sensitivity", "reset"))
+class _ModuleMeta(ABCMeta):
+    """Moves a module class's ``__call__`` (its response) to ``_evaluate`` and checks that the
+    response and ``_sensitivity`` take positional parameters only."""
+
+    def __new__(mcs, name, bases, ns, **kw):
+        if any(isinstance(b, _ModuleMeta) for b in bases):  # not the root class Module itself
+            response = ns.pop("__call__", None)
+            if response is not None:
+                _arity(response)
+                ns["_evaluate"] = response
+        if "_sensitivity" in ns:
+            _arity(ns["_sensitivity"])
+        return super().__new__(mcs, name, bases, ns, **kw)
 
 
-def _check_function_signature(fn, n_args: int = None):
-    """Positional-only contract for ``__call__``/``_sensitivity`` (``core_objects.py:408-448``)."""
-    min_args, max_args = 0, 0
-    name = f"{type(getattr(fn, '__self__', None)).__name__}.{fn.__name__}"
-    for p in inspect.signature(fn).parameters.values():
-        if p.kind in (p.POSITIONAL_ONLY, p.POSITIONAL_OR_KEYWORD):
-            if p.default is p.empty:
-                min_args += 1
-            if max_args is not None:
-                max_args += 1
-        elif p.kind == p.VAR_POSITIONAL:
-            max_args = None
-        elif p.kind == p.KEYWORD_ONLY:
-            raise SyntaxError(f'{name} may not contain keyword-only arguments "{p}"')
-        elif p.kind == p.VAR_KEYWORD:
-            raise SyntaxError(f'{name} may not contain dict of keyword-only arguments "{p}"')
-    if n_args is not None:
-        if n_args < min_args:
-            raise TypeError(f"Not enough arguments ({n_args}) for {name}; expected {min_args}")
-        if max_args is not None and n_args > max_args:
-            raise TypeError(f"Too many arguments ({n_args}) for {name}; expected {max_args}")
-    return min_args, max_args
+class Module(metaclass=_ModuleMeta):
+    """A node of the graph.  Subclasses define ``__call__`` (response from input states) and
+    optionally ``_sensitivity`` (input sensitivities from output sensitivities) and ``_reset``.
+    Subclasses need not call ``Module.__init__``."""
 
+    sig_in: Optional[List] = None
+    sig_out: Optional[List] = None
 
-class Module(ABC):
-    """Base class: ``__call__`` computes the response, ``_sensitivity`` its adjoint."""
+    @abstractmethod
+    def _evaluate(self, *states):
+        """The subclass's ``__call__``."""
 
-    sig_in: List = None
-    sig_out: List = None
-    _init_loc: str = None
-
-    def __init_subclass__(cls, **kwargs):
-        super().__init_subclass__(**kwargs)
-        fwd = cls.__dict__.get("__call__", None)
-        if fwd is None or getattr(fwd, "__isabstractmethod__", False) or getattr(fwd, "_is_wrapped", False):
-            return
-        _check_function_signature(fwd)
-        cls._orig_call = fwd
-        cls._response = cls._wrap_connect(fwd, create=False)
-        cls.__call__ = cls._wrap_connect(fwd, create=True)
-
-    @classmethod
-    def _wrap_connect(cls, response, create=True):
-        def wrapped(self, *args):
-            inp = [s.state if _is_valid_signal(s) else s for s in args]
-            all_constant = len(args) > 0 and not any(_is_valid_signal(s) for s in args)
-            self._init_loc = _caller_location()
-            if create and not all_constant:
-                self.sig_in = _parse_to_list(args)
-            out = response(self, *inp)
-            if all_constant:
-                return out
-            state_out = _parse_to_list(out)
-            if create:
-                if self.sig_out is None:
-                    self.sig_out = [Signal(f"{type(self).__name__}_output{i}") for i in range(len(state_out))]
-                _check_function_signature(self._sensitivity, self.n_out)
-                for n in Network.active:
-                    if self in n.mods:
-                        raise RuntimeError("Module already in network, cannot add twice.")
-                    n.append(self)
-            if len(state_out) != self.n_out:
-                raise TypeError(f"Number of responses calculated ({len(state_out)}) is unequal to number of "
-                                f"output signals ({self.n_out})")
-            for i, val in enumerate(state_out):
-                self.sig_out[i].state = val
-            if self.n_out == 0:
-                return None
-            return self.sig_out[0] if self.n_out == 1 else tuple(self.sig_out)
-        wrapped._is_wrapped = True
-        return wrapped
+    # -- wiring ------------------------------------------------------------------------------
+    def __call__(self, *args):
+        if args and not any(_is_signal(a) for a in args):
+            return self._evaluate(*args)  # plain values: a function call, no wiring
+        if self in _WIRED:
+            raise RuntimeError(f"{type(self).__name__} at {hex(id(self))} is already connected; "
+                               "create another module (or a deep copy) for other signals")
+        self.connect(list(args))
+        for net in Network._open:
+            net.append(self)
+        if not self.sig_out:
+            return None
+        return self.sig_out[0] if len(self.sig_out) == 1 else tuple(self.sig_out)
 
     def connect(self, sig_in, sig_out=None):
-        """Connect signals without adding to an active network."""
-        self.sig_in = _parse_to_list(sig_in)
-        inp = [s.state if _is_valid_signal(s) else s for s in self.sig_in]
-        out = _parse_to_list(self._orig_call(*inp))
-        if sig_out is not None:
-            self.sig_out = _parse_to_list(sig_out)
-            if self.n_out != len(out):
-                raise TypeError(f"Number of responses calculated ({len(out)}) is unequal to number of "
-                                f"output signals ({self.n_out})")
+        """Wire ``sig_in`` (and ``sig_out``, default new signals), evaluate once, without adding the
+        module to an open network.  Returns the module."""
+        self._origin = _where()
+        self.sig_in = _as_list(sig_in)
+        values = _as_list(self._evaluate(*self._input_states()))
+        if sig_out is None:
+            self.sig_out = [Signal(f"{type(self).__name__}_output{i}") for i in range(len(values))]
         else:
-            self.sig_out = [Signal(f"{type(self).__name__}_output{i}") for i in range(len(out))]
-        _check_function_signature(self._sensitivity, self.n_out)
-        for i, val in enumerate(out):
-            self.sig_out[i].state = val
+            self.sig_out = _as_list(sig_out)
+        _check_count(self._sensitivity, len(self.sig_out), "output sensitivities")
+        self._store(values)
+        _WIRED.add(self)
         return self
+
+    def _input_states(self) -> list:
+        return [s.state if _is_signal(s) else s for s in self.sig_in]
+
+    def _store(self, values: list):
+        if len(values) != len(self.sig_out):
+            raise TypeError(f"{type(self).__name__} returned {len(values)} values for {len(self.sig_out)} "
+                            "output signals")
+        for s, v in zip(self.sig_out, values):
+            s.state = v
 
     @property
     def n_in(self):
@@ -272,136 +324,152 @@ class Module(ABC):
     def n_out(self):
         return None if self.sig_out is None else len(self.sig_out)
 
-    def _err_str(self):
-        ins = "Unconnected" if self.sig_in is None else ", ".join(str(getattr(s, "tag", type(s).__name__))
-                                                                  for s in self.sig_in)
-        outs = "Unconnected" if self.sig_out is None else ", ".join(str(getattr(s, "tag", type(s).__name__))
-                                                                    for s in self.sig_out)
-        return f"\n\t| Module '{type(self).__name__}'( {ins} ) --> {outs}\n\t| Used in {self._init_loc}"
+    def _describe(self) -> str:
+        def tags(sigs):
+            if sigs is None:
+                return "(not connected)"
+            return ", ".join(str(getattr(s, "tag", s)) for s in sigs)
+        return (f"\n  in module {type(self).__name__}({tags(self.sig_in)}) -> {tags(self.sig_out)}, "
+                f"connected at {getattr(self, '_origin', 'unknown location')}")
 
+    # -- evaluation --------------------------------------------------------------------------
     def response(self):
-        """Recompute outputs from the current input states."""
+        """Re-evaluate the outputs from the current input states."""
         try:
-            self._response(*self.sig_in)
-            return self
+            self._store(_as_list(self._evaluate(*self._input_states())))
         except Exception as e:
-            raise type(e)(str(e) + "\n\t| Above error was raised when calling response(). Module details:"
-                          + self._err_str()).with_traceback(sys.exc_info()[2])
+            _rethrow(e, self._describe())
+        return self
 
     def sensitivity(self):
-        """Back-propagate output sensitivities into the input signals."""
-        try:
-            sens_in = self.get_output_sensitivities(as_list=True)
-            has_in = self.sig_in is not None and any(_is_valid_signal(s) for s in self.sig_in)
-            has_out = self.sig_out is not None and any(_is_valid_signal(s) for s in self.sig_out)
-            if not (has_in or has_out):
-                return
-            if self.n_out > 0 and all(s is None for s in sens_in):
-                return
-            sens_out = _parse_to_list(self._sensitivity(*sens_in))
-            if len(sens_out) != self.n_in:
-                raise TypeError(f"Number of sensitivities calculated ({len(sens_out)}) is unequal to number of "
-                                f"input signals ({self.n_in})")
-            for i, ds in enumerate(sens_out):
-                if _is_valid_signal(self.sig_in[i]):
-                    self.sig_in[i].add_sensitivity(ds)
+        """Add this module's contribution to the input sensitivities."""
+        if not self.sig_in or not any(_is_signal(s) for s in self.sig_in):
+            return self  # nothing to propagate into
+        d_out = [getattr(s, "sensitivity", None) for s in self.sig_out or []]
+        if d_out and all(d is None for d in d_out):
             return self
+        try:
+            d_in = self._sensitivity(*d_out)
+            d_in = list(d_in) if isinstance(d_in, (list, tuple)) else [d_in]
+            if len(d_in) != len(self.sig_in):
+                raise TypeError(f"_sensitivity returned {len(d_in)} values for {len(self.sig_in)} inputs")
+            for s, d in zip(self.sig_in, d_in):
+                if _is_signal(s):
+                    s.add_sensitivity(d)
         except Exception as e:
-            raise type(e)(str(e) + "\n\t| Above error was raised when calling sensitivity(). Module details:"
-                          + self._err_str()).with_traceback(sys.exc_info()[2])
+            _rethrow(e, self._describe())
+        return self
 
     def reset(self):
-        """Reset sensitivities of all connected signals."""
-        for s in self.sig_out or []:
-            if _is_valid_signal(s):
-                s.reset()
-        for s in self.sig_in or []:
-            if _is_valid_signal(s):
+        """Clear the sensitivities of the connected signals and the module's own state."""
+        for s in (self.sig_out or []) + (self.sig_in or []):
+            if _is_signal(s):
                 s.reset()
         self._reset()
         return self
 
-    def get_input_states(self, as_list=False):
+    def get_input_states(self, as_list: bool = False):
         if self.sig_in is None:
             return None
-        if not as_list and self.n_in == 1:
-            return self.sig_in[0].state
-        return [s.state if _is_valid_signal(s) else s for s in self.sig_in]
+        states = self._input_states()
+        return states[0] if len(states) == 1 and not as_list else states
 
-    def get_output_states(self, as_list=False):
+    def get_output_states(self, as_list: bool = False):
         if self.sig_out is None:
             return None
-        if not as_list and self.n_out == 1:
-            return self.sig_out[0].state
-        return [s.state if _is_valid_signal(s) else s for s in self.sig_out]
+        states = [s.state for s in self.sig_out]
+        return states[0] if len(states) == 1 and not as_list else states
 
-    def get_output_sensitivities(self, as_list=False):
+    def get_output_sensitivities(self, as_list: bool = False):
         if self.sig_out is None:
             return None
-        if not as_list and self.n_out == 1:
-            return self.sig_out[0].sensitivity
-        return [s.sensitivity if _is_valid_signal(s) else s for s in self.sig_out]
+        sens = [s.sensitivity for s in self.sig_out]
+        return sens[0] if len(sens) == 1 and not as_list else sens
 
-    @abstractmethod
-    def __call__(self, *args):
-        raise NotImplementedError("No response behavior defined")
-
-    def _sensitivity(self, *args):
-        if self.n_out and self.n_in:
-            warnings.warn(f"Sensitivity routine is used, but not defined, in {type(self).__name__}", RuntimeWarning)
-        return [None for _ in self.sig_in]
+    def _sensitivity(self, *d_outputs):
+        warnings.warn(f"{type(self).__name__} defines no _sensitivity; its inputs receive none", RuntimeWarning)
+        return [None] * len(self.sig_in)
 
     def _reset(self):
         pass
 
     def __repr__(self):
-        return f"Module {type(self).__name__} at {hex(id(self))}"
+        return f"{type(self).__name__} module at {hex(id(self))}"
 
+
+# --------------------------------------------------------------------------------------------
+# networks
 
 class Network(Module):
-    """Ordered list of modules run as one (``core_objects.py:750-994``)."""
+    """An ordered list of modules evaluated as one: responses in order, sensitivities in reverse
+    (``core_objects.py:750-994``).  Inside ``with Network() as fn:`` every module called with signals
+    is appended to ``fn`` (and to any enclosing open network)."""
 
-    active = []
+    _open: List["Network"] = []
 
-    def __init__(self, *args, print_timing=False):
-        self._init_loc = _caller_location()
+    def __init__(self, *modules, print_timing=False):
         self.mods = []
-        self.append(*args)
         self.print_timing = print_timing
+        self._origin = _where()
+        self.append(*modules)
+
+    def _evaluate(self, *args):
+        raise RuntimeError("a Network is built from modules; it cannot be called with signals")
+
+    def connect(self, sig_in, sig_out=None):
+        self._evaluate()
 
     def __enter__(self):
-        if self in Network.active:
-            raise ValueError("Network is already activated, cannot activate twice")
-        Network.active.append(self)
+        if any(n is self for n in Network._open):
+            raise ValueError("this Network is already open")
+        Network._open.append(self)
         return self
 
-    def __exit__(self, typ, value, traceback):
-        Network.active.remove(self)
+    def __exit__(self, *exc):
+        Network._open.remove(self)
+        return False
 
-    def _run(self, fns, label):
+    def append(self, *modules):
+        """Append modules; returns the output signal(s) of the last one."""
+        mods = [m for group in modules for m in _as_list(group)]
+        for i, m in enumerate(mods):
+            if not _is_module(m):
+                raise TypeError(f"argument {i} of append ({type(m).__name__}) is not a module")
+        if not mods:
+            return None
+        self.mods.extend(mods)
+        last = mods[-1].sig_out or []
+        return last[0] if len(last) == 1 else last
+
+    def _timed(self, calls, label):
         if self.print_timing is False:
-            for fn, _ in fns:
-                fn()
+            for call, _ in calls:
+                call()
             return
-        tmin = 0.0 if isinstance(self.print_timing, bool) else float(self.print_timing)
-        t0 = time.time()
-        for fn, name in fns:
-            t1 = time.time()
-            fn()
-            if time.time() - t1 > tmin:
-                print(f'-- {label} of "{name}" took {time.time() - t1} s')
-        if time.time() - t0 > tmin:
-            print(f"-- TOTAL {label} took {time.time() - t0} s")
+        floor = 0.0 if self.print_timing is True else float(self.print_timing)
+        start = time.perf_counter()
+        for call, name in calls:
+            t = time.perf_counter()
+            call()
+            dt = time.perf_counter() - t
+            if dt >= floor:
+                print(f"-- {label} of {name}: {dt:.6f} s")
+        total = time.perf_counter() - start
+        if total >= floor:
+            print(f"-- {label} total: {total:.6f} s")
 
     def response(self):
-        self._run([(m.response, type(m).__name__) for m in self.mods], "Response")
+        self._timed([(m.response, type(m).__name__) for m in self.mods], "response")
+        return self
 
     def sensitivity(self):
-        self._run([(m.sensitivity, type(m).__name__) for m in reversed(self.mods)], "Sensitivity")
+        self._timed([(m.sensitivity, type(m).__name__) for m in reversed(self.mods)], "sensitivity")
+        return self
 
     def reset(self):
         for m in reversed(self.mods):
             m.reset()
+        return self
 
     def __len__(self):
         return len(self.mods)
@@ -409,71 +477,96 @@ class Network(Module):
     def __iter__(self):
         return iter(self.mods)
 
-    def __getitem__(self, sel):
-        out = Network(self.mods[sel])
-        if len(out) == 0:
-            return None
-        return out.mods[0] if len(out) == 1 else out
-
-    def __call__(self, *args):
-        raise RuntimeError("Cannot re-connect signals in existing network")
-
-    def append(self, *newmods):
-        modlist = _parse_to_list(*newmods)
-        if not modlist:
-            return None
-        for i, m in enumerate(modlist):
-            if not _is_valid_module(m):
-                raise TypeError(f"Argument #{i} is not a valid module, type='{type(m).__name__}'.")
-        self.mods.extend(modlist)
-        so = modlist[-1].sig_out or []
-        return so[0] if len(so) == 1 else so
-
     @staticmethod
-    def _signal_set(sigs):
-        def base(s):
-            return base(s.base) if isinstance(s, SignalSlice) else s
-        return set(base(s) for s in _parse_to_list(sigs) if _is_valid_signal(s))
+    def _unwrap(net: "Network"):
+        if len(net) == 0:
+            return None
+        return net.mods[0] if len(net) == 1 else net
+
+    def __getitem__(self, sel):
+        """An int gives a module; a slice of positions gives a sub-network (or its only module, or
+        None); a slice of signals ``fn[a:b]`` gives the modules between those signals."""
+        if isinstance(sel, slice) and (_is_signal(sel.start) or _is_signal(sel.stop)):
+            if sel.step is not None:
+                raise IndexError("a slice of signals takes no step")
+            for end in (sel.start, sel.stop):
+                if end is not None and not _is_signal(end):
+                    raise IndexError(f"a slice of signals cannot end at a {type(end).__name__}")
+            return self._unwrap(self.get_input_cone(sel.start).get_output_cone(sel.stop))
+        if isinstance(sel, slice):
+            return self._unwrap(Network(self.mods[sel]))
+        return self.mods[sel]
+
+    def _flows(self):
+        consumed, produced = set(), set()
+        for m in self.mods:
+            consumed |= _signal_set(m.sig_in)
+            produced |= _signal_set(m.sig_out)
+        return consumed, produced
 
     @property
     def sig_in(self):
-        ins, outs = set(), set()
-        for m in self.mods:
-            ins |= self._signal_set(m.sig_in)
-            outs |= self._signal_set(m.sig_out)
-        return list(ins - outs)
+        """Signals read by the network but produced by none of its modules."""
+        consumed, produced = self._flows()
+        return list(consumed - produced)
 
     @property
     def sig_out(self):
-        ins, outs = set(), set()
-        for m in self.mods:
-            ins |= self._signal_set(m.sig_in)
-            outs |= self._signal_set(m.sig_out)
-        return list(outs - ins)
+        """Signals produced by the network but read by none of its modules."""
+        consumed, produced = self._flows()
+        return list(produced - consumed)
 
     def get_input_cone(self, fromsig=None, frommod=None):
-        touched = self._signal_set(fromsig)
-        frommod = set(_parse_to_list(frommod))
-        if not touched and not frommod:
+        """Modules that depend on ``fromsig`` or on the modules ``frommod`` (those included), in order;
+        the whole network when both are empty."""
+        reached = _signal_set(fromsig)
+        starts = set(_as_list(frommod))
+        if not reached and not starts:
             return self
         cone = Network()
-        for m in self:
-            if m in frommod or any(s in touched for s in self._signal_set(m.sig_in)):
-                touched |= self._signal_set(m.sig_out)
-                cone.append(m)
+        for m in self.mods:
+            if m in starts or reached & _signal_set(m.sig_in):
+                reached |= _signal_set(m.sig_out)
+                cone.mods.append(m)
         return cone
 
     def get_output_cone(self, tosig=None, tomod=None):
-        dep = self._signal_set(tosig)
-        tomod = set(_parse_to_list(tomod))
-        if not dep and not tomod:
+        """Modules that ``tosig`` or the modules ``tomod`` (those included) depend on, in order; the
+        whole network when both are empty."""
+        needed = _signal_set(tosig)
+        ends = set(_as_list(tomod))
+        if not needed and not ends:
             return self
-        cone = []
+        picked = []
         for m in reversed(self.mods):
-            if m in tomod or any(s in dep for s in self._signal_set(m.sig_out)):
-                dep |= self._signal_set(m.sig_in)
-                cone.append(m)
-        return Network(list(reversed(cone)))
+            if m in ends or needed & _signal_set(m.sig_out):
+                needed |= _signal_set(m.sig_in)
+                picked.append(m)
+        return Network(picked[::-1])
 
+    def get_subset(self, fromsig=None, tosig=None, include_sinks: bool = True, include_sources: bool = True):
+        """The modules on the paths from ``fromsig`` to ``tosig`` (``core_objects.py:947-998``); with
+        ``include_sources`` also the source modules (no inputs) whose outputs reach ``tosig`` and the
+        modules in between, with ``include_sinks`` the sink modules (no outputs) that ``fromsig``
+        reaches.  An empty end defaults to the open ends of the path between them."""
+        starts, ends = _signal_set(fromsig), _signal_set(tosig)
+        if not starts and not ends and include_sinks and include_sources:
+            return self
+        core = self.get_input_cone(starts).get_output_cone(ends)
+        if len(core) == 0:
+            return Network()
+        starts = starts or set(core.sig_in)
+        ends = ends or set(core.sig_out)
+        sources, sinks = [], []
+        if include_sources:
+            # a source joins (with the modules between it and the path) when it reaches the end signals
+            sources = [m for m in self.mods
+                       if not m.sig_in and len(self.get_input_cone(frommod=m).get_output_cone(ends)) > 0]
+        if include_sinks:
+            # a sink joins (likewise) when the start signals reach it
+            sinks = [m for m in self.mods
+                     if not m.sig_out and len(self.get_output_cone(tomod=m).get_input_cone(starts)) > 0]
+        return self.get_input_cone(starts, sources).get_output_cone(ends, sinks)
 
-Network.active.append(Network())
+    def __repr__(self):
+        return f"Network of {len(self.mods)} modules at {hex(id(self))}"
