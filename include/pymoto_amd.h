@@ -85,6 +85,15 @@ int pmt_axpby(long long n, double a, const double* x, double b, double* y, void*
  * Replaces numpy einsum('i,i->') of EinSum (generic.py:193-194) and the CG dots (iterative.py:377-394). */
 int pmt_dot(long long n, const double* x, const double* y, double* out, double* work, void* stream);
 int pmt_reduce_work_size(void);
+/* pmt_dot for 1 <= k <= PMT_DOT_NOALLOC_MAX_COLS column pairs without a workspace: out[j] = x_j.y_j with
+ * column j at x + j*n, y + j*n; each out[j] is bit-identical to pmt_dot(n, x_j, y_j).  Allocates nothing
+ * (no hipMalloc, no caller workspace): the partial sums live in a static 64 KiB device array, one per
+ * GPU, whose address is looked up on the first call on each device.  Calls that overlap in time on one
+ * GPU (other streams or host threads) share that array and race; serialise them.  k above the maximum
+ * returns PMT_ERR_ARG.  No counterpart in the reference: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). */
+#define PMT_DOT_NOALLOC_MAX_COLS 8
+int pmt_dot_noalloc(int k, long long n, const double* x, const double* y, double* out, void* stream);
 /* Dirichlet split used by the linear-dependency-aware solve (pymoto/solvers/solvers.py:174-176):
  * for bc dofs sol = rhs/bcdiag and rhs_loc = 0, elsewhere rhs_loc = rhs and sol = 0. */
 int pmt_bc_split(int nelx, int nely, int nelz, const uint8_t* bcmask, double bcdiag, const double* rhs,

@@ -31,6 +31,7 @@ _SIGS = {
     "pmt_axpby": (c_int, [c_ll, c_dbl, c_vp, c_dbl, c_vp, c_vp]),
     "pmt_dot": (c_int, [c_ll, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "pmt_reduce_work_size": (c_int, []),
+    "pmt_dot_noalloc": (c_int, [c_int, c_ll, c_vp, c_vp, c_vp, c_vp]),
     "pmt_bc_split": (c_int, [c_int] * 3 + [c_vp, c_dbl, c_vp, c_vp, c_vp, c_vp]),
     "pmt_axpy_free": (c_int, [c_int] * 3 + [c_vp, c_dbl, c_vp, c_vp, c_vp]),
     "pmt_stiffness_apply": (c_int, [c_int] * 3 + [c_vp, c_int, c_vp, c_vp, c_dbl, c_vp, c_vp, c_vp]),

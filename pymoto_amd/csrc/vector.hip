@@ -6,6 +6,7 @@
 // fixed order, so every dot product is bit-reproducible run to run.
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "launch.h"
 
@@ -302,6 +303,53 @@ int launch_to_double(long long n, const float* x, double* y, hipStream_t s) {
   return check_launch("to_double_kernel");
 }
 
+// ---------------------------------------------------------------------------------------------
+// Dot products of k column pairs without a caller workspace: the per-block partial sums live in a
+// file-scope device array (one copy per GPU, kept until the process ends).  Same grid, loop and
+// summation order per column as dot_stage1<1> / dot_stage2<1>, so each column's result is bit-identical
+// to pmt_dot's.  Every partial is written before it is read, so nothing needs clearing between calls.
+constexpr int kNoAllocMaxCols = 8;
+static __device__ __align__(256) double scratch[kNoAllocMaxCols * kReduceBlocks];  // 64 KiB
+constexpr int kMaxDevices = 64;
+static double* g_scratch_ptr[kMaxDevices];  // device address of `scratch`, per device
+static std::mutex g_scratch_mu;
+
+// Device address of this device's `scratch`, looked up once per device with that device current.
+// In host code the array's name is only a host-side stand-in; kernels get this pointer.
+static int scratch_for_current_device(double** out) {
+  int dev = 0;
+  PMT_TRY(hip_status(hipGetDevice(&dev), "hipGetDevice"));
+  if (dev < 0 || dev >= kMaxDevices) return arg_error("pmt_dot_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    PMT_TRY(hip_status(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)"));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return PMT_OK;
+}
+
+__global__ void __launch_bounds__(kReduceThreads)
+dot_cols_stage1(long long n, const double* __restrict__ x, const double* __restrict__ y,
+                double* __restrict__ partial) {
+  const long long off = (long long)blockIdx.y * n;
+  double s = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    s = fma(x[off + i], y[off + i], s);
+  const double r = block_sum(s);
+  if (threadIdx.x == 0) partial[blockIdx.y * kReduceBlocks + blockIdx.x] = r;
+}
+
+__global__ void __launch_bounds__(kReduceThreads)
+dot_cols_stage2(const double* __restrict__ partial, double* __restrict__ out) {
+  const double* p = partial + (long long)blockIdx.x * kReduceBlocks;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < kReduceBlocks; b += blockDim.x) s += p[b];
+  const double r = block_sum(s);
+  if (threadIdx.x == 0) out[blockIdx.x] = r;
+}
+
 int launch_fill(long long n, double v, double* y, hipStream_t s) {
   if (n <= 0) return PMT_OK;
   fill_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, v, y);
@@ -370,6 +418,20 @@ int pmt_axpby(long long n, double a, const double* x, double b, double* y, void*
 int pmt_dot(long long n, const double* x, const double* y, double* out, double* work, void* stream) {
   if (n < 0 || !x || !y || !out || !work) return arg_error("pmt_dot: bad argument");
   return launch_dot(n, x, y, out, work, (hipStream_t)stream);
+}
+
+int pmt_dot_noalloc(int k, long long n, const double* x, const double* y, double* out, void* stream) {
+  if (k < 1 || n < 0 || !x || !y || !out) return arg_error("pmt_dot_noalloc: bad argument");
+  if (k > kNoAllocMaxCols)
+    return arg_error(("pmt_dot_noalloc: k = " + std::to_string(k) + " column pairs exceed the maximum of " +
+                      std::to_string(kNoAllocMaxCols) + " (the static scratch holds 8 x 1024 partial sums)").c_str());
+  double* partial = nullptr;
+  PMT_TRY(scratch_for_current_device(&partial));
+  hipStream_t s = (hipStream_t)stream;
+  dot_cols_stage1<<<dim3(kReduceBlocks, k), kReduceThreads, 0, s>>>(n, x, y, partial);
+  PMT_TRY(check_launch("dot_cols_stage1"));
+  dot_cols_stage2<<<k, kReduceThreads, 0, s>>>(partial, out);
+  return check_launch("dot_cols_stage2");
 }
 
 int pmt_bc_split(int nelx, int nely, int nelz, const uint8_t* bcmask, double bcdiag, const double* rhs,
