@@ -94,6 +94,8 @@ int pmt_reduce_work_size(void);
  * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). */
 #define PMT_DOT_NOALLOC_MAX_COLS 8
 int pmt_dot_noalloc(int k, long long n, const double* x, const double* y, double* out, void* stream);
+/* z = w * r / diag: DampedJacobi.solve (pymoto/solvers/iterative.py:41-47) as a preconditioner of CG. */
+int pmt_jacobi_apply(long long n, double w, const double* diag, const double* r, double* z, void* stream);
 /* Dirichlet split used by the linear-dependency-aware solve (pymoto/solvers/solvers.py:174-176):
  * for bc dofs sol = rhs/bcdiag and rhs_loc = 0, elsewhere rhs_loc = rhs and sol = 0. */
 int pmt_bc_split(int nelx, int nely, int nelz, const uint8_t* bcmask, double bcdiag, const double* rhs,
@@ -101,6 +103,12 @@ int pmt_bc_split(int nelx, int nely, int nelz, const uint8_t* bcmask, double bcd
 /* y += a*x on free (non-bc) dofs only (solvers.py:233-236). */
 int pmt_axpy_free(int nelx, int nely, int nelz, const uint8_t* bcmask, double a, const double* x,
                   double* y, void* stream);
+/* The two above for ndof unknowns per node (1 <= ndof <= 8; the forms above use ndof = the grid
+ * dimension): the scalar systems of AssemblePoisson use ndof = 1. */
+int pmt_bc_split_ndof(int nelx, int nely, int nelz, int ndof, const uint8_t* bcmask, double bcdiag,
+                      const double* rhs, double* rhs_loc, double* sol, void* stream);
+int pmt_axpy_free_ndof(int nelx, int nely, int nelz, int ndof, const uint8_t* bcmask, double a, const double* x,
+                       double* y, void* stream);
 
 /* ---------------------------------------------------------------------------------------------
  * Matrix-free stiffness operator K(x) = sum_e x_e K0 with Dirichlet rows/cols zeroed and bcdiag on
@@ -150,6 +158,9 @@ int pmt_k0_extend(int dim, const double* K0_host, double* ext_host);
  * the reflection basis from the transformed child densities, no stencil stored (default; used when
  * k0_pattern has PMT_K0_REFLECTION and the level is not the coarsest), 0 = half stencil.  Read by
  * pmt_mg_create: it applies to hierarchies created afterwards.
+ * PMT_TUNE_FILTER: FilterConv kernels for the 3^3 / 5^3 (3^2 / 5^2) extents, 1 = LDS-tiled z-streamed
+ * blocks (default), 0 = one thread per element gathering through the caches; both sum the taps in the
+ * same order.
  * pmt_get_tuning returns the current value or a negative status.
  * (No reference counterpart: the launch shapes picked by the autotuner, pymoto_amd/tune.py.) */
 #define PMT_TUNE_FINE_KERNEL 1
@@ -157,6 +168,7 @@ int pmt_k0_extend(int dim, const double* K0_host, double* ext_host);
 #define PMT_TUNE_STREAM_TILE 3
 #define PMT_TUNE_REFL_ROWS 4
 #define PMT_TUNE_COARSE1 5
+#define PMT_TUNE_FILTER 6
 int pmt_set_tuning(int param, int value);
 int pmt_get_tuning(int param);
 /* diag(K) (DampedJacobi.update, iterative.py:38-39) */
@@ -169,6 +181,11 @@ int pmt_stiffness_diagonal(int nelx, int nely, int nelz, const double* K0, const
 int pmt_sens_contract(int nelx, int nely, int nelz, const double* K0, const uint8_t* bcmask,
                       const double* u, const double* v, double alpha, double* dx, int accumulate,
                       void* stream);
+/* The same contraction with the k0_pattern of pmt_stiffness_apply: with PMT_K0_REFLECTION (hex8, K0 the
+ * extended buffer of pmt_k0_extend) each element is contracted in the reflection basis (two 8-point
+ * transforms and 96 multiply-adds; one transform when u == v); otherwise as pmt_sens_contract. */
+int pmt_sens_contract_k0(int nelx, int nely, int nelz, const double* K0, int k0_pattern, const uint8_t* bcmask,
+                         const double* u, const double* v, double alpha, double* dx, int accumulate, void* stream);
 
 /* ---------------------------------------------------------------------------------------------
  * Assembled CSR (the reference's output format).  Pattern: per row the sorted union of the dofs
@@ -183,6 +200,11 @@ int pmt_csr_pattern(int nelx, int nely, int nelz, int ndof, int index_bytes, voi
 int pmt_csr_values(int nelx, int nely, int nelz, const double* K0, const double* xe,
                    const uint8_t* bcmask, double bcdiag, int index_bytes, const void* indptr,
                    double* data, void* stream);
+/* pmt_csr_values for ndof unknowns per node: ndof = the grid dimension (as pmt_csr_values) or 1 (the
+ * scalar matrix of AssemblePoisson, assembly.py:523-560, bit 0 of bcmask). */
+int pmt_csr_values_ndof(int nelx, int nely, int nelz, int ndof, const double* K0, const double* xe,
+                        const uint8_t* bcmask, double bcdiag, int index_bytes, const void* indptr, double* data,
+                        void* stream);
 
 /* ---------------------------------------------------------------------------------------------
  * Geometric multigrid transfers with the trilinear (bilinear) weights 1, 1/2, 1/4, 1/8 of
@@ -221,7 +243,8 @@ int pmt_mg_smoother_interval(pmt_mg* mg, int level, double* lo, double* hi);
  * (GeometricMultigrid.update, iterative.py:168-176).  xe and bcmask must stay alive until the
  * next update.  Hex8 grids: the coarsest dense inverse may still be in progress on the handle's own
  * stream when this returns; pmt_mg_vcycle, pmt_pcg_solve, pmt_block_pcg_solve and the next update
- * make their stream wait for it. */
+ * make their stream wait for it (with the legacy default stream, NULL, the first of them waits for it on
+ * the host). */
 int pmt_mg_update(pmt_mg* mg, const double* xe, const uint8_t* bcmask, double bcdiag, void* stream);
 /* z = M r, one V-cycle (GeometricMultigrid.solve with x0=None, iterative.py:222-256). */
 int pmt_mg_vcycle(pmt_mg* mg, const double* r, double* z, void* stream);
@@ -238,6 +261,13 @@ long long pmt_mg_device_bytes(pmt_mg* mg);
 int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double tol, int maxit,
                   int restart, int* iters, double* relres, double* history, void* stream);
 
+/* Coarse-space deflation inside pmt_pcg_solve (no reference counterpart: the learned coarse correction
+ * of pymoto_amd/learned.py): with 1 <= m <= 4 columns W and AW = A W (FP64, caller-owned device memory,
+ * column j at W + j*ndof, kept alive while set) and Einv = (W^T A W)^-1 (host, m x m row-major, copied),
+ * every preconditioned residual is projected, z <- z - W Einv (AW)^T z, so the search directions stay
+ * A-orthogonal to span W (with the start x0 = W Einv W^T b the residuals stay orthogonal to it: deflated
+ * PCG).  m = 0 removes it.  Not used by pmt_block_pcg_solve. */
+int pmt_mg_set_coarse_space(pmt_mg* mg, int m, const double* W, const double* AW, const double* Einv_host);
 /* Block PCG for 1 <= k <= 8 right-hand sides sharing the operator: CG.solve with an (ndof, k)
  * right-hand side (iterative.py:259-403), Gram-Schmidt orth with rank-deficient columns dropped.
  * B and X hold k columns of pmt_mg_level_ndof(mg, 0) doubles each, column j at B + j*ndof (the
@@ -248,6 +278,39 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
  * Allocates its workspace on the first call with a larger k. */
 int pmt_block_pcg_solve(pmt_mg* mg, int k, const double* B, double* X, int x0_is_zero, double tol, int maxit,
                         int restart, int use_graph, int* iters, double* relres, double* history, void* stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * Scalar problems (one unknown per node): the matrix K(x) = sum_e x_e P0 of AssemblePoisson
+ * (pymoto/modules/assembly.py:523-560; AssembleGeneral with a (2^D x 2^D) element matrix,
+ * assembly.py:38-296) - thermal conduction, electric permittivity.  bcmask bit 0 marks the node's
+ * unknown as Dirichlet (row and column zeroed, bcdiag on the diagonal).  P0 is a device copy of the
+ * element matrix (row-major, local node order of pymoto/common/domain.py:131-142).
+ */
+int pmt_scalar_apply(int nelx, int nely, int nelz, const double* P0, const double* xe, const uint8_t* bcmask,
+                     double bcdiag, const double* u, double* y, void* stream);
+int pmt_scalar_residual(int nelx, int nely, int nelz, const double* P0, const double* xe, const uint8_t* bcmask,
+                        double bcdiag, const double* b, const double* u, double* r, void* stream);
+int pmt_scalar_diagonal(int nelx, int nely, int nelz, const double* P0, const double* xe, const uint8_t* bcmask,
+                        double bcdiag, double* diag, void* stream);
+/* dx_e (+)= alpha * u_e^T P0 v_e with Dirichlet entries zeroed: AssembleGeneral._sensitivity +
+ * DyadicMatrix.contract (assembly.py:298-315, pymoto/common/dyadcarrier.py:392-414) for ndof = 1. */
+int pmt_scalar_sens_contract(int nelx, int nely, int nelz, const double* P0, const uint8_t* bcmask, const double* u,
+                             const double* v, double alpha, double* dx, int accumulate, void* stream);
+/* Multigrid-preconditioned CG for the scalar matrix: nested GeometricMultigrid levels with damped
+ * Jacobi and Galerkin R^T A R stencils (27-point hex8, 9-point Q4), explicit inverse on the coarsest
+ * grid, CG.solve for one right-hand side (pymoto/solvers/iterative.py:21-47,124-256,340-403).  Same
+ * argument meaning as pmt_mg_create / pmt_mg_update / pmt_mg_vcycle / pmt_pcg_solve; a non-finite
+ * right-hand side, start vector or residual returns PMT_ERR_NUMERIC.  Allocates at create only. */
+typedef struct pmt_smg pmt_smg;
+int pmt_smg_create(int nelx, int nely, int nelz, int nlevels, const double* omega, const int* steps,
+                   const double* P0_host, pmt_smg** out);
+int pmt_smg_destroy(pmt_smg* h);
+long long pmt_smg_level_nodes(pmt_smg* h, int level);
+int pmt_smg_update(pmt_smg* h, const double* xe, const uint8_t* bcmask, double bcdiag, void* stream);
+int pmt_smg_vcycle(pmt_smg* h, const double* r, double* z, void* stream);
+int pmt_smg_level_apply(pmt_smg* h, int level, const double* u, double* y, void* stream);
+int pmt_smg_pcg_solve(pmt_smg* h, const double* b, double* x, int x0_is_zero, double tol, int maxit, int restart,
+                      int* iters, double* relres, double* history, void* stream);
 
 /* ---------------------------------------------------------------------------------------------
  * Learned warm start (no reference counterpart): kernels of a 3D convolutional encoder-decoder on

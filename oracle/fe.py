@@ -9,6 +9,8 @@ Pinning (see tests/test_oracle_fe.py):
 * ``assemble_values``    - golden CSR data bit-exact, plus dense brute-force sum_e x_e K0.
 * ``simp``               - golden xSIMP / dSIMP from the reference's sympy-lambdified expression.
 * ``sens_contract``      - finite differences of u^T K(x) v and the golden dC/dx.
+* ``element_poisson``    - one-element heat conduction T = Q L / (k A) (2D and 3D), textbook conduction
+                           matrices of the unit square / cube, constant null vector, symmetry.
 """
 import numpy as np
 import scipy.sparse as sps
@@ -105,6 +107,23 @@ def element_stiffness(dim, esize=(1.0, 1.0, 1.0), E=1.0, nu=0.3, plane="strain")
         B = _B(_shape_fun_der(dim, esize, pos))
         K0 += w * B.T @ D @ B
     return K0
+
+
+def element_poisson(dim, esize=(1.0, 1.0, 1.0), k=1.0):
+    """P0 = sum_gp w k dN^T dN over 2x2(x2) Gauss points at +-h/(2 sqrt 3); w = prod(h/2), k times h_z in
+    2D (``AssemblePoisson.__init__``, assembly.py:535-560).  Pinned by the reference's one-element heat
+    conduction answers (``tests/test_assembly.py:286-340``: T = Q L / (k A)) and the textbook bilinear /
+    trilinear conduction matrices of a square / cube."""
+    esize = np.asarray(esize, dtype=float)
+    w = np.prod(esize[:dim] / 2)
+    kk = k * (esize[2] if dim == 2 else 1.0)
+    P0 = np.zeros((2 ** dim, 2 ** dim))
+    for a in range(2 ** dim):
+        sgn = np.array([1 if (a >> b) & 1 else -1 for b in range(3)], dtype=float)
+        pos = sgn * (esize / 2) / np.sqrt(3)
+        dN = _shape_fun_der(dim, esize, pos)
+        P0 += w * kk * dN.T @ dN
+    return P0
 
 
 # --------------------------------------------------------------------------------------------

@@ -9,6 +9,8 @@ Test infrastructure only.  Restates the module chain of
 Pinned end-to-end against the reference's own outputs in ``tests/golden/*.npz``.
 """
 import numpy as np
+import scipy.sparse as sps
+import scipy.sparse.linalg as spla
 
 from . import fe
 from .filter import ConvFilter, filter_weights
@@ -84,3 +86,57 @@ def robust_iteration(nel, x, bc, F, radius, etas=(0.3, 0.5, 0.7), beta=8.0, xmin
         dx = filt.adjoint(dxs * fe.dsimp(xp, xmin, penal) * dxp)
         out.append(dict(eta=eta, xf=xf, xp=xp, xsimp=xs, U=U, c=c, dcdx=dx))
     return out
+
+
+class SystemOfEquations:
+    """Partitioned solve with prescribed unknowns (``SystemOfEquations``, pymoto/modules/linalg.py:222-349),
+    dense-free numpy/scipy restatement with the direct solve: A_ff x_f = b_f - A_fp x_p, b_p = A_pf x_f +
+    A_pp x_p; outputs the full x and b.  ``sensitivity(dgdx, dgdb)`` returns (lam, x) of the matrix dyad
+    lam (x) x, dg/db_f and dg/dx_p (``linalg.py:312-349``).  Pinned by finite differences of a scalar
+    response through all three inputs and by the identities A x = b, x_p = prescribed, b_f = applied."""
+
+    def __init__(self, free=None, prescribed=None):
+        if free is None and prescribed is None:
+            raise ValueError("Either prescribed or free indices must be provided")
+        self.f, self.p = free, prescribed
+
+    def forward(self, A, bf, xp):
+        A = sps.csr_matrix(A)
+        n = A.shape[0]
+        if self.f is None:
+            self.f = np.setdiff1d(np.arange(n), self.p)
+        if self.p is None:
+            self.p = np.setdiff1d(np.arange(n), self.f)
+        f, p = self.f, self.p
+        self.Aff = A[f][:, f].tocsc()
+        self.Afp = A[f][:, p]
+        self.App = A[p][:, p]
+        self.lu = spla.splu(self.Aff)
+        x = np.zeros((n,) + bf.shape[1:])
+        x[p] = xp
+        x[f] = self.lu.solve(bf - self.Afp @ xp)
+        b = np.zeros_like(x)
+        b[f] = bf
+        b[p] = self.Afp.T @ x[f] + self.App @ xp
+        self.x = x
+        return x, b
+
+    def sensitivity(self, dgdx=None, dgdb=None):
+        f, p = self.f, self.p
+        load = np.zeros_like(self.x[f])
+        if dgdx is not None:
+            load += dgdx[f]
+        if dgdb is not None:
+            load += self.Afp @ dgdb[p]
+        lam = np.zeros_like(self.x)
+        lam[f] = -self.lu.solve(load)
+        if dgdb is not None:
+            lam[p] = dgdb[p]
+        dbf = -lam[f]
+        dxp = self.Afp.T @ lam[f]
+        if dgdx is not None:
+            dxp = dxp + dgdx[p]
+        if dgdb is not None:
+            dbf = dbf + dgdb[f]
+            dxp = dxp + self.App @ dgdb[p]
+        return lam, self.x, dbf, dxp

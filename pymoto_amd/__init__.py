@@ -12,15 +12,15 @@ from .core import Module, Network, Signal, SignalSlice, make_signals
 from .domain import VoxelDomain
 from .dyadic import DyadicMatrix
 from . import solvers
-from .assembly import AssembleGeneral, AssembleStiffness, StiffnessMatrix
+from .assembly import AssembleGeneral, AssemblePoisson, AssembleStiffness, StiffnessMatrix
 from .filter import DensityFilter, FilterConv
 from .generic import EinSum, MathExpression
-from .linalg import LinSolve
+from .linalg import LinSolve, SystemOfEquations
 from .optim import OC
 
 __all__ = ["Signal", "SignalSlice", "Module", "Network", "make_signals", "VoxelDomain", "DyadicMatrix", "solvers",
-           "AssembleGeneral", "AssembleStiffness", "StiffnessMatrix", "FilterConv", "DensityFilter", "MathExpression",
-           "EinSum", "LinSolve", "OC"]
+           "AssembleGeneral", "AssembleStiffness", "AssemblePoisson", "StiffnessMatrix", "FilterConv", "DensityFilter", "MathExpression",
+           "EinSum", "LinSolve", "SystemOfEquations", "OC"]
 
 # deprecated aliases kept by the reference until 2.2 (pymoto/__init__.py:102-117)
 DEPRECATED_NAMES = [("DomainDefinition", "VoxelDomain", "2.2"), ("DyadCarrier", "DyadicMatrix", "2.2"),

@@ -32,8 +32,11 @@ _SIGS = {
     "pmt_dot": (c_int, [c_ll, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "pmt_reduce_work_size": (c_int, []),
     "pmt_dot_noalloc": (c_int, [c_int, c_ll, c_vp, c_vp, c_vp, c_vp]),
+    "pmt_jacobi_apply": (c_int, [c_ll, c_dbl, c_vp, c_vp, c_vp, c_vp]),
     "pmt_bc_split": (c_int, [c_int] * 3 + [c_vp, c_dbl, c_vp, c_vp, c_vp, c_vp]),
     "pmt_axpy_free": (c_int, [c_int] * 3 + [c_vp, c_dbl, c_vp, c_vp, c_vp]),
+    "pmt_bc_split_ndof": (c_int, [c_int] * 4 + [c_vp, c_dbl, c_vp, c_vp, c_vp, c_vp]),
+    "pmt_axpy_free_ndof": (c_int, [c_int] * 4 + [c_vp, c_dbl, c_vp, c_vp, c_vp]),
     "pmt_stiffness_apply": (c_int, [c_int] * 3 + [c_vp, c_int, c_vp, c_vp, c_dbl, c_vp, c_vp, c_vp]),
     "pmt_stiffness_residual": (c_int, [c_int] * 3 + [c_vp, c_int, c_vp, c_vp, c_dbl, c_vp, c_vp, c_vp, c_vp]),
     "pmt_stiffness_apply_f32": (c_int, [c_int] * 3 + [c_vp, c_int, c_vp, c_vp, c_dbl, c_vp, c_vp, c_vp]),
@@ -44,9 +47,11 @@ _SIGS = {
     "pmt_get_tuning": (c_int, [c_int]),
     "pmt_stiffness_diagonal": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp, c_dbl, c_vp, c_vp]),
     "pmt_sens_contract": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp, c_int, c_vp]),
+    "pmt_sens_contract_k0": (c_int, [c_int] * 3 + [c_vp, c_int, c_vp, c_vp, c_vp, c_dbl, c_vp, c_int, c_vp]),
     "pmt_csr_nnz": (c_ll, [c_int] * 4),
     "pmt_csr_pattern": (c_int, [c_int] * 5 + [c_vp, c_vp, c_vp]),
     "pmt_csr_values": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp, c_dbl, c_int, c_vp, c_vp, c_vp]),
+    "pmt_csr_values_ndof": (c_int, [c_int] * 4 + [c_vp, c_vp, c_vp, c_dbl, c_int, c_vp, c_vp, c_vp]),
     "pmt_restrict": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp]),
     "pmt_prolong_add": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp]),
     "pmt_mg_create": (c_int, [c_int] * 4 + [c_vp, c_vp, c_vp, c_vp]),
@@ -60,8 +65,21 @@ _SIGS = {
     "pmt_mg_level_ndof": (c_ll, [c_vp, c_int]),
     "pmt_mg_device_bytes": (c_ll, [c_vp]),
     "pmt_pcg_solve": (c_int, [c_vp, c_vp, c_vp, c_int, c_dbl, c_int, c_int, c_vp, c_vp, c_vp, c_vp]),
+    "pmt_mg_set_coarse_space": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp]),
     "pmt_block_pcg_solve": (c_int, [c_vp, c_int, c_vp, c_vp, c_int, c_dbl, c_int, c_int, c_int, c_vp, c_vp, c_vp,
                                     c_vp]),
+    # scalar (one unknown per node) operator and multigrid PCG
+    "pmt_scalar_apply": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp, c_dbl, c_vp, c_vp, c_vp]),
+    "pmt_scalar_residual": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp, c_dbl, c_vp, c_vp, c_vp, c_vp]),
+    "pmt_scalar_diagonal": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp, c_dbl, c_vp, c_vp]),
+    "pmt_scalar_sens_contract": (c_int, [c_int] * 3 + [c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp, c_int, c_vp]),
+    "pmt_smg_create": (c_int, [c_int] * 4 + [c_vp, c_vp, c_vp, c_vp]),
+    "pmt_smg_destroy": (c_int, [c_vp]),
+    "pmt_smg_level_nodes": (c_ll, [c_vp, c_int]),
+    "pmt_smg_update": (c_int, [c_vp, c_vp, c_vp, c_dbl, c_vp]),
+    "pmt_smg_vcycle": (c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "pmt_smg_level_apply": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp]),
+    "pmt_smg_pcg_solve": (c_int, [c_vp, c_vp, c_vp, c_int, c_dbl, c_int, c_int, c_vp, c_vp, c_vp, c_vp]),
     # learned warm start
     "pmt_conv3d_fwd": (c_int, [c_int] * 5 + [c_vp, c_vp, c_vp, c_int, c_vp, c_vp]),
     "pmt_conv3d_bwd_data": (c_int, [c_int] * 5 + [c_vp, c_vp, c_vp, c_vp, c_vp]),

@@ -119,9 +119,13 @@ class StiffnessMatrix:
     def _apply(self, u: torch.Tensor) -> torch.Tensor:
         g, lib = self.grid, _lib.lib()
         y = torch.empty_like(u)
-        _lib.check(lib.pmt_stiffness_apply(*g.nel3, _lib.ptr(g.K0_dev), g.k0_pattern, _lib.ptr(self.xe),
-                                           _lib.ptr(g.bcmask_dev),
-                                           g.bcdiag, _lib.ptr(u), _lib.ptr(y), _lib.stream()), "stiffness apply")
+        if g.scalar:
+            rc = lib.pmt_scalar_apply(*g.nel3, _lib.ptr(g.K0_dev), _lib.ptr(self.xe), _lib.ptr(g.bcmask_dev), g.bcdiag,
+                                      _lib.ptr(u), _lib.ptr(y), _lib.stream())
+        else:
+            rc = lib.pmt_stiffness_apply(*g.nel3, _lib.ptr(g.K0_dev), g.k0_pattern, _lib.ptr(self.xe),
+                                         _lib.ptr(g.bcmask_dev), g.bcdiag, _lib.ptr(u), _lib.ptr(y), _lib.stream())
+        _lib.check(rc, "stiffness apply")
         return y
 
     def __matmul__(self, u):
@@ -137,6 +141,8 @@ class StiffnessMatrix:
         ``fp32`` (FP32 arithmetic) or ``bf16`` (hex8: BF16 element products on the matrix cores,
         FP32 accumulation).  Returns an FP32 tensor."""
         g, lib = self.grid, _lib.lib()
+        if g.scalar:
+            raise NotImplementedError("reduced-precision products exist for the elasticity operator only")
         u32 = _lib.as_device(u, dtype=torch.float32)
         x32 = self.xe.to(torch.float32)
         y = torch.empty_like(u32)
@@ -156,17 +162,22 @@ class StiffnessMatrix:
         """b - K u in one kernel."""
         g, lib = self.grid, _lib.lib()
         r = torch.empty_like(u)
-        _lib.check(lib.pmt_stiffness_residual(*g.nel3, _lib.ptr(g.K0_dev), g.k0_pattern, _lib.ptr(self.xe),
-                                              _lib.ptr(g.bcmask_dev), g.bcdiag, _lib.ptr(b), _lib.ptr(u),
-                                              _lib.ptr(r), _lib.stream()), "stiffness residual")
+        if g.scalar:
+            rc = lib.pmt_scalar_residual(*g.nel3, _lib.ptr(g.K0_dev), _lib.ptr(self.xe), _lib.ptr(g.bcmask_dev),
+                                         g.bcdiag, _lib.ptr(b), _lib.ptr(u), _lib.ptr(r), _lib.stream())
+        else:
+            rc = lib.pmt_stiffness_residual(*g.nel3, _lib.ptr(g.K0_dev), g.k0_pattern, _lib.ptr(self.xe),
+                                            _lib.ptr(g.bcmask_dev), g.bcdiag, _lib.ptr(b), _lib.ptr(u),
+                                            _lib.ptr(r), _lib.stream())
+        _lib.check(rc, "stiffness residual")
         return r
 
     def diagonal(self) -> torch.Tensor:
         g, lib = self.grid, _lib.lib()
         d = torch.empty(g.n, dtype=torch.float64, device=self.xe.device)
-        _lib.check(lib.pmt_stiffness_diagonal(*g.nel3, _lib.ptr(g.K0_dev), _lib.ptr(self.xe),
-                                              _lib.ptr(g.bcmask_dev), g.bcdiag, _lib.ptr(d), _lib.stream()),
-                   "stiffness diagonal")
+        fn = lib.pmt_scalar_diagonal if g.scalar else lib.pmt_stiffness_diagonal
+        _lib.check(fn(*g.nel3, _lib.ptr(g.K0_dev), _lib.ptr(self.xe), _lib.ptr(g.bcmask_dev), g.bcdiag, _lib.ptr(d),
+                      _lib.stream()), "stiffness diagonal")
         return d
 
     def tocsr(self, index_dtype=torch.int32):
@@ -179,8 +190,9 @@ class StiffnessMatrix:
         data = torch.empty(nnz, dtype=torch.float64, device=self.xe.device)
         _lib.check(lib.pmt_csr_pattern(*g.nel3, g.ndof, nb, _lib.ptr(indptr), _lib.ptr(indices), _lib.stream()),
                    "csr pattern")
-        _lib.check(lib.pmt_csr_values(*g.nel3, _lib.ptr(g.K0_dev), _lib.ptr(self.xe), _lib.ptr(g.bcmask_dev),
-                                      g.bcdiag, nb, _lib.ptr(indptr), _lib.ptr(data), _lib.stream()), "csr values")
+        _lib.check(lib.pmt_csr_values_ndof(*g.nel3, g.ndof, _lib.ptr(g.K0_dev), _lib.ptr(self.xe),
+                                           _lib.ptr(g.bcmask_dev), g.bcdiag, nb, _lib.ptr(indptr), _lib.ptr(data),
+                                           _lib.stream()), "csr values")
         return indptr, indices, data
 
     def to_scipy(self):
@@ -205,12 +217,16 @@ class _GridData:
         self.bc = None if bc is None else np.asarray(bc).ravel()
         self.bcdiag = float(bcdiag) if bcdiag is not None else 0.0
         self.bcmask = bc_node_mask(domain, self.ndof, self.bc)
-        # device copy in the extended layout: K0, then its reflection-basis blocks (hex8)
-        self.K0_ext = np.zeros(_lib.K0_EXT_LEN, dtype=np.float64)
-        self.k0_pattern = int(_lib.load().pmt_k0_extend(self.dim, self.K0.ctypes.data_as(_lib.c_vp),
-                                                        self.K0_ext.ctypes.data_as(_lib.c_vp)))
-        if self.k0_pattern < 0:
-            raise ValueError("element matrix rejected by pmt_k0_extend")
+        self.scalar = self.ndof == 1
+        if self.scalar:  # one unknown per node: the element matrix as is
+            self.K0_ext = self.K0.ravel().copy()
+            self.k0_pattern = 0
+        else:  # device copy in the extended layout: K0, then its reflection-basis blocks (hex8)
+            self.K0_ext = np.zeros(_lib.K0_EXT_LEN, dtype=np.float64)
+            self.k0_pattern = int(_lib.load().pmt_k0_extend(self.dim, self.K0.ctypes.data_as(_lib.c_vp),
+                                                            self.K0_ext.ctypes.data_as(_lib.c_vp)))
+            if self.k0_pattern < 0:
+                raise ValueError("element matrix rejected by pmt_k0_extend")
         self._dev = {}
 
     def _on_device(self, key, arr, dtype):
@@ -247,8 +263,9 @@ class AssembleGeneral(Module):
             raise NotImplementedError("only square element matrices are supported on the device")
         if K0.shape[0] % domain.elemnodes != 0:
             raise ValueError("Number of rows in element matrix should be a multiple of the number of nodes per element")
-        if K0.shape[0] // domain.elemnodes != domain.dim:
-            raise NotImplementedError("the device operator needs dim dofs per node (vector elasticity)")
+        if K0.shape[0] // domain.elemnodes not in (1, domain.dim):
+            raise NotImplementedError("the device operators take dim unknowns per node (elasticity) or one "
+                                      "(scalar problems)")
         if domain.dim not in (2, 3):
             raise NotImplementedError("1D domains are not supported")
         if add_constant is not None:
@@ -278,9 +295,14 @@ class AssembleGeneral(Module):
         xs = self.sig_in[0].state
         dx = torch.zeros(self.nel, dtype=torch.float64, device=_lib.device())
         for k, (u, v, fac) in enumerate(dgdmat.terms()):
-            _lib.check(lib.pmt_sens_contract(*g.nel3, _lib.ptr(g.K0_dev), _lib.ptr(g.bcmask_dev), _lib.ptr(u),
-                                             _lib.ptr(v), float(fac), _lib.ptr(dx), int(k > 0), _lib.stream()),
-                       "sensitivity contraction")
+            if g.scalar:
+                rc = lib.pmt_scalar_sens_contract(*g.nel3, _lib.ptr(g.K0_dev), _lib.ptr(g.bcmask_dev), _lib.ptr(u),
+                                                  _lib.ptr(v), float(fac), _lib.ptr(dx), int(k > 0), _lib.stream())
+            else:
+                rc = lib.pmt_sens_contract_k0(*g.nel3, _lib.ptr(g.K0_dev), g.k0_pattern, _lib.ptr(g.bcmask_dev),
+                                              _lib.ptr(u), _lib.ptr(v), float(fac), _lib.ptr(dx), int(k > 0),
+                                              _lib.stream())
+            _lib.check(rc, "sensitivity contraction")
         if isinstance(xs, np.ndarray):
             return dx.cpu().numpy().reshape(xs.shape)
         return dx.reshape(xs.shape)
@@ -294,3 +316,28 @@ class AssembleStiffness(AssembleGeneral):
         self.E, self.nu = e_modulus, poisson_ratio
         self.stiffness_element = element_stiffness(domain, e_modulus, poisson_ratio, plane)
         super().__init__(domain, self.stiffness_element, *args, **kwargs)
+
+
+def poisson_element(domain: VoxelDomain, material_property=1.0):
+    """Element matrix of the Poisson equation, P0 = sum_gp w k grad(N)^T grad(N) with 2x2(x2) Gauss points
+    at +-h/(2 sqrt 3) and weight prod(h/2); in 2D k is multiplied by the thickness h_z
+    (``assembly.py:535-560``)."""
+    siz = domain.element_size
+    w = np.prod(siz[: domain.dim] / 2)
+    k = np.prod(np.atleast_1d(material_property)) * (siz[2] if domain.dim == 2 else 1.0)
+    P0 = np.zeros((domain.elemnodes, domain.elemnodes))
+    for n in domain.node_numbering:
+        pos = np.asarray(n, dtype=float) * (siz / 2) / np.sqrt(3)
+        dN = domain.eval_shape_fun_der(pos)
+        P0 += w * k * dN.T @ dN
+    return P0
+
+
+class AssemblePoisson(AssembleGeneral):
+    """Scalar conduction / permittivity matrix P(x) = sum_e x_e P0 for Q4/hex8, one unknown per node
+    (``assembly.py:523-560``); applied, solved and differentiated by the scalar kernels."""
+
+    def __init__(self, domain: VoxelDomain, *args, material_property: float = 1.0, **kwargs):
+        self.material_property = material_property
+        self.poisson_element = poisson_element(domain, material_property)
+        super().__init__(domain, self.poisson_element, *args, **kwargs)

@@ -64,16 +64,17 @@ __global__ void csr_pattern_kernel(GridDims g, int D, IT* __restrict__ indptr, I
       }
 }
 
-template <int DIM, typename IT>
+// D: unknowns per node (DIM for elasticity, 1 for scalar problems)
+template <int DIM, int D, typename IT>
 __global__ void csr_values_kernel(GridDims g, const double* __restrict__ K0, const double* __restrict__ xe,
                                   const uint8_t* __restrict__ bcm, double bcdiag, const IT* __restrict__ indptr,
                                   double* __restrict__ data) {
   constexpr int NN = 1 << DIM;
-  constexpr int NE = NN * DIM;
+  constexpr int NE = NN * D;
   const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= g.nnodes * DIM) return;
-  const long long n = row / DIM;
-  const int d = (int)(row - n * DIM);
+  if (row >= g.nnodes * D) return;
+  const long long n = row / D;
+  const int d = (int)(row - n * D);
   const int i = (int)(n % g.nx);
   const long long t = n / g.nx;
   const int j = (int)(t % g.ny);
@@ -88,7 +89,7 @@ __global__ void csr_values_kernel(GridDims g, const double* __restrict__ K0, con
       for (int ii = i0; ii <= i1; ++ii) {
         const long long m = ((long long)kk * g.ny + jj) * g.nx + ii;
         const unsigned mk = bcm ? bcm[m] : 0u;
-        for (int d2 = 0; d2 < DIM; ++d2) {
+        for (int d2 = 0; d2 < D; ++d2) {
           double v = 0.0;
           if (row_fixed || ((mk >> d2) & 1u)) {
             v = (m == n && d2 == d && row_fixed) ? bcdiag : 0.0;
@@ -104,7 +105,7 @@ __global__ void csr_values_kernel(GridDims g, const double* __restrict__ K0, con
                   const int locn = (i - ei) + 2 * (j - ej) + (DIM == 3 ? 4 * (k - ek) : 0);
                   const int locm = bx + 2 * by + 4 * bz;
                   const double xv = xe[((long long)ek * g.ey + ej) * g.ex + ei];
-                  const double prod = K0[(locn * DIM + d) * NE + locm * DIM + d2] * xv;
+                  const double prod = K0[(locn * D + d) * NE + locm * D + d2] * xv;
                   v = v + prod;
                 }
           }
@@ -142,29 +143,35 @@ int pmt_csr_pattern(int nelx, int nely, int nelz, int ndof, int index_bytes, voi
   return check_launch("csr_pattern_kernel");
 }
 
-int pmt_csr_values(int nelx, int nely, int nelz, const double* K0, const double* xe, const uint8_t* bcmask,
-                   double bcdiag, int index_bytes, const void* indptr, double* data, void* stream) {
+int pmt_csr_values_ndof(int nelx, int nely, int nelz, int ndof, const double* K0, const double* xe,
+                        const uint8_t* bcmask, double bcdiag, int index_bytes, const void* indptr, double* data,
+                        void* stream) {
   if (!grid_ok(nelx, nely, nelz) || !K0 || !xe || !indptr || !data || (index_bytes != 4 && index_bytes != 8))
     return arg_error("pmt_csr_values: bad argument");
   GridDims g = make_grid(nelx, nely, nelz);
-  const long long rows = g.nnodes * g.dim;
+  if (ndof != 1 && ndof != g.dim) return arg_error("pmt_csr_values: ndof must be 1 or the grid dimension");
+  const long long rows = g.nnodes * ndof;
   hipStream_t s = (hipStream_t)stream;
-  if (g.dim == 3) {
-    if (index_bytes == 4)
-      csr_values_kernel<3, int32_t><<<blocks_for(rows, 256), 256, 0, s>>>(g, K0, xe, bcmask, bcdiag,
-                                                                          (const int32_t*)indptr, data);
-    else
-      csr_values_kernel<3, int64_t><<<blocks_for(rows, 256), 256, 0, s>>>(g, K0, xe, bcmask, bcdiag,
-                                                                          (const int64_t*)indptr, data);
+  const unsigned blocks = blocks_for(rows, 256);
+#define PMT_CSRV(DIM, D, IT) \
+  csr_values_kernel<DIM, D, IT><<<blocks, 256, 0, s>>>(g, K0, xe, bcmask, bcdiag, (const IT*)indptr, data)
+  if (g.dim == 3 && ndof == 3) {
+    if (index_bytes == 4) PMT_CSRV(3, 3, int32_t); else PMT_CSRV(3, 3, int64_t);
+  } else if (g.dim == 3) {
+    if (index_bytes == 4) PMT_CSRV(3, 1, int32_t); else PMT_CSRV(3, 1, int64_t);
+  } else if (ndof == 2) {
+    if (index_bytes == 4) PMT_CSRV(2, 2, int32_t); else PMT_CSRV(2, 2, int64_t);
   } else {
-    if (index_bytes == 4)
-      csr_values_kernel<2, int32_t><<<blocks_for(rows, 256), 256, 0, s>>>(g, K0, xe, bcmask, bcdiag,
-                                                                          (const int32_t*)indptr, data);
-    else
-      csr_values_kernel<2, int64_t><<<blocks_for(rows, 256), 256, 0, s>>>(g, K0, xe, bcmask, bcdiag,
-                                                                          (const int64_t*)indptr, data);
+    if (index_bytes == 4) PMT_CSRV(2, 1, int32_t); else PMT_CSRV(2, 1, int64_t);
   }
+#undef PMT_CSRV
   return check_launch("csr_values_kernel");
+}
+
+int pmt_csr_values(int nelx, int nely, int nelz, const double* K0, const double* xe, const uint8_t* bcmask,
+                   double bcdiag, int index_bytes, const void* indptr, double* data, void* stream) {
+  return pmt_csr_values_ndof(nelx, nely, nelz, nelz > 0 ? 3 : 2, K0, xe, bcmask, bcdiag, index_bytes, indptr, data,
+                             stream);
 }
 
 }  // extern "C"

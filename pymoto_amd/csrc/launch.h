@@ -5,6 +5,16 @@
 
 namespace pmt {
 
+// a small dense matrix passed to kernels by value (up to 4 x 4, row-major)
+struct SmallMat {
+  double e[16];
+};
+
+// vector.hip: z -= W Einv (AW)^T z for m <= 4 columns W and AW = A W (column j at W + j*n); buf holds
+// 4 + 4*kReduceBlocks doubles (the coefficients and the dot partials)
+int launch_coarse_correct(long long n, int m, const double* W, const double* AW, const SmallMat& Einv, double* buf,
+                          double* z, hipStream_t s);
+
 // MODE_CHEB: y = u + omega D^-1 (b - A u) + beta (u - up), the second-order (Chebyshev) sweep;
 // up may be NULL (= 0) and may alias y (each entry reads its own up before writing it).
 // MODE_APPLY_DOTS (fine reflection kernel, FP64): y = A u plus per-block partial sums of u.y and u.r.

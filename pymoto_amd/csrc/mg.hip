@@ -17,6 +17,7 @@
 // with BF16 operands and FP32 accumulation (hex8 only); the fine residual that is restricted to the
 // coarse grid stays FP32, so the coarse correction sees no BF16 rounding.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <limits>
@@ -227,8 +228,12 @@ static int read_scalar(pmt_mg* mg, int k, double* v, hipStream_t s) {
 // pinned memory, and the host polls for that sequence number (wait_published).  Work enqueued between
 // the two keeps the GPU busy.  After 1 s without it the host falls back to the synchronising read, so a
 // value that never arrives cannot hang the solve.
+static std::atomic<long long> g_publish_seq{0};
+
 static int publish_scalar(pmt_mg* mg, int k, hipStream_t s) {
-  mg->pub_seq += 1.0;
+  // sequence numbers are unique in the process: pinned memory freed by another handle (with its last
+  // sequence number still in it) and handed out again here can never look like this publish
+  mg->pub_seq = (double)g_publish_seq.fetch_add(1, std::memory_order_relaxed) + 1.0;
   return launch_publish(mg->sc + k, mg->pub_seq, mg->host_rr, s);
 }
 
@@ -361,6 +366,14 @@ int mg_precondition(pmt_mg* mg, const double* r, double* z, hipStream_t s) { ret
 // A stream under graph capture must already be listed (block CG joins its stream before capturing).
 int mg_join(pmt_mg* mg, hipStream_t s) {
   if (!mg->pending_join) return PMT_OK;
+  if (s == nullptr) {
+    // the legacy default stream: wait for the inversion on the host instead of through an event, so the
+    // ordering does not depend on how the null stream treats a wait on a non-blocking stream's event
+    PMT_HIP(hipStreamSynchronize(mg->s2));
+    mg->pending_join = false;
+    mg->joined.clear();
+    return PMT_OK;
+  }
   for (hipStream_t j : mg->joined)
     if (j == s) return PMT_OK;
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
@@ -598,6 +611,7 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
   ALLOC(mg->r, n0);
   ALLOC(mg->sc, 16);
   ALLOC(mg->work, 2 * kReduceBlocks);
+  ALLOC(mg->cs_buf, 4 + 4 * kReduceBlocks);
   if (hex && (mg->k0pat & PMT_K0_REFLECTION)) {
     mg->dotws_parts = fine_apply_dots_parts(mg->L[0].g);
     ALLOC(mg->dotws, 2 * mg->dotws_parts);
@@ -605,6 +619,10 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
 #undef ALLOC
   if (rc == PMT_OK)
     rc = hip_status(hipHostMalloc((void**)&mg->host_rr, 2 * sizeof(double), hipHostMallocCoherent), "hipHostMalloc");
+  if (rc == PMT_OK) {  // pinned memory is not cleared: no sequence number may be left in it
+    mg->host_rr[0] = 0.0;
+    mg->host_rr[1] = -1.0;
+  }
   if (rc == PMT_OK) rc = hip_status(hipHostMalloc((void**)&mg->host_sc, 16 * sizeof(double), hipHostMallocDefault),
                                     "hipHostMalloc");
   if (rc == PMT_OK)
@@ -757,6 +775,16 @@ long long pmt_mg_level_ndof(pmt_mg* mg, int level) {
 
 long long pmt_mg_device_bytes(pmt_mg* mg) { return mg ? mg->bytes : PMT_ERR_ARG; }
 
+int pmt_mg_set_coarse_space(pmt_mg* mg, int m, const double* W, const double* AW, const double* Einv_host) {
+  if (!mg || m < 0 || m > 4 || (m > 0 && (!W || !AW || !Einv_host)))
+    return arg_error("pmt_mg_set_coarse_space: bad argument (0 <= m <= 4)");
+  mg->cs_m = m;
+  mg->cs_W = m > 0 ? W : nullptr;
+  mg->cs_AW = m > 0 ? AW : nullptr;
+  for (int q = 0; q < 16; ++q) mg->cs_Einv.e[q] = q < m * m ? Einv_host[q] : 0.0;
+  return PMT_OK;
+}
+
 int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double tol, int maxit, int restart,
                   int* iters, double* relres, double* history, void* stream) {
   if (!mg || !b || !x || maxit < 0 || restart < 1) return arg_error("pmt_pcg_solve: bad argument");
@@ -800,6 +828,7 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
   }
   if (tval <= tol) return PMT_OK;
   PMT_TRY(precondition(mg, mg->r, mg->z, s));
+  PMT_TRY(launch_coarse_correct(n, mg->cs_m, mg->cs_W, mg->cs_AW, mg->cs_Einv, mg->cs_buf, mg->z, s));
   PMT_TRY(launch_dot(n, mg->z, mg->z, sc + 5, mg->work, s));
   PMT_TRY(launch_scale_rsqrt(n, sc, 5, mg->z, mg->p, s));
   int i = 0;
@@ -836,10 +865,17 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
     auto next_direction = [&]() -> int {
       if (mixed) {
         PMT_TRY(precondition_f32(mg, mg->r, r32_ready, s, r32_ready && first));
-        PMT_TRY(launch_dot(n, mg->q, mg->z32, sc + 3, mg->work, s));
-        return launch_cg_p_update(n, sc, mg->z32, mg->p, s);
+        if (mg->cs_m == 0) {
+          PMT_TRY(launch_dot(n, mg->q, mg->z32, sc + 3, mg->work, s));
+          return launch_cg_p_update(n, sc, mg->z32, mg->p, s);
+        }
+        // with a coarse space the projection runs in FP64: rounded to FP32 it would leave components
+        // along W of relative size 1e-7 in every direction, and CG would stall at that level
+        PMT_TRY(launch_to_double(n, mg->z32, mg->z, s));
+      } else {
+        PMT_TRY(precondition(mg, mg->r, mg->z, s));
       }
-      PMT_TRY(precondition(mg, mg->r, mg->z, s));
+      PMT_TRY(launch_coarse_correct(n, mg->cs_m, mg->cs_W, mg->cs_AW, mg->cs_Einv, mg->cs_buf, mg->z, s));
       PMT_TRY(launch_dot(n, mg->q, mg->z, sc + 3, mg->work, s));
       return launch_cg_p_update(n, sc, mg->z, mg->p, s);
     };

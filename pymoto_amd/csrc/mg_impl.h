@@ -112,6 +112,14 @@ struct pmt_mg {
   std::vector<void*> allocs;
   // multi-right-hand-side CG (blockcg.hip): workspace for kcap columns and captured iterations
   pmt::BlockCG* bcg = nullptr;
+  // deflation of the single-right-hand-side PCG's preconditioned residual by a coarse space,
+  // z -= W Einv (AW)^T z (pmt_mg_set_coarse_space): cs_m caller-owned columns W and AW, Einv kept on the
+  // host (kernel argument)
+  int cs_m = 0;
+  const double* cs_W = nullptr;
+  const double* cs_AW = nullptr;
+  pmt::SmallMat cs_Einv{};
+  double* cs_buf = nullptr;  // device: [c (4)][dot partials (4 x kReduceBlocks)]
 };
 
 namespace pmt {

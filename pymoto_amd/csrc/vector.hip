@@ -303,6 +303,12 @@ int launch_to_double(long long n, const float* x, double* y, hipStream_t s) {
   return check_launch("to_double_kernel");
 }
 
+__global__ void jacobi_apply_kernel(long long n, double w, const double* __restrict__ diag,
+                                    const double* __restrict__ r, double* __restrict__ z) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) z[i] = w * r[i] / diag[i];
+}
+
 // ---------------------------------------------------------------------------------------------
 // Dot products of k column pairs without a caller workspace: the per-block partial sums live in a
 // file-scope device array (one copy per GPU, kept until the process ends).  Same grid, loop and
@@ -348,6 +354,60 @@ dot_cols_stage2(const double* __restrict__ partial, double* __restrict__ out) {
   for (int b = threadIdx.x; b < kReduceBlocks; b += blockDim.x) s += p[b];
   const double r = block_sum(s);
   if (threadIdx.x == 0) out[blockIdx.x] = r;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Deflation of the preconditioned residual by a coarse space W (A-orthogonal projection):
+// z <- z - W Einv (AW)^T z with Einv = (W^T A W)^-1, all in FP64.  Stage 1: per-block partial sums of the
+// m dots AW_j . z (fixed grid, deterministic); stage 2: one block sums them in a fixed order and forms
+// c = Einv d; then every entry of z loses sum_j W_j c_j.
+__global__ void __launch_bounds__(kReduceThreads)
+cs_dots_stage1(long long n, const double* __restrict__ AW, const double* __restrict__ z, double* __restrict__ partial) {
+  const double* w = AW + (long long)blockIdx.y * n;
+  double s = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    s = fma(w[i], z[i], s);
+  const double t = block_sum(s);
+  if (threadIdx.x == 0) partial[blockIdx.y * kReduceBlocks + blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(kReduceThreads)
+cs_coef_kernel(int m, const double* __restrict__ partial, SmallMat Einv, double* __restrict__ c) {
+  __shared__ double d[4];
+  for (int j = 0; j < m; ++j) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < kReduceBlocks; b += blockDim.x) s += partial[j * kReduceBlocks + b];
+    const double t = block_sum(s);
+    if (threadIdx.x == 0) d[j] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < m) {
+    double v = 0.0;
+    for (int k = 0; k < m; ++k) v = fma(Einv.e[threadIdx.x * m + k], d[k], v);
+    c[threadIdx.x] = v;
+  }
+}
+
+__global__ void cs_sub_kernel(long long n, int m, const double* __restrict__ W, const double* __restrict__ c,
+                              double* __restrict__ z) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = 0.0;
+  for (int j = 0; j < m; ++j) v = fma(W[(long long)j * n + i], c[j], v);
+  z[i] -= v;
+}
+
+int launch_coarse_correct(long long n, int m, const double* W, const double* AW, const SmallMat& Einv, double* buf,
+                          double* z, hipStream_t s) {
+  if (m <= 0) return PMT_OK;
+  double* c = buf;
+  double* partial = buf + 4;
+  cs_dots_stage1<<<dim3(kReduceBlocks, m), kReduceThreads, 0, s>>>(n, AW, z, partial);
+  PMT_TRY(check_launch("cs_dots_stage1"));
+  cs_coef_kernel<<<1, kReduceThreads, 0, s>>>(m, partial, Einv, c);
+  PMT_TRY(check_launch("cs_coef_kernel"));
+  cs_sub_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, m, W, c, z);
+  return check_launch("cs_sub_kernel");
 }
 
 int launch_fill(long long n, double v, double* y, hipStream_t s) {
@@ -420,6 +480,13 @@ int pmt_dot(long long n, const double* x, const double* y, double* out, double* 
   return launch_dot(n, x, y, out, work, (hipStream_t)stream);
 }
 
+int pmt_jacobi_apply(long long n, double w, const double* diag, const double* r, double* z, void* stream) {
+  if (n < 0 || (n > 0 && (!diag || !r || !z))) return arg_error("pmt_jacobi_apply: bad argument");
+  if (n == 0) return PMT_OK;
+  jacobi_apply_kernel<<<blocks_for(n, 256), 256, 0, (hipStream_t)stream>>>(n, w, diag, r, z);
+  return check_launch("jacobi_apply_kernel");
+}
+
 int pmt_dot_noalloc(int k, long long n, const double* x, const double* y, double* out, void* stream) {
   if (k < 1 || n < 0 || !x || !y || !out) return arg_error("pmt_dot_noalloc: bad argument");
   if (k > kNoAllocMaxCols)
@@ -434,22 +501,33 @@ int pmt_dot_noalloc(int k, long long n, const double* x, const double* y, double
   return check_launch("dot_cols_stage2");
 }
 
+int pmt_bc_split_ndof(int nelx, int nely, int nelz, int ndof, const uint8_t* bcmask, double bcdiag,
+                      const double* rhs, double* rhs_loc, double* sol, void* stream) {
+  if (!grid_ok(nelx, nely, nelz) || ndof < 1 || ndof > 8 || !rhs || !rhs_loc || !sol)
+    return arg_error("pmt_bc_split: bad argument");
+  GridDims g = make_grid(nelx, nely, nelz);
+  bc_split_kernel<<<blocks_for(g.nnodes * ndof, 256), 256, 0, (hipStream_t)stream>>>(g.nnodes, ndof, bcmask, bcdiag,
+                                                                                     rhs, rhs_loc, sol);
+  return check_launch("bc_split_kernel");
+}
+
 int pmt_bc_split(int nelx, int nely, int nelz, const uint8_t* bcmask, double bcdiag, const double* rhs,
                  double* rhs_loc, double* sol, void* stream) {
-  if (!grid_ok(nelx, nely, nelz) || !rhs || !rhs_loc || !sol) return arg_error("pmt_bc_split: bad argument");
+  return pmt_bc_split_ndof(nelx, nely, nelz, nelz > 0 ? 3 : 2, bcmask, bcdiag, rhs, rhs_loc, sol, stream);
+}
+
+int pmt_axpy_free_ndof(int nelx, int nely, int nelz, int ndof, const uint8_t* bcmask, double a, const double* x,
+                       double* y, void* stream) {
+  if (!grid_ok(nelx, nely, nelz) || ndof < 1 || ndof > 8 || !x || !y) return arg_error("pmt_axpy_free: bad argument");
   GridDims g = make_grid(nelx, nely, nelz);
-  bc_split_kernel<<<blocks_for(g.nnodes * g.dim, 256), 256, 0, (hipStream_t)stream>>>(g.nnodes, g.dim, bcmask, bcdiag,
-                                                                                      rhs, rhs_loc, sol);
-  return check_launch("bc_split_kernel");
+  axpy_free_kernel<<<blocks_for(g.nnodes * ndof, 256), 256, 0, (hipStream_t)stream>>>(g.nnodes, ndof, bcmask, a, x,
+                                                                                      y);
+  return check_launch("axpy_free_kernel");
 }
 
 int pmt_axpy_free(int nelx, int nely, int nelz, const uint8_t* bcmask, double a, const double* x, double* y,
                   void* stream) {
-  if (!grid_ok(nelx, nely, nelz) || !x || !y) return arg_error("pmt_axpy_free: bad argument");
-  GridDims g = make_grid(nelx, nely, nelz);
-  axpy_free_kernel<<<blocks_for(g.nnodes * g.dim, 256), 256, 0, (hipStream_t)stream>>>(g.nnodes, g.dim, bcmask, a, x,
-                                                                                       y);
-  return check_launch("axpy_free_kernel");
+  return pmt_axpy_free_ndof(nelx, nely, nelz, nelz > 0 ? 3 : 2, bcmask, a, x, y, stream);
 }
 
 }  // extern "C"

@@ -280,13 +280,55 @@ def predict_displacement(net, domain, K, xfilt, f):
     return alpha * u
 
 
+def coarse_space(A, u):
+    """The learned coarse space of one solve: W = the prediction u split into its three displacement
+    components (three FP64 columns, Dirichlet dofs zero), AW = A W and Einv = (W^T A W)^-1 (host).
+    Columns whose energy vanishes are dropped.  Returns (W (m, n), AW (m, n), Einv (m, m) numpy)."""
+    u = _lib.as_device(u)
+    g = A.grid
+    cols = []
+    for d in range(3):
+        w = torch.zeros_like(u)
+        w[d::3] = u[d::3]  # one displacement component (a strided copy)
+        wm = torch.empty_like(w)
+        _lib.check(_lib.lib().pmt_bc_split(*g.nel3, _p(g.bcmask_dev), 1.0, _p(w), _p(wm), _p(torch.empty_like(w)),
+                                           _lib.stream()), "bc split")
+        cols.append(wm)
+    AWl = [A @ w for w in cols]
+    E = np.array([[_lib_dot(cols[i], AWl[j]) for j in range(3)] for i in range(3)])
+    keep = [i for i in range(3) if E[i, i] > 1e-12 * max(np.max(np.diag(E)), 1e-300)]
+    W = torch.stack([cols[i] for i in keep]).contiguous()
+    AW = torch.stack([AWl[i] for i in keep]).contiguous()
+    Einv = np.linalg.inv(0.5 * (E[np.ix_(keep, keep)] + E[np.ix_(keep, keep)].T))
+    return W, AW, Einv
+
+
+def deflated_start(A, W, Einv, b, x0=None):
+    """x0 + W Einv W^T (b - A x0): the start whose residual is orthogonal to span W (the Galerkin projection
+    of the solution onto W when x0 = 0), which deflated CG needs."""
+    b = _lib.as_device(b)
+    x = torch.zeros_like(b) if x0 is None else _lib.as_device(x0).clone()
+    r = A.residual(b, x)
+    coef = Einv @ np.array([_lib_dot(W[i], r) for i in range(W.shape[0])])
+    for i in range(W.shape[0]):
+        _lib.check(_lib.lib().pmt_axpby(x.numel(), float(coef[i]), _p(W[i]), 1.0, _p(x), _lib.stream()), "axpby")
+    return x
+
+
 class WarmStartCG:
     """CG whose initial guess comes from the network when that guess has the smaller residual;
-    a guarded start falls back to the plain solve (from the caller's x0) if the network start stalls."""
+    a guarded start falls back to the plain solve (from the caller's x0) if the network start stalls.
 
-    def __init__(self, cg, net, domain, xfilt_signal, guard_its: int = 5):
+    ``coarse_space=True`` (PCG with a learned coarse correction): the prediction's three displacement
+    components span a coarse space W; the solve starts from the caller's start corrected by
+    W (W^T A W)^-1 W^T r0 (its residual orthogonal to W) and every preconditioned residual is projected
+    A-orthogonally to W (``pmt_mg_set_coarse_space``: deflated PCG, whose directions never revisit the
+    error components that W already resolves)."""
+
+    def __init__(self, cg, net, domain, xfilt_signal, guard_its: int = 5, coarse_space: bool = False):
         self.cg, self.net, self.domain, self.xsig = cg, net, domain, xfilt_signal
         self.guard_its = guard_its
+        self.coarse = coarse_space
         self.used_network = False
         self.iterations = 0
         self.A = None
@@ -295,10 +337,29 @@ class WarmStartCG:
         self.A = A
         self.cg.update(A)
 
+    def _solve_coarse(self, rhs, x0):
+        u = self.net.forward(make_input(self.domain, self.xsig.state, rhs)).to(torch.float64)
+        W, AW, Einv = coarse_space(self.A, u)
+        start = deflated_start(self.A, W, Einv, rhs, x0)
+        self.used_network = True
+        lib = _lib.lib()
+        h = self.cg.preconditioner.handle
+        Eh = np.ascontiguousarray(Einv, dtype=np.float64)
+        _lib.check(lib.pmt_mg_set_coarse_space(h, W.shape[0], _p(W), _p(AW), Eh.ctypes.data_as(ctypes.c_void_p)),
+                   "coarse space")
+        try:
+            out = self.cg.solve(rhs, x0=start)
+        finally:
+            _lib.check(lib.pmt_mg_set_coarse_space(h, 0, None, None, None), "coarse space")
+        self.iterations = self.cg.iterations
+        return out
+
     def solve(self, rhs, x0=None, trans="N"):
         rhs = _lib.as_device(rhs)
         if rhs.ndim != 1:
             return self.cg.solve(rhs, x0=x0, trans=trans)
+        if self.coarse:
+            return self._solve_coarse(rhs, x0)
         guess = predict_displacement(self.net, self.domain, self.A, self.xsig.state, rhs)
         rg = self.A.residual(rhs, guess)
         r_net = np.sqrt(_lib_dot(rg, rg))
@@ -335,3 +396,48 @@ class WarmStartCG:
         out = self.cg.solve(rhs, x0=xg)
         self.iterations = its + self.cg.iterations
         return out
+
+
+# ---------------------------------------------------------------------- solver-generated training data
+def trajectory_samples(nel, n_pairs, seeds=(0,), volfracs=(0.3, 0.4, 0.5), iters=60, precision="fp32", omega=0.5,
+                       tol=1e-8, radius=1.5, move=0.2, keep_every=1):
+    """(input, normalised target) pairs from seeded SIMP/OC trajectories of the cantilever (filter r=1.5,
+    SIMP p=3, the MG-PCG solve of this package, OC with the reference's bisection,
+    ``pymoto/common/optimizers.py:337-437``): trajectory t starts from the uniform design at volume
+    fraction volfracs[t % len] plus seeded noise, and every ``keep_every``-th design iteration gives one
+    pair, until ``n_pairs`` pairs exist.  Returns (samples, info)."""
+    import pymoto_amd as pm
+    from . import problems
+    d = pm.VoxelDomain(*nel)
+    bc, f = problems.cantilever3d(d)
+    fd = _lib.as_device(f)
+    levels = pm.solvers.auto_levels(d)
+    samples, traj, total_its = [], 0, 0
+    while len(samples) < n_pairs:
+        seed = seeds[traj % len(seeds)] + 1000 * (traj // len(seeds))
+        vf = volfracs[traj % len(volfracs)]
+        rng = np.random.default_rng(seed)
+        x0 = np.clip(vf + 0.1 * (rng.random(d.nel) - 0.5), 0.0, 1.0)
+        sx = pm.Signal("x", _lib.as_device(x0))
+        solver = pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(d, levels, omega=omega, smooth_steps=2,
+                                                                         precision=precision), tol=tol)
+        with pm.Network() as fn:
+            sxf = pm.FilterConv(d, radius=radius)(sx)
+            ss = pm.MathExpression(f"{1e-9} + {1.0 - 1e-9}*inp0^3")(sxf)
+            sK = pm.AssembleStiffness(d, bc=bc)(ss)
+            su = pm.LinSolve(hermitian=True, solver=solver)(sK, fd)
+            sc = pm.EinSum("i,i->")(su, fd)
+        oc = pm.OC(sx, sc, fn, move=move, xmin=0.0, xmax=1.0, verbosity=0, maxvol=vf)
+        for it in range(iters):
+            fn.reset()
+            fn.response()
+            total_its += solver.iterations
+            if it % keep_every == 0:
+                samples.append((make_input(d, sxf.state, fd), normalized_target(su.state)))
+                if len(samples) >= n_pairs:
+                    break
+            sc.sensitivity = 1.0
+            fn.sensitivity()
+            sx.state = oc.step(sx.state, sx.sensitivity)
+        traj += 1
+    return samples, dict(trajectories=traj, pairs=len(samples), solver_iterations=total_its)

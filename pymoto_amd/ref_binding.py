@@ -12,8 +12,10 @@ keeps pyMOTO's solver contract (``pymoto/solvers/solvers.py:6-50``):
 * ``solve(rhs, x0=None, trans='N')`` (``solvers.py:36-50``) honours ``x0`` (``x0_is_zero = 0``) and
   ``trans`` ('N', 'T' and 'H' are the same solve for a real symmetric stiffness matrix); a 2D
   right-hand side is solved column by column.
-* Device buffers (densities, Dirichlet mask, one right-hand side and one solution) are allocated once
-  and freed by ``close()`` (also called on garbage collection), with the multigrid handle.
+* Device buffers (densities, Dirichlet mask, one right-hand side and one solution) and one HIP stream
+  are allocated once and freed by ``close()`` (also called on garbage collection), with the multigrid
+  handle.  The HIP runtime calls go through ``libpymoto_amd.so``'s own handle, so they reach the
+  runtime the library is linked against.
 """
 import ctypes
 import ctypes.util
@@ -27,12 +29,17 @@ _vp, _i, _d, _sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_size
 
 
 def _load():
-    hip = ctypes.CDLL(ctypes.util.find_library("amdhip64") or "libamdhip64.so")
+    pmt = ctypes.CDLL(_LIBPATH)
+    # the HIP entry points are looked up through the library's own handle, so they come from the HIP
+    # runtime libpymoto_amd.so is linked against (one runtime per process)
+    hip = pmt
     hip.hipMalloc.argtypes = [ctypes.POINTER(_vp), _sz]
     hip.hipFree.argtypes = [_vp]
     hip.hipMemcpy.argtypes = [_vp, _vp, _sz, _i]
     hip.hipDeviceSynchronize.argtypes = []
-    pmt = ctypes.CDLL(_LIBPATH)
+    hip.hipStreamCreate.argtypes = [ctypes.POINTER(_vp)]
+    hip.hipStreamSynchronize.argtypes = [_vp]
+    hip.hipStreamDestroy.argtypes = [_vp]
     pmt.pmt_last_error.restype = ctypes.c_char_p
     pmt.pmt_mg_create.argtypes = [_i] * 4 + [_vp] * 4
     pmt.pmt_mg_destroy.argtypes = [_vp]
@@ -80,6 +87,11 @@ class SolverMGPCG:
         self.d_x = self._alloc(domain.nel * 8)
         self.d_b = self._alloc(self.n * 8)
         self.d_u = self._alloc(self.n * 8)
+        # the solver's own (blocking) stream: the legacy default stream's synchronous copies order
+        # themselves with it, and the copies back wait for it explicitly
+        self.stream = _vp()
+        if self._hip.hipStreamCreate(ctypes.byref(self.stream)) != 0:
+            raise RuntimeError("hipStreamCreate failed")
 
     def _check(self, rc):
         if rc != 0:
@@ -98,6 +110,8 @@ class SolverMGPCG:
             raise RuntimeError("hipMemcpy host to device failed")
 
     def _d2h(self, arr, src):
+        if self._hip.hipStreamSynchronize(self.stream) != 0:
+            raise RuntimeError("hipStreamSynchronize failed")
         if self._hip.hipMemcpy(arr.ctypes.data, src, _sz(arr.nbytes), _HIP_D2H) != 0:
             raise RuntimeError("hipMemcpy device to host failed")
 
@@ -108,7 +122,7 @@ class SolverMGPCG:
         if x.size != self.domain.nel:
             raise ValueError("the design signal must hold one factor per element")
         self._h2d(self.d_x, x)  # the current design, every update
-        self._check(self._pmt.pmt_mg_update(self.h, self.d_x, self.d_mask, self.bcdiag, None))
+        self._check(self._pmt.pmt_mg_update(self.h, self.d_x, self.d_mask, self.bcdiag, self.stream))
 
     def _solve1(self, b, x0):
         self._h2d(self.d_b, np.asarray(b, dtype=np.float64))
@@ -116,7 +130,7 @@ class SolverMGPCG:
             self._h2d(self.d_u, np.asarray(x0, dtype=np.float64))
         it, rr = ctypes.c_int(), ctypes.c_double()
         self._check(self._pmt.pmt_pcg_solve(self.h, self.d_b, self.d_u, int(x0 is None), self.tol, self.maxit,
-                                            self.restart, ctypes.byref(it), ctypes.byref(rr), None, None))
+                                            self.restart, ctypes.byref(it), ctypes.byref(rr), None, self.stream))
         self.iterations, self.relres = it.value, rr.value
         out = np.empty(self.n)
         self._d2h(out, self.d_u)
@@ -140,6 +154,9 @@ class SolverMGPCG:
         if self.h:
             self._pmt.pmt_mg_destroy(self.h)
             self.h = _vp()
+        if getattr(self, "stream", None):
+            self._hip.hipStreamDestroy(self.stream)
+            self.stream = _vp()
         for p in self._bufs:
             self._hip.hipFree(p)
         self._bufs = []

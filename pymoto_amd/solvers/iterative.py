@@ -49,8 +49,22 @@ class DampedJacobi(Preconditioner):
         self.D = A.diagonal()
 
     def solve(self, rhs, x0=None, trans="N"):
-        raise NotImplementedError("on the device DampedJacobi is the smoother fused into the GeometricMultigrid "
-                                  "operator kernels; use it as GeometricMultigrid(smoother=DampedJacobi(w=...))")
+        """z = w r / diag(A) (``iterative.py:41-47``); a real matrix makes 'N', 'T' and 'H' the same."""
+        if trans not in ("N", "T", "H"):
+            raise TypeError("Only N, T, or H transposition is possible")
+        if self.D is None:
+            raise RuntimeError("DampedJacobi.update(A) must be called before solve")
+        rhs = _lib.as_device(rhs)
+        lib = _lib.lib()
+
+        def one(c):
+            z = torch.empty_like(c)
+            _lib.check(lib.pmt_jacobi_apply(c.numel(), float(self.w), _lib.ptr(self.D), _lib.ptr(c), _lib.ptr(z),
+                                            _lib.stream()), "Jacobi apply")
+            return z
+        if rhs.ndim == 1:
+            return one(rhs)
+        return torch.stack([one(rhs[:, k].contiguous()) for k in range(rhs.shape[1])], dim=1)
 
 
 class ChebyshevSmoother:
@@ -118,6 +132,7 @@ class GeometricMultigrid(Preconditioner):
         return len(self.levels()) + 1
 
     def _ensure_handle(self, A):
+        self._scalar = bool(getattr(A.grid, "scalar", False))
         chain = self.levels()
         omega = np.array([getattr(m.smoother, "w", 0.5) for m in chain], dtype=np.float64)
         cheb = isinstance(self.smoother, ChebyshevSmoother)
@@ -131,6 +146,15 @@ class GeometricMultigrid(Preconditioner):
         lib = _lib.lib()
         h = ctypes.c_void_p()
         K0 = np.ascontiguousarray(A.grid.K0, dtype=np.float64)
+        if self._scalar:
+            if self.precision != "fp64" or cheb:
+                raise NotImplementedError("scalar problems use the FP64 damped-Jacobi multigrid")
+            _lib.check(lib.pmt_smg_create(d.nelx, d.nely, d.nelz if d.dim == 3 else 0, len(chain) + 1,
+                                          omega.ctypes.data_as(ctypes.c_void_p), steps.ctypes.data_as(ctypes.c_void_p),
+                                          K0.ctypes.data_as(ctypes.c_void_p), ctypes.byref(h)),
+                       "scalar multigrid setup")
+            self._handle, self._key = h, key
+            return
         _lib.check(lib.pmt_mg_create(d.nelx, d.nely, d.nelz if d.dim == 3 else 0, len(chain) + 1,
                                      omega.ctypes.data_as(ctypes.c_void_p), steps.ctypes.data_as(ctypes.c_void_p),
                                      K0.ctypes.data_as(ctypes.c_void_p), ctypes.byref(h)), "multigrid setup")
@@ -143,10 +167,16 @@ class GeometricMultigrid(Preconditioner):
     def _release(self):
         if self._handle is not None:
             try:
-                _lib.load().pmt_mg_destroy(self._handle)
+                lib = _lib.load()
+                (lib.pmt_smg_destroy if getattr(self, "_scalar", False) else lib.pmt_mg_destroy)(self._handle)
             except Exception:
                 pass
             self._handle = None
+
+    @property
+    def scalar(self):
+        """True when the hierarchy is the scalar (one unknown per node) one."""
+        return bool(getattr(self, "_scalar", False))
 
     def __del__(self):
         self._release()
@@ -155,15 +185,18 @@ class GeometricMultigrid(Preconditioner):
         self.A = A
         self._ensure_handle(A)
         g = A.grid
-        _lib.check(_lib.lib().pmt_mg_update(self._handle, _lib.ptr(A.xe), _lib.ptr(g.bcmask_dev), g.bcdiag,
-                                            _lib.stream()), "multigrid update")
+        upd = _lib.lib().pmt_smg_update if self.scalar else _lib.lib().pmt_mg_update
+        _lib.check(upd(self._handle, _lib.ptr(A.xe), _lib.ptr(g.bcmask_dev), g.bcdiag, _lib.stream()),
+                   "multigrid update")
 
     @property
     def handle(self):
         return self._handle
 
     def device_bytes(self):
-        return _lib.load().pmt_mg_device_bytes(self._handle) if self._handle is not None else 0
+        if self._handle is None or self.scalar:
+            return 0
+        return _lib.load().pmt_mg_device_bytes(self._handle)
 
     def solve(self, rhs, x0=None, trans="N"):
         if x0 is not None:
@@ -171,9 +204,11 @@ class GeometricMultigrid(Preconditioner):
         rhs = _lib.as_device(rhs)
         lib = _lib.lib()
 
+        vcycle = lib.pmt_smg_vcycle if self.scalar else lib.pmt_mg_vcycle
+
         def one(c):
             z = torch.empty_like(c)
-            _lib.check(lib.pmt_mg_vcycle(self._handle, _lib.ptr(c), _lib.ptr(z), _lib.stream()), "V-cycle")
+            _lib.check(vcycle(self._handle, _lib.ptr(c), _lib.ptr(z), _lib.stream()), "V-cycle")
             return z
         if rhs.ndim == 1:
             return one(rhs)
@@ -190,8 +225,8 @@ class GeometricMultigrid(Preconditioner):
         """y = A_level u with the Galerkin operator of that level (level 0 = fine operator)."""
         u = _lib.as_device(u)
         y = torch.empty_like(u)
-        _lib.check(_lib.lib().pmt_mg_level_apply(self._handle, int(level), _lib.ptr(u), _lib.ptr(y), _lib.stream()),
-                   "level apply")
+        fn = _lib.lib().pmt_smg_level_apply if self.scalar else _lib.lib().pmt_mg_level_apply
+        _lib.check(fn(self._handle, int(level), _lib.ptr(u), _lib.ptr(y), _lib.stream()), "level apply")
         return y
 
 
@@ -250,23 +285,83 @@ class CG(LinearSolver):
         self.A = A
         if self.preconditioner is None:
             self.preconditioner = build_hierarchy(A.grid.domain, max(2, auto_levels(A.grid.domain)))
-        if not isinstance(self.preconditioner, GeometricMultigrid):
-            raise NotImplementedError("the device CG is preconditioned by GeometricMultigrid")
         self.preconditioner.update(A)
         if self.verbosity >= 1:
             torch.cuda.synchronize()
             print(f"CG Preconditioner set up in {np.round(time.perf_counter() - t0, 3)}s")
 
+    def _device_mg(self):
+        return isinstance(self.preconditioner, GeometricMultigrid)
+
+    def _solve_generic(self, b, x0):
+        """CG.solve for one right-hand side (``iterative.py:340-403``) with any preconditioner that has
+        ``solve(r)`` (e.g. DampedJacobi): the host drives the iteration, every product, dot and update
+        runs in the HIP library."""
+        lib = _lib.lib()
+        A, M = self.A, self.preconditioner
+        n = b.numel()
+        work = _lib.reduce_work()
+        sc = torch.empty(2, dtype=torch.float64, device=b.device)
+
+        def dot(u, v):
+            _lib.check(lib.pmt_dot(n, _lib.ptr(u), _lib.ptr(v), _lib.ptr(sc), _lib.ptr(work), _lib.stream()), "dot")
+            return float(sc[0].item())
+
+        def axpby(a, u, bb, v):
+            _lib.check(lib.pmt_axpby(n, float(a), _lib.ptr(u), float(bb), _lib.ptr(v), _lib.stream()), "axpby")
+
+        x = torch.zeros_like(b) if x0 is None else _lib.as_device(x0).clone()
+        r = A.residual(b, x)
+        bn = np.sqrt(dot(b, b))
+        rr = dot(r, r)
+        if not np.isfinite(bn) or not np.isfinite(rr):
+            raise _lib.NumericalError("PCG solve: non-finite right-hand side or start vector")
+        hist = []
+        if bn == 0.0:
+            self.iterations, self.relres, self.history = 0, 0.0, np.zeros(1)
+            return torch.zeros_like(b)
+        tval = np.sqrt(rr) / bn
+        hist.append(tval)
+        i = -1
+        if tval > self.tol:
+            z = M.solve(r)
+            p = torch.empty_like(z)
+            axpby(1.0 / np.sqrt(dot(z, z)), z, 0.0, p)  # orth normalises the first direction
+            for i in range(self.maxit):
+                q = A @ p
+                pq = dot(p, q)
+                alpha = dot(p, r) / pq
+                axpby(alpha, p, 1.0, x)
+                if i % self.restart == 0:
+                    r = A.residual(b, x)
+                else:
+                    axpby(-alpha, q, 1.0, r)
+                tval = np.sqrt(dot(r, r)) / bn
+                hist.append(tval)
+                if not np.isfinite(tval):
+                    raise _lib.NumericalError(f"PCG solve: non-finite residual at iteration {i + 1}")
+                if tval <= self.tol:
+                    break
+                z = M.solve(r)
+                beta = -dot(q, z) / pq
+                axpby(1.0, z, beta, p)
+        self.iterations, self.relres, self.history = i + 1, tval, np.asarray(hist)
+        if tval > self.tol:
+            warnings.warn(f"CG Maximum iterations ({self.maxit}) reached, with final residuals {tval}")
+        return x
+
     def _solve_1rhs(self, b, x0):
+        if not self._device_mg():
+            return self._solve_generic(b, x0)
         lib = _lib.lib()
         x = torch.zeros_like(b) if x0 is None else _lib.as_device(x0).clone()
         it = ctypes.c_int(0)
         rel = ctypes.c_double(0.0)
         hist = np.zeros(self.maxit + 1)
-        _lib.check(lib.pmt_pcg_solve(self.preconditioner.handle, _lib.ptr(b), _lib.ptr(x), int(x0 is None),
-                                     float(self.tol), int(self.maxit), int(self.restart), ctypes.byref(it),
-                                     ctypes.byref(rel), hist.ctypes.data_as(ctypes.c_void_p), _lib.stream()),
-                   "PCG solve")
+        pcg = lib.pmt_smg_pcg_solve if self.preconditioner.scalar else lib.pmt_pcg_solve
+        _lib.check(pcg(self.preconditioner.handle, _lib.ptr(b), _lib.ptr(x), int(x0 is None), float(self.tol),
+                       int(self.maxit), int(self.restart), ctypes.byref(it), ctypes.byref(rel),
+                       hist.ctypes.data_as(ctypes.c_void_p), _lib.stream()), "PCG solve")
         self.iterations, self.relres = it.value, rel.value
         self.history = hist[: it.value + 1]
         if rel.value > self.tol:
@@ -302,6 +397,15 @@ class CG(LinearSolver):
         if rhs.ndim == 2 and rhs.shape[1] == 1:
             return self._solve_1rhs(rhs[:, 0].contiguous(), None if x0 is None else _lib.as_device(x0).reshape(-1)
                                     ).reshape(-1, 1)
+        if not self._device_mg() or self.preconditioner.scalar:
+            # column by column (block PCG runs on the elasticity multigrid handle)
+            X0 = None if x0 is None else _lib.as_device(x0)
+            cols, its = [], []
+            for k in range(rhs.shape[1]):
+                cols.append(self._solve_1rhs(rhs[:, k].contiguous(), None if X0 is None else X0[:, k].contiguous()))
+                its.append(self.iterations)
+            self.iterations = max(its)
+            return torch.stack(cols, dim=1)
         Bt = rhs.t().contiguous()  # column j contiguous
         X0t = None if x0 is None else _lib.as_device(x0).t().contiguous()
         out, its = [], []

@@ -84,8 +84,8 @@ class LDAWrapper(LinearSolver):
         """(v with Dirichlet entries zeroed, v[bc]/bcdiag on Dirichlet entries and 0 elsewhere)."""
         g = self.A.grid
         rl, sol = torch.empty_like(v), torch.empty_like(v)
-        _lib.check(_lib.lib().pmt_bc_split(*g.nel3, _lib.ptr(g.bcmask_dev), g.bcdiag if g.bcdiag else 1.0,
-                                           _lib.ptr(v), _lib.ptr(rl), _lib.ptr(sol), _lib.stream()), "bc split")
+        _lib.check(_lib.lib().pmt_bc_split_ndof(*g.nel3, g.ndof, _lib.ptr(g.bcmask_dev), g.bcdiag if g.bcdiag else 1.0,
+                                                _lib.ptr(v), _lib.ptr(rl), _lib.ptr(sol), _lib.stream()), "bc split")
         return rl, sol
 
     def _solve_cols(self, cols, x0s):
@@ -123,8 +123,9 @@ class LDAWrapper(LinearSolver):
                                       x0=None if x0l is None else torch.stack(x0l, dim=1))
                 xnews = [X[:, i].contiguous() for i in range(len(todo))]
             for k, xnew in zip(todo, xnews):
-                _lib.check(_lib.lib().pmt_axpy_free(*g.nel3, _lib.ptr(g.bcmask_dev), 1.0, _lib.ptr(xnew),
-                                                    _lib.ptr(sols[k]), _lib.stream()), "axpy_free")
+                _lib.check(_lib.lib().pmt_axpy_free_ndof(*g.nel3, g.ndof, _lib.ptr(g.bcmask_dev), 1.0,
+                                                         _lib.ptr(xnew), _lib.ptr(sols[k]), _lib.stream()),
+                           "axpy_free")
             for xnew in xnews:
                 xadd, _ = self._split(xnew)
                 badd, _ = self._split(A @ xnew)

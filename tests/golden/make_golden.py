@@ -11,9 +11,10 @@ Why an import is needed for these (no closed form or library routine pins them e
   ``np.add.at``-assembled CSR data;
 * the iteration count and residual history of the reference's own GeometricMultigrid + CG;
 * FilterConv outputs for the 'edge' / 'wrap' / constant paddings and non-symmetric kernels,
-  which the reference's own tests do not pin.
+  which the reference's own tests do not pin;
+* the bytes of the VTI files written by VoxelDomain.write_to_vti (a file format: no formula pins it).
 
-Usage:  MPLBACKEND=Agg python tests/golden/make_golden.py
+Usage:  MPLBACKEND=Agg python tests/golden/make_golden.py [case ...]
 """
 import hashlib
 import os
@@ -185,7 +186,40 @@ def small_csr_cases():
     np.savez_compressed(os.path.join(OUT, "csr_cases.npz"), **out)
 
 
+def vti_cases():
+    """Bytes of VoxelDomain.write_to_vti (domain.py:452-604) for a 2D and a 3D grid: a cell scalar, a
+    nodal vector (2D: padded to 3 components), a (k, ndof) block of nodal vectors and a scaled grid.
+    The file format is pinned by nothing else (no closed form), so its bytes are the fixture."""
+    import tempfile
+    out = {}
+    specs = [((5, 3, 0), 1.0, 11), ((4, 3, 2), 0.5, 12)]
+    for i, (nel, scale, seed) in enumerate(specs):
+        dom = pym.VoxelDomain(*nel, unitx=1.0, unity=0.5, unitz=2.0)
+        rng = np.random.default_rng(seed)
+        vecs = {"x": rng.uniform(0.0, 1.0, dom.nel),
+                "u": rng.standard_normal(dom.nnodes * dom.dim),
+                "modes": rng.standard_normal((3, dom.nnodes * dom.dim))}
+        with tempfile.TemporaryDirectory() as tmp:
+            fn = os.path.join(tmp, "case.vti")
+            dom.write_to_vti(vecs, filename=fn, scale=scale)
+            with open(fn, "rb") as fh:
+                data = fh.read()
+        out[f"nel_{i}"] = np.array(nel)
+        out[f"unit_{i}"] = np.array([1.0, 0.5, 2.0])
+        out[f"scale_{i}"] = np.array(scale)
+        for k, v in vecs.items():
+            out[f"{k}_{i}"] = v
+        out[f"bytes_{i}"] = np.frombuffer(data, dtype=np.uint8)
+    out["ncases"] = len(specs)
+    np.savez_compressed(os.path.join(OUT, "vti_cases.npz"), **out)
+
+
 if __name__ == "__main__":
+    if len(sys.argv) > 1:  # only the named cases, e.g. `make_golden.py vti_cases`
+        for name in sys.argv[1:]:
+            globals()[name]()
+        sys.exit(0)
+    vti_cases()
     c1_mbb()
     c2_small_direct((16, 8, 8), seed=2)
     c2_small_direct((24, 12, 12), seed=3)

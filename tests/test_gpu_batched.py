@@ -43,7 +43,10 @@ def test_robust_heaviside_multiload_batch_vs_oracle(gpu_lib, prec, tol, bound):
     d = pm.VoxelDomain(*nel)
     bc = problems.cantilever3d(*nel)[0]
     F = problems.face_point_loads(*nel, 4, 17)
-    x = np.random.default_rng(5).uniform(0.0, 1.0, d.nel)
+    # densities U[0.35, 1]: with U[0, 1] the filtered field is 0.5 +- 0.1 and its eta = 0.7 projection a
+    # disconnected void (mean SIMP factor 0.017) on which the oracle's own block CG (the reference
+    # algorithm) stalls at 1e-8..1e-10 after 500 iterations; with U[0.35, 1] it takes 16 / 17 / 44
+    x = np.random.default_rng(5).uniform(0.35, 1.0, d.nel)
     ref = robust_iteration(nel, x, bc, F, radius=1.5, etas=(0.3, 0.5, 0.7), beta=8.0)
     for r in ref:
         sx = pm.Signal("x", dev(x))

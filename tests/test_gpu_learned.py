@@ -158,3 +158,107 @@ def test_oc_update_vs_oracle(gpu_lib):
         xn = oc.step().cpu().numpy()
     ref = oc_step(x, dg, maxvol=0.45, move=0.2)
     npt.assert_allclose(xn, ref, rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_coarse_space_deflation_vs_oracle_pcg(gpu_lib, prec):
+    """pmt_mg_set_coarse_space: PCG whose preconditioned residuals are projected A-orthogonally to a space W,
+    z <- (I - W (W^T A W)^-1 (AW)^T) M r, started from x0 = W (W^T A W)^-1 W^T b (deflated CG), against the
+    oracle's CG (iterative.py:340-403) with the same projected preconditioner in numpy: same iteration
+    count within one (FP64 V-cycle) and the same solution to the CG tolerance; with the FP32 V-cycle the
+    FP64 CG still reaches the tolerance.  A space holding the exact solution's components gives a start
+    that is the solution."""
+    from pymoto_amd import _lib
+    from pymoto_amd import learned as PL
+    from oracle.solvers import CG as OracleCG
+    from oracle.solvers import build_mg
+    import ctypes
+    nel = (16, 8, 8)
+    d = pm.VoxelDomain(*nel)
+    bc, f = problems.cantilever3d(*nel)
+    rng = np.random.default_rng(4)
+    x = fe.simp(rng.uniform(0.05, 1.0, d.nel))
+    asm = fe.Assembly(*nel, fe.element_stiffness(3), bc=bc)
+    A = asm.matrix(x)
+    n = A.shape[0]
+    W = rng.standard_normal((2, n))
+    W[:, bc] = 0.0
+    AW = (A @ W.T).T
+    Einv = np.linalg.inv(W @ AW.T)
+    x0 = W.T @ (Einv @ (W @ f))
+    K = pm.AssembleStiffness(d, bc=bc)(torch.as_tensor(x).cuda())
+    cg = pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(d, 3, omega=0.5, smooth_steps=2, precision=prec),
+                       tol=1e-10, maxit=500)
+    cg.update(K)
+    Wd, AWd = torch.as_tensor(W).cuda().contiguous(), torch.as_tensor(AW).cuda().contiguous()
+    Eh = np.ascontiguousarray(Einv)
+    lib = _lib.lib()
+    h = cg.preconditioner.handle
+    _lib.check(lib.pmt_mg_set_coarse_space(h, 2, _lib.ptr(Wd), _lib.ptr(AWd), Eh.ctypes.data_as(ctypes.c_void_p)))
+    try:
+        u = cg.solve(torch.as_tensor(f).cuda(), x0=torch.as_tensor(x0).cuda()).cpu().numpy()
+        its = cg.iterations
+    finally:
+        _lib.check(lib.pmt_mg_set_coarse_space(h, 0, None, None, None))
+    assert cg.relres <= 1e-10
+    uref = spla_solve(A, f)
+    assert np.max(np.abs(u - uref)) / np.max(np.abs(uref)) < 1e-8
+    if prec == "fp64":
+        omg = build_mg(nel, 3, omega=0.5, smooth_steps=2)
+
+        class Projected:
+            def update(self, A_):
+                omg.update(A_)
+
+            def solve(self, r):
+                z = omg.solve(r)
+                return z - W.T @ (Einv @ (AW @ z))
+        ocg = OracleCG(preconditioner=Projected(), tol=1e-10, maxit=500)
+        ocg.update(A)
+        ocg.solve(f, x0=x0)
+        assert abs(its - ocg.iterations) <= 1, (its, ocg.iterations)
+    # the exact solution's three components as the space: the deflated start is the solution
+    Wx, AWx, Ex = PL.coarse_space(K, torch.as_tensor(uref).cuda())
+    assert Wx.shape[0] == 3
+    xs = PL.deflated_start(K, Wx, Ex, torch.as_tensor(f).cuda())
+    r = K.residual(torch.as_tensor(f).cuda(), xs).cpu().numpy()
+    assert np.linalg.norm(r) / np.linalg.norm(f) < 1e-10
+
+
+def spla_solve(A, f):
+    import scipy.sparse.linalg as spla
+    return spla.spsolve(A.tocsc(), f)
+
+
+def test_warm_start_with_coarse_correction_reaches_tolerance(gpu_lib):
+    """WarmStartCG(coarse_space=True) on OC-trajectory designs (trajectory_samples) at 32x16x16: every solve
+    ends at the FP64 tolerance with the solution of the plain solve (to 1e-7), the handle's coarse space
+    is cleared afterwards, and deflation by a trained network's space needs no more iterations than the
+    plain solve on average (deflation removes error components; it cannot add any)."""
+    from pymoto_amd import learned as PL
+    nel = (32, 16, 16)
+    d = pm.VoxelDomain(*nel)
+    samples, info = PL.trajectory_samples(nel, 24, seeds=(3,), volfracs=(0.4,), iters=12)
+    assert info["pairs"] == 24 and info["trajectories"] == 2
+    net = PL.WarmStartNet(nel, seed=2)
+    hist = PL.train(net, samples[:18], epochs=8, lr=1e-3)
+    assert hist[-1] < hist[0]
+    bc, f = problems.cantilever3d(*nel)
+    fd = torch.as_tensor(f).cuda()
+    its_c, its_p = [], []
+    rng = np.random.default_rng(8)
+    for k in range(3):
+        x = np.clip(0.4 + 0.1 * (rng.random(d.nel) - 0.5), 0, 1)
+        sxf = pm.Signal("xf", torch.as_tensor(x).cuda())
+        K = pm.AssembleStiffness(d, bc=bc)(torch.as_tensor(fe.simp(x)).cuda())
+        cg = pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(d, 3, omega=0.5, smooth_steps=2), tol=1e-9)
+        ws = PL.WarmStartCG(cg, net, d, sxf, coarse_space=True)
+        ws.update(K)
+        u = ws.solve(fd)
+        its_c.append(ws.iterations)
+        assert cg.relres <= 1e-9
+        up = cg.solve(fd)
+        its_p.append(cg.iterations)
+        npt.assert_allclose(u.cpu().numpy(), up.cpu().numpy(), rtol=0, atol=1e-7 * up.abs().max().item())
+    print(f"coarse-corrected iterations {its_c} vs plain {its_p}")
+    assert np.mean(its_c) <= np.mean(its_p) + 1

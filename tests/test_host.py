@@ -281,3 +281,28 @@ def test_face_point_loads_match_oracle():
     npt.assert_array_equal(pp.face_point_loads(d, 4, 7), problems.face_point_loads(12, 6, 4, 4, 7))
     F = pp.face_point_loads(d, 4, 7)
     assert np.all(np.sum(F, axis=0) == -1.0) and np.count_nonzero(F) == 4
+
+
+def _vti_golden_cases():
+    from conftest import golden
+    g = golden("vti_cases.npz")
+    for i in range(int(g["ncases"])):
+        nel = tuple(int(v) for v in g[f"nel_{i}"])
+        unit = g[f"unit_{i}"]
+        vecs = {k: g[f"{k}_{i}"] for k in ("x", "u", "modes")}
+        yield nel, unit, float(g[f"scale_{i}"]), vecs, g[f"bytes_{i}"].tobytes()
+
+
+def test_vti_writer_bytes_equal_reference_golden(tmp_path):
+    """The bytes of the reference's own VoxelDomain.write_to_vti (domain.py:452-604; fixture written by
+    tests/golden/make_golden.py) for a 2D and a 3D grid: cell scalar, nodal vector (2D padded to 3
+    components) and a block of 3 nodal vectors - from this package's writer (numpy inputs) and from the
+    oracle's restatement."""
+    from oracle.io import write_vti
+    for nel, unit, scale, vecs, want in _vti_golden_cases():
+        d = pm.VoxelDomain(*nel, unitx=unit[0], unity=unit[1], unitz=unit[2])
+        a, b = tmp_path / f"pkg{nel[2]}.vti", tmp_path / f"orc{nel[2]}.vti"
+        d.write_to_vti(vecs, str(a), scale=scale)
+        write_vti(d, vecs, str(b), scale=scale)
+        assert a.read_bytes() == want
+        assert b.read_bytes() == want

@@ -345,3 +345,96 @@ def test_robust_iteration_oracle_vs_finite_differences():
         for k, o in enumerate(out):
             fd = (cp[k]["c"] - cm[k]["c"]) / (2 * e)
             npt.assert_allclose(o["dcdx"][i], fd, rtol=2e-6)
+
+
+# ---------------------------------------------------------------------------------------------
+# scalar (Poisson) element and the partitioned solve with prescribed unknowns
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_poisson_element_one_element_heat_conduction(dim):
+    """The reference's one-element wall (tests/test_assembly.py:286-340): T = 0 on the x = 0 face, heat
+    Q spread over the x = Lx face; T there equals Q Lx / (k A) with A the cross-section (Ly Lz)."""
+    Lx, Ly, Lz, k, Q = 2.0, 3.0, 4.0, 1.0, 1.0
+    P0 = fe.element_poisson(dim, (Lx, Ly, Lz), k)
+    nodes = np.arange(2 ** dim)
+    left, right = nodes[(nodes & 1) == 0], nodes[(nodes & 1) == 1]
+    asm = fe.Assembly(1, 1, 1 if dim == 3 else 0, P0, bc=left)
+    q = np.zeros(2 ** dim)
+    q[right] = Q / right.size
+    T = np.linalg.solve(asm.matrix(np.ones(1)).toarray(), q)
+    np.testing.assert_allclose(T[left], 0.0, atol=1e-12)
+    np.testing.assert_allclose(T[right], Q * Lx / (k * Ly * Lz), rtol=1e-12)
+
+
+def test_poisson_element_textbook_forms():
+    # bilinear unit square: (1/6) [[4,-1,-1,-2], ...] in node order 0(-,-) 1(+,-) 2(-,+) 3(+,+), unit thickness
+    sq = np.array([[4, -1, -1, -2], [-1, 4, -2, -1], [-1, -2, 4, -1], [-2, -1, -1, 4]]) / 6.0
+    np.testing.assert_allclose(fe.element_poisson(2), sq, rtol=1e-14, atol=1e-15)
+    # trilinear unit cube: 1/3 on the diagonal, 0 along an edge, -1/12 across a face and the body diagonal
+    P = fe.element_poisson(3)
+    for a in range(8):
+        for b in range(8):
+            hamming = bin(a ^ b).count("1")
+            want = {0: 1 / 3, 1: 0.0, 2: -1 / 12, 3: -1 / 12}[hamming]
+            assert abs(P[a, b] - want) < 1e-14
+    # scaling: k and the element size (2D: thickness h_z multiplies, square shape invariant in 2D)
+    np.testing.assert_allclose(fe.element_poisson(2, (2.0, 2.0, 3.0), 5.0), 15.0 * sq, rtol=1e-14)
+    np.testing.assert_allclose(fe.element_poisson(3, (2.0, 2.0, 2.0), 1.0), 2.0 * P, rtol=1e-14)
+    for d in (2, 3):
+        Pd = fe.element_poisson(d, (0.7, 1.3, 2.1), 1.7)
+        np.testing.assert_allclose(Pd, Pd.T, atol=1e-15)
+        np.testing.assert_allclose(Pd @ np.ones(2 ** d), 0.0, atol=1e-14)  # constants in the null space
+
+
+def _fd_check(fun, x, grad, h=1e-6, rtol=1e-5, n=6, seed=0):
+    rng = np.random.default_rng(seed)
+    for _ in range(n):
+        dx = rng.standard_normal(x.shape)
+        fd = (fun(x + h * dx) - fun(x - h * dx)) / (2 * h)
+        an = float(np.sum(grad * dx))
+        assert abs(fd - an) <= rtol * max(abs(fd), abs(an), 1e-12), (fd, an)
+
+
+@pytest.mark.parametrize("nel,physics", [((10, 10, 0), "elastic"), ((6, 4, 4), "elastic"), ((8, 6, 0), "thermal")])
+def test_system_of_equations_oracle(nel, physics):
+    """SystemOfEquations (linalg.py:222-349): identities and finite differences of g = c1 . x + c2 . b through
+    the densities, the applied load and the prescribed values (the reference tests test_linsolve_sparse.py:206-420
+    check the same three gradients by finite differences)."""
+    from oracle.pipeline import SystemOfEquations
+    nelx, nely, nelz = nel
+    dim = fe.grid_dim(*nel)
+    K0 = fe.element_poisson(dim) if physics == "thermal" else fe.element_stiffness(dim)
+    ndof = K0.shape[0] // 2 ** dim
+    asm = fe.Assembly(nelx, nely, nelz, K0)
+    n = asm.n
+    nny, nnz = nely + 1, max(nelz, 0) + 1
+    left = np.array([(k * nny + j) * (nelx + 1) for k in range(nnz) for j in range(nny)])
+    right = left + nelx
+    # every unknown of the x = 0 face and the last one of the x = max face (no rigid-body mode left free)
+    p = np.unique(np.concatenate([(left[:, None] * ndof + np.arange(ndof)).ravel(), right * ndof + ndof - 1]))
+    f = np.setdiff1d(np.arange(n), p)
+    rng = np.random.default_rng(1)
+    x0 = rng.uniform(0.1, 1.0, asm.dc.shape[0])
+    bf0 = rng.standard_normal(f.size)
+    xp0 = rng.standard_normal(p.size)
+    c1, c2 = rng.standard_normal(n), rng.standard_normal(n)
+
+    def run(xe, bf, xp):
+        soe = SystemOfEquations(free=f, prescribed=p)
+        X, B = soe.forward(asm.matrix(xe), bf, xp)
+        return soe, X, B
+
+    soe, X, B = run(x0, bf0, xp0)
+    A = asm.matrix(x0)
+    np.testing.assert_allclose(A @ X, B, rtol=1e-9, atol=1e-9 * np.abs(B).max())
+    np.testing.assert_array_equal(X[p], xp0)
+    np.testing.assert_array_equal(B[f], bf0)
+    lam, xx, dbf, dxp = soe.sensitivity(c1, c2)
+    dxe = asm.sens_contract([lam], [xx])
+
+    def g(xe=x0, bf=bf0, xp=xp0):
+        _, X_, B_ = run(xe, bf, xp)
+        return float(c1 @ X_ + c2 @ B_)
+    _fd_check(lambda v: g(xe=v), x0, dxe)
+    _fd_check(lambda v: g(bf=v), bf0, dbf, rtol=1e-6)
+    _fd_check(lambda v: g(xp=v), xp0, dxp, rtol=1e-6)
