@@ -50,16 +50,23 @@ def parse():
     p.add_argument("--levels", type=int, default=0, help="grids in the MG hierarchy (0 = reference example rule)")
     p.add_argument("--tol", type=float, default=1e-8)
     p.add_argument("--omega", type=float, default=0.6, help="Jacobi damping of variants without '@omega'")
-    p.add_argument("--variants", default="fp64,fp64@0.5,fp32@0.5,bf16@0.5",
-                   help="comma-separated precision[@omega] list; the first fp64 entry is the reference")
+    p.add_argument("--variants", default="fp64,fp64@0.5,fp32@0.5,bf16@0.5,fp32@0.5+learned",
+                   help="comma-separated precision[@omega][+learned] list; the first fp64 entry is the reference")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--no-batched", action="store_true", help="skip the batched multi-load workload")
     p.add_argument("--learned-samples", type=int, default=24, help="training pairs for '+learned' variants")
     p.add_argument("--learned-epochs", type=int, default=4)
-    p.add_argument("--cpu-nel", type=int, nargs=3, default=[48, 24, 24])
+    p.add_argument("--c5-pairs", type=int, default=2000,
+                   help="solver-generated OC-trajectory pairs of the 128x64x64 learned workload (0 = skip it)")
+    p.add_argument("--c5-epochs", type=int, default=2)
+    p.add_argument("--cpu-nel", type=int, nargs=3, default=[64, 32, 32])
     p.add_argument("--ku-only", action="store_true",
                    help="kernel-only driver for hardware counters: 5 fine FP64 K.u launches, then 5 copies of one "
                         "nodal vector (calibration of the byte counters); prints no JSON line")
+    p.add_argument("--kernels-only", action="store_true",
+                   help="kernel-only driver: 5 launches each of the fine K.u (FP64, FP32, BF16), FilterConv forward "
+                        "and adjoint, the sensitivity contraction and the learned network's forward pass, then 5 "
+                        "copies of one nodal vector; prints no JSON line")
     return p.parse_args()
 
 
@@ -71,7 +78,7 @@ def parse_variant(v):
     return prec.strip(), (float(om) if om else None)
 
 
-def build_network(pm, torch, nel, levels, tol, precision, omega=0.6, steps=2, warm_net=None):
+def build_network(pm, torch, nel, levels, tol, precision, omega=0.6, steps=2, warm_net=None, coarse_space=True):
     from pymoto_amd import problems
     d = pm.VoxelDomain(*nel)
     bc, f = problems.cantilever3d(d)
@@ -88,7 +95,7 @@ def build_network(pm, torch, nel, levels, tol, precision, omega=0.6, steps=2, wa
         inner = solver
         if warm_net is not None:
             from pymoto_amd.learned import WarmStartCG
-            inner = WarmStartCG(solver, warm_net, d, sxf)
+            inner = WarmStartCG(solver, warm_net, d, sxf, coarse_space=coarse_space)
         lin = pm.LinSolve(hermitian=True, solver=inner)
         su = lin(sK, fd)
         sc = pm.EinSum("i,i->")(su, fd)
@@ -115,6 +122,87 @@ def train_warm_start(pm, torch, args, precision, omega, nsamples, epochs):
     hist = learned.train(model, samples, epochs=epochs, lr=2e-3)
     torch.cuda.synchronize()
     return model, time.perf_counter() - t0, hist
+
+
+def learned_workload(pm, torch, nel=(128, 64, 64), pairs=2000, epochs=2, precision="fp32", omega=0.5, tol=1e-8,
+                     n_eval=8):
+    """Learned warm start / coarse correction at 128x64x64: ``pairs`` (input, displacement) pairs from
+    seeded SIMP/OC trajectories solved by this package's MG-PCG (volume fractions 0.3/0.4/0.5, 60 design
+    iterations each), the network trained on them on the GPU, then evaluated on the designs of a held-out
+    trajectory (another seed): CG iterations and seconds per solve for the plain MG-PCG, the network's warm
+    start, and warm start plus the learned coarse correction (flexible PCG), every solve to tol; the
+    compliance of each against an FP64 MG-PCG solve to 1e-12."""
+    import time as _t
+    from pymoto_amd import learned, problems
+    t0 = _t.perf_counter()
+    samples, info = learned.trajectory_samples(nel, pairs, seeds=(11, 12, 13), precision=precision, omega=omega,
+                                               tol=tol)
+    torch.cuda.synchronize()
+    t_data = _t.perf_counter() - t0
+    net = learned.WarmStartNet(nel, seed=0)
+    t0 = _t.perf_counter()
+    hist = learned.train(net, samples, epochs=epochs, lr=1e-3)
+    torch.cuda.synchronize()
+    t_train = _t.perf_counter() - t0
+    del samples
+    torch.cuda.empty_cache()
+    # held-out trajectory: designs of another seed
+    d = pm.VoxelDomain(*nel)
+    bc, f = problems.cantilever3d(d)
+    fd = torch.as_tensor(f, device="cuda")
+    designs = []
+    rng = np.random.default_rng(97)
+    x = torch.as_tensor(np.clip(0.4 + 0.1 * (rng.random(d.nel) - 0.5), 0.0, 1.0), device="cuda")
+    sx = pm.Signal("x", x)
+    levels = pm.solvers.auto_levels(d)
+    with pm.Network() as fn:
+        sxf = pm.FilterConv(d, radius=1.5)(sx)
+        ss = pm.MathExpression(f"{1e-9} + {1.0 - 1e-9}*inp0^3")(sxf)
+        sK = pm.AssembleStiffness(d, bc=bc)(ss)
+        su = pm.LinSolve(hermitian=True, solver=pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(
+            d, levels, omega=omega, smooth_steps=2, precision=precision), tol=tol))(sK, fd)
+        sc = pm.EinSum("i,i->")(su, fd)
+    oc = pm.OC(sx, sc, fn, move=0.2, xmin=0.0, xmax=1.0, verbosity=0, maxvol=0.4)
+    for it in range(n_eval):
+        fn.reset()
+        fn.response()
+        designs.append((sxf.state.clone(), ss.state.clone()))
+        sc.sensitivity = 1.0
+        fn.sensitivity()
+        sx.state = oc.step(sx.state, sx.sensitivity)
+    out = {"plain": [], "warm": [], "warm+coarse": []}
+    times = {k: 0.0 for k in out}
+    comp = {k: [] for k in out}
+    comp_ref = []
+    for xf, xs in designs:
+        K = pm.AssembleStiffness(d, bc=bc)(xs)
+        ref = pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(d, levels, omega=omega, smooth_steps=2),
+                            tol=1e-12, maxit=5000)
+        ref.update(K)
+        comp_ref.append(float(torch.dot(ref.solve(fd), fd)))
+        for mode in out:
+            cg = pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(d, levels, omega=omega, smooth_steps=2,
+                                                                         precision=precision), tol=tol)
+            solver = cg if mode == "plain" else learned.WarmStartCG(cg, net, d, pm.Signal("xf", xf),
+                                                                   coarse_space=(mode == "warm+coarse"))
+            solver.update(K)
+            torch.cuda.synchronize()
+            t1 = _t.perf_counter()
+            u = solver.solve(fd)
+            torch.cuda.synchronize()
+            times[mode] += _t.perf_counter() - t1
+            out[mode].append(int(solver.iterations))
+            comp[mode].append(float(torch.dot(u, fd)))
+    res = {"workload": f"learned warm start / coarse correction, cantilever {nel[0]}x{nel[1]}x{nel[2]} hex8, "
+                       f"{precision} V-cycle (omega {omega}) in FP64 CG, tol {tol:g}",
+           "training_pairs": info["pairs"], "trajectories": info["trajectories"], "epochs": epochs,
+           "data_s": t_data, "train_s": t_train, "loss_history": [float(h) for h in hist],
+           "params": int(net.nparams()), "held_out_designs": len(designs)}
+    for mode in out:
+        res[mode] = {"cg_iterations": out[mode], "s_per_solve": times[mode] / len(designs),
+                     "compliance_rel_err_vs_fp64": float(max(abs(a - b) / abs(b) for a, b in zip(comp[mode],
+                                                                                               comp_ref)))}
+    return res
 
 
 def design_iteration(net, x):
@@ -180,8 +268,12 @@ def kernel_roofline(pm, torch, nel, reps=20):
     return out
 
 
-def ku_driver(pm, torch, nel, reps=5):
-    """Kernel-only run for rocprofv3 --pmc: the fine FP64 K.u, then a plain copy of one nodal vector."""
+def ku_driver(pm, torch, nel, reps=5, which="all"):
+    """Kernel-only run for rocprofv3 (--pmc counter passes, --kernel-trace): five launches each of the fine
+    K.u in FP64 (reflection basis), FP32, BF16 (matrix cores), the FilterConv forward and adjoint, the
+    sensitivity contraction, the learned network's forward pass (conv3d on the matrix cores), then five
+    plain copies of one nodal vector (calibration of the byte counters).  ``which`` = "ku" runs the FP64
+    K.u and the copies only.  Prints one line per group."""
     from pymoto_amd import problems
     d = pm.VoxelDomain(*nel)
     bc, _ = problems.cantilever3d(d)
@@ -190,21 +282,71 @@ def ku_driver(pm, torch, nel, reps=5):
     u = torch.randn(3 * d.nnodes, dtype=torch.float64, device="cuda")
     for _ in range(reps):
         K @ u
+    torch.cuda.synchronize()
+    print(f"fine K.u FP64 x{reps}", flush=True)
+    if which == "all":
+        # V-cycles of the headline hierarchy (2 damped-Jacobi sweeps per level, omega 0.5): the fine sweeps
+        # (fine_refl_kernel in Jacobi mode) in FP64 and FP32
+        for prec in ("fp64", "fp32"):
+            mg = pm.solvers.build_hierarchy(d, pm.solvers.auto_levels(d), omega=0.5, smooth_steps=2,
+                                            precision=prec)
+            mg.update(K)
+            for _ in range(reps):
+                mg.solve(u)
+            torch.cuda.synchronize()
+            print(f"V-cycle {prec} x{reps}", flush=True)
+            del mg
+        u32 = u.to(torch.float32)
+        for prec in ("fp32", "bf16"):
+            for _ in range(reps):
+                K.apply_reduced(u32, prec)
+            torch.cuda.synchronize()
+            print(f"fine K.u {prec} x{reps}", flush=True)
+        filt = pm.FilterConv(d, radius=1.5)
+        sx = pm.Signal("x", x)
+        sy = filt(sx)
+        sy.sensitivity = torch.randn_like(x)
+        for _ in range(reps):
+            filt.response()
+            sx.reset()
+            filt.sensitivity()
+        torch.cuda.synchronize()
+        print(f"FilterConv forward + adjoint x{reps}", flush=True)
+        mod = pm.AssembleStiffness(d, bc=bc)
+        sxs = pm.Signal("xs", x)
+        sK = mod(sxs)
+        v = torch.randn_like(u)
+        for _ in range(reps):
+            sK.sensitivity = pm.DyadicMatrix(u, v, fac=-1.0)
+            sxs.reset()
+            mod.sensitivity()
+        torch.cuda.synchronize()
+        print(f"sensitivity contraction x{reps}", flush=True)
+        if all(n % 8 == 0 for n in nel):
+            from pymoto_amd import learned
+            net = learned.WarmStartNet(nel, seed=0)
+            inp = learned.make_input(d, x, torch.zeros(3 * d.nnodes, dtype=torch.float64, device="cuda"))
+            for _ in range(reps):
+                net.forward(inp)
+            torch.cuda.synchronize()
+            print(f"learned network forward x{reps}", flush=True)
     c = torch.empty_like(u)
     for _ in range(reps):
         c.copy_(u)
     torch.cuda.synchronize()
-    print(f"K.u x{reps}, copy x{reps}: vector of {u.numel()} FP64 = {u.numel() * 8 / 1024:.1f} KiB", flush=True)
+    print(f"copy x{reps}: vector of {u.numel()} FP64 = {u.numel() * 8 / 1024:.1f} KiB", flush=True)
 
 
-def batched_workload(pm, torch, nel=(128, 64, 64), beta=8.0, etas=(0.4, 0.5, 0.6), nloads=4, nuneven=16, seed=7,
-                     precision="fp32", omega=0.5, tol=1e-8, xmin=1e-5):
-    """Robust multi-load batch: 3 Heaviside designs (eta 0.4/0.5/0.6, beta 8, SIMP xmin 1e-5: the dilated /
-    nominal / eroded projections and stiffness floor of the reference's robust example,
-    examples/topology_optimization/ex_compliance_robust.py:27,137-149) x 4 seeded point loads on
-    the x = max face, each design one 4-column block PCG (captured-graph iterations), timed against
-    the same 12 solves one right-hand side at a time; plus 16 independent problems of seeded sizes
-    from 64x32 Q4 to 96x48x48 hex8, solved concurrently on 8 streams and timed against one stream."""
+def batched_workload(pm, torch, nel=(128, 64, 64), beta=8.0, etas=(0.3, 0.5, 0.7), nloads=4, nuneven=16, seed=7,
+                     precision="fp32", omega=0.5, tol=1e-8, xmin=1e-9):
+    """Robust multi-load batch: 3 Heaviside designs (eta 0.3/0.5/0.7, beta 8, SIMP xmin 1e-9: dilated /
+    nominal / eroded projections as in the reference's robust example,
+    examples/topology_optimization/ex_compliance_robust.py:137-149) x 4 seeded point loads on
+    the x = max face, each design one 4-column block PCG (captured-graph iterations), one after the other
+    and the three at once on three streams, timed against the same 12 solves one right-hand side at a time
+    and as 12 concurrent single-right-hand-side solves (one handle and stream each); the fine operator's
+    launch shape is picked by the autotuner first.  Plus 16 independent problems of seeded sizes from 64x32
+    Q4 to 96x48x48 hex8, solved concurrently on 8 streams and timed against one stream."""
     import time as _t
     from pymoto_amd import problems
     from pymoto_amd.batch import solve_independent
@@ -232,6 +374,9 @@ def batched_workload(pm, torch, nel=(128, 64, 64), beta=8.0, etas=(0.4, 0.5, 0.6
         return pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(d, levels, omega=omega, smooth_steps=2,
                                                                        precision=precision), tol=tol, graph=graph,
                              maxit=2000)
+    # launch shapes of the fine operator picked by the autotuner on this grid (applied process-wide)
+    from pymoto_amd import tune
+    tuned, _ = tune.autotune_fine_operator(Ks[0], precision=precision if precision != "fp64" else "fp64")
     solvers = [make_solver() for _ in etas]
     for K, sv in zip(Ks, solvers):  # setup + graph capture, untimed
         sv.update(K)
@@ -245,6 +390,11 @@ def batched_workload(pm, torch, nel=(128, 64, 64), beta=8.0, etas=(0.4, 0.5, 0.6
         its_block.append(int(sv.iterations))
     torch.cuda.synchronize()
     t_block = _t.perf_counter() - t0
+    # the three block solves at once, one stream (and handle) per design
+    solve_independent([(K, Fd, sv) for K, sv in zip(Ks, solvers)], workers=3)
+    t0 = _t.perf_counter()
+    _, its_block_conc = solve_independent([(K, Fd, sv) for K, sv in zip(Ks, solvers)], workers=3)
+    t_block_conc = _t.perf_counter() - t0
     t0 = _t.perf_counter()
     its_single = []
     for K, sv in zip(Ks, solvers):
@@ -255,6 +405,13 @@ def batched_workload(pm, torch, nel=(128, 64, 64), beta=8.0, etas=(0.4, 0.5, 0.6
             its_single[-1].append(int(sv.iterations))
     torch.cuda.synchronize()
     t_single = _t.perf_counter() - t0
+    # the 12 right-hand sides as 12 independent solves at once (a handle each, 12 streams)
+    jobs12 = [(K, Fd[:, c].contiguous(), make_solver(graph=False)) for K in Ks for c in range(nloads)]
+    solve_independent(jobs12, workers=12)  # setup, untimed
+    t0 = _t.perf_counter()
+    _, its_conc12 = solve_independent(jobs12, workers=12)
+    t_single_conc = _t.perf_counter() - t0
+    del jobs12
 
     # uneven batch of independent problems
     jobs = []
@@ -291,15 +448,21 @@ def batched_workload(pm, torch, nel=(128, 64, 64), beta=8.0, etas=(0.4, 0.5, 0.6
                     f"{nloads} point loads per design; {precision} V-cycle (omega {omega}), FP64 CG, tol {tol:g}",
         "rhs_solves": len(etas) * nloads, "ndof": ndof,
         "block_graph_s": t_block, "block_iterations": its_block,
+        "block_graph_3_streams_s": t_block_conc, "block_3_streams_iterations": its_block_conc,
         "single_rhs_s": t_single, "single_rhs_iterations": its_single,
-        "dof_rhs_per_s_block": ndof * len(etas) * nloads / t_block,
+        "single_rhs_12_streams_s": t_single_conc, "single_rhs_12_streams_iterations": its_conc12,
+        "dof_rhs_per_s_best": ndof * len(etas) * nloads / min(t_block, t_block_conc, t_single, t_single_conc),
+        "autotuned_fine_operator": tuned,
         "uneven": {"problems": nuneven, "total_dofs": dofs, "concurrent_8_streams_s": t_conc,
                    "one_stream_s": t_seq, "iterations": its_uneven},
     }
 
 
-def cpu_baseline(nel, steps=3):
-    """The CPU oracle (numpy/scipy restatement of the reference) on a bounded 3D sample."""
+def cpu_baseline(nel, levels, omega, steps=3):
+    """The CPU oracle (numpy/scipy restatement of the reference) on a bounded 3D sample, with the multigrid
+    level rule and Jacobi damping of the GPU variant it is compared with.  Returns the mean seconds per
+    design iteration (first iteration excluded when there are several), the sample's dof count and the CG
+    iteration counts.  The oracle is single-threaded: scipy's sparse products and SuperLU run on one core."""
     from oracle import problems
     from oracle.pipeline import design_iteration as cpu_iter
     bc, f = problems.cantilever3d(*nel)
@@ -309,7 +472,7 @@ def cpu_baseline(nel, steps=3):
     for k in range(steps):
         x = np.random.default_rng(100 + k).uniform(0.0, 1.0, nelt)
         t0 = time.perf_counter()
-        out = cpu_iter(tuple(nel), x, bc, f, radius=1.5, solver="mgcg", mg_levels=4, omega=0.6, smooth_steps=2,
+        out = cpu_iter(tuple(nel), x, bc, f, radius=1.5, solver="mgcg", mg_levels=levels, omega=omega, smooth_steps=2,
                        tol=1e-8, x0=u)
         times.append(time.perf_counter() - t0)
         its.append(out["iterations"])
@@ -348,8 +511,8 @@ def main():
 
     nel = tuple(args.nel)
     nelt = nel[0] * nel[1] * nel[2]
-    if args.ku_only:
-        ku_driver(pm, torch, nel)
+    if args.ku_only or args.kernels_only:
+        ku_driver(pm, torch, nel, which="ku" if args.ku_only else "all")
         return
     nsteps = args.warmup + args.steps
     xs = [torch.as_tensor(np.random.default_rng(1000 + k).uniform(0.0, 1.0, nelt), device="cuda")
@@ -418,6 +581,8 @@ def main():
         line["config"]["learned"] = learned_info
     if not args.no_batched:
         line["config"]["batched"] = batched_workload(pm, torch)
+    if args.c5_pairs > 0:
+        line["config"]["learned_128x64x64"] = learned_workload(pm, torch, pairs=args.c5_pairs, epochs=args.c5_epochs)
     if roof is not None:
         line["roofline"] = {"bound": "hbm", "achieved": roof["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
                             "frac": roof["gbs"] / PEAK_HBM_GBS, "traffic": roof["traffic"], "kernel": roof["kernel"],
@@ -425,12 +590,21 @@ def main():
                             "fp32_product": roof["fp32"], "tuning": roof["tuning"],
                             "note": "compulsory bytes 2*ndof*8 + nel*8 per K.u; HIP events on the launch stream"}
     if not args.no_cpu:
-        t_cpu, ndof_cpu, its = cpu_baseline(tuple(args.cpu_nel))
+        cpu_nel = tuple(args.cpu_nel)
+        cpu_levels = pm.solvers.auto_levels(pm.VoxelDomain(*cpu_nel))
+        best_prec, best_om = parse_variant(best)
+        cpu_omega = best_om if isinstance(best_om, float) else args.omega
+        t_cpu, ndof_cpu, its = cpu_baseline(cpu_nel, cpu_levels, cpu_omega)
         line["cpu_baseline"] = {
             "value": t_cpu * ndof / ndof_cpu, "unit": "s/design-iter", "cores": 1, "kind": "port",
-            "sample": f"oracle (numpy/scipy restatement of pyMOTO) 3D cantilever {args.cpu_nel[0]}x{args.cpu_nel[1]}x"
-                      f"{args.cpu_nel[2]} ({ndof_cpu} dofs), 4-grid MG-CG tol 1e-8: {t_cpu:.3f} s/design-iter "
-                      f"measured (CG its {its}); value scaled linearly in dofs to {ndof} dofs",
+            "scaled": True, "measured_value": t_cpu, "measured_ndof": ndof_cpu,
+            "host_cpus": len(os.sched_getaffinity(0)),
+            "thread_env": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "MKL_NUM_THREADS", "OPENBLAS_NUM_THREADS")},
+            "sample": f"oracle (numpy/scipy restatement of pyMOTO, single-threaded scipy sparse + SuperLU) 3D "
+                      f"cantilever {cpu_nel[0]}x{cpu_nel[1]}x{cpu_nel[2]} ({ndof_cpu} dofs), {cpu_levels}-grid MG-CG "
+                      f"(the reference example's level rule) with omega {cpu_omega} as the GPU variant {best}, tol "
+                      f"1e-8: {t_cpu:.3f} s/design-iter measured (CG its {its}); value = that time scaled linearly in "
+                      f"dofs to {ndof} dofs (SCALED, not measured at that size)",
         }
     print(json.dumps(line), flush=True)
 
