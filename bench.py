@@ -64,9 +64,10 @@ def parse():
                    help="kernel-only driver for hardware counters: 5 fine FP64 K.u launches, then 5 copies of one "
                         "nodal vector (calibration of the byte counters); prints no JSON line")
     p.add_argument("--kernels-only", action="store_true",
-                   help="kernel-only driver: 5 launches each of the fine K.u (FP64, FP32, BF16), FilterConv forward "
-                        "and adjoint, the sensitivity contraction and the learned network's forward pass, then 5 "
-                        "copies of one nodal vector; prints no JSON line")
+                   help="kernel-only driver: 5 launches each of the fine K.u (FP64, FP32, BF16), V-cycles (FP64, "
+                        "FP32), FilterConv forward and adjoint, SIMP forward and adjoint, the sensitivity contraction "
+                        "and the learned network's forward pass, then 5 copies of one nodal vector; prints no JSON "
+                        "line")
     return p.parse_args()
 
 
@@ -270,8 +271,8 @@ def kernel_roofline(pm, torch, nel, reps=20):
 
 def ku_driver(pm, torch, nel, reps=5, which="all"):
     """Kernel-only run for rocprofv3 (--pmc counter passes, --kernel-trace): five launches each of the fine
-    K.u in FP64 (reflection basis), FP32, BF16 (matrix cores), the FilterConv forward and adjoint, the
-    sensitivity contraction, the learned network's forward pass (conv3d on the matrix cores), then five
+    K.u in FP64 (reflection basis), FP32, BF16 (matrix cores), V-cycles (FP64, FP32), the FilterConv
+    forward and adjoint, the SIMP power law forward and adjoint, the sensitivity contraction, the learned network's forward pass (conv3d on the matrix cores), then five
     plain copies of one nodal vector (calibration of the byte counters).  ``which`` = "ku" runs the FP64
     K.u and the copies only.  Prints one line per group."""
     from pymoto_amd import problems
@@ -312,6 +313,16 @@ def ku_driver(pm, torch, nel, reps=5, which="all"):
             filt.sensitivity()
         torch.cuda.synchronize()
         print(f"FilterConv forward + adjoint x{reps}", flush=True)
+        simp = pm.MathExpression(f"{1e-9} + {1.0 - 1e-9}*inp0^3")
+        sxp = pm.Signal("xp", x)
+        syp = simp(sxp)
+        syp.sensitivity = torch.randn_like(x)
+        for _ in range(reps):
+            simp.response()
+            sxp.reset()
+            simp.sensitivity()
+        torch.cuda.synchronize()
+        print(f"SIMP forward + adjoint x{reps}", flush=True)
         mod = pm.AssembleStiffness(d, bc=bc)
         sxs = pm.Signal("xs", x)
         sK = mod(sxs)

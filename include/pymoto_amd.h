@@ -46,7 +46,10 @@ int pmt_filter_fwd(int nelx, int nely, int nelz, int kx, int ky, int kz, const d
 /* FilterConv adjoint: correlate 'full', drop constant cells, fold back onto elements.
  * Replaces FilterConv._sensitivity (filter.py:214-220).  inv_ptr/inv_idx: per-axis lists of the
  * padded coordinates mapping to each element coordinate, concatenated x|y|z (inv_ptr holds
- * nelx+1, nely+1, max(nelz,1)+1 entries, absolute offsets into inv_idx). */
+ * nelx+1, nely+1, max(nelz,1)+1 entries, absolute offsets into inv_idx).  Every element coordinate
+ * farther than (k-1)/2 from both ends of its axis must map from its own padded coordinate only (true
+ * for the reference's padding modes): the default kernels reduce those from LDS tiles and walk the lists
+ * only within that distance of a face. */
 int pmt_filter_bwd(int nelx, int nely, int nelz, int kx, int ky, int kz, const double* w,
                    const int32_t* inv_ptr, const int32_t* inv_idx, const double* dy, double* dx,
                    void* stream);
@@ -139,19 +142,26 @@ int pmt_k0_pattern(int dim, const double* K0_host);
  * mid-plane reflections (FP64, then FP32) - and the fine operator multiplies each element in that
  * basis (2*72 add/sub + 72 multiply-adds instead of 576 multiply-adds).  Kernels that only need K0
  * read the first (2^D*D)^2 entries of the same buffer.
+ * PMT_K0_RIGID (with PMT_K0_REFLECTION): the blocks' entries of the three rigid translations vanish (to
+ * 1e-13 of the largest block entry; true for every stiffness matrix); with PMT_K0_CLASS_PATTERN the
+ * reflection-basis kernels then skip all the pattern's zero entries.
  * pmt_k0_extend fills ext_host[PMT_K0_EXT_LEN] from a host K0 and returns the k0_pattern bits it
  * supports (PMT_K0_REFLECTION when K0 commutes with the reflections to 1e-13 of its largest entry).
  * Host only.  (No reference counterpart: a property of K0 from AssembleStiffness.__init__,
  * assembly.py:446-461.) */
 #define PMT_K0_CLASS_PATTERN 1
 #define PMT_K0_REFLECTION 2
+#define PMT_K0_RIGID 4
 #define PMT_K0_EXT_LEN 684
 int pmt_k0_extend(int dim, const double* K0_host, double* ext_host);
 /* Launch-shape tuning of the fine-grid operator, process-wide, read at every launch.
  * PMT_TUNE_FINE_KERNEL: 0 = 3D node tiles, 1 = xy node columns streamed along z, 2 = streamed
  * columns multiplying in the reflection basis (default; used when k0_pattern has PMT_K0_REFLECTION,
  * else 1; 3D only).
- * PMT_TUNE_REFL_ROWS: node rows per block of the reflection-basis kernel, 16 (default) or 32.
+ * PMT_TUNE_REFL_ROWS: block shape of the reflection-basis kernel, node rows per block: 16 (default),
+ * 24 or 32 rows of 16 nodes, 12 rows of 32 nodes, or 0 = the shape with the fewest threads per node
+ * plane.  All shapes give the same bits for the operator and the sweeps (the fused dots of PCG are
+ * grouped by block).
  * PMT_TUNE_STREAM_PLANES: node planes per streamed column, 0 = chosen from the CU count (default).
  * PMT_TUNE_STREAM_TILE: streamed column shape, 0 = 32x8 (default), 1 = 64x4, 2 = 16x16 nodes.
  * PMT_TUNE_COARSE1: operator of the first coarse level of hex8 hierarchies, 1 = element by element in
@@ -244,7 +254,8 @@ int pmt_mg_smoother_interval(pmt_mg* mg, int level, double* lo, double* hi);
  * next update.  Hex8 grids: the coarsest dense inverse may still be in progress on the handle's own
  * stream when this returns; pmt_mg_vcycle, pmt_pcg_solve, pmt_block_pcg_solve and the next update
  * make their stream wait for it (with the legacy default stream, NULL, the first of them waits for it on
- * the host). */
+ * the host).  Calls on one handle share its work vectors: the caller orders them (one stream, or events
+ * between streams); they must not overlap on the device. */
 int pmt_mg_update(pmt_mg* mg, const double* xe, const uint8_t* bcmask, double bcdiag, void* stream);
 /* z = M r, one V-cycle (GeometricMultigrid.solve with x0=None, iterative.py:222-256). */
 int pmt_mg_vcycle(pmt_mg* mg, const double* r, double* z, void* stream);

@@ -8,7 +8,9 @@
 // gathers the kx*ky*kz window ('valid' convolution).  Adjoint: one thread per element visits the
 // padded cells that map onto it (per-axis inverse lists; interior elements have exactly one) and
 // evaluates the 'full' correlation there - a gather, so no atomics and a deterministic sum.
-#include "common.h"
+#include <algorithm>
+
+#include "launch.h"
 
 namespace pmt {
 
@@ -141,17 +143,24 @@ filter_fwd_fixed_kernel(int nx, int ny, int nz, const double* __restrict__ w, co
   y[e] = acc;
 }
 
+// The element's padded cells are exactly its own position shifted by the half widths (the element is at
+// least (K-1)/2 from every mirrored, replicated or wrapped face, or the padding is constant).
 template <int KXY, int KZ>
-__global__ void __launch_bounds__(256)
-filter_bwd_fixed_kernel(int nx, int ny, int nz, const double* __restrict__ w, const int32_t* __restrict__ inv_ptr,
-                        const int32_t* __restrict__ inv_idx, const double* __restrict__ dy, double* __restrict__ dx) {
-  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long nel = (long long)nx * ny * nz;
-  if (e >= nel) return;
-  const int i = (int)(e % nx);
-  const long long t = e / nx;
-  const int j = (int)(t % ny);
-  const int k = (int)(t / ny);
+__device__ inline bool filter_single_cell(const int32_t* __restrict__ inv_ptr, const int32_t* __restrict__ inv_idx,
+                                          int nx, int ny, int i, int j, int k) {
+  const int32_t* px = inv_ptr;
+  const int32_t* py = inv_ptr + (nx + 1);
+  const int32_t* pz = inv_ptr + (nx + 1) + (ny + 1);
+  return px[i + 1] - px[i] == 1 && inv_idx[px[i]] == i + (KXY - 1) / 2 && py[j + 1] - py[j] == 1 &&
+         inv_idx[py[j]] == j + (KXY - 1) / 2 && pz[k + 1] - pz[k] == 1 && inv_idx[pz[k]] == k + (KZ - 1) / 2;
+}
+
+// dx_e of element (i, j, k) by its inverse lists: the 'full' correlation of dy at every padded cell
+// that maps onto it, in list order (the taps in the order r, q, p)
+template <int KXY, int KZ>
+__device__ inline double filter_bwd_gather(int nx, int ny, int nz, const double* __restrict__ w,
+                                           const int32_t* __restrict__ inv_ptr, const int32_t* __restrict__ inv_idx,
+                                           const double* __restrict__ dy, int i, int j, int k) {
   const int32_t* px = inv_ptr;
   const int32_t* py = inv_ptr + (nx + 1);
   const int32_t* pz = inv_ptr + (nx + 1) + (ny + 1);
@@ -180,61 +189,128 @@ filter_bwd_fixed_kernel(int nx, int ny, int nz, const double* __restrict__ w, co
       }
     }
   }
-  dx[e] = acc;
+  return acc;
 }
 
-// LDS-tiled forms of the fixed-extent kernels (default).  A block owns TX x TY output columns and walks
-// a chunk of NZC element planes in z; the padded input planes it needs pass through a ring of KZ LDS
-// planes of (TX + KXY - 1) x (TY + KXY - 1) values, each loaded once from HBM (through the per-axis maps,
-// 32-bit index math) while the previous plane is reduced, so an element is read about once per chunk
-// instead of KXY^2 KZ times through the caches.  Taps are summed in the order of the gather kernels
-// above (r, then q, then p), so both forms give the same bits.
-constexpr int kFiltTX = 64, kFiltTY = 4, kFiltNZC = 16;
-int g_filter_tiled = 1;  // PMT_TUNE_FILTER: 1 tiled (default), 0 per-element gathers
+template <int KXY, int KZ>
+__global__ void __launch_bounds__(256)
+filter_bwd_fixed_kernel(int nx, int ny, int nz, const double* __restrict__ w, const int32_t* __restrict__ inv_ptr,
+                        const int32_t* __restrict__ inv_idx, const double* __restrict__ dy, double* __restrict__ dx) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nel = (long long)nx * ny * nz;
+  if (e >= nel) return;
+  const int i = (int)(e % nx);
+  const long long t = e / nx;
+  const int j = (int)(t % ny);
+  const int k = (int)(t / ny);
+  dx[e] = filter_bwd_gather<KXY, KZ>(nx, ny, nz, w, inv_ptr, inv_idx, dy, i, j, k);
+}
 
+// Shell of the tiled adjoint: the elements within the half widths of a face, where every padding mode puts
+// the elements with more than one padded cell (mirrored, replicated or wrapped cells map there), in three
+// disjoint slab sets - x slabs (all j, k), y slabs (interior i), z slabs (interior i, j).  Those with a
+// single cell are left to the tiled kernel.
+template <int KXY, int KZ>
+__global__ void __launch_bounds__(256)
+filter_bwd_shell_kernel(int nx, int ny, int nz, const double* __restrict__ w, const int32_t* __restrict__ inv_ptr,
+                        const int32_t* __restrict__ inv_idx, const double* __restrict__ dy, double* __restrict__ dx) {
+  constexpr int HX = (KXY - 1) / 2, HZ = (KZ - 1) / 2;
+  const int wx = min(nx, 2 * HX), wy = min(ny, 2 * HX), wz = min(nz, 2 * HZ);
+  const long long cx = (long long)wx * ny * nz, cy = (long long)(nx - wx) * wy * nz;
+  const long long cz = (long long)(nx - wx) * (ny - wy) * wz;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  // slab index a of width wa on an axis of extent n: the first half-width, then the last
+  auto side = [](int a, int wa, int n, int h) { return (a < h || wa == n) ? a : n - wa + a; };
+  int i, j, k;
+  if (t < cx) {
+    i = side((int)(t % wx), wx, nx, HX);
+    t /= wx;
+    j = (int)(t % ny);
+    k = (int)(t / ny);
+  } else if ((t -= cx) < cy) {
+    i = wx / 2 + (int)(t % (nx - wx));
+    t /= (nx - wx);
+    j = side((int)(t % wy), wy, ny, HX);
+    k = (int)(t / wy);
+  } else if ((t -= cy) < cz) {
+    i = wx / 2 + (int)(t % (nx - wx));
+    t /= (nx - wx);
+    j = wy / 2 + (int)(t % (ny - wy));
+    k = side((int)(t / (ny - wy)), wz, nz, HZ);
+  } else {
+    return;
+  }
+  if (filter_single_cell<KXY, KZ>(inv_ptr, inv_idx, nx, ny, i, j, k)) return;
+  dx[((long long)k * ny + j) * nx + i] = filter_bwd_gather<KXY, KZ>(nx, ny, nz, w, inv_ptr, inv_idx, dy, i, j, k);
+}
+
+// LDS-tiled forms of the fixed-extent kernels (default).  A block of 64 x 4 threads owns 64 x 16 output
+// columns (a thread the column of 4 outputs above its row) and walks a chunk of nzc element planes in z;
+// the input planes the tile needs pass through a ring of KZ LDS planes of (64 + KXY - 1) x (16 + KXY - 1)
+// values, each fetched once from HBM while the previous plane is reduced.  A thread reads the
+// (4 + KXY - 1) x KXY values of a plane that its 4 outputs share once and carries the 4 tap chains
+// together (a quarter of the LDS reads of one output per thread), and every chain sums its taps in the
+// order of the gather kernels above (r, then q, then p), so both forms give the same bits.
+constexpr int kFiltTX = 64, kFiltTY = 4, kFiltOY = 4;
+constexpr int kFiltRY = kFiltTY * kFiltOY;  // output rows of a tile
+int g_filter_tiled = 1;                     // PMT_TUNE_FILTER: 1 tiled (default), 0 per-element gathers
+
+// Forward: the padded value at (a, b, c) is cz[c] if mz[c] < 0, else cy[b] if my[b] < 0, else cx[a] if
+// mx[a] < 0, else x[mz[c], my[b], mx[a]]; everything but mz[c] is the same for every plane, so each
+// thread resolves its cells' in-plane sources once and a plane costs one load per cell.
 template <int KXY, int KZ>
 __global__ void __launch_bounds__(kFiltTX * kFiltTY)
-filter_fwd_tiled_kernel(int nx, int ny, int nz, int tiles_x, int tiles_y, const double* __restrict__ w,
+filter_fwd_tiled_kernel(int nx, int ny, int nz, int tiles_x, int tiles_y, int nzc, int zero, const double* __restrict__ w,
                         const int32_t* __restrict__ map, const double* __restrict__ cval, const double* __restrict__ x,
                         double* __restrict__ y) {
-  constexpr int LX = kFiltTX + KXY - 1, LY = kFiltTY + KXY - 1, LP = LX * LY, NT = kFiltTX * kFiltTY;
+  constexpr int LX = kFiltTX + KXY - 1, LY = kFiltRY + KXY - 1, LP = LX * LY, NT = kFiltTX * kFiltTY;
   constexpr int PER = (LP + NT - 1) / NT;  // plane values loaded per thread
+  constexpr int WY = kFiltOY + KXY - 1;    // rows of a thread's window
   __shared__ double ring[KZ][LP];
   const int tid = threadIdx.x, tx = tid % kFiltTX, ty = tid / kFiltTX;
   const int tile = blockIdx.x % (tiles_x * tiles_y), chunk = blockIdx.x / (tiles_x * tiles_y);
-  const int i0 = (tile % tiles_x) * kFiltTX, j0 = (tile / tiles_x) * kFiltTY;
-  const int kbeg = chunk * kFiltNZC, kend = min(kbeg + kFiltNZC, nz);
+  const int i0 = (tile % tiles_x) * kFiltTX, j0 = (tile / tiles_x) * kFiltRY;
+  const int kbeg = chunk * nzc, kend = min(kbeg + nzc, nz);
   const int Lx = nx + KXY - 1, Ly = ny + KXY - 1, Lz = nz + KZ - 1;
+  const int nxy = nx * ny;
   const int32_t* mx = map;
   const int32_t* my = map + Lx;
   const int32_t* mz = map + Lx + Ly;
-  const double* cx = cval;
-  const double* cy = cval + Lx;
   const double* cz = cval + Lx + Ly;
-  // padded plane c (padded z index) of the tile: value at local (la, lb) = padded (i0 + la, j0 + lb)
-  auto fetch = [&](int c, double (&v)[PER]) {
-    const bool zin = c < Lz;
-    const int mc = zin ? mz[c] : 0;
+  // in-plane source of each cell this thread loads: element offset (>= 0), or -1 with its constant in
+  // cst (constant row or column), or -2 outside the padded grid / the tile (value 0)
+  int src[PER];
+  double cst[PER];
 #pragma unroll
-    for (int t = 0; t < PER; ++t) {
-      const int q = tid + t * NT;
-      const int la = q % LX, lb = q / LX;
-      const int a = i0 + la, b = j0 + lb;
-      double val = 0.0;
-      if (q < LP && zin && a < Lx && b < Ly) {
-        if (mc < 0) {
-          val = cz[c];
+  for (int t = 0; t < PER; ++t) {
+    const int q = tid + t * NT;
+    const int a = i0 + q % LX, b = j0 + q / LX;
+    src[t] = -2;
+    cst[t] = 0.0;
+    if (q < LP && a < Lx && b < Ly) {
+      const int mb = my[b];
+      if (mb < 0) {
+        src[t] = -1;
+        cst[t] = cval[Lx + b];
+      } else {
+        const int ma = mx[a];
+        if (ma < 0) {
+          src[t] = -1;
+          cst[t] = cval[a];
         } else {
-          const int mb = my[b];
-          if (mb < 0) {
-            val = cy[b];
-          } else {
-            const int ma = mx[a];
-            val = ma < 0 ? cx[a] : x[(mc * ny + mb) * nx + ma];
-          }
+          src[t] = mb * nx + ma;
         }
       }
-      v[t] = val;
+    }
+  }
+  auto fetch = [&](int c, double (&v)[PER]) {
+    const int mc = c < Lz ? mz[c] : 0;
+    const double czc = (c < Lz && mc < 0) ? cz[c] : 0.0;
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      double val = src[t] == -1 ? cst[t] : 0.0;
+      if (src[t] >= 0 && mc >= 0) val = x[mc * nxy + src[t]];  // a constant plane (mc < 0) reads nothing
+      v[t] = (mc < 0 && src[t] != -2) ? czc : val;
     }
   };
   auto store = [&](int c, const double (&v)[PER]) {
@@ -250,56 +326,77 @@ filter_fwd_tiled_kernel(int nx, int ny, int nz, int tiles_x, int tiles_y, const 
     store(c, v);
   }
   fetch(kbeg + KZ - 1, v);
-  const int i = i0 + tx, j = j0 + ty;
-  const bool out = i < nx && j < ny;
+  const int i = i0 + tx, jb = j0 + ty * kFiltOY;
   for (int k = kbeg; k < kend; ++k) {
     store(k + KZ - 1, v);  // padded plane k + KZ - 1 completes the window of output plane k
     __syncthreads();
     if (k + 1 < kend) fetch(k + KZ, v);  // next plane's loads in flight during the reduction
-    double acc = 0.0;
+    // the weights are re-read per plane through the scalar cache (zero is 0; hoisted out of the loop they
+    // would occupy KXY^2 KZ register pairs per thread)
+    const double* wk = w + zero * k;
+    double acc[kFiltOY];
+#pragma unroll
+    for (int o = 0; o < kFiltOY; ++o) acc[o] = 0.0;
 #pragma unroll
     for (int r = 0; r < KZ; ++r) {
-      const double* pl = ring[(k + KZ - 1 - r) % KZ];
+      const double* pl = ring[(k + KZ - 1 - r) % KZ] + ty * kFiltOY * LX + tx;
+      double win[WY][KXY];
+#pragma unroll
+      for (int yy = 0; yy < WY; ++yy)
+#pragma unroll
+        for (int p = 0; p < KXY; ++p) win[yy][p] = pl[yy * LX + p];
 #pragma unroll
       for (int q = 0; q < KXY; ++q)
 #pragma unroll
-        for (int p = 0; p < KXY; ++p)
-          acc = fma(w[(p * KXY + q) * KZ + r], pl[(ty + KXY - 1 - q) * LX + tx + KXY - 1 - p], acc);
+        for (int p = 0; p < KXY; ++p) {
+          const double wt = wk[(p * KXY + q) * KZ + r];
+#pragma unroll
+          for (int o = 0; o < kFiltOY; ++o) acc[o] = fma(wt, win[o + KXY - 1 - q][KXY - 1 - p], acc[o]);
+        }
+      if constexpr (KXY > 3) __builtin_amdgcn_sched_barrier(0);  // one plane's window live at a time
     }
-    if (out) y[((long long)k * ny + j) * nx + i] = acc;
+#pragma unroll
+    for (int o = 0; o < kFiltOY; ++o)
+      if (i < nx && jb + o < ny) y[(k * ny + jb + o) * nx + i] = acc[o];
     __syncthreads();  // the slot of plane k is overwritten next
   }
 }
 
-// Adjoint: dx_e = sum over the padded cells mapping to e of the 'full' correlation of dy there.  An
-// element whose three per-axis inverse lists hold only its own padded position (every element at least
-// (K-1)/2 from a mirrored, edge-replicated or wrapped face) is reduced from an LDS ring of dy planes
-// (zero outside the grid); the others take the per-element path of filter_bwd_fixed_kernel.
+// Adjoint: dx_e = sum over the padded cells mapping to e of the 'full' correlation of dy there.  The
+// tiled kernel reduces, from an LDS ring of dy planes (zero outside the grid), the elements whose only
+// padded cell is their own position (all but a shell of (K-1)/2 along mirrored, replicated or wrapped
+// faces; every element with constant padding); filter_bwd_shell_kernel then does the shell's elements
+// by their inverse lists, so no wave of the tiled kernel diverges into the list walk.
 template <int KXY, int KZ>
 __global__ void __launch_bounds__(kFiltTX * kFiltTY)
-filter_bwd_tiled_kernel(int nx, int ny, int nz, int tiles_x, int tiles_y, const double* __restrict__ w,
+filter_bwd_tiled_kernel(int nx, int ny, int nz, int tiles_x, int tiles_y, int nzc, int zero, const double* __restrict__ w,
                         const int32_t* __restrict__ inv_ptr, const int32_t* __restrict__ inv_idx,
                         const double* __restrict__ dy, double* __restrict__ dx) {
   constexpr int HX = (KXY - 1) / 2, HZ = (KZ - 1) / 2;
-  constexpr int LX = kFiltTX + KXY - 1, LY = kFiltTY + KXY - 1, LP = LX * LY, NT = kFiltTX * kFiltTY;
+  constexpr int LX = kFiltTX + KXY - 1, LY = kFiltRY + KXY - 1, LP = LX * LY, NT = kFiltTX * kFiltTY;
   constexpr int PER = (LP + NT - 1) / NT;
+  constexpr int WY = kFiltOY + KXY - 1;
   __shared__ double ring[KZ][LP];
   const int tid = threadIdx.x, tx = tid % kFiltTX, ty = tid / kFiltTX;
   const int tile = blockIdx.x % (tiles_x * tiles_y), chunk = blockIdx.x / (tiles_x * tiles_y);
-  const int i0 = (tile % tiles_x) * kFiltTX, j0 = (tile / tiles_x) * kFiltTY;
-  const int kbeg = chunk * kFiltNZC, kend = min(kbeg + kFiltNZC, nz);
+  const int i0 = (tile % tiles_x) * kFiltTX, j0 = (tile / tiles_x) * kFiltRY;
+  const int kbeg = chunk * nzc, kend = min(kbeg + nzc, nz);
   const int32_t* px = inv_ptr;
   const int32_t* py = inv_ptr + (nx + 1);
   const int32_t* pz = inv_ptr + (nx + 1) + (ny + 1);
-  // dy plane kk (element z index, zero outside) at local (la, lb) = element (i0 - HX + la, j0 - HX + lb)
-  auto fetch = [&](int kk, double (&v)[PER]) {
+  // element (in-plane) offset of each cell this thread loads; -1 outside the grid (value 0)
+  int src[PER];
 #pragma unroll
-    for (int t = 0; t < PER; ++t) {
-      const int q = tid + t * NT;
-      const int ii = i0 - HX + q % LX, jj = j0 - HX + q / LX;
-      const bool in = q < LP && kk >= 0 && kk < nz && ii >= 0 && ii < nx && jj >= 0 && jj < ny;
-      v[t] = in ? dy[(kk * ny + jj) * nx + ii] : 0.0;
-    }
+  for (int t = 0; t < PER; ++t) {
+    const int q = tid + t * NT;
+    const int ii = i0 - HX + q % LX, jj = j0 - HX + q / LX;
+    src[t] = (q < LP && ii >= 0 && ii < nx && jj >= 0 && jj < ny) ? jj * nx + ii : -1;
+  }
+  const int nxy = nx * ny;
+  auto fetch = [&](int kk, double (&v)[PER]) {
+    const bool zin = kk >= 0 && kk < nz;
+#pragma unroll
+    for (int t = 0; t < PER; ++t) v[t] = (zin && src[t] >= 0) ? dy[kk * nxy + src[t]] : 0.0;
   };
   auto store = [&](int kk, const double (&v)[PER]) {
 #pragma unroll
@@ -308,11 +405,15 @@ filter_bwd_tiled_kernel(int nx, int ny, int nz, int tiles_x, int tiles_y, const 
       if (q < LP) ring[(kk + KZ) % KZ][q] = v[t];
     }
   };
-  const int i = i0 + tx, j = j0 + ty;
-  const bool out = i < nx && j < ny;
-  // the only padded cell of an element away from the faces is its own position shifted by the half width
-  const bool xy1 = out && px[i + 1] - px[i] == 1 && inv_idx[px[i]] == i + HX && py[j + 1] - py[j] == 1 &&
-                   inv_idx[py[j]] == j + HX;
+  const int i = i0 + tx, jb = j0 + ty * kFiltOY;
+  // x and y parts of the single-cell test, once per thread
+  const bool xin = i < nx && px[i + 1] - px[i] == 1 && inv_idx[px[i]] == i + HX;
+  bool yin[kFiltOY];
+#pragma unroll
+  for (int o = 0; o < kFiltOY; ++o) {
+    const int j = jb + o;
+    yin[o] = xin && j < ny && py[j + 1] - py[j] == 1 && inv_idx[py[j]] == j + HX;
+  }
   double v[PER];
   for (int kk = kbeg - HZ; kk < kbeg + HZ; ++kk) {
     fetch(kk, v);
@@ -323,44 +424,35 @@ filter_bwd_tiled_kernel(int nx, int ny, int nz, int tiles_x, int tiles_y, const 
     store(k + HZ, v);  // dy plane k + HZ completes the window of element plane k
     __syncthreads();
     if (k + 1 < kend) fetch(k + HZ + 1, v);
-    const bool one = xy1 && pz[k + 1] - pz[k] == 1 && inv_idx[pz[k]] == k + HZ;
-    double acc = 0.0;
-    if (one) {
-      // cell (a, b, c) = (i + HX, j + HX, k + HZ): dy at (a - (KXY-1) + p, ...) = (i - HX + p, ...)
+    const bool zone = pz[k + 1] - pz[k] == 1 && inv_idx[pz[k]] == k + HZ;  // the same for the whole block
+    const double* wk = w + zero * k;  // re-read per plane (see the forward kernel)
+    if (zone) {
+      // element (i, j, k) at cell (i + HX, j + HX, k + HZ): dy at (i - HX + p, j - HX + q, k - HZ + r)
+      double acc[kFiltOY];
+#pragma unroll
+      for (int o = 0; o < kFiltOY; ++o) acc[o] = 0.0;
 #pragma unroll
       for (int r = 0; r < KZ; ++r) {
-        const double* pl = ring[(k - HZ + r + KZ) % KZ];
+        const double* pl = ring[(k - HZ + r + KZ) % KZ] + ty * kFiltOY * LX + tx;
+        double win[WY][KXY];
+#pragma unroll
+        for (int yy = 0; yy < WY; ++yy)
+#pragma unroll
+          for (int p = 0; p < KXY; ++p) win[yy][p] = pl[yy * LX + p];
 #pragma unroll
         for (int q = 0; q < KXY; ++q)
 #pragma unroll
-          for (int p = 0; p < KXY; ++p) acc = fma(w[(p * KXY + q) * KZ + r], pl[(ty + q) * LX + tx + p], acc);
-      }
-    } else if (out) {
-      for (int ic = pz[k]; ic < pz[k + 1]; ++ic) {
-        const int c = inv_idx[ic];
-        for (int ib = py[j]; ib < py[j + 1]; ++ib) {
-          const int b = inv_idx[ib];
-          for (int ia = px[i]; ia < px[i + 1]; ++ia) {
-            const int a = inv_idx[ia];
+          for (int p = 0; p < KXY; ++p) {
+            const double wt = wk[(p * KXY + q) * KZ + r];
 #pragma unroll
-            for (int r = 0; r < KZ; ++r) {
-              const int cc = c - (KZ - 1) + r;
-#pragma unroll
-              for (int q = 0; q < KXY; ++q) {
-                const int bb = b - (KXY - 1) + q;
-#pragma unroll
-                for (int p = 0; p < KXY; ++p) {
-                  const int aa = a - (KXY - 1) + p;
-                  const bool in = cc >= 0 && cc < nz && bb >= 0 && bb < ny && aa >= 0 && aa < nx;
-                  if (in) acc = fma(w[(p * KXY + q) * KZ + r], dy[(cc * ny + bb) * nx + aa], acc);
-                }
-              }
-            }
+            for (int o = 0; o < kFiltOY; ++o) acc[o] = fma(wt, win[o + q][p], acc[o]);
           }
-        }
+        if constexpr (KXY > 3) __builtin_amdgcn_sched_barrier(0);  // one plane's window live at a time
       }
+#pragma unroll
+      for (int o = 0; o < kFiltOY; ++o)
+        if (yin[o]) dx[(k * ny + jb + o) * nx + i] = acc[o];
     }
-    if (out) dx[((long long)k * ny + j) * nx + i] = acc;
     __syncthreads();
   }
 }
@@ -368,14 +460,23 @@ filter_bwd_tiled_kernel(int nx, int ny, int nz, int tiles_x, int tiles_y, const 
 template <int KXY, int KZ>
 void launch_filter_tiled(bool fwd, int nx, int ny, int nz, const double* w, const int32_t* m0, const int32_t* m1,
                          const double* cv, const double* in, double* out, hipStream_t s) {
-  const int tiles_x = (nx + kFiltTX - 1) / kFiltTX, tiles_y = (ny + kFiltTY - 1) / kFiltTY;
-  const unsigned blocks = (unsigned)(tiles_x * tiles_y * ((nz + kFiltNZC - 1) / kFiltNZC));
-  if (fwd)
-    filter_fwd_tiled_kernel<KXY, KZ><<<blocks, kFiltTX * kFiltTY, 0, s>>>(nx, ny, nz, tiles_x, tiles_y, w, m0, cv, in,
-                                                                        out);
-  else
-    filter_bwd_tiled_kernel<KXY, KZ><<<blocks, kFiltTX * kFiltTY, 0, s>>>(nx, ny, nz, tiles_x, tiles_y, w, m0, m1, in,
-                                                                        out);
+  const int tiles_x = (nx + kFiltTX - 1) / kFiltTX, tiles_y = (ny + kFiltRY - 1) / kFiltRY;
+  // planes per block: about three blocks per CU (a block re-reads KZ - 1 halo planes), 4 to 16 planes
+  const long long want = 3LL * device_cu_count();
+  const int nzc = (int)std::max(4LL, std::min(16LL, (long long)nz * tiles_x * tiles_y / std::max(1LL, want)));
+  const unsigned blocks = (unsigned)(tiles_x * tiles_y * ((nz + nzc - 1) / nzc));
+  if (fwd) {
+    filter_fwd_tiled_kernel<KXY, KZ><<<blocks, kFiltTX * kFiltTY, 0, s>>>(nx, ny, nz, tiles_x, tiles_y, nzc, 0, w, m0,
+                                                                        cv, in, out);
+  } else {
+    filter_bwd_tiled_kernel<KXY, KZ><<<blocks, kFiltTX * kFiltTY, 0, s>>>(nx, ny, nz, tiles_x, tiles_y, nzc, 0, w, m0,
+                                                                        m1, in, out);
+    constexpr int HX = (KXY - 1) / 2, HZ = (KZ - 1) / 2;
+    const long long wx = std::min(nx, 2 * HX), wy = std::min(ny, 2 * HX), wz = std::min(nz, 2 * HZ);
+    const long long shell = wx * ny * nz + (nx - wx) * wy * nz + (nx - wx) * (ny - wy) * wz;
+    if (shell > 0)
+      filter_bwd_shell_kernel<KXY, KZ><<<blocks_for(shell, 256), 256, 0, s>>>(nx, ny, nz, w, m0, m1, in, out);
+  }
 }
 
 // the tiled kernels index with 32-bit element numbers

@@ -56,10 +56,19 @@ int launch_fine_refl(int mode, const GridDims& g, int planes /* 0 = from the occ
 // 2 * ws_parts doubles).  Returns PMT_ERR_STATE (nothing launched) when the reflection kernel is not the
 // fine operator or the launch needs more partial sums than ws holds; the caller then runs K.u and
 // the dot products separately.
+// the fine reflection-basis kernel skips the pattern's zero block entries: the box class pattern and
+// vanishing rigid-translation entries
+inline bool refl_pattern(int k0_pattern) {
+  return (k0_pattern & PMT_K0_CLASS_PATTERN) && (k0_pattern & PMT_K0_RIGID);
+}
 int launch_fine_apply_dots(const GridDims& g, const double* K0ext, int k0_pattern, const double* xe,
                            const uint8_t* bcm, double bcdiag, const double* p, const double* r, double* q,
                            double* ws, long long ws_parts, double* out, hipStream_t s);
 long long fine_apply_dots_parts(const GridDims& g);  // a sufficient ws_parts for the grid
+// refl.hip: dx = alpha u_e^T K0 v_e (accumulate: dx += ...) in the reflection basis over z-streamed node
+// columns; tab = the FP64 block entries behind K0; hex8 only
+int launch_sens_stream(const GridDims& g, const double* tab, const uint8_t* bcm, const double* u, const double* v,
+                       double alpha, double* dx, int accumulate, hipStream_t s);
 int set_tuning(int param, int value);
 int get_tuning(int param);
 int tuned_stream_planes();  // 0 = automatic

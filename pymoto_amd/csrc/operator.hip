@@ -678,66 +678,6 @@ __global__ void prolong_kernel(GridDims gf, GridDims gc, const T* __restrict__ u
 }
 
 // dx_e (+)= alpha * u_e^T K0 v_e with Dirichlet entries zeroed (element-centric, no scatter)
-// dx_e (+)= alpha * u_e^T K0 v_e in the reflection basis of a hex8 K0 (see refl.hip): with the 8-point
-// Walsh-Hadamard transform over the corner bits per component, U^_d[k] = sum_a (-1)^(k.a) u_d[a],
-//   u_e^T K0 v_e = sum_chi sum_d U^_d[chi ^ (1 << d)] sum_e M_chi[d][e] V^_e[chi ^ (1 << e)]
-// (K0 = H blockdiag(M) H with H the symmetric transform and M holding the 1/64): 72 add/sub per transformed
-// vector and 96 multiply-adds per element instead of 600.  tab: the 72 FP64 block entries
-// [chi][d][e].  SAME: u and v are the same vector (compliance), transformed once.
-template <bool SAME>
-__global__ void __launch_bounds__(256)
-sens_refl_kernel(GridDims g, const double* __restrict__ tab, const uint8_t* __restrict__ bcm,
-                 const double* __restrict__ u, const double* __restrict__ v, double alpha,
-                 double* __restrict__ dx, int accumulate) {
-  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= g.nel) return;
-  const int ei = (int)(e % g.ex);
-  const long long t = e / g.ex;
-  const int ej = (int)(t % g.ey);
-  const int ek = (int)(t / g.ey);
-  double U[8][3], V[8][3];
-#pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    const long long m = ((long long)(ek + ((a >> 2) & 1)) * g.ny + (ej + ((a >> 1) & 1))) * g.nx + (ei + (a & 1));
-    const unsigned mk = bcm ? bcm[m] : 0u;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const bool fixed = (mk >> d) & 1u;
-      U[a][d] = fixed ? 0.0 : u[m * 3 + d];
-      if (!SAME) V[a][d] = fixed ? 0.0 : v[m * 3 + d];
-    }
-  }
-#pragma unroll
-  for (int bit = 1; bit < 8; bit <<= 1)
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      if (a & bit) continue;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const double p = U[a][d], q = U[a | bit][d];
-        U[a][d] = p + q;
-        U[a | bit][d] = p - q;
-        if (!SAME) {
-          const double pv = V[a][d], qv = V[a | bit][d];
-          V[a][d] = pv + qv;
-          V[a | bit][d] = pv - qv;
-        }
-      }
-    }
-  double s = 0.0;
-#pragma unroll
-  for (int chi = 0; chi < 8; ++chi) {
-    const double u0 = U[chi ^ 1][0], u1 = U[chi ^ 2][1], u2 = U[chi ^ 4][2];
-    const double v0 = SAME ? u0 : V[chi ^ 1][0], v1 = SAME ? u1 : V[chi ^ 2][1], v2 = SAME ? u2 : V[chi ^ 4][2];
-    const double* M = tab + chi * 9;
-    const double w0 = fma(M[0], v0, fma(M[1], v1, M[2] * v2));
-    const double w1 = fma(M[3], v0, fma(M[4], v1, M[5] * v2));
-    const double w2 = fma(M[6], v0, fma(M[7], v1, M[8] * v2));
-    s = fma(u0, w0, fma(u1, w1, fma(u2, w2, s)));
-  }
-  dx[e] = accumulate ? fma(alpha, s, dx[e]) : alpha * s;
-}
-
 template <int DIM>
 __global__ void __launch_bounds__(256)
 sens_kernel(GridDims g, const double* __restrict__ K0, const uint8_t* __restrict__ bcm,
@@ -803,7 +743,7 @@ extern int g_filter_tiled;  // filter.hip
 
 // Launch-shape tuning of the fine operator (set by pmt_set_tuning; read at every launch).
 static int g_fine_kernel = 2;  // 0 = 3D tiles, 1 = z-streamed columns, 2 = reflection basis (if available)
-static int g_refl_rows = 16;   // node rows per block of the reflection-basis kernel
+static int g_refl_rows = 16;   // block shape of the reflection-basis kernel (node rows per block; 0 = by grid)
 static int g_coarse1 = 1;      // hex8 level 1 without a stencil (reflection basis), new hierarchies
 static int g_stream_nzc = 0;   // node planes per streamed block (0 = chosen from the CU count)
 static int g_stream_tile = 0;  // streamed column shape: 0 = 32x8, 1 = 64x4, 2 = 16x16 nodes
@@ -828,7 +768,7 @@ int set_tuning(int param, int value) {
     ++g_tuning_epoch;
     return 0;
   }
-  if (param == PMT_TUNE_REFL_ROWS && (value == 16 || value == 32)) {
+  if (param == PMT_TUNE_REFL_ROWS && (value == 0 || value == 12 || value == 16 || value == 24 || value == 32)) {
     g_refl_rows = value;
     ++g_tuning_epoch;
     return 0;
@@ -930,8 +870,8 @@ int launch_fine(int mode, const GridDims& g, const double* K0, int k0_pattern, c
   const T bd = (T)bcdiag, om = (T)omega, be = (T)beta;
   const bool cls = (k0_pattern & PMT_K0_CLASS_PATTERN) != 0;
   if (g.dim == 3 && g_fine_kernel == 2 && (k0_pattern & PMT_K0_REFLECTION)) {
-    return launch_fine_refl<T>(mode, g, g_stream_nzc, g_refl_rows, cls ? 1 : 0, K0, xe, bcm, bcdiag, u, b, dinv, omega,
-                               y, s, up, beta);
+    return launch_fine_refl<T>(mode, g, g_stream_nzc, g_refl_rows, refl_pattern(k0_pattern) ? 1 : 0, K0, xe, bcm,
+                               bcdiag, u, b, dinv, omega, y, s, up, beta);
   }
   if (g.dim == 3 && g_fine_kernel >= 1) {
     if (cls)
@@ -1176,14 +1116,8 @@ int pmt_sens_contract_k0(int nelx, int nely, int nelz, const double* K0, int k0_
   GridDims g = make_grid(nelx, nely, nelz);
   if (g.dim != 3 || !(k0_pattern & PMT_K0_REFLECTION))
     return pmt_sens_contract(nelx, nely, nelz, K0, bcmask, u, v, alpha, dx, accumulate, stream);
-  dim3 grid(blocks_for(g.nel, 256));
-  hipStream_t s = (hipStream_t)stream;
   const double* tab = K0 + kK0ReflOffset64;  // the FP64 reflection-basis blocks behind K0 (pmt_k0_extend layout)
-  if (u == v)
-    sens_refl_kernel<true><<<grid, 256, 0, s>>>(g, tab, bcmask, u, v, alpha, dx, accumulate);
-  else
-    sens_refl_kernel<false><<<grid, 256, 0, s>>>(g, tab, bcmask, u, v, alpha, dx, accumulate);
-  return check_launch("sens_refl_kernel");
+  return launch_sens_stream(g, tab, bcmask, u, v, alpha, dx, accumulate, (hipStream_t)stream);
 }
 
 int pmt_restrict(int nelx, int nely, int nelz, const double* rf, double* rc, void* stream) {

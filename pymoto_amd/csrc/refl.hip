@@ -12,22 +12,29 @@
 // An element costs 2*72 add/sub + 72 multiply-adds + 24 multiplies (by x_e) instead of 576
 // multiply-adds, which moves the FP64 operator from the FMA roof to the HBM roof.
 //
-// Kernel layout: a block owns a 16 x SY column of nodes (one 16-lane DPP row per node row) and walks
+// Kernel layout: a block owns an RX x SY column of nodes (RX consecutive lanes per node row) and walks
 // z.  Thread (tx, ty) owns node (i, j) and the element (i, j) whose corner 0 it is.  Per element
 // layer k it forms the 8 Dirichlet-masked corner vectors (own node from registers, the +x neighbour
-// by a DPP row shift, the +y row through LDS), multiplies, combines the four xy corner results of
-// each node (x by DPP, y through LDS) and carries the upper-plane half to the next layer, so every
-// value crosses HBM once per application.  Edge rows/columns of the column are halo: 14 x (SY-2)
-// nodes are written per block.
+// by a DPP shift, the +y row through LDS), multiplies, and sums the corner results into the nodes one
+// axis at a time: z first (the upper half is carried to the next layer in registers), then y (LDS),
+// then x (DPP), so every value crosses HBM once per application and only 3 values per node cross
+// lanes by DPP.  Edge rows/columns of the column are halo: (RX-2) x (SY-2) nodes are written per block.
 #include <type_traits>
 
 #include "launch.h"
 
 namespace pmt {
 
-constexpr int kReflRow = 16;  // nodes per block row = lanes per DPP row
+constexpr int pow2_floor(int n) {
+  int p = 1;
+  while (2 * p <= n) p *= 2;
+  return p;
+}
 
-// DPP row shifts (within 16-lane rows; lanes shifted in from outside the row read 0):
+// DPP lane shifts by one node along x.  RX = 16: row shifts within the 16-lane rows of the wave; RX = 32
+// or 64: shifts of the whole wave.  Lanes shifted in from outside the row (wave) read 0; with RX = 32 the
+// lane at a row boundary receives the other row's edge node, which only ever lands in halo lanes (the
+// last column's element and the first column's node sum are never used).
 // from_lower: lane n receives lane n-1; from_upper: lane n receives lane n+1.
 template <int CTRL>
 __device__ inline float dpp_move(float v) {
@@ -40,13 +47,15 @@ __device__ inline double dpp_move(double v) {
   const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), CTRL, 0xf, 0xf, true);
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
-template <typename T>
+template <int RX, typename T>
 __device__ inline T from_lower(T v) {
-  return dpp_move<0x111>(v);  // row_shr:1
+  static_assert(RX == 16 || RX == 32 || RX == 64, "node rows of 16, 32 or 64 lanes");
+  return dpp_move<RX == 16 ? 0x111 : 0x138>(v);  // row_shr:1 / wave_shr:1
 }
-template <typename T>
+template <int RX, typename T>
 __device__ inline T from_upper(T v) {
-  return dpp_move<0x101>(v);  // row_shl:1
+  static_assert(RX == 16 || RX == 32 || RX == 64, "node rows of 16, 32 or 64 lanes");
+  return dpp_move<RX == 16 ? 0x101 : 0x130>(v);  // row_shl:1 / wave_shl:1
 }
 
 // Entry (d, e) of block chi vanishes for every K0 with the box sign/class pattern: characters 1..6
@@ -57,13 +66,24 @@ __host__ __device__ constexpr int refl_iso(int chi) {
 __host__ __device__ constexpr bool refl_zero(int chi, int d, int e) {
   return chi != 0 && chi != 7 && d != e && (d == refl_iso(chi) || e == refl_iso(chi));
 }
+// The rigid translation along d (frequency 0 of component d, character 2^d) is in K0's null space, so
+// its block entry vanishes for a stiffness matrix (pmt_k0_extend checks it: PMT_K0_RIGID).
+__host__ __device__ constexpr bool refl_rigid(int chi, int d, int e) { return chi == (1 << d) && e == d; }
+
+// Fine-kernel switches (both keep the results): skip the rigid-translation entries along with the box
+// pattern's zeros; skip the Dirichlet selects in waves whose nodes of the plane have none.
+constexpr bool kReflSkipRigid = true;
+constexpr bool kReflMaskBallot = true;
 
 // v[0..3] / v[4..7]: the lower / upper plane's masked corner displacements after the x and y stages.
 // Returns x * (K0 u_e) in v before the inverse transform (frequency index k = kx + 2 ky + 4 kz): the
 // caller interleaves its stages with the node sums.  PAT skips the entries that the box pattern makes
-// zero.
+// zero, and with them the rigid translations' entries (its callers take PAT only with PMT_K0_RIGID).
 template <bool PAT, typename T>
 __device__ inline void refl_element(const T* __restrict__ tab, T x, T (&v)[8][3]) {
+  constexpr auto zero = [](int chi, int d, int e) {
+    return PAT && (refl_zero(chi, d, e) || (kReflSkipRigid && refl_rigid(chi, d, e)));
+  };
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -79,9 +99,9 @@ __device__ inline void refl_element(const T* __restrict__ tab, T x, T (&v)[8][3]
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const T* m = tab + chi * 9 + d * 3;
-      T acc = (PAT && refl_zero(chi, d, 0)) ? (T)0 : m[0] * in0;
-      if (!(PAT && refl_zero(chi, d, 1))) acc = fma(m[1], in1, acc);
-      if (!(PAT && refl_zero(chi, d, 2))) acc = fma(m[2], in2, acc);
+      T acc = zero(chi, d, 0) ? (T)0 : m[0] * in0;
+      if (!zero(chi, d, 1)) acc = zero(chi, d, 0) ? m[1] * in1 : fma(m[1], in1, acc);
+      if (!zero(chi, d, 2)) acc = (zero(chi, d, 0) && zero(chi, d, 1)) ? m[2] * in2 : fma(m[2], in2, acc);
       w[chi ^ (1 << d)][d] = acc;
     }
   }
@@ -197,19 +217,18 @@ __device__ inline void coarse1_element(const T* tab, const T (&X)[8], T (&v)[8][
 // OP 0: the fine grid (element factor x_e, Dirichlet-masked nodes, bcdiag rows).  OP 1: the first
 // coarse level of a hex8 hierarchy (element factors X^[8], layout [layer][m][element]; no mask: the
 // elements with Dirichlet children get their corrections from coarse1_correct_kernel afterwards).
-template <int OP, int SY, int MODE, int PAT, typename T>
-__global__ void __launch_bounds__(kReflRow* SY)
+template <int OP, int RX, int SY, int MODE, int PAT, typename T>
+__global__ void __launch_bounds__(RX* SY)
 fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, const T* __restrict__ tab, int zero_stride,
                  const T* __restrict__ xe, const uint8_t* __restrict__ bcm, T bcdiag, const T* __restrict__ u,
                  const T* __restrict__ b, const T* __restrict__ dinv, T omega, T* y, const T* up, T beta,
                  double* __restrict__ partial) {
-  constexpr int RX = kReflRow, NT = RX * SY;
+  constexpr int NT = RX * SY;
   // MODE_APPLY_DOTS: y = A u, and the block's sums of u.y and u.r over its output nodes to
   // partial[work] and partial[nwork + work] (b holds r)
   constexpr bool DOTS = MODE == MODE_APPLY_DOTS;
   constexpr int XN = OP == 0 ? 1 : 8;  // element factors per element
   constexpr int TS = (int)sizeof(T);
-  constexpr bool CARRY = sizeof(T) == 4;  // carry the transformed plane to the next layer
   __shared__ T su[2][6][NT];  // x-stage sums / differences of the block's node planes, both parities
   // fine-grid sweeps take D^-1 from the densities instead of a stored vector: with K0 reflection
   // symmetric every corner has the same diagonal k_d, so D(n, d) = k_d * (sum of the 8 element factors
@@ -284,6 +303,11 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
     }
   };
   auto masked = [](const T(&ur)[3], unsigned mk, T(&c)[3]) {
+    if (kReflMaskBallot && __ballot(mk != 0u) == 0ull) {  // no constrained dof among the wave's nodes
+#pragma unroll
+      for (int d = 0; d < 3; ++d) c[d] = ur[d];
+      return;
+    }
 #pragma unroll
     for (int d = 0; d < 3; ++d) c[d] = ((mk >> d) & 1u) ? (T)0 : ur[d];
   };
@@ -308,7 +332,7 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
     T sx[3], dx[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const T cx = from_upper(c[d]);
+      const T cx = from_upper<RX>(c[d]);
       sx[d] = c[d] + cx;
       dx[d] = c[d] - cx;
       pl[d][tid] = sx[d];
@@ -324,13 +348,17 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
       P[3][d] = dx[d] - dy;
     }
   };
-  T Wlo[4][3];  // plane k, transformed (FP32: carried from the previous layer's upper plane)
+  T Wlo[4][3];  // plane k after the x and y stages, carried from the previous layer's upper plane
   {
     T c[3];
     masked(raw[0], mk[0], c);
     plane_corners(c, su[(kbeg - 1) & 1], Wlo);
   }
-  T C[3] = {(T)0, (T)0, (T)0};  // own node, plane k: the sum over element layer k-1
+  T Z[4][3];  // upper (az = 1) corners of element layer k-1, xy frequency kx + 2 ky: plane k's share
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) Z[a][d] = (T)0;
   double spq = 0.0, spr = 0.0;  // DOTS: this thread's output nodes
   // DIAG: 1 / k_d (k_d = sum over the classes of the blocks' (d, d) entries: K0 = H M H with H^2 = 8 I
   // per axis and M holding the 1/64) and 1 / bcdiag, once per block; XC: the x sum of layer k-1
@@ -354,20 +382,6 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
     masked(raw[SB], mk[SB], c1);
     T Whi[4][3];
     plane_corners(c1, su[(k + 1) & 1], Whi);
-    if (!CARRY) {
-      // FP64: rebuild the lower plane from the x-stage values the previous layer left in LDS (its own
-      // and the +y row's; 24 fewer live registers than carrying it)
-      T(*const pl)[NT] = su[k & 1];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const T sx = pl[d][tid], dx = pl[3 + d][tid];
-        const T sy = pl[d][slot_up], dy = pl[3 + d][slot_up];
-        Wlo[0][d] = sx + sy;
-        Wlo[2][d] = sx - sy;
-        Wlo[1][d] = dx + dy;
-        Wlo[3][d] = dx - dy;
-      }
-    }
     T v[8][3];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -375,7 +389,7 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
       for (int d = 0; d < 3; ++d) {
         v[a][d] = Wlo[a][d];
         v[4 + a][d] = Whi[a][d];
-        if (CARRY) Wlo[a][d] = Whi[a][d];
+        Wlo[a][d] = Whi[a][d];
       }
     // the table entries are re-read through the scalar cache each layer (zero_stride is 0; the
     // compiler cannot hoist the loads, which would exceed the scalar register file)
@@ -385,56 +399,65 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
       coarse1_element<PAT>((LDS_TAB ? sG0 : tab) + zero_stride * k, xs[SA], v);  // (re-read per layer)
     }
 
-    // Inverse transform interleaved with the node sums, one axis at a time (all stages are linear):
-    // x stage, then the node takes its own corner ax = 0 plus the left neighbour's ax = 1 (DPP); y
-    // stage, its own ay = 0 plus the row below's ay = 1 (LDS); z stage: az = 0 completes plane k, az = 1
-    // is carried to plane k+1.  21 add/sub per component instead of 24 for the transform and 7 for
-    // the sums.
-    T t[2][2][3];  // [ky][kz]
+    // Inverse transform interleaved with the node sums, one axis at a time (all stages are linear).
+    // z: corner az = 0 completes plane k with the carried upper half of layer k-1, az = 1 is carried to
+    // plane k+1; y: the node takes its own corner ay = 0 plus the row below's ay = 1 (LDS); x: its own
+    // ax = 0 plus the left neighbour's ax = 1 (DPP).  21 add/sub per component, as for the transform and
+    // the sums separately minus the shared stages; 3 values per node cross lanes.
+    T zs[4][3];  // plane k, xy frequency kx + 2 ky
 #pragma unroll
-    for (int ky = 0; ky < 2; ++ky)
-#pragma unroll
-      for (int kz = 0; kz < 2; ++kz)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const int m = 2 * ky + 4 * kz;
-          const T p = v[m][d] + v[m + 1][d], q = v[m][d] - v[m + 1][d];
-          t[ky][kz][d] = p + from_lower(q);
-        }
-    T r[2][3];  // [kz]
-#pragma unroll
-    for (int kz = 0; kz < 2; ++kz)
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        r[kz][d] = t[0][kz][d] + t[1][kz][d];
-        so[3 * kz + d][tid] = t[0][kz][d] - t[1][kz][d];
+        const T p = v[a][d] + v[a + 4][d], q = v[a][d] - v[a + 4][d];
+        zs[a][d] = Z[a][d] + p;
+        Z[a][d] = q;
+      }
+    T ys[2][3];  // [kx]
+#pragma unroll
+    for (int kx = 0; kx < 2; ++kx)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        ys[kx][d] = zs[kx][d] + zs[kx + 2][d];
+        so[3 * kx + d][tid] = zs[kx][d] - zs[kx + 2][d];
       }
     T xsum = (T)0;  // DIAG: element factors of layer k around the node, x then y
     if constexpr (DIAG) {
-      xsum = xs[SA][0] + from_lower(xs[SA][0]);
+      xsum = xs[SA][0] + from_lower<RX>(xs[SA][0]);
       so[6][tid] = xsum;
     }
     __syncthreads();
 #pragma unroll
-    for (int kz = 0; kz < 2; ++kz)
+    for (int kx = 0; kx < 2; ++kx)
 #pragma unroll
-      for (int d = 0; d < 3; ++d) r[kz][d] += so[3 * kz + d][slot_dn];
+      for (int d = 0; d < 3; ++d) ys[kx][d] += so[3 * kx + d][slot_dn];
+    T au[3];  // (K u) of the own node, plane k
+#pragma unroll
+    for (int d = 0; d < 3; ++d) au[d] = (ys[0][d] + ys[1][d]) + from_lower<RX>(ys[0][d] - ys[1][d]);
     if constexpr (DIAG) {
       xsum += so[6][slot_dn];
       const T inv = (T)1 / (XC + xsum);  // plane k: layers k-1 and k
       XC = xsum;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) dv[d] = ((mk[SA] >> d) & 1u) ? ibc : inv * ikd[d];
+      for (int d = 0; d < 3; ++d) dv[d] = inv * ikd[d];
     }
-
+    // Dirichlet rows: bcdiag u (and 1 / bcdiag as D^-1); skipped in waves without constrained dofs
+    const bool anyfix = !kReflMaskBallot || __ballot(mk[SA] != 0u) != 0ull;
+    if (anyfix) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const bool fx = (mk[SA] >> d) & 1u;
+        au[d] = fx ? bcdiag * raw[SA][d] : au[d];
+        if constexpr (DIAG) dv[d] = fx ? ibc : dv[d];
+      }
+    }
     {
       // plane k is complete (stores of the prologue layer and of halo lanes are dropped)
       const __amdgpu_buffer_rsrc_t ry = plane_rsrc(y, 3 * (long long)k * uplane, 3 * uplane, k >= kbeg);
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        const T au = C[d] + (r[0][d] + r[1][d]);
         const T uk = raw[SA][d];
-        const T ku = ((mk[SA] >> d) & 1u) ? bcdiag * uk : au;
+        const T ku = au[d];
         T out;
         if (MODE == MODE_APPLY || DOTS)
           out = ku;
@@ -452,8 +475,6 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
         }
       }
     }
-#pragma unroll
-    for (int d = 0; d < 3; ++d) C[d] = r[0][d] - r[1][d];
 
     __builtin_amdgcn_sched_barrier(0);  // keep the prefetch behind the stores
     load_node(k + 3, raw[SA], mk[SA]);
@@ -469,12 +490,22 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
     if (k + 3 >= kend) break;
   }
   if constexpr (DOTS) {  // fixed-order tree over the block (LDS rows 0 and 1 of so are free now)
+    // NT need not be a power of two (384-thread shapes): the threads above the largest power of two
+    // P2 <= NT are folded onto the first ones before the tree
+    constexpr int P2 = pow2_floor(NT);
     __syncthreads();
     so[0][tid] = spq;
     so[1][tid] = spr;
     __syncthreads();
+    if constexpr (P2 < NT) {
+      if (tid < NT - P2) {
+        so[0][tid] += so[0][tid + P2];
+        so[1][tid] += so[1][tid + P2];
+      }
+      __syncthreads();
+    }
 #pragma unroll
-    for (int st = NT / 2; st > 0; st >>= 1) {
+    for (int st = P2 / 2; st > 0; st >>= 1) {
       if (tid < st) {
         so[0][tid] += so[0][tid + st];
         so[1][tid] += so[1][tid + st];
@@ -514,14 +545,14 @@ bool k0_reflection_table(const double* K0, double* tab) {
 }
 
 // Resident blocks per CU of one kernel instance (cached per device).
-template <int OP, int SY, int MODE, int PAT, typename T>
+template <int OP, int RX, int SY, int MODE, int PAT, typename T>
 static int refl_resident_blocks() {
   static int cache[64] = {0};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 2;
   if (cache[dev] == 0) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fine_refl_kernel<OP, SY, MODE, PAT, T>, kReflRow * SY, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fine_refl_kernel<OP, RX, SY, MODE, PAT, T>, RX * SY, 0) !=
             hipSuccess ||
         nb < 1)
       nb = 2;
@@ -553,20 +584,20 @@ static int refl_planes(int nxy, int nz, int resident) {
   return best > 0 ? best : nz;
 }
 
-template <int OP, int SY, int PAT, typename T>
+template <int OP, int RX, int SY, int PAT, typename T>
 static void launch_refl_shape(int mode, const GridDims& g, int planes, const T* tab, const T* xe, const uint8_t* bcm,
                               T bcdiag, const T* u, const T* b, const T* dinv, T omega, T* y, hipStream_t s,
                               const T* up, T beta, double* partial = nullptr, int* nwork_out = nullptr) {
-  const int tx = (g.nx + kReflRow - 3) / (kReflRow - 2);
+  const int tx = (g.nx + RX - 3) / (RX - 2);
   const int ty = (g.ny + SY - 3) / (SY - 2);
   const int nxy = tx * ty;
 #define PMT_REFL(M)                                                                                             \
   do {                                                                                                          \
     const int nzc = planes > 0 ? (planes < g.nz ? planes : g.nz)                                               \
-                               : refl_planes(nxy, g.nz, refl_resident_blocks<OP, SY, M, PAT, T>());             \
+                               : refl_planes(nxy, g.nz, refl_resident_blocks<OP, RX, SY, M, PAT, T>());         \
     const int nwork = nxy * ((g.nz + nzc - 1) / nzc);                                                          \
     if (nwork_out) *nwork_out = nwork;                                                                          \
-    fine_refl_kernel<OP, SY, M, PAT, T><<<(unsigned)((nwork + 7) & ~7), kReflRow * SY, 0, s>>>(                \
+    fine_refl_kernel<OP, RX, SY, M, PAT, T><<<(unsigned)((nwork + 7) & ~7), RX * SY, 0, s>>>(                  \
         g, tx, nxy, nzc, nwork, tab, 0, xe, bcm, bcdiag, u, b, dinv, omega, y, up, beta, partial);               \
   } while (0)
   if (mode == MODE_APPLY_DOTS) {
@@ -582,30 +613,71 @@ static void launch_refl_shape(int mode, const GridDims& g, int planes, const T* 
 #undef PMT_REFL
 }
 
+// Block shapes: `rows` node rows per block of refl_cols(rows) nodes each (12 rows of 32; 16, 24 or 32
+// rows of 16).  Every shape does the same arithmetic per node: the results are bitwise the same.
+static int refl_cols(int rows) { return rows == 12 ? 32 : 16; }
+static const int kReflShapes[4] = {16, 24, 12, 32};
+
+static long long refl_tiles(const GridDims& g, int rows) {
+  const int rx = refl_cols(rows);
+  return (long long)((g.nx + rx - 3) / (rx - 2)) * ((g.ny + rows - 3) / (rows - 2));
+}
+
+// Shape 0 (automatic): the fewest threads per node plane, halo columns and rows of every block and the
+// padding of the last tiles included; ties go to the earlier shape of kReflShapes.
+int refl_auto_rows(const GridDims& g) {
+  int best = 16;
+  long long best_threads = -1;
+  for (int rows : kReflShapes) {
+    const long long threads = refl_tiles(g, rows) * refl_cols(rows) * rows;
+    if (best_threads < 0 || threads < best_threads) {
+      best_threads = threads;
+      best = rows;
+    }
+  }
+  return best;
+}
+
+template <int OP, int PAT, typename T>
+static void launch_refl_rows(int rows, int mode, const GridDims& g, int planes, const T* tab, const T* xe,
+                             const uint8_t* bcm, T bcdiag, const T* u, const T* b, const T* dinv, T omega, T* y,
+                             hipStream_t s, const T* up, T beta, double* partial = nullptr, int* nwork_out = nullptr) {
+  switch (rows > 0 ? rows : refl_auto_rows(g)) {
+    case 12:
+      launch_refl_shape<OP, 32, 12, PAT, T>(mode, g, planes, tab, xe, bcm, bcdiag, u, b, dinv, omega, y, s, up, beta,
+                                            partial, nwork_out);
+      break;
+    case 24:
+      launch_refl_shape<OP, 16, 24, PAT, T>(mode, g, planes, tab, xe, bcm, bcdiag, u, b, dinv, omega, y, s, up, beta,
+                                            partial, nwork_out);
+      break;
+    case 32:
+      launch_refl_shape<OP, 16, 32, PAT, T>(mode, g, planes, tab, xe, bcm, bcdiag, u, b, dinv, omega, y, s, up, beta,
+                                            partial, nwork_out);
+      break;
+    default:
+      launch_refl_shape<OP, 16, 16, PAT, T>(mode, g, planes, tab, xe, bcm, bcdiag, u, b, dinv, omega, y, s, up, beta,
+                                            partial, nwork_out);
+  }
+}
+
 template <typename T>
 int launch_fine_refl(int mode, const GridDims& g, int planes, int rows, int pattern, const double* K0ext, const T* xe,
                      const uint8_t* bcm, double bcdiag, const T* u, const T* b, const T* dinv, double omega, T* y,
                      hipStream_t s, const T* up, double beta) {
   const T* tab = sizeof(T) == 8 ? (const T*)(K0ext + kK0ReflOffset64) : (const T*)(K0ext + kK0ReflOffset32);
   const T bd = (T)bcdiag, om = (T)omega, be = (T)beta;
-  pattern = pattern && sizeof(T) == 4;  // FP64: skipping the zeros costs more scalar-register spills than it saves
-  if (rows == 32) {
-    if (pattern)
-      launch_refl_shape<0, 32, true, T>(mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
-    else
-      launch_refl_shape<0, 32, false, T>(mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
-  } else {
-    if (pattern)
-      launch_refl_shape<0, 16, true, T>(mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
-    else
-      launch_refl_shape<0, 16, false, T>(mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
-  }
+  if (pattern)
+    launch_refl_rows<0, true, T>(rows, mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
+  else
+    launch_refl_rows<0, false, T>(rows, mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
   return check_launch("fine_refl_kernel");
 }
 
 long long fine_apply_dots_parts(const GridDims& g) {
-  const long long tx = (g.nx + kReflRow - 3) / (kReflRow - 2), ty = (g.ny + 16 - 3) / (16 - 2);
-  return tx * ty * (long long)g.nz;  // blocks of the 16-row shape with one plane each: an upper bound
+  long long most = 0;  // blocks of the shape with the most tiles, one plane each: an upper bound
+  for (int rows : kReflShapes) most = refl_tiles(g, rows) > most ? refl_tiles(g, rows) : most;
+  return most * (long long)g.nz;
 }
 
 // out[o] = sum over i of partial[o * n + i], o = 0, 1, in a fixed order (one block)
@@ -634,15 +706,150 @@ int launch_fine_apply_dots(const GridDims& g, const double* K0ext, int k0_patter
     return PMT_ERR_STATE;
   const double* tab = K0ext + kK0ReflOffset64;
   int nwork = 0;
-  if (get_tuning(PMT_TUNE_REFL_ROWS) == 32)
-    launch_refl_shape<0, 32, false, double>(MODE_APPLY_DOTS, g, tuned_stream_planes(), tab, xe, bcm, bcdiag, p, r,
-                                            nullptr, 0.0, q, s, nullptr, 0.0, ws, &nwork);
+  if (refl_pattern(k0_pattern))
+    launch_refl_rows<0, true, double>(get_tuning(PMT_TUNE_REFL_ROWS), MODE_APPLY_DOTS, g, tuned_stream_planes(), tab,
+                                      xe, bcm, bcdiag, p, r, nullptr, 0.0, q, s, nullptr, 0.0, ws, &nwork);
   else
-    launch_refl_shape<0, 16, false, double>(MODE_APPLY_DOTS, g, tuned_stream_planes(), tab, xe, bcm, bcdiag, p, r,
-                                            nullptr, 0.0, q, s, nullptr, 0.0, ws, &nwork);
+    launch_refl_rows<0, false, double>(get_tuning(PMT_TUNE_REFL_ROWS), MODE_APPLY_DOTS, g, tuned_stream_planes(), tab,
+                                       xe, bcm, bcdiag, p, r, nullptr, 0.0, q, s, nullptr, 0.0, ws, &nwork);
   PMT_TRY(check_launch("fine_refl_kernel<apply + dots>"));
   sum_parts_kernel<<<1, 256, 0, s>>>(ws, nwork, out);
   return check_launch("sum_parts_kernel");
+}
+
+// Sensitivity contraction dx_e = alpha * u_e^T K0 v_e (u, v Dirichlet-masked) over z-streamed node
+// columns: the same transforms as the operator (x stage with the +x neighbour by DPP, y stage through
+// LDS, both shared by the two element layers of a node plane, z stage per element) and
+// sum_chi U^_chi^T M_chi V^_chi with U^_chi = (U^_0[chi ^ 1], U^_1[chi ^ 2], U^_2[chi ^ 4]) (K0 = H blockdiag(M) H,
+// H the 8-point transform over the corner bits per component, M holding the 1/64): 72 add/sub per
+// transformed vector and 96 multiply-adds per element instead of 600.  A block of 16 x 16 threads owns the
+// 15 x 15 elements whose corner 0 it holds (the last row and column only supply corners) and walks a
+// chunk of element layers; one barrier per layer (the exchange buffer alternates by parity).
+template <bool SAME>
+__global__ void __launch_bounds__(256)
+sens_stream_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, const double* __restrict__ tab,
+                   int zero_stride, const uint8_t* __restrict__ bcm, const double* __restrict__ u,
+                   const double* __restrict__ v, double alpha, double* dx, int accumulate) {
+  constexpr int RX = 16, SY = 16, NT = RX * SY, NV = SAME ? 1 : 2;
+  __shared__ double sx_[2][NV][6][NT];  // x-stage sums / differences of the node plane, both parities
+  const int work = (int)xcd_block();
+  if (work >= nwork) return;
+  const int txy = work % ntiles_xy;
+  const int tid = threadIdx.x, tx = tid % RX, ty = tid / RX;
+  const int i = (txy % tiles_x) * (RX - 1) + tx;
+  const int j = (txy / tiles_x) * (SY - 1) + ty;
+  const int kbeg = (work / ntiles_xy) * nzc;
+  const int kend = min(kbeg + nzc, g.ez);
+  const bool nin = i < g.nx && j < g.ny;
+  const bool eout = i < g.ex && j < g.ey && tx < RX - 1 && ty < SY - 1;
+  const int slot_up = min(ty + 1, SY - 1) * RX + tx;
+  const long long uplane = (long long)g.nx * g.ny, xplane = (long long)g.ex * g.ey;
+  const int o_node = nin ? (int)((long long)j * g.nx + i) : kOOB;
+  const int o_vec = nin ? 24 * (int)((long long)j * g.nx + i) : kOOB;
+  const int o_el = eout ? 8 * (int)((long long)j * g.ex + i) : kOOB;
+  const double* vecs[2] = {u, v};
+
+  // masked node values of plane p (zero outside the grid)
+  auto load_node = [&](int p, double (&c)[NV][3]) {
+    const bool pin = p >= 0 && p < g.nz;
+    const __amdgpu_buffer_rsrc_t rm = plane_rsrc(bcm, (long long)p * uplane, uplane, pin && bcm != nullptr);
+    const unsigned mk = __builtin_amdgcn_raw_buffer_load_b8(rm, o_node, 0, 0);
+#pragma unroll
+    for (int w = 0; w < NV; ++w) {
+      const __amdgpu_buffer_rsrc_t rv = plane_rsrc(vecs[w], 3 * (long long)p * uplane, 3 * uplane, pin);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double val = bload(rv, o_vec + 8 * d, 0.0);
+        c[w][d] = ((mk >> d) & 1u) ? 0.0 : val;
+      }
+    }
+  };
+  // x stage (DPP) to the LDS buffer of parity par, barrier, y stage with the +y row: P[kx + 2 ky][d]
+  auto plane_corners = [&](const double (&c)[NV][3], int par, double (&P)[NV][4][3]) {
+    double sx[NV][3], dx_[NV][3];
+#pragma unroll
+    for (int w = 0; w < NV; ++w)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double cx = from_upper<RX>(c[w][d]);
+        sx[w][d] = c[w][d] + cx;
+        dx_[w][d] = c[w][d] - cx;
+        sx_[par][w][d][tid] = sx[w][d];
+        sx_[par][w][3 + d][tid] = dx_[w][d];
+      }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < NV; ++w)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double sy = sx_[par][w][d][slot_up], dy = sx_[par][w][3 + d][slot_up];
+        P[w][0][d] = sx[w][d] + sy;
+        P[w][2][d] = sx[w][d] - sy;
+        P[w][1][d] = dx_[w][d] + dy;
+        P[w][3][d] = dx_[w][d] - dy;
+      }
+  };
+  double raw[2][NV][3];  // node planes k+1 (in use) and k+2 (in flight)
+  double Wlo[NV][4][3];
+  {
+    double c0[NV][3];
+    load_node(kbeg, c0);
+    load_node(kbeg + 1, raw[0]);
+    plane_corners(c0, kbeg & 1, Wlo);
+  }
+  for (int k = kbeg; k < kend; ++k) {
+    double Whi[NV][4][3];
+    const int cur = (k - kbeg) & 1;
+    plane_corners(raw[cur], (k + 1) & 1, Whi);
+    load_node(k + 2, raw[cur ^ 1]);  // in flight during this layer's products
+    double U[NV][8][3];
+#pragma unroll
+    for (int w = 0; w < NV; ++w)
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          U[w][a][d] = Wlo[w][a][d] + Whi[w][a][d];
+          U[w][a + 4][d] = Wlo[w][a][d] - Whi[w][a][d];
+          Wlo[w][a][d] = Whi[w][a][d];
+        }
+    const double* M0 = tab + zero_stride * k;  // re-read per layer (zero_stride is 0)
+    double sum = 0.0;
+#pragma unroll
+    for (int chi = 0; chi < 8; ++chi) {
+      const double u0 = U[0][chi ^ 1][0], u1 = U[0][chi ^ 2][1], u2 = U[0][chi ^ 4][2];
+      const double v0 = U[NV - 1][chi ^ 1][0], v1 = U[NV - 1][chi ^ 2][1], v2 = U[NV - 1][chi ^ 4][2];
+      const double* M = M0 + chi * 9;
+      const double w0 = fma(M[0], v0, fma(M[1], v1, M[2] * v2));
+      const double w1 = fma(M[3], v0, fma(M[4], v1, M[5] * v2));
+      const double w2 = fma(M[6], v0, fma(M[7], v1, M[8] * v2));
+      sum = fma(u0, w0, fma(u1, w1, fma(u2, w2, sum)));
+    }
+    const __amdgpu_buffer_rsrc_t rx = plane_rsrc(dx, (long long)k * xplane, xplane, true);
+    const double prev = accumulate ? bload(rx, o_el, 0.0) : 0.0;
+    bstore(rx, o_el, accumulate ? fma(alpha, sum, prev) : alpha * sum);
+  }
+}
+
+int launch_sens_stream(const GridDims& g, const double* tab, const uint8_t* bcm, const double* u, const double* v,
+                       double alpha, double* dx, int accumulate, hipStream_t s) {
+  if (g.dim != 3 || g.nel <= 0) return arg_error("launch_sens_stream: 3D grids only");
+  const int tiles_x = (g.ex + 14) / 15, tiles_y = (g.ey + 14) / 15;
+  const int nxy = tiles_x * tiles_y;
+  // element layers per block: about four blocks per CU, at least 8 layers (a block re-reads one plane)
+  const long long want = 4LL * device_cu_count();
+  int nzc = (int)(((long long)g.ez * nxy + want - 1) / want);
+  nzc = nzc < 8 ? 8 : nzc;
+  nzc = nzc > g.ez ? g.ez : nzc;
+  const int nwork = nxy * ((g.ez + nzc - 1) / nzc);
+  const unsigned blocks = (unsigned)((nwork + 7) & ~7);
+  if (u == v)
+    sens_stream_kernel<true><<<blocks, 256, 0, s>>>(g, tiles_x, nxy, nzc, nwork, tab, 0, bcm, u, v, alpha, dx,
+                                                    accumulate);
+  else
+    sens_stream_kernel<false><<<blocks, 256, 0, s>>>(g, tiles_x, nxy, nzc, nwork, tab, 0, bcm, u, v, alpha, dx,
+                                                     accumulate);
+  return check_launch("sens_stream_kernel");
 }
 
 // Corrections of the first coarse level's elements with Dirichlet children (those along constrained
@@ -756,11 +963,11 @@ int launch_coarse1(int mode, const GridDims& g, int pattern, const double* G0ext
   const T* tab = sizeof(T) == 8 ? (const T*)G0ext : (const T*)(G0ext + 576);
   const T om = (T)omega, be = (T)beta;
   if (pattern == 2)
-    launch_refl_shape<1, 16, 2, T>(mode, g, 0, tab, Xh, nullptr, (T)0, u, b, dinv, om, y, s, up, be);
+    launch_refl_shape<1, 16, 16, 2, T>(mode, g, 0, tab, Xh, nullptr, (T)0, u, b, dinv, om, y, s, up, be);
   else if (pattern == 1)
-    launch_refl_shape<1, 16, 1, T>(mode, g, 0, tab, Xh, nullptr, (T)0, u, b, dinv, om, y, s, up, be);
+    launch_refl_shape<1, 16, 16, 1, T>(mode, g, 0, tab, Xh, nullptr, (T)0, u, b, dinv, om, y, s, up, be);
   else
-    launch_refl_shape<1, 16, 0, T>(mode, g, 0, tab, Xh, nullptr, (T)0, u, b, dinv, om, y, s, up, be);
+    launch_refl_shape<1, 16, 16, 0, T>(mode, g, 0, tab, Xh, nullptr, (T)0, u, b, dinv, om, y, s, up, be);
   PMT_TRY(check_launch("fine_refl_kernel<coarse level 1>"));
   if (nt <= 0) return PMT_OK;
   const unsigned blocks = (unsigned)blocks_for(3LL * nt, 256);
@@ -874,6 +1081,11 @@ int pmt_k0_extend(int dim, const double* K0_host, double* ext_host) {
         t32[q] = (float)tab[q];
       }
       bits |= PMT_K0_REFLECTION;
+      double scale = 0.0;
+      for (int q = 0; q < 72; ++q) scale = fmax(scale, fabs(tab[q]));
+      bool rigid = true;  // the translations' entries M_(2^d)[d][d] vanish (K0 annihilates translations)
+      for (int d = 0; d < 3; ++d) rigid = rigid && fabs(tab[(1 << d) * 9 + d * 4]) <= 1e-13 * scale;
+      if (rigid) bits |= PMT_K0_RIGID;
     }
   }
   return bits;

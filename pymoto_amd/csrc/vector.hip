@@ -4,6 +4,7 @@
 // Reductions use a fixed grid of kReduceBlocks blocks with grid-stride loops, per-block
 // partials in a caller workspace, and a single-block second stage summing the partials in a
 // fixed order, so every dot product is bit-reproducible run to run.
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -24,10 +25,22 @@ int hip_status(hipError_t err, const char* where) {
 
 int check_launch(const char* where) { return hip_status(hipGetLastError(), where); }
 
-__global__ void powlaw_fwd_kernel(long long n, double a, double b, double p, const double* __restrict__ x,
+// x^p: repeated products for a small whole exponent (ip >= 0; SIMP's p = 3 and its derivative's 2, within
+// an ulp or two of pow and a fraction of its cost), else pow
+__device__ inline double powlaw_pow(double x, double p, int ip) {
+  if (ip < 0) return pow(x, p);
+  double r = 1.0;
+  for (int k = 0; k < ip; ++k) r *= x;
+  return r;
+}
+
+// ip for powlaw_pow: p itself when it is a whole number in [0, 8], else -1
+static int powlaw_int_exponent(double p) { return (p >= 0.0 && p <= 8.0 && p == std::floor(p)) ? (int)p : -1; }
+
+__global__ void powlaw_fwd_kernel(long long n, double a, double b, double p, int ip, const double* __restrict__ x,
                                   double* __restrict__ y) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = a + b * pow(x[i], p);
+  if (i < n) y[i] = a + b * powlaw_pow(x[i], p, ip);
 }
 
 // Heaviside projection (ex_compliance_robust.py:139, a MathExpression of the reference):
@@ -65,11 +78,11 @@ __global__ void oc_update_kernel(long long n, const double* __restrict__ x, cons
                                  double xmin, double xmax, double move, double lmid, double* __restrict__ xnew,
                                  double* __restrict__ partial);
 
-__global__ void powlaw_bwd_kernel(long long n, double c, double q, const double* __restrict__ x,
+__global__ void powlaw_bwd_kernel(long long n, double c, double q, int iq, const double* __restrict__ x,
                                   const double* __restrict__ dy, double* __restrict__ dx, int accumulate) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
-    const double v = dy[i] * (c * pow(x[i], q));
+    const double v = dy[i] * (c * powlaw_pow(x[i], q, iq));
     dx[i] = accumulate ? dx[i] + v : v;
   }
 }
@@ -431,7 +444,7 @@ int pmt_reduce_work_size(void) { return 2 * kReduceBlocks; }
 int pmt_powlaw_fwd(long long n, double a, double b, double p, const double* x, double* y, void* stream) {
   if (n < 0 || (n > 0 && (!x || !y))) return arg_error("pmt_powlaw_fwd: bad argument");
   if (n == 0) return PMT_OK;
-  powlaw_fwd_kernel<<<blocks_for(n, 256), 256, 0, (hipStream_t)stream>>>(n, a, b, p, x, y);
+  powlaw_fwd_kernel<<<blocks_for(n, 256), 256, 0, (hipStream_t)stream>>>(n, a, b, p, powlaw_int_exponent(p), x, y);
   return check_launch("powlaw_fwd_kernel");
 }
 
@@ -439,7 +452,8 @@ int pmt_powlaw_bwd(long long n, double c, double q, const double* x, const doubl
                    void* stream) {
   if (n < 0 || (n > 0 && (!x || !dy || !dx))) return arg_error("pmt_powlaw_bwd: bad argument");
   if (n == 0) return PMT_OK;
-  powlaw_bwd_kernel<<<blocks_for(n, 256), 256, 0, (hipStream_t)stream>>>(n, c, q, x, dy, dx, accumulate);
+  powlaw_bwd_kernel<<<blocks_for(n, 256), 256, 0, (hipStream_t)stream>>>(n, c, q, powlaw_int_exponent(q), x, dy, dx,
+                                                                         accumulate);
   return check_launch("powlaw_bwd_kernel");
 }
 

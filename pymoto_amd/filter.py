@@ -99,6 +99,10 @@ class FilterConv(Module):
             p = self.weights.shape[ax] // 2
             m, v = _axis_map(sizes[ax], p, *self.bcs[ax])
             ptr, idx = _inverse_lists(m, sizes[ax])
+            # the tiled adjoint kernel takes every element farther than the half width p from both faces
+            # to have only its own padded cell (p + a); the padding modes put the others within p of a face
+            inner = np.arange(p, sizes[ax] - p)
+            assert np.all(np.diff(ptr)[inner] == 1) and np.all(idx[ptr[inner]] == inner + p)
             maps.append(m)
             vals.append(v)
             ptrs.append(ptr)

@@ -9,8 +9,8 @@ Results are cached per (grid, device) for the life of the process.
 
 Tuning parameters (``include/pymoto_amd.h``): ``FINE_KERNEL`` (0 = 3D tiles, 1 = z-streamed
 columns, 2 = z-streamed columns multiplying in the reflection basis of K0), ``STREAM_TILE`` (0 = 32x8,
-1 = 64x4, 2 = 16x16 nodes per plane of kernel 1), ``REFL_ROWS`` (16 or 32 node rows per block of
-kernel 2), ``STREAM_PLANES`` (z planes per block, 0 = derived from the CU count).
+1 = 64x4, 2 = 16x16 nodes per plane of kernel 1), ``REFL_ROWS`` (block shape of kernel 2: 0 = chosen
+per grid, 16 / 24 / 32 rows of 16 nodes, 12 rows of 32 nodes), ``STREAM_PLANES`` (z planes per block, 0 = derived from the CU count).
 """
 import itertools
 
@@ -51,7 +51,7 @@ def candidates(nz, reflection=True):
     for tile, p in itertools.product((0, 1, 2), planes):
         out.append(dict(fine_kernel=1, stream_tile=tile, stream_planes=p, refl_rows=16))
     if reflection:
-        for rows, p in itertools.product((16, 32), planes):
+        for rows, p in itertools.product((0, 12, 16, 24, 32), planes):
             out.append(dict(fine_kernel=2, stream_tile=0, stream_planes=p, refl_rows=rows))
     return out
 

@@ -128,9 +128,10 @@ def test_cg_zero_rhs_returns_zero(gpu_lib):
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_coarsest_inverse_join_each_consumer_stream(gpu_lib, prec):
-    """Update on stream A, a V-cycle on stream B (ordered after A), then a solve on A that does not
-    wait for B: A must still wait for the coarsest inversion itself.  Results equal a hierarchy
-    updated and solved on one stream, bit for bit."""
+    """Update on stream A (the coarsest inversion forked to the handle's own stream), a V-cycle on stream
+    B ordered after A's update, then a V-cycle on A ordered after B's (calls on one handle share its
+    work vectors, so they must not overlap).  Each stream joins the inversion before it reads the
+    inverse; results equal a hierarchy updated and solved on one stream, bit for bit."""
     rng = np.random.default_rng(11)
     nel = (32, 16, 16)
     d = pm.VoxelDomain(*nel)
@@ -148,6 +149,7 @@ def test_coarsest_inverse_join_each_consumer_stream(gpu_lib, prec):
     B.wait_stream(A)
     with torch.cuda.stream(B):
         zb = mg.solve(r)
+    A.wait_stream(B)
     with torch.cuda.stream(A):
         za = mg.solve(r)
     torch.cuda.synchronize()

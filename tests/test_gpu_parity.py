@@ -155,12 +155,13 @@ def test_streamed_operator_matches_tiled_bitwise(gpu_lib, nel, planes):
     assert lib.pmt_get_tuning(PMT_TUNE_FINE_KERNEL) == 2
 
 
-@pytest.mark.parametrize("nel,planes,rows", [((40, 12, 10), 0, 16), ((40, 12, 10), 3, 32), ((33, 9, 7), 1, 16),
-                                             ((16, 8, 8), 2, 32), ((30, 29, 5), 0, 16), ((2, 1, 1), 0, 16)])
+@pytest.mark.parametrize("nel,planes,rows", [((40, 12, 10), 0, 16), ((40, 12, 10), 3, 32), ((33, 9, 7), 1, 12),
+                                             ((16, 8, 8), 2, 24), ((30, 29, 5), 0, 0), ((2, 1, 1), 0, 12),
+                                             ((61, 23, 4), 0, 12), ((2, 1, 1), 0, 16)])
 def test_reflection_operator_vs_streamed(gpu_lib, nel, planes, rows):
     """The reflection-basis fine operator (kernel 2) against the dense-K0 streamed one (kernel 1):
     K u, b - K u, the fused Jacobi sweeps of a V-cycle (FP64) and the FP32 product, on ragged grids,
-    every plane chunking and both block shapes; differences are rounding only."""
+    every plane chunking and every block shape; differences are rounding only."""
     from pymoto_amd import tune
     lib = pm._lib.lib()
     d = pm.VoxelDomain(*nel)
@@ -168,7 +169,7 @@ def test_reflection_operator_vs_streamed(gpu_lib, nel, planes, rows):
     x = rng.uniform(1e-3, 1.0, d.nel)
     bc = problems.cantilever3d(*nel)[0]
     K = pm.AssembleStiffness(d, bc=bc)(dev(x))
-    assert K.grid.k0_pattern == 3
+    assert K.grid.k0_pattern == 7  # class pattern, reflection form, rigid translations
     u, b = dev(rng.standard_normal(3 * d.nnodes)), dev(rng.standard_normal(3 * d.nnodes))
     even = all(n % 2 == 0 for n in nel) and min(nel) >= 2
     out = []
@@ -189,6 +190,40 @@ def test_reflection_operator_vs_streamed(gpu_lib, nel, planes, rows):
     npt.assert_allclose(out[1][2], out[0][2], rtol=0, atol=1e-5 * np.abs(out[0][2]).max())
     if even:
         npt.assert_allclose(out[1][3], out[0][3], rtol=0, atol=1e-11 * np.abs(out[0][3]).max())
+
+
+@pytest.mark.parametrize("nel", [(40, 12, 10), (62, 22, 4), (30, 28, 6)])
+def test_reflection_block_shapes_same_bits(gpu_lib, nel):
+    """Every block shape of the reflection-basis kernel (16 / 24 / 32 rows of 16 nodes, 12 rows of 32
+    nodes whose x shifts run over the whole wave) does the same arithmetic per node: K u, b - K u, the
+    FP32 product and a V-cycle (fused Jacobi sweeps) are bitwise equal across shapes; PCG, whose product
+    carries the block sums of p.q and p.r (384-thread blocks are not a power of two), converges the same."""
+    from pymoto_amd import tune
+    d = pm.VoxelDomain(*nel)
+    rng = np.random.default_rng(5)
+    K = pm.AssembleStiffness(d, bc=problems.cantilever3d(*nel)[0])(dev(rng.uniform(1e-3, 1.0, d.nel)))
+    u, b = dev(rng.standard_normal(3 * d.nnodes)), dev(rng.standard_normal(3 * d.nnodes))
+    out, its, sols = {}, {}, {}
+    try:
+        for rows in (16, 24, 32, 12, 0):
+            tune.set_tuning(fine_kernel=2, stream_planes=0, refl_rows=rows)
+            res = [host(K @ u), host(K.residual(b, u)), host(K.apply_reduced(u.to(torch.float32), "fp32"))]
+            mg = pm.solvers.build_hierarchy(d, 2, omega=0.5, smooth_steps=2)
+            mg.update(K)
+            res.append(host(mg.solve(b)))
+            # PCG: its q = A p is fused with the block sums of p.q and p.r, whose grouping follows the shape
+            cg = pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(d, 2, omega=0.5, smooth_steps=2), tol=1e-10)
+            cg.update(K)
+            sol = host(cg.solve(b))
+            out[rows] = res
+            its[rows], sols[rows] = cg.iterations, sol
+    finally:
+        tune.set_tuning(**tune.DEFAULT)
+    for rows in (24, 32, 12, 0):
+        for got, want in zip(out[rows], out[16]):
+            npt.assert_array_equal(got, want)
+        assert abs(its[rows] - its[16]) <= 1 and its[rows] < 100
+        assert relerr(sols[rows], sols[16]) < 1e-8
 
 
 def test_sensitivity_contraction_vs_oracle(gpu_lib):

@@ -63,7 +63,7 @@ def test_k0_reflection_form(esize, nu):
     box elements; the FP32 copy is the rounded FP64 table; K0 itself leads the buffer."""
     K0 = fe.element_stiffness(3, esize=esize, nu=nu)
     bits, ext = _k0_extend(K0)
-    assert bits == 3  # box elements: class pattern and reflection form
+    assert bits == 7  # box elements: class pattern, reflection form, rigid-translation entries zero
     npt.assert_array_equal(ext[:576], K0.ravel())
     tab = ext[576:648]
     npt.assert_array_equal(ext[648:684].view(np.float32), tab.astype(np.float32))
