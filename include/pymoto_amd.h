@@ -145,6 +145,9 @@ int pmt_k0_pattern(int dim, const double* K0_host);
  * PMT_K0_RIGID (with PMT_K0_REFLECTION): the blocks' entries of the three rigid translations vanish (to
  * 1e-13 of the largest block entry; true for every stiffness matrix); with PMT_K0_CLASS_PATTERN the
  * reflection-basis kernels then skip all the pattern's zero entries.
+ * PMT_K0_CUBE (with PMT_K0_RIGID): the blocks have the isotropic cube's form - 8 numbers in a fixed
+ * pattern, equal within classes to 1e-12 of the largest entry; pmt_k0_extend then writes the class
+ * means (and exact zeros) and the fine kernel multiplies with the 8 numbers (33 operations per element).
  * pmt_k0_extend fills ext_host[PMT_K0_EXT_LEN] from a host K0 and returns the k0_pattern bits it
  * supports (PMT_K0_REFLECTION when K0 commutes with the reflections to 1e-13 of its largest entry).
  * Host only.  (No reference counterpart: a property of K0 from AssembleStiffness.__init__,
@@ -152,6 +155,7 @@ int pmt_k0_pattern(int dim, const double* K0_host);
 #define PMT_K0_CLASS_PATTERN 1
 #define PMT_K0_REFLECTION 2
 #define PMT_K0_RIGID 4
+#define PMT_K0_CUBE 8
 #define PMT_K0_EXT_LEN 684
 int pmt_k0_extend(int dim, const double* K0_host, double* ext_host);
 /* Launch-shape tuning of the fine-grid operator, process-wide, read at every launch.

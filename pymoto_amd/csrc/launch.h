@@ -56,10 +56,11 @@ int launch_fine_refl(int mode, const GridDims& g, int planes /* 0 = from the occ
 // 2 * ws_parts doubles).  Returns PMT_ERR_STATE (nothing launched) when the reflection kernel is not the
 // fine operator or the launch needs more partial sums than ws holds; the caller then runs K.u and
 // the dot products separately.
-// the fine reflection-basis kernel skips the pattern's zero block entries: the box class pattern and
-// vanishing rigid-translation entries
-inline bool refl_pattern(int k0_pattern) {
-  return (k0_pattern & PMT_K0_CLASS_PATTERN) && (k0_pattern & PMT_K0_RIGID);
+// element product of the fine reflection-basis kernel: 2 = the isotropic cube's 8 numbers, 1 = the box
+// class pattern's zero block entries skipped (with vanishing rigid-translation entries), 0 = dense blocks
+inline int refl_pattern(int k0_pattern) {
+  if (k0_pattern & PMT_K0_CUBE) return 2;
+  return ((k0_pattern & PMT_K0_CLASS_PATTERN) && (k0_pattern & PMT_K0_RIGID)) ? 1 : 0;
 }
 int launch_fine_apply_dots(const GridDims& g, const double* K0ext, int k0_pattern, const double* xe,
                            const uint8_t* bcm, double bcdiag, const double* p, const double* r, double* q,

@@ -870,8 +870,8 @@ int launch_fine(int mode, const GridDims& g, const double* K0, int k0_pattern, c
   const T bd = (T)bcdiag, om = (T)omega, be = (T)beta;
   const bool cls = (k0_pattern & PMT_K0_CLASS_PATTERN) != 0;
   if (g.dim == 3 && g_fine_kernel == 2 && (k0_pattern & PMT_K0_REFLECTION)) {
-    return launch_fine_refl<T>(mode, g, g_stream_nzc, g_refl_rows, refl_pattern(k0_pattern) ? 1 : 0, K0, xe, bcm,
-                               bcdiag, u, b, dinv, omega, y, s, up, beta);
+    return launch_fine_refl<T>(mode, g, g_stream_nzc, g_refl_rows, refl_pattern(k0_pattern), K0, xe, bcm, bcdiag, u,
+                               b, dinv, omega, y, s, up, beta);
   }
   if (g.dim == 3 && g_fine_kernel >= 1) {
     if (cls)

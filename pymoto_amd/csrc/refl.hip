@@ -75,6 +75,19 @@ __host__ __device__ constexpr bool refl_rigid(int chi, int d, int e) { return ch
 constexpr bool kReflSkipRigid = true;
 constexpr bool kReflMaskBallot = true;
 
+// Isotropic cube element (PMT_K0_CUBE): the blocks hold 8 numbers in a fixed pattern - chi 0 and 7:
+// a on the diagonal, b off it; chi 1, 2, 4: the pair of non-translation dofs coupled as [[c, d], [d, c]];
+// chi 3, 5, 6: the pair all e, the single dof f.  The kernel reads them once (as a - b, b, ...) and
+// folds the element factor into them per element.
+template <typename T>
+struct CubeBlocks {
+  T amb0, b0, amb7, b7, c, d, e, f;
+};
+template <typename T>
+__device__ inline CubeBlocks<T> cube_blocks(const T* __restrict__ tab) {
+  return {tab[0] - tab[1], tab[1], tab[63] - tab[64], tab[64], tab[13], tab[14], tab[27], tab[35]};
+}
+
 // v[0..3] / v[4..7]: the lower / upper plane's masked corner displacements after the x and y stages.
 // Returns x * (K0 u_e) in v before the inverse transform (frequency index k = kx + 2 ky + 4 kz): the
 // caller interleaves its stages with the node sums.  PAT skips the entries that the box pattern makes
@@ -105,6 +118,55 @@ __device__ inline void refl_element(const T* __restrict__ tab, T x, T (&v)[8][3]
       w[chi ^ (1 << d)][d] = acc;
     }
   }
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) v[a][d] = w[a][d];
+}
+
+// refl_element for the isotropic cube (CubeBlocks): 8 multiplies fold x into the numbers, then 33
+// operations instead of 72; the rigid translations' outputs (frequency 0) are zero.
+template <typename T>
+__device__ inline void refl_element_cube(const CubeBlocks<T>& cb, T x, T (&v)[8][3]) {
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T p = v[a][d], q = v[a + 4][d];
+      v[a][d] = p + q;
+      v[a + 4][d] = p - q;
+    }
+  const T A0 = cb.amb0 * x, B0 = cb.b0 * x, A7 = cb.amb7 * x, B7 = cb.b7 * x;
+  const T C = cb.c * x, D = cb.d * x, E = cb.e * x, F = cb.f * x;
+  T w[8][3];
+  // chi 0: inputs (v[1][0], v[2][1], v[4][2]); chi 7: (v[6][0], v[5][1], v[3][2])
+  {
+    const T bs = B0 * (v[1][0] + v[2][1] + v[4][2]);
+    w[1][0] = fma(A0, v[1][0], bs);
+    w[2][1] = fma(A0, v[2][1], bs);
+    w[4][2] = fma(A0, v[4][2], bs);
+  }
+  {
+    const T bs = B7 * (v[6][0] + v[5][1] + v[3][2]);
+    w[6][0] = fma(A7, v[6][0], bs);
+    w[5][1] = fma(A7, v[5][1], bs);
+    w[3][2] = fma(A7, v[3][2], bs);
+  }
+  // chi 1, 2, 4: the translation dof (frequency 0) drops out; the pair is [[c, d], [d, c]]
+  w[0][0] = w[0][1] = w[0][2] = (T)0;
+  w[3][1] = fma(C, v[3][1], D * v[5][2]);  // chi 1: dofs 1, 2
+  w[5][2] = fma(C, v[5][2], D * v[3][1]);
+  w[3][0] = fma(C, v[3][0], D * v[6][2]);  // chi 2: dofs 0, 2
+  w[6][2] = fma(C, v[6][2], D * v[3][0]);
+  w[5][0] = fma(C, v[5][0], D * v[6][1]);  // chi 4: dofs 0, 1
+  w[6][1] = fma(C, v[6][1], D * v[5][0]);
+  // chi 3, 5, 6: the pair is e (x + y) in both dofs, the single dof f
+  w[2][0] = w[1][1] = E * (v[2][0] + v[1][1]);  // chi 3: pair 0, 1; single 2
+  w[7][2] = F * v[7][2];
+  w[4][0] = w[1][2] = E * (v[4][0] + v[1][2]);  // chi 5: pair 0, 2; single 1
+  w[7][1] = F * v[7][1];
+  w[4][1] = w[2][2] = E * (v[4][1] + v[2][2]);  // chi 6: pair 1, 2; single 0
+  w[7][0] = F * v[7][0];
 #pragma unroll
   for (int a = 0; a < 8; ++a)
 #pragma unroll
@@ -243,6 +305,8 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
   if (work >= nwork) return;  // whole block leaves together (grid padded for the XCD remap)
   const int txy = work % ntiles_xy;
   const int tid = threadIdx.x, tx = tid % RX, ty = tid / RX;
+  // PAT 2 (isotropic cube): the 8 block numbers, read once and kept in scalar registers
+  const CubeBlocks<T> cube = (OP == 0 && PAT == 2) ? cube_blocks<T>(tab) : CubeBlocks<T>{};
   if (OP == 1)
     for (int q = tid; q < 576 && LDS_TAB; q += NT) sG0[q] = tab[q];  // visible after the first barrier below
   const int i = (txy % tiles_x) * (RX - 2) - 1 + tx;
@@ -393,7 +457,9 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
       }
     // the table entries are re-read through the scalar cache each layer (zero_stride is 0; the
     // compiler cannot hoist the loads, which would exceed the scalar register file)
-    if constexpr (OP == 0) {
+    if constexpr (OP == 0 && PAT == 2) {
+      refl_element_cube<T>(cube, xs[SA][0], v);
+    } else if constexpr (OP == 0) {
       refl_element<PAT != 0>(tab + zero_stride * k, xs[SA][0], v);
     } else {
       coarse1_element<PAT>((LDS_TAB ? sG0 : tab) + zero_stride * k, xs[SA], v);  // (re-read per layer)
@@ -517,6 +583,39 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
       partial[nwork + work] = so[1][0];
     }
   }
+}
+
+// The isotropic cube's block pattern (CubeBlocks): every class of entries equal to 1e-12 of the largest
+// entry and the other entries below 1e-13 of it.  On success the classes are set to their means and the
+// others to 0, so every kernel sees the same numbers.
+static bool k0_cube_form(double* tab, double scale) {
+  static const int kClass[8][12] = {
+      {0, 4, 8, -1},                                           // chi 0 diagonal
+      {1, 2, 3, 5, 6, 7, -1},                                  // chi 0 off-diagonal
+      {63, 67, 71, -1},                                        // chi 7 diagonal
+      {64, 65, 66, 68, 69, 70, -1},                            // chi 7 off-diagonal
+      {13, 17, 18, 26, 36, 40, -1},                            // c: chi 1, 2, 4 pair diagonal
+      {14, 16, 20, 24, 37, 39, -1},                            // d: chi 1, 2, 4 pair coupling
+      {27, 28, 30, 31, 45, 47, 51, 53, 58, 59, 61, 62},        // e: chi 3, 5, 6 pair
+      {35, 49, 54, -1}};                                       // f: chi 3, 5, 6 single dof
+  bool member[72] = {false};
+  double mean[8];
+  for (int c = 0; c < 8; ++c) {
+    double sum = 0.0;
+    int n = 0;
+    for (int q = 0; q < 12 && kClass[c][q] >= 0; ++q, ++n) sum += tab[kClass[c][q]];
+    mean[c] = sum / n;
+    for (int q = 0; q < 12 && kClass[c][q] >= 0; ++q) {
+      if (fabs(tab[kClass[c][q]] - mean[c]) > 1e-12 * scale) return false;
+      member[kClass[c][q]] = true;
+    }
+  }
+  for (int q = 0; q < 72; ++q)
+    if (!member[q] && fabs(tab[q]) > 1e-13 * scale) return false;
+  for (int q = 0; q < 72; ++q) tab[q] = 0.0;
+  for (int c = 0; c < 8; ++c)
+    for (int q = 0; q < 12 && kClass[c][q] >= 0; ++q) tab[kClass[c][q]] = mean[c];
+  return true;
 }
 
 // Block tables of an element matrix in the reflection basis; false if K0 does not commute with the
@@ -667,10 +766,12 @@ int launch_fine_refl(int mode, const GridDims& g, int planes, int rows, int patt
                      hipStream_t s, const T* up, double beta) {
   const T* tab = sizeof(T) == 8 ? (const T*)(K0ext + kK0ReflOffset64) : (const T*)(K0ext + kK0ReflOffset32);
   const T bd = (T)bcdiag, om = (T)omega, be = (T)beta;
-  if (pattern)
-    launch_refl_rows<0, true, T>(rows, mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
+  if (pattern == 2)
+    launch_refl_rows<0, 2, T>(rows, mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
+  else if (pattern == 1)
+    launch_refl_rows<0, 1, T>(rows, mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
   else
-    launch_refl_rows<0, false, T>(rows, mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
+    launch_refl_rows<0, 0, T>(rows, mode, g, planes, tab, xe, bcm, bd, u, b, dinv, om, y, s, up, be);
   return check_launch("fine_refl_kernel");
 }
 
@@ -706,12 +807,17 @@ int launch_fine_apply_dots(const GridDims& g, const double* K0ext, int k0_patter
     return PMT_ERR_STATE;
   const double* tab = K0ext + kK0ReflOffset64;
   int nwork = 0;
-  if (refl_pattern(k0_pattern))
-    launch_refl_rows<0, true, double>(get_tuning(PMT_TUNE_REFL_ROWS), MODE_APPLY_DOTS, g, tuned_stream_planes(), tab,
-                                      xe, bcm, bcdiag, p, r, nullptr, 0.0, q, s, nullptr, 0.0, ws, &nwork);
+  const int lvl = refl_pattern(k0_pattern);
+  const int rows = get_tuning(PMT_TUNE_REFL_ROWS);
+  if (lvl == 2)
+    launch_refl_rows<0, 2, double>(rows, MODE_APPLY_DOTS, g, tuned_stream_planes(), tab, xe, bcm, bcdiag, p, r,
+                                   nullptr, 0.0, q, s, nullptr, 0.0, ws, &nwork);
+  else if (lvl == 1)
+    launch_refl_rows<0, 1, double>(rows, MODE_APPLY_DOTS, g, tuned_stream_planes(), tab, xe, bcm, bcdiag, p, r,
+                                   nullptr, 0.0, q, s, nullptr, 0.0, ws, &nwork);
   else
-    launch_refl_rows<0, false, double>(get_tuning(PMT_TUNE_REFL_ROWS), MODE_APPLY_DOTS, g, tuned_stream_planes(), tab,
-                                       xe, bcm, bcdiag, p, r, nullptr, 0.0, q, s, nullptr, 0.0, ws, &nwork);
+    launch_refl_rows<0, 0, double>(rows, MODE_APPLY_DOTS, g, tuned_stream_planes(), tab, xe, bcm, bcdiag, p, r,
+                                   nullptr, 0.0, q, s, nullptr, 0.0, ws, &nwork);
   PMT_TRY(check_launch("fine_refl_kernel<apply + dots>"));
   sum_parts_kernel<<<1, 256, 0, s>>>(ws, nwork, out);
   return check_launch("sum_parts_kernel");
@@ -1086,6 +1192,14 @@ int pmt_k0_extend(int dim, const double* K0_host, double* ext_host) {
       bool rigid = true;  // the translations' entries M_(2^d)[d][d] vanish (K0 annihilates translations)
       for (int d = 0; d < 3; ++d) rigid = rigid && fabs(tab[(1 << d) * 9 + d * 4]) <= 1e-13 * scale;
       if (rigid) bits |= PMT_K0_RIGID;
+      if (rigid && k0_cube_form(tab, scale)) {
+        bits |= PMT_K0_CUBE;
+        float* t32 = (float*)(ext_host + kK0ReflOffset32);
+        for (int q = 0; q < 72; ++q) {  // the symmetrised table for every kernel that reads it
+          ext_host[kK0ReflOffset64 + q] = tab[q];
+          t32[q] = (float)tab[q];
+        }
+      }
     }
   }
   return bits;

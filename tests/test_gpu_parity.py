@@ -169,7 +169,7 @@ def test_reflection_operator_vs_streamed(gpu_lib, nel, planes, rows):
     x = rng.uniform(1e-3, 1.0, d.nel)
     bc = problems.cantilever3d(*nel)[0]
     K = pm.AssembleStiffness(d, bc=bc)(dev(x))
-    assert K.grid.k0_pattern == 7  # class pattern, reflection form, rigid translations
+    assert K.grid.k0_pattern == 15  # class pattern, reflection form, rigid translations, isotropic cube
     u, b = dev(rng.standard_normal(3 * d.nnodes)), dev(rng.standard_normal(3 * d.nnodes))
     even = all(n % 2 == 0 for n in nel) and min(nel) >= 2
     out = []
@@ -190,6 +190,35 @@ def test_reflection_operator_vs_streamed(gpu_lib, nel, planes, rows):
     npt.assert_allclose(out[1][2], out[0][2], rtol=0, atol=1e-5 * np.abs(out[0][2]).max())
     if even:
         npt.assert_allclose(out[1][3], out[0][3], rtol=0, atol=1e-11 * np.abs(out[0][3]).max())
+
+
+@pytest.mark.parametrize("unit", [(1.0, 2.0, 0.5), (1.0, 1.0, 1.0)])
+def test_reflection_element_forms_vs_streamed(gpu_lib, unit):
+    """A box element (1 x 2 x 0.5: class pattern and rigid zeros, dense-pattern products) and the unit cube
+    (the 8-number isotropic form) in the reflection-basis kernel against the dense-K0 streamed kernel:
+    K u, b - K u, FP32 K u and a V-cycle, rounding only."""
+    from pymoto_amd import tune
+    nel = (30, 14, 10)
+    d = pm.VoxelDomain(*nel, *unit)
+    rng = np.random.default_rng(17)
+    K = pm.AssembleStiffness(d, bc=problems.cantilever3d(*nel)[0])(dev(rng.uniform(1e-3, 1.0, d.nel)))
+    assert K.grid.k0_pattern == (15 if unit == (1.0, 1.0, 1.0) else 7)
+    u, b = dev(rng.standard_normal(3 * d.nnodes)), dev(rng.standard_normal(3 * d.nnodes))
+    out = []
+    try:
+        for kernel in (1, 2):
+            tune.set_tuning(fine_kernel=kernel)
+            mg = pm.solvers.build_hierarchy(d, 2, omega=0.5, smooth_steps=2)
+            mg.update(K)
+            out.append([host(K @ u), host(K.residual(b, u)), host(K.apply_reduced(u.to(torch.float32), "fp32")),
+                        host(mg.solve(b))])
+    finally:
+        tune.set_tuning(**tune.DEFAULT)
+    scale = np.abs(out[0][0]).max()
+    npt.assert_allclose(out[1][0], out[0][0], rtol=0, atol=1e-13 * scale)
+    npt.assert_allclose(out[1][1], out[0][1], rtol=0, atol=1e-13 * max(scale, np.abs(out[0][1]).max()))
+    npt.assert_allclose(out[1][2], out[0][2], rtol=0, atol=1e-5 * np.abs(out[0][2]).max())
+    npt.assert_allclose(out[1][3], out[0][3], rtol=0, atol=1e-11 * np.abs(out[0][3]).max())
 
 
 @pytest.mark.parametrize("nel", [(40, 12, 10), (62, 22, 4), (30, 28, 6)])

@@ -63,7 +63,9 @@ def test_k0_reflection_form(esize, nu):
     box elements; the FP32 copy is the rounded FP64 table; K0 itself leads the buffer."""
     K0 = fe.element_stiffness(3, esize=esize, nu=nu)
     bits, ext = _k0_extend(K0)
-    assert bits == 7  # box elements: class pattern, reflection form, rigid-translation entries zero
+    # box elements: class pattern, reflection form, rigid-translation entries zero; the cube also the
+    # 8-number isotropic form (PMT_K0_CUBE)
+    assert bits == (15 if esize == (1.0, 1.0, 1.0) else 7)
     npt.assert_array_equal(ext[:576], K0.ravel())
     tab = ext[576:648]
     npt.assert_array_equal(ext[648:684].view(np.float32), tab.astype(np.float32))
