@@ -269,6 +269,49 @@ def kernel_roofline(pm, torch, nel, reps=20):
     return out
 
 
+def cg_iteration_roofline(pm, torch, nel, precision, omega, levels, tol=1e-8):
+    """Whole-CG-iteration figure for a solver variant: milliseconds per PCG iteration of one cold solve on
+    a seeded filtered design (HIP events around the solve on its stream, divided by the iterations), and
+    the modeled algorithmic HBM bytes of one iteration - every kernel reading its inputs and writing its
+    outputs once: the fine FP64 K.u fused with p.q and p.r (p, r, q, densities, mask), the x/r update
+    (p, q, x, r read, x and r written), the V-cycle's first sweep from zero, the rest of its fine level
+    (3 Jacobi sweeps and the residual: u, b, y, densities, mask each; restriction; prolongation), 1/7
+    more for the coarse levels (1/8 of the dofs per level), the direction update and q.z.  Returns a
+    dict."""
+    from pymoto_amd import problems
+    d = pm.VoxelDomain(*nel)
+    bc, f = problems.cantilever3d(d)
+    x = torch.as_tensor(np.random.default_rng(11).uniform(0.0, 1.0, d.nel), device="cuda")
+    xf = pm.FilterConv(d, radius=1.5)(pm.Signal("x", x)).state
+    xs = pm.MathExpression(f"{1e-9} + {1.0 - 1e-9}*inp0^3")(pm.Signal("xf", xf)).state
+    K = pm.AssembleStiffness(d, bc=bc)(xs)
+    cg = pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(d, levels, omega=omega, smooth_steps=2,
+                                                                 precision=precision), tol=tol)
+    cg.update(K)
+    fd = torch.as_tensor(f, device="cuda")
+    cg.solve(fd)  # warm-up (first-call costs)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    cg.solve(fd)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    its = int(cg.iterations)
+    ms = e0.elapsed_time(e1) / max(its, 1)
+    n, e, nn = 3 * d.nnodes, d.nel, d.nnodes
+    S = 8.0
+    s = 8.0 if precision == "fp64" else 4.0
+    vfine = 4 * (3 * n * s + e * s + nn) + (n * s + n * s / 8) + (2 * n * s + n * s / 8)
+    # the V-cycle's first sweep from zero (D^-1 and r read, u written) is fused into the x/r update for
+    # reduced-precision V-cycles and a kernel of its own in FP64: 3 n s either way
+    model = (3 * n * S + e * S + nn) + 6 * n * S + 3 * n * s + (8.0 / 7.0) * vfine + (n * s + 2 * n * S) \
+        + (n * S + n * s)
+    return {"variant": f"{precision}@{omega:g}", "cg_iterations": its, "ms_per_iteration": ms,
+            "modeled_bytes": model, "achieved_gbs": model / (ms * 1e-3) / 1e9,
+            "frac": model / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+            "note": "cold solve from zero on a seeded filtered design; bytes modeled, not counted"}
+
+
 def ku_driver(pm, torch, nel, reps=5, which="all"):
     """Kernel-only run for rocprofv3 (--pmc counter passes, --kernel-trace): five launches each of the fine
     K.u in FP64 (reflection basis), FP32, BF16 (matrix cores), V-cycles (FP64, FP32), the FilterConv
@@ -610,6 +653,9 @@ def main():
         line["config"]["batched"] = batched_workload(pm, torch)
     if args.c5_pairs > 0:
         line["config"]["learned_128x64x64"] = learned_workload(pm, torch, pairs=args.c5_pairs, epochs=args.c5_epochs)
+    bp, bo = parse_variant(best)
+    line["config"]["cg_iteration_roofline"] = cg_iteration_roofline(
+        pm, torch, nel, bp, bo if isinstance(bo, float) else args.omega, levels, args.tol)
     if roof is not None:
         line["roofline"] = {"bound": "hbm", "achieved": roof["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
                             "frac": roof["gbs"] / PEAK_HBM_GBS, "traffic": roof["traffic"], "kernel": roof["kernel"],

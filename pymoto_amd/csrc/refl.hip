@@ -74,6 +74,7 @@ __host__ __device__ constexpr bool refl_rigid(int chi, int d, int e) { return ch
 // pattern's zeros; skip the Dirichlet selects in waves whose nodes of the plane have none.
 constexpr bool kReflSkipRigid = true;
 constexpr bool kReflMaskBallot = true;
+constexpr bool kReflCarry64 = true;  // FP64 also carries the transformed lower plane in registers
 
 // Isotropic cube element (PMT_K0_CUBE): the blocks hold 8 numbers in a fixed pattern - chi 0 and 7:
 // a on the diagonal, b off it; chi 1, 2, 4: the pair of non-translation dofs coupled as [[c, d], [d, c]];
@@ -299,6 +300,8 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
   __shared__ T so[DIAG ? 7 : 6][NT];  // y-stage differences (corners ay = 1) of the layer's results, both kz;
                                       // DIAG: then the x-summed element factors
   constexpr bool LDS_TAB = OP == 1 && sizeof(T) == 8;
+  // carry the transformed lower plane to the next layer in registers (else rebuild it from LDS)
+  constexpr bool CARRY = kReflCarry64 || sizeof(T) == 4;
   __shared__ T sG0[LDS_TAB ? 576 : 1];  // OP 1, FP64: the 24 x 24 table (read as broadcasts)
 
   const int work = (int)xcd_block();
@@ -446,6 +449,20 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
     masked(raw[SB], mk[SB], c1);
     T Whi[4][3];
     plane_corners(c1, su[(k + 1) & 1], Whi);
+    if constexpr (!CARRY) {
+      // rebuild the lower plane from the x-stage values the previous layer left in LDS (its own and the
+      // +y row's): 24 fewer live registers than carrying it
+      T(*const pl)[NT] = su[k & 1];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const T sx = pl[d][tid], dx = pl[3 + d][tid];
+        const T sy = pl[d][slot_up], dy = pl[3 + d][slot_up];
+        Wlo[0][d] = sx + sy;
+        Wlo[2][d] = sx - sy;
+        Wlo[1][d] = dx + dy;
+        Wlo[3][d] = dx - dy;
+      }
+    }
     T v[8][3];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -453,7 +470,7 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
       for (int d = 0; d < 3; ++d) {
         v[a][d] = Wlo[a][d];
         v[4 + a][d] = Whi[a][d];
-        Wlo[a][d] = Whi[a][d];
+        if constexpr (CARRY) Wlo[a][d] = Whi[a][d];
       }
     // the table entries are re-read through the scalar cache each layer (zero_stride is 0; the
     // compiler cannot hoist the loads, which would exceed the scalar register file)
