@@ -79,6 +79,10 @@ def parse_variant(v):
     return prec.strip(), (float(om) if om else None)
 
 
+def smoother_name(omega):
+    return "Chebyshev smoother" if omega == "chebyshev" else f"damped Jacobi omega {omega}"
+
+
 def build_network(pm, torch, nel, levels, tol, precision, omega=0.6, steps=2, warm_net=None, coarse_space=True):
     from pymoto_amd import problems
     d = pm.VoxelDomain(*nel)
@@ -126,18 +130,20 @@ def train_warm_start(pm, torch, args, precision, omega, nsamples, epochs):
 
 
 def learned_workload(pm, torch, nel=(128, 64, 64), pairs=2000, epochs=2, precision="fp32", omega=0.5, tol=1e-8,
-                     n_eval=8):
+                     n_eval=8, omega_data="chebyshev", maxit_data=3000):
     """Learned warm start / coarse correction at 128x64x64: ``pairs`` (input, displacement) pairs from
     seeded SIMP/OC trajectories solved by this package's MG-PCG (volume fractions 0.3/0.4/0.5, 60 design
-    iterations each), the network trained on them on the GPU, then evaluated on the designs of a held-out
+    iterations each; Chebyshev smoother, at most ``maxit_data`` iterations per solve: late OC designs
+    need 100-160 iterations at this size and Jacobi omega 0.5 occasionally many more), the network
+    trained on them on the GPU, then evaluated on the designs of a held-out
     trajectory (another seed): CG iterations and seconds per solve for the plain MG-PCG, the network's warm
     start, and warm start plus the learned coarse correction (flexible PCG), every solve to tol; the
     compliance of each against an FP64 MG-PCG solve to 1e-12."""
     import time as _t
     from pymoto_amd import learned, problems
     t0 = _t.perf_counter()
-    samples, info = learned.trajectory_samples(nel, pairs, seeds=(11, 12, 13), precision=precision, omega=omega,
-                                               tol=tol)
+    samples, info = learned.trajectory_samples(nel, pairs, seeds=(11, 12, 13), precision=precision,
+                                               omega=omega_data, tol=tol, maxit=maxit_data)
     torch.cuda.synchronize()
     t_data = _t.perf_counter() - t0
     net = learned.WarmStartNet(nel, seed=0)
@@ -197,6 +203,7 @@ def learned_workload(pm, torch, nel=(128, 64, 64), pairs=2000, epochs=2, precisi
     res = {"workload": f"learned warm start / coarse correction, cantilever {nel[0]}x{nel[1]}x{nel[2]} hex8, "
                        f"{precision} V-cycle (omega {omega}) in FP64 CG, tol {tol:g}",
            "training_pairs": info["pairs"], "trajectories": info["trajectories"], "epochs": epochs,
+           "data_smoother": smoother_name(omega_data), "data_solver_iterations": info["solver_iterations"],
            "data_s": t_data, "train_s": t_train, "loss_history": [float(h) for h in hist],
            "params": int(net.nparams()), "held_out_designs": len(designs)}
     for mode in out:
@@ -392,15 +399,18 @@ def ku_driver(pm, torch, nel, reps=5, which="all"):
 
 
 def batched_workload(pm, torch, nel=(128, 64, 64), beta=8.0, etas=(0.3, 0.5, 0.7), nloads=4, nuneven=16, seed=7,
-                     precision="fp32", omega=0.5, tol=1e-8, xmin=1e-9):
+                     precision="fp32", omega="chebyshev", omega_uneven=0.5, tol=1e-8, xmin=1e-9):
     """Robust multi-load batch: 3 Heaviside designs (eta 0.3/0.5/0.7, beta 8, SIMP xmin 1e-9: dilated /
     nominal / eroded projections as in the reference's robust example,
     examples/topology_optimization/ex_compliance_robust.py:137-149) x 4 seeded point loads on
     the x = max face, each design one 4-column block PCG (captured-graph iterations), one after the other
     and the three at once on three streams, timed against the same 12 solves one right-hand side at a time
     and as 12 concurrent single-right-hand-side solves (one handle and stream each); the fine operator's
-    launch shape is picked by the autotuner first.  Plus 16 independent problems of seeded sizes from 64x32
-    Q4 to 96x48x48 hex8, solved concurrently on 8 streams and timed against one stream."""
+    launch shape is picked by the autotuner first.  The robust designs use the Chebyshev smoother: the
+    eroded design's lambda_max(D^-1 K) is about 4.7, so damped Jacobi with omega 0.5 amplifies its top
+    modes (omega * lambda_max > 2) and PCG needs 2-6x the iterations or stalls.  Plus 16 independent
+    problems of seeded sizes from 64x32 Q4 to 96x48x48 hex8 (filtered uniform densities, Jacobi
+    omega_uneven), solved concurrently on 8 streams and timed against one stream."""
     import time as _t
     from pymoto_amd import problems
     from pymoto_amd.batch import solve_independent
@@ -485,7 +495,8 @@ def batched_workload(pm, torch, nel=(128, 64, 64), beta=8.0, etas=(0.3, 0.5, 0.7
         K2 = pm.AssembleStiffness(dd, bc=b2)(xxs)
         prec = precision if (dd.dim == 3 or precision != "bf16") else "fp32"
         sv = pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(dd, max(2, pm.solvers.auto_levels(dd)),
-                                                                     omega=omega, smooth_steps=2, precision=prec),
+                                                                     omega=omega_uneven, smooth_steps=2,
+                                                                     precision=prec),
                            tol=tol, maxit=2000)
         jobs.append((K2, torch.as_tensor(f2, device="cuda"), sv))
         dofs += K2.shape[0]
@@ -499,7 +510,8 @@ def batched_workload(pm, torch, nel=(128, 64, 64), beta=8.0, etas=(0.3, 0.5, 0.7
     ndof = 3 * d.nnodes
     return {
         "workload": f"robust {nel[0]}x{nel[1]}x{nel[2]} hex8, Heaviside beta {beta} eta {list(etas)}, SIMP xmin {xmin:g}, "
-                    f"{nloads} point loads per design; {precision} V-cycle (omega {omega}), FP64 CG, tol {tol:g}",
+                    f"{nloads} point loads per design; {precision} V-cycle ({smoother_name(omega)}), FP64 CG, "
+                    f"tol {tol:g}",
         "rhs_solves": len(etas) * nloads, "ndof": ndof,
         "block_graph_s": t_block, "block_iterations": its_block,
         "block_graph_3_streams_s": t_block_conc, "block_3_streams_iterations": its_block_conc,
@@ -507,7 +519,8 @@ def batched_workload(pm, torch, nel=(128, 64, 64), beta=8.0, etas=(0.3, 0.5, 0.7
         "single_rhs_12_streams_s": t_single_conc, "single_rhs_12_streams_iterations": its_conc12,
         "dof_rhs_per_s_best": ndof * len(etas) * nloads / min(t_block, t_block_conc, t_single, t_single_conc),
         "autotuned_fine_operator": tuned,
-        "uneven": {"problems": nuneven, "total_dofs": dofs, "concurrent_8_streams_s": t_conc,
+        "uneven": {"problems": nuneven, "smoother": smoother_name(omega_uneven), "total_dofs": dofs,
+                   "concurrent_8_streams_s": t_conc,
                    "one_stream_s": t_seq, "iterations": its_uneven},
     }
 

@@ -161,7 +161,8 @@ static int smooth(pmt_mg* mg, int l, const T* b, T*& cur, T*& oth, bool from_zer
   const double sigma = theta / delta;
   double rho_prev = 1.0 / sigma;
   if (from_zero) {
-    PMT_TRY(launch_jacobi0<T>(L.ndof, B.dinv, b, 1.0 / theta, cur, s));
+    // first_done: cur already holds D^-1 b / theta (written by the CG update)
+    if (!first_done) PMT_TRY(launch_jacobi0<T>(L.ndof, B.dinv, b, 1.0 / theta, cur, s));
   } else {
     PMT_TRY(level_apply<T>(mg, l, MODE_JACOBI, cur, b, oth, s, 1.0 / theta));
     T* tmp = cur;
@@ -848,9 +849,12 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
     // reduced-precision V-cycles: the update also writes r in FP32 (the V-cycle's input), and z stays
     // FP32 (q.z and p = z + beta p widen it exactly: the same sums as through an FP64 copy)
     const bool mixed = mg->prec != 64;
-    // damped Jacobi: the update also writes the V-cycle's first fine sweep from zero, omega D^-1 r32, into
-    // the fine level's work vector (the same FP32 expression as the sweep kernel)
-    const bool first = mixed && mg->smoother == 0;
+    // the update also writes the V-cycle's first fine sweep from zero into the fine level's work vector:
+    // omega D^-1 r32 (damped Jacobi) or D^-1 r32 / theta (Chebyshev), the same FP32 expression as the
+    // sweep kernel
+    const bool first = mixed && (mg->smoother == 0 || mg->smoother == 1);
+    const double first_factor =
+        mg->smoother == 0 ? F.omega : 1.0 / (0.5 * (F.cheb_hi + F.cheb_lo));  // Chebyshev: 1 / theta (fine)
     bool r32_ready = false;
     if (i % restart == 0) {
       PMT_TRY(launch_cg_x_update(n, sc, mg->p, x, s));
@@ -858,7 +862,7 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
       PMT_TRY(launch_dot(n, mg->r, mg->r, sc + 2, mg->work, s));
     } else {
       PMT_TRY(launch_cg_xr_update(n, sc, mg->p, mg->q, x, mg->r, mg->work, sc + 2, s, mixed ? F.f.b : nullptr,
-                                  first ? F.f.dinv : nullptr, (float)F.omega, first ? F.f.t : nullptr));
+                                  first ? F.f.dinv : nullptr, (float)first_factor, first ? F.f.t : nullptr));
       r32_ready = mixed;
     }
     // z = M r, sc[3] = q.z, p = z + beta p: reads r and q, writes only z, sc[3] and p (no result)

@@ -280,8 +280,16 @@ __device__ inline void coarse1_element(const T* tab, const T (&X)[8], T (&v)[8][
 // OP 0: the fine grid (element factor x_e, Dirichlet-masked nodes, bcdiag rows).  OP 1: the first
 // coarse level of a hex8 hierarchy (element factors X^[8], layout [layer][m][element]; no mask: the
 // elements with Dirichlet children get their corrections from coarse1_correct_kernel afterwards).
+// Waves per SIMD the register allocation must leave room for: the fine FP64 residual and the CG product
+// fit 168 VGPRs (3 waves) without spilling (2 waves otherwise); every other instance keeps the
+// compiler's choice (a bound on the sweeps makes them spill, which costs more than the wave gains).
+template <int OP, int MODE, typename T>
+constexpr int refl_min_waves() {
+  return (OP == 0 && sizeof(T) == 8 && (MODE == MODE_RESIDUAL || MODE == MODE_APPLY_DOTS)) ? 3 : 1;
+}
+
 template <int OP, int RX, int SY, int MODE, int PAT, typename T>
-__global__ void __launch_bounds__(RX* SY)
+__global__ void __launch_bounds__(RX* SY) __attribute__((amdgpu_waves_per_eu(refl_min_waves<OP, MODE, T>())))
 fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, const T* __restrict__ tab, int zero_stride,
                  const T* __restrict__ xe, const uint8_t* __restrict__ bcm, T bcdiag, const T* __restrict__ u,
                  const T* __restrict__ b, const T* __restrict__ dinv, T omega, T* y, const T* up, T beta,

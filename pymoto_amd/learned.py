@@ -400,12 +400,13 @@ class WarmStartCG:
 
 # ---------------------------------------------------------------------- solver-generated training data
 def trajectory_samples(nel, n_pairs, seeds=(0,), volfracs=(0.3, 0.4, 0.5), iters=60, precision="fp32", omega=0.5,
-                       tol=1e-8, radius=1.5, move=0.2, keep_every=1):
+                       tol=1e-8, radius=1.5, move=0.2, keep_every=1, maxit=10000):
     """(input, normalised target) pairs from seeded SIMP/OC trajectories of the cantilever (filter r=1.5,
     SIMP p=3, the MG-PCG solve of this package, OC with the reference's bisection,
     ``pymoto/common/optimizers.py:337-437``): trajectory t starts from the uniform design at volume
     fraction volfracs[t % len] plus seeded noise, and every ``keep_every``-th design iteration gives one
-    pair, until ``n_pairs`` pairs exist.  Returns (samples, info)."""
+    pair, until ``n_pairs`` pairs exist; ``omega`` is the Jacobi factor or "chebyshev", ``maxit`` bounds each
+    solve.  Returns (samples, info)."""
     import pymoto_amd as pm
     from . import problems
     d = pm.VoxelDomain(*nel)
@@ -420,7 +421,8 @@ def trajectory_samples(nel, n_pairs, seeds=(0,), volfracs=(0.3, 0.4, 0.5), iters
         x0 = np.clip(vf + 0.1 * (rng.random(d.nel) - 0.5), 0.0, 1.0)
         sx = pm.Signal("x", _lib.as_device(x0))
         solver = pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(d, levels, omega=omega, smooth_steps=2,
-                                                                         precision=precision), tol=tol)
+                                                                         precision=precision), tol=tol,
+                                   maxit=maxit)
         with pm.Network() as fn:
             sxf = pm.FilterConv(d, radius=radius)(sx)
             ss = pm.MathExpression(f"{1e-9} + {1.0 - 1e-9}*inp0^3")(sxf)

@@ -781,6 +781,47 @@ def test_chebyshev_intervals_cover_the_spectrum(gpu_lib):
     assert 0.9 * l1 <= hi1 / 1.1 <= l1 * (1 + 1e-9) and hi1 >= l1
 
 
+def test_chebyshev_on_eroded_robust_design(gpu_lib):
+    """The eroded projection of the robust formulation (Heaviside beta 8, eta 0.7 of filtered uniform[0.35, 1]
+    densities, SIMP xmin 1e-9; reference examples/topology_optimization/ex_compliance_robust.py:137-149) has
+    lambda_max(D^-1 K) of about 4.75 (eigsh on the oracle's K): damped Jacobi with omega 0.5 then amplifies
+    the top modes (omega lambda_max > 2), its V-cycle is not a positive definite preconditioner, and PCG
+    needs more iterations, in FP32 more than in FP64.  The Chebyshev smoother, whose fine interval ends at
+    lambda_max(diag(K0)^-1 K0) = 5.318 >= lambda_max(D^-1 K) for any positive densities, converges in FP32
+    with the FP64 iteration count (within one), and both reach the direct solution's compliance."""
+    import scipy.sparse as sps
+    from scipy.sparse.linalg import eigsh, spsolve
+    from oracle.filter import filter_weights
+    from oracle.pipeline import heaviside
+    nel = (32, 16, 16)
+    rng = np.random.default_rng(7)
+    xf = ConvFilter(nel, filter_weights(1.5, nel)).forward(rng.uniform(0.35, 1.0, 32 * 16 * 16))
+    xs = fe.simp(heaviside(xf, 8.0, 0.7)[0], 1e-9, 3.0)
+    bc, f = problems.cantilever3d(*nel)
+    K0 = fe.element_stiffness(3)
+    A = fe.Assembly(*nel, K0, bc=bc).matrix(xs)
+    s = 1.0 / np.sqrt(A.diagonal())
+    lam = eigsh(sps.diags(s) @ A @ sps.diags(s), k=1, which="LA")[0][0]
+    s0 = 1.0 / np.sqrt(np.diag(K0))
+    lam0 = np.linalg.eigvalsh(s0[:, None] * K0 * s0[None, :]).max()
+    assert 4.0 < lam <= lam0 and abs(lam0 - 5.318181818) < 1e-6
+    c_ref = float(f @ spsolve(A.tocsc(), f))
+    d = pm.VoxelDomain(*nel)
+    K = pm.AssembleStiffness(d, bc=bc)(dev(xs))
+    its = {}
+    for prec, om in (("fp32", 0.5), ("fp64", "chebyshev"), ("fp32", "chebyshev")):
+        sv = pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(d, 3, omega=om, smooth_steps=2, precision=prec),
+                           tol=1e-10, maxit=3000)
+        sv.update(K)
+        u = sv.solve(dev(f))
+        its[(prec, om)] = sv.iterations
+        print(prec, om, sv.iterations, flush=True)
+        if om == "chebyshev":
+            assert abs(float(torch.dot(u, dev(f))) - c_ref) / abs(c_ref) < 1e-9
+    assert abs(its[("fp32", "chebyshev")] - its[("fp64", "chebyshev")]) <= 1
+    assert its[("fp32", "chebyshev")] < its[("fp32", 0.5)]
+
+
 @pytest.mark.parametrize("nel", [(70, 9, 37), (130, 7, 3), (131, 45, 0), (64, 4, 16)])
 @pytest.mark.parametrize("radius", [1.5, 2.5])
 @pytest.mark.parametrize("bcs", [("symmetric",) * 6, ("edge", "wrap", "symmetric", 0.25, "wrap", "edge"),
