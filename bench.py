@@ -280,11 +280,11 @@ def cg_iteration_roofline(pm, torch, nel, precision, omega, levels, tol=1e-8):
     """Whole-CG-iteration figure for a solver variant: milliseconds per PCG iteration of one cold solve on
     a seeded filtered design (HIP events around the solve on its stream, divided by the iterations), and
     the modeled algorithmic HBM bytes of one iteration - every kernel reading its inputs and writing its
-    outputs once: the fine FP64 K.u fused with p.q and p.r (p, r, q, densities, mask), the x/r update
-    (p, q, x, r read, x and r written), the V-cycle's first sweep from zero, the rest of its fine level
-    (3 Jacobi sweeps and the residual: u, b, y, densities, mask each; restriction; prolongation), 1/7
-    more for the coarse levels (1/8 of the dofs per level), the direction update and q.z.  Returns a
-    dict."""
+    outputs once: the fine FP64 K.u fused with p.q and p.r (p, r, q, densities, mask), the r update (q, r
+    read, r written), the V-cycle's first sweep from zero, the rest of its fine level (3 Jacobi sweeps
+    and the residual: u, b, y, densities, mask each; restriction; prolongation), 1/7 more for the coarse
+    levels (1/8 of the dofs per level), q.z, and the direction update with the x update (z, p, x read,
+    p and x written).  Returns a dict."""
     from pymoto_amd import problems
     d = pm.VoxelDomain(*nel)
     bc, f = problems.cantilever3d(d)
@@ -311,7 +311,7 @@ def cg_iteration_roofline(pm, torch, nel, precision, omega, levels, tol=1e-8):
     vfine = 4 * (3 * n * s + e * s + nn) + (n * s + n * s / 8) + (2 * n * s + n * s / 8)
     # the V-cycle's first sweep from zero (D^-1 and r read, u written) is fused into the x/r update for
     # reduced-precision V-cycles and a kernel of its own in FP64: 3 n s either way
-    model = (3 * n * S + e * S + nn) + 6 * n * S + 3 * n * s + (8.0 / 7.0) * vfine + (n * s + 2 * n * S) \
+    model = (3 * n * S + e * S + nn) + 3 * n * S + 3 * n * s + (8.0 / 7.0) * vfine + (n * s + 4 * n * S) \
         + (n * S + n * s)
     return {"variant": f"{precision}@{omega:g}", "cg_iterations": its, "ms_per_iteration": ms,
             "modeled_bytes": model, "achieved_gbs": model / (ms * 1e-3) / 1e9,

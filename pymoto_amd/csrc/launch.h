@@ -107,12 +107,14 @@ int launch_dot(long long n, const double* x, const float* y, double* out, double
 int launch_dot2(long long n, const double* a0, const double* b0, const double* a1, const double* b1, double* out,
                 double* work, hipStream_t s);
 int launch_axpby(long long n, double a, const double* x, double b, double* y, hipStream_t s);
+// x = nullptr: r only (the x update deferred to launch_cg_p_update)
 int launch_cg_xr_update(long long n, const double* sc, const double* p, const double* q, double* x, double* r,
                         double* work, double* out_rr, hipStream_t s, float* r32 = nullptr,
                         const float* dinv32 = nullptr, float omega32 = 0.0f, float* u1 = nullptr);
 int launch_cg_x_update(long long n, const double* sc, const double* p, double* x, hipStream_t s);
-int launch_cg_p_update(long long n, const double* sc, const double* z, double* p, hipStream_t s);
-int launch_cg_p_update(long long n, const double* sc, const float* z, double* p, hipStream_t s);
+// x (optional): also x += (sc[1]/sc[0]) p_old first, the update a cg_xr_update with x = nullptr deferred
+int launch_cg_p_update(long long n, const double* sc, const double* z, double* p, hipStream_t s, double* x = nullptr);
+int launch_cg_p_update(long long n, const double* sc, const float* z, double* p, hipStream_t s, double* x = nullptr);
 int launch_scale_rsqrt(long long n, const double* sc, int k, const double* z, double* p, hipStream_t s);
 int launch_fill(long long n, double v, double* y, hipStream_t s);
 // host[0] = *v, then host[1] = seq (pinned host memory; the host polls host[1])

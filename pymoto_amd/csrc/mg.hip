@@ -834,6 +834,7 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
   PMT_TRY(launch_scale_rsqrt(n, sc, 5, mg->z, mg->p, s));
   int i = 0;
   double tprev = 0.0;  // the relative residual one iteration before tval (0: none yet)
+  bool x_pending = false;  // x lacks this iteration's alpha p (added by the p update or after the loop)
   for (i = 0; i < maxit; ++i) {
     // q = A p, sc[0] = p.q, sc[1] = p.r: fused into the reflection-basis operator when it is the fine
     // operator, else K.u and one two-dot pass
@@ -861,9 +862,11 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
       PMT_TRY(level_apply<double>(mg, 0, MODE_RESIDUAL, x, b, mg->r, s));
       PMT_TRY(launch_dot(n, mg->r, mg->r, sc + 2, mg->work, s));
     } else {
-      PMT_TRY(launch_cg_xr_update(n, sc, mg->p, mg->q, x, mg->r, mg->work, sc + 2, s, mixed ? F.f.b : nullptr,
+      // x += alpha p is deferred to the p update, which reads p anyway (x_pending until it runs)
+      PMT_TRY(launch_cg_xr_update(n, sc, mg->p, mg->q, nullptr, mg->r, mg->work, sc + 2, s, mixed ? F.f.b : nullptr,
                                   first ? F.f.dinv : nullptr, (float)first_factor, first ? F.f.t : nullptr));
       r32_ready = mixed;
+      x_pending = true;
     }
     // z = M r, sc[3] = q.z, p = z + beta p: reads r and q, writes only z, sc[3] and p (no result)
     auto next_direction = [&]() -> int {
@@ -871,7 +874,9 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
         PMT_TRY(precondition_f32(mg, mg->r, r32_ready, s, r32_ready && first));
         if (mg->cs_m == 0) {
           PMT_TRY(launch_dot(n, mg->q, mg->z32, sc + 3, mg->work, s));
-          return launch_cg_p_update(n, sc, mg->z32, mg->p, s);
+          PMT_TRY(launch_cg_p_update(n, sc, mg->z32, mg->p, s, x_pending ? x : nullptr));
+          x_pending = false;
+          return PMT_OK;
         }
         // with a coarse space the projection runs in FP64: rounded to FP32 it would leave components
         // along W of relative size 1e-7 in every direction, and CG would stall at that level
@@ -881,7 +886,9 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
       }
       PMT_TRY(launch_coarse_correct(n, mg->cs_m, mg->cs_W, mg->cs_AW, mg->cs_Einv, mg->cs_buf, mg->z, s));
       PMT_TRY(launch_dot(n, mg->q, mg->z, sc + 3, mg->work, s));
-      return launch_cg_p_update(n, sc, mg->z, mg->p, s);
+      PMT_TRY(launch_cg_p_update(n, sc, mg->z, mg->p, s, x_pending ? x : nullptr));
+      x_pending = false;
+      return PMT_OK;
     };
     PMT_TRY(publish_scalar(mg, 2, s));
     // when the residual's rate of decrease says this iteration will not stop, the next direction is
@@ -906,6 +913,8 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
     if (tval <= tol) break;
     if (!ahead) PMT_TRY(next_direction());
   }
+  // stopped before the p update of the last iteration ran: its x update
+  if (x_pending) PMT_TRY(launch_cg_x_update(n, sc, mg->p, x, s));
   if (iters) *iters = i < maxit ? i + 1 : maxit;
   if (relres) *relres = tval;
   return PMT_OK;

@@ -174,10 +174,11 @@ int launch_dot2(long long n, const double* a0, const double* b0, const double* a
   return check_launch("dot_stage2<2>");
 }
 
-// CG: x += alpha p, r -= alpha q, partial ||r||^2; alpha = sc[1]/sc[0] read on device; r32 (optional):
-// the new r rounded to FP32, the input of a reduced-precision preconditioner; u1 (optional, with r32):
-// omega * D^-1 * r32, the preconditioner's first damped-Jacobi sweep from zero (same FP32 expression as
-// jacobi0_kernel)
+// CG: x += alpha p (WITHX; else the caller adds it later, in cg_p_update), r -= alpha q, partial ||r||^2;
+// alpha = sc[1]/sc[0] read on device; r32 (optional): the new r rounded to FP32, the input of a
+// reduced-precision preconditioner; u1 (optional, with r32): omega * D^-1 * r32, the preconditioner's
+// first sweep from zero (same FP32 expression as jacobi0_kernel)
+template <bool WITHX>
 __global__ void __launch_bounds__(kReduceThreads)
 cg_xr_update(long long n, const double* __restrict__ sc, const double* __restrict__ p, const double* __restrict__ q,
              double* __restrict__ x, double* __restrict__ r, double* __restrict__ partial, float* __restrict__ r32,
@@ -185,7 +186,7 @@ cg_xr_update(long long n, const double* __restrict__ sc, const double* __restric
   const double alpha = sc[1] / sc[0];
   double s = 0.0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    x[i] = fma(alpha, p[i], x[i]);
+    if (WITHX) x[i] = fma(alpha, p[i], x[i]);
     const double rv = fma(-alpha, q[i], r[i]);
     r[i] = rv;
     if (r32) {
@@ -208,13 +209,19 @@ __global__ void cg_x_update(long long n, const double* __restrict__ sc, const do
 }
 
 // CG: p = z + beta p with beta = -sc[3]/sc[0]   (iterative.py:394-395); z FP64 or the FP32 output of a
-// reduced-precision preconditioner (widened exactly)
-template <typename Z>
+// reduced-precision preconditioner (widened exactly).  x (optional): first the x update the iteration's
+// cg_xr_update<false> left out, x += alpha p with alpha = sc[1]/sc[0] and the old p - the same
+// expression, so the iterates are bitwise those of the undeferred update, for one read of p fewer
+template <typename Z, bool WITHX>
 __global__ void cg_p_update(long long n, const double* __restrict__ sc, const Z* __restrict__ z,
-                            double* __restrict__ p) {
+                            double* __restrict__ p, double* __restrict__ x) {
   const double beta = -sc[3] / sc[0];
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = fma(beta, p[i], (double)z[i]);
+  if (i < n) {
+    const double pv = p[i];
+    if (WITHX) x[i] = fma(sc[1] / sc[0], pv, x[i]);
+    p[i] = fma(beta, pv, (double)z[i]);
+  }
 }
 
 // p = z / sqrt(sc[k])  (orth(z, normalize=True) for one column, iterative.py:286-290)
@@ -261,7 +268,10 @@ int launch_axpby(long long n, double a, const double* x, double b, double* y, hi
 int launch_cg_xr_update(long long n, const double* sc, const double* p, const double* q, double* x, double* r,
                         double* work, double* out_rr, hipStream_t s, float* r32, const float* dinv32, float omega32,
                         float* u1) {
-  cg_xr_update<<<kReduceBlocks, kReduceThreads, 0, s>>>(n, sc, p, q, x, r, work, r32, dinv32, omega32, u1);
+  if (x)
+    cg_xr_update<true><<<kReduceBlocks, kReduceThreads, 0, s>>>(n, sc, p, q, x, r, work, r32, dinv32, omega32, u1);
+  else
+    cg_xr_update<false><<<kReduceBlocks, kReduceThreads, 0, s>>>(n, sc, p, q, x, r, work, r32, dinv32, omega32, u1);
   PMT_TRY(check_launch("cg_xr_update"));
   dot_stage2<1><<<1, kReduceThreads, 0, s>>>(work, out_rr);
   return check_launch("dot_stage2");
@@ -290,13 +300,19 @@ int launch_cg_x_update(long long n, const double* sc, const double* p, double* x
   return check_launch("cg_x_update");
 }
 
-int launch_cg_p_update(long long n, const double* sc, const double* z, double* p, hipStream_t s) {
-  cg_p_update<double><<<blocks_for(n, 256), 256, 0, s>>>(n, sc, z, p);
+int launch_cg_p_update(long long n, const double* sc, const double* z, double* p, hipStream_t s, double* x) {
+  if (x)
+    cg_p_update<double, true><<<blocks_for(n, 256), 256, 0, s>>>(n, sc, z, p, x);
+  else
+    cg_p_update<double, false><<<blocks_for(n, 256), 256, 0, s>>>(n, sc, z, p, x);
   return check_launch("cg_p_update");
 }
 
-int launch_cg_p_update(long long n, const double* sc, const float* z, double* p, hipStream_t s) {
-  cg_p_update<float><<<blocks_for(n, 256), 256, 0, s>>>(n, sc, z, p);
+int launch_cg_p_update(long long n, const double* sc, const float* z, double* p, hipStream_t s, double* x) {
+  if (x)
+    cg_p_update<float, true><<<blocks_for(n, 256), 256, 0, s>>>(n, sc, z, p, x);
+  else
+    cg_p_update<float, false><<<blocks_for(n, 256), 256, 0, s>>>(n, sc, z, p, x);
   return check_launch("cg_p_update<float>");
 }
 
