@@ -309,7 +309,13 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
                                       // DIAG: then the x-summed element factors
   constexpr bool LDS_TAB = OP == 1 && sizeof(T) == 8;
   // carry the transformed lower plane to the next layer in registers (else rebuild it from LDS)
-  constexpr bool CARRY = kReflCarry64 || sizeof(T) == 4;
+  constexpr bool CARRY = OP == 0 && (kReflCarry64 || sizeof(T) == 4);
+  // element-factor ring depth: 3 (loaded with the node planes, three layers ahead) on the fine grid; 1
+  // on the first coarse level, whose 8 factors per element would otherwise hold 48 (FP64) or 24 (FP32)
+  // registers (with the LDS rebuild of the lower plane: 2 instead of 1 wave per SIMD in FP64, 5
+  // instead of 4 in FP32); loaded one layer ahead, before the node prefetch, so that waiting for them
+  // does not wait for the later node planes
+  constexpr int XR = OP == 0 ? 3 : 1;
   __shared__ T sG0[LDS_TAB ? 576 : 1];  // OP 1, FP64: the 24 x 24 table (read as broadcasts)
 
   const int work = (int)xcd_block();
@@ -393,12 +399,12 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
   // three.  The loop is unrolled by three so every slot is a fixed register set: no copies, and no
   // wait falls on a memory operation issued within the last layer.  The update operands (b, D^-1,
   // previous iterate) of plane k are loaded at the start of layer k and used at its end.
-  T raw[3][3], xs[3][XN];
+  T raw[3][3], xs[XR][XN];
   unsigned mk[3];
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
     load_node(kbeg - 1 + r, raw[r], mk[r]);
-    load_x(kbeg - 1 + r, xs[r]);
+    if (XR == 3 || r == 0) load_x(kbeg - 1 + r, xs[r % XR]);
   }
   // corners of a node plane after the x and y transform stages, from the own masked values c: the x
   // stage with the +x neighbour (DPP), its sum and difference to LDS plane buffer pl, and after the
@@ -450,7 +456,7 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
   }
 
   auto layer = [&](int k, auto slot) {
-    constexpr int SA = decltype(slot)::value, SB = (SA + 1) % 3;
+    constexpr int SA = decltype(slot)::value, SB = (SA + 1) % 3, SX = SA % XR;
     T bo[3], dv[3], po[3];
     load_out(k, bo, dv, po);
     T c1[3];
@@ -483,11 +489,11 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
     // the table entries are re-read through the scalar cache each layer (zero_stride is 0; the
     // compiler cannot hoist the loads, which would exceed the scalar register file)
     if constexpr (OP == 0 && PAT == 2) {
-      refl_element_cube<T>(cube, xs[SA][0], v);
+      refl_element_cube<T>(cube, xs[SX][0], v);
     } else if constexpr (OP == 0) {
-      refl_element<PAT != 0>(tab + zero_stride * k, xs[SA][0], v);
+      refl_element<PAT != 0>(tab + zero_stride * k, xs[SX][0], v);
     } else {
-      coarse1_element<PAT>((LDS_TAB ? sG0 : tab) + zero_stride * k, xs[SA], v);  // (re-read per layer)
+      coarse1_element<PAT>((LDS_TAB ? sG0 : tab) + zero_stride * k, xs[SX], v);  // (re-read per layer)
     }
 
     // Inverse transform interleaved with the node sums, one axis at a time (all stages are linear).
@@ -514,7 +520,7 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
       }
     T xsum = (T)0;  // DIAG: element factors of layer k around the node, x then y
     if constexpr (DIAG) {
-      xsum = xs[SA][0] + from_lower<RX>(xs[SA][0]);
+      xsum = xs[SX][0] + from_lower<RX>(xs[SX][0]);
       so[6][tid] = xsum;
     }
     __syncthreads();
@@ -568,8 +574,9 @@ fine_refl_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, con
     }
 
     __builtin_amdgcn_sched_barrier(0);  // keep the prefetch behind the stores
+    if (XR == 1) load_x(k + 1, xs[0]);
     load_node(k + 3, raw[SA], mk[SA]);
-    load_x(k + 3, xs[SA]);
+    if (XR == 3) load_x(k + 3, xs[SX]);
   };
 
   for (int k = kbeg - 1;; k += 3) {
