@@ -557,6 +557,20 @@ __global__ void __launch_bounds__(256) gemv_kernel(long long n, const T* __restr
   if (lane == 0) y[row] = s;
 }
 
+// y = (float)(A (double)x): the FP64 coarsest inverse applied to an FP32 right-hand side, FP64 sums (the
+// widening, gemv_kernel<double> and the rounding of separate launches, in one)
+__global__ void __launch_bounds__(256) gemv_f32io_kernel(long long n, const double* __restrict__ A,
+                                                         const float* __restrict__ x, float* __restrict__ y) {
+  const long long row = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const double* a = A + row * n;
+  double s = 0;
+  for (long long c = lane; c < n; c += 64) s = fma(a[c], (double)x[c], s);
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) y[row] = (float)s;
+}
+
 // y = (float) x
 __global__ void to_float_kernel(long long n, const double* __restrict__ x, float* __restrict__ y) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -677,6 +691,11 @@ int launch_gemv(long long n, const T* A, const T* x, T* y, hipStream_t s) {
 }
 template int launch_gemv<double>(long long, const double*, const double*, double*, hipStream_t);
 template int launch_gemv<float>(long long, const float*, const float*, float*, hipStream_t);
+
+int launch_gemv_f32io(long long n, const double* A, const float* x, float* y, hipStream_t s) {
+  gemv_f32io_kernel<<<blocks_for(n, 4), 256, 0, s>>>(n, A, x, y);
+  return check_launch("gemv_f32io_kernel");
+}
 
 int launch_to_float(long long n, const double* x, float* y, hipStream_t s) {
   if (n <= 0) return PMT_OK;

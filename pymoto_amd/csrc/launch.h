@@ -92,7 +92,9 @@ int launch_stencil_half(int mode, const GridDims& g, const T* H, const T* u, con
 template <typename T>
 int launch_jacobi0(long long n, const T* dinv, const T* b, double omega, T* u, hipStream_t s);
 template <typename T>
-int launch_restrict(const GridDims& gf, const GridDims& gc, const T* rf, T* rc, hipStream_t s);
+// u0 (optional): also u0 = w0 D^-1 rc (dinv of the coarse level), its first smoothing sweep from zero
+int launch_restrict(const GridDims& gf, const GridDims& gc, const T* rf, T* rc, hipStream_t s, const T* dinv = nullptr,
+                    double w0 = 0.0, T* u0 = nullptr);
 template <typename T>
 int launch_prolong(const GridDims& gf, const GridDims& gc, const T* uc, T* uf, hipStream_t s);
 
@@ -110,7 +112,8 @@ int launch_axpby(long long n, double a, const double* x, double b, double* y, hi
 // x = nullptr: r only (the x update deferred to launch_cg_p_update)
 int launch_cg_xr_update(long long n, const double* sc, const double* p, const double* q, double* x, double* r,
                         double* work, double* out_rr, hipStream_t s, float* r32 = nullptr,
-                        const float* dinv32 = nullptr, float omega32 = 0.0f, float* u1 = nullptr);
+                        const float* dinv32 = nullptr, float omega32 = 0.0f, float* u1 = nullptr,
+                        double* host = nullptr, double seq = 0.0);  // host: also publish r.r (launch_publish)
 int launch_cg_x_update(long long n, const double* sc, const double* p, double* x, hipStream_t s);
 // x (optional): also x += (sc[1]/sc[0]) p_old first, the update a cg_xr_update with x = nullptr deferred
 int launch_cg_p_update(long long n, const double* sc, const double* z, double* p, hipStream_t s, double* x = nullptr);
@@ -155,6 +158,7 @@ int launch_dense_from_stencil(const GridDims& g, const double* S, double* A, hip
 int gj_invert(long long n, double* A, double* wsP, double* wsC, hipStream_t s);
 template <typename T>
 int launch_gemv(long long n, const T* A, const T* x, T* y, hipStream_t s);
+int launch_gemv_f32io(long long n, const double* A, const float* x, float* y, hipStream_t s);
 int launch_to_float(long long n, const double* x, float* y, hipStream_t s);
 
 }  // namespace pmt

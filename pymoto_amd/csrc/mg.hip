@@ -94,10 +94,7 @@ int coarse_solve<double>(pmt_mg* mg, const double* b, double* out, hipStream_t s
 template <>
 int coarse_solve<float>(pmt_mg* mg, const float* b, float* out, hipStream_t s) {
   PMT_TRY(mg_join(mg, s));
-  Level& C = mg->L[mg->nlev - 1];
-  PMT_TRY(launch_to_double(mg->nc, b, C.d.b, s));
-  PMT_TRY(launch_gemv<double>(mg->nc, mg->Ainv, C.d.b, C.d.u, s));
-  return launch_to_float(mg->nc, C.d.u, out, s);
+  return launch_gemv_f32io(mg->nc, mg->Ainv, b, out, s);  // FP64 inverse and sums, FP32 in and out
 }
 
 // level operator in `mode`; Jacobi / Chebyshev sweeps use `omega` (and `up`, `beta`)
@@ -194,8 +191,12 @@ static int vcycle(pmt_mg* mg, int l, const T* b, T* out, hipStream_t s, bool fir
   T* oth = out;
   PMT_TRY(smooth<T>(mg, l, b, cur, oth, true, s, first_done));
   PMT_TRY(level_apply<T>(mg, l, MODE_RESIDUAL, cur, b, B.r, s));
-  PMT_TRY(launch_restrict<T>(L.g, C.g, B.r, CB.b, s));
-  PMT_TRY(vcycle<T>(mg, l + 1, CB.b, CB.u, s));
+  // a smoothed coarse level starts from zero: the restriction also writes its first sweep into the
+  // buffer that sweep would write (CB.t), with the factor the smoother would use
+  const bool fuse = l + 1 < mg->nlev - 1;
+  const double w0 = mg->smoother == 0 ? C.omega : 1.0 / (0.5 * (C.cheb_hi + C.cheb_lo));
+  PMT_TRY(launch_restrict<T>(L.g, C.g, B.r, CB.b, s, fuse ? CB.dinv : nullptr, w0, fuse ? CB.t : nullptr));
+  PMT_TRY(vcycle<T>(mg, l + 1, CB.b, CB.u, s, fuse));
   PMT_TRY(launch_prolong<T>(L.g, C.g, CB.u, cur, s));
   PMT_TRY(smooth<T>(mg, l, b, cur, oth, false, s));
   if (cur != out) return arg_error("vcycle: internal buffer parity error");
@@ -231,11 +232,17 @@ static int read_scalar(pmt_mg* mg, int k, double* v, hipStream_t s) {
 // value that never arrives cannot hang the solve.
 static std::atomic<long long> g_publish_seq{0};
 
-static int publish_scalar(pmt_mg* mg, int k, hipStream_t s) {
-  // sequence numbers are unique in the process: pinned memory freed by another handle (with its last
-  // sequence number still in it) and handed out again here can never look like this publish
+// the sequence number of the next publish (kept for wait_published); unique in the process: pinned memory
+// freed by another handle (with its last sequence number still in it) and handed out again here can
+// never look like this publish
+static double next_publish_seq(pmt_mg* mg) {
   mg->pub_seq = (double)g_publish_seq.fetch_add(1, std::memory_order_relaxed) + 1.0;
-  return launch_publish(mg->sc + k, mg->pub_seq, mg->host_rr, s);
+  return mg->pub_seq;
+}
+
+static int publish_scalar(pmt_mg* mg, int k, hipStream_t s) {
+  const double seq = next_publish_seq(mg);
+  return launch_publish(mg->sc + k, seq, mg->host_rr, s);
 }
 
 static int wait_published(pmt_mg* mg, int k, double* v, hipStream_t s) {
@@ -857,16 +864,20 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
     const double first_factor =
         mg->smoother == 0 ? F.omega : 1.0 / (0.5 * (F.cheb_hi + F.cheb_lo));  // Chebyshev: 1 / theta (fine)
     bool r32_ready = false;
+    bool published = false;  // r.r already on its way to the host (with the r update's reduction)
     if (i % restart == 0) {
       PMT_TRY(launch_cg_x_update(n, sc, mg->p, x, s));
       PMT_TRY(level_apply<double>(mg, 0, MODE_RESIDUAL, x, b, mg->r, s));
       PMT_TRY(launch_dot(n, mg->r, mg->r, sc + 2, mg->work, s));
     } else {
       // x += alpha p is deferred to the p update, which reads p anyway (x_pending until it runs)
+      const double seq = next_publish_seq(mg);
       PMT_TRY(launch_cg_xr_update(n, sc, mg->p, mg->q, nullptr, mg->r, mg->work, sc + 2, s, mixed ? F.f.b : nullptr,
-                                  first ? F.f.dinv : nullptr, (float)first_factor, first ? F.f.t : nullptr));
+                                  first ? F.f.dinv : nullptr, (float)first_factor, first ? F.f.t : nullptr,
+                                  mg->host_rr, seq));
       r32_ready = mixed;
       x_pending = true;
+      published = true;
     }
     // z = M r, sc[3] = q.z, p = z + beta p: reads r and q, writes only z, sc[3] and p (no result)
     auto next_direction = [&]() -> int {
@@ -890,7 +901,7 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
       x_pending = false;
       return PMT_OK;
     };
-    PMT_TRY(publish_scalar(mg, 2, s));
+    if (!published) PMT_TRY(publish_scalar(mg, 2, s));
     // when the residual's rate of decrease says this iteration will not stop, the next direction is
     // enqueued before the host waits for r.r, so the GPU stays busy meanwhile; if it does stop after all,
     // that work is discarded (the iterates and the iteration count are the same either way)

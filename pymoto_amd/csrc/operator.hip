@@ -613,9 +613,12 @@ __global__ void jacobi0_kernel(long long n, const T* __restrict__ dinv, const T*
   if (q < n) u[q] = omega * dinv[q] * b[q];
 }
 
-// rc = R^T rf : coarse node gathers the 3^D fine nodes around its fine image (weights 1, 1/2, 1/4, 1/8)
+// rc = R^T rf : coarse node gathers the 3^D fine nodes around its fine image (weights 1, 1/2, 1/4, 1/8).
+// u0 (optional): also the coarse level's first smoothing sweep from zero, u0 = w0 D^-1 rc (the
+// expression of jacobi0_kernel, so the bits are the same as a separate launch)
 template <int DIM, typename T>
-__global__ void restrict_kernel(GridDims gf, GridDims gc, const T* __restrict__ rf, T* __restrict__ rc) {
+__global__ void restrict_kernel(GridDims gf, GridDims gc, const T* __restrict__ rf, T* __restrict__ rc,
+                                const T* __restrict__ dinv, T w0, T* __restrict__ u0) {
   const long long N = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (N >= gc.nnodes) return;
   const int I = (int)(N % gc.nx);
@@ -641,6 +644,10 @@ __global__ void restrict_kernel(GridDims gf, GridDims gc, const T* __restrict__ 
       }
 #pragma unroll
   for (int d = 0; d < DIM; ++d) rc[N * DIM + d] = acc[d];
+  if (u0) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) u0[N * DIM + d] = w0 * dinv[N * DIM + d] * acc[d];
+  }
 }
 
 // uf += R uc : fine node interpolates from the 1, 2, 4 or 8 coarse nodes around it.  Branch free: every
@@ -999,12 +1006,13 @@ int launch_jacobi0(long long n, const T* dinv, const T* b, double omega, T* u, h
 }
 
 template <typename T>
-int launch_restrict(const GridDims& gf, const GridDims& gc, const T* rf, T* rc, hipStream_t s) {
+int launch_restrict(const GridDims& gf, const GridDims& gc, const T* rf, T* rc, hipStream_t s, const T* dinv,
+                    double w0, T* u0) {
   dim3 grid(blocks_for(gc.nnodes, 256));
   if (gf.dim == 3)
-    restrict_kernel<3, T><<<grid, 256, 0, s>>>(gf, gc, rf, rc);
+    restrict_kernel<3, T><<<grid, 256, 0, s>>>(gf, gc, rf, rc, dinv, (T)w0, u0);
   else
-    restrict_kernel<2, T><<<grid, 256, 0, s>>>(gf, gc, rf, rc);
+    restrict_kernel<2, T><<<grid, 256, 0, s>>>(gf, gc, rf, rc, dinv, (T)w0, u0);
   return check_launch("restrict_kernel");
 }
 
@@ -1028,7 +1036,7 @@ int launch_prolong(const GridDims& gf, const GridDims& gc, const T* uc, T* uf, h
   template int launch_stencil_half<T>(int, const GridDims&, const T*, const T*, const T*, const T*, double, T*,    \
                                       hipStream_t, const T*, double);                                              \
   template int launch_jacobi0<T>(long long, const T*, const T*, double, T*, hipStream_t);                          \
-  template int launch_restrict<T>(const GridDims&, const GridDims&, const T*, T*, hipStream_t);                    \
+  template int launch_restrict<T>(const GridDims&, const GridDims&, const T*, T*, hipStream_t, const T*, double, T*); \
   template int launch_prolong<T>(const GridDims&, const GridDims&, const T*, T*, hipStream_t);
 PMT_INSTANTIATE_LEVEL(double)
 PMT_INSTANTIATE_LEVEL(float)

@@ -137,6 +137,24 @@ __global__ void __launch_bounds__(kReduceThreads) dot_stage2(const double* __res
   }
 }
 
+// dot_stage2<1> that also hands the result to the host through pinned memory (the protocol of
+// publish_kernel: value, system-scope fence, sequence number)
+__global__ void __launch_bounds__(kReduceThreads) dot_stage2_publish(const double* __restrict__ partial,
+                                                                     double* __restrict__ out, double seq,
+                                                                     double* host) {
+  double s = 0.0;
+  for (int b = threadIdx.x; b < kReduceBlocks; b += blockDim.x) s += partial[b];
+  const double r = block_sum(s);
+  if (threadIdx.x == 0) {
+    out[0] = r;
+    volatile double* h = host;
+    h[0] = r;
+    __threadfence_system();
+    h[1] = seq;
+    __threadfence_system();
+  }
+}
+
 __global__ void oc_update_kernel(long long n, const double* __restrict__ x, const double* __restrict__ dg,
                                  double xmin, double xmax, double move, double lmid, double* __restrict__ xnew,
                                  double* __restrict__ partial) {
@@ -267,12 +285,16 @@ int launch_axpby(long long n, double a, const double* x, double b, double* y, hi
 
 int launch_cg_xr_update(long long n, const double* sc, const double* p, const double* q, double* x, double* r,
                         double* work, double* out_rr, hipStream_t s, float* r32, const float* dinv32, float omega32,
-                        float* u1) {
+                        float* u1, double* host, double seq) {
   if (x)
     cg_xr_update<true><<<kReduceBlocks, kReduceThreads, 0, s>>>(n, sc, p, q, x, r, work, r32, dinv32, omega32, u1);
   else
     cg_xr_update<false><<<kReduceBlocks, kReduceThreads, 0, s>>>(n, sc, p, q, x, r, work, r32, dinv32, omega32, u1);
   PMT_TRY(check_launch("cg_xr_update"));
+  if (host) {
+    dot_stage2_publish<<<1, kReduceThreads, 0, s>>>(work, out_rr, seq, host);
+    return check_launch("dot_stage2_publish");
+  }
   dot_stage2<1><<<1, kReduceThreads, 0, s>>>(work, out_rr);
   return check_launch("dot_stage2");
 }
