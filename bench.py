@@ -222,12 +222,18 @@ def design_iteration(net, x):
     return net["sc"].state
 
 
+ROTATE = 4  # buffer sets cycled by the kernel timings: 4 x (u, y, densities) of 12.8M dofs = 955 MB in FP64
+
+
 def kernel_roofline(pm, torch, nel, reps=20):
     """Average duration of the fine-grid K.u kernel (FP64, and the FP32 product of the V-cycle) from HIP
     events on the stream it is launched on, and the HBM roofline of its compulsory bytes: u read and
     y written (3 dofs per node) plus one density per element (the Dirichlet mask byte per node not
     counted).  The element product runs in the reflection basis (~0.24 kflop per element), so the
-    kernel's roof is HBM, not the FP64 FMA rate."""
+    kernel's roof is HBM, not the FP64 FMA rate.  Consecutive launches cycle through ROTATE sets of
+    (densities, u, y): one set (239 MB in FP64) fits the 256 MB last-level cache, so repeated launches
+    on one set would read it from there instead of HBM, as a K.u inside the solve (whose iteration
+    touches ~2.4 GB) never does."""
     import json
     from pymoto_amd import _lib, problems
     d = pm.VoxelDomain(*nel)
@@ -235,30 +241,34 @@ def kernel_roofline(pm, torch, nel, reps=20):
     x = torch.as_tensor(np.random.default_rng(3).uniform(1e-3, 1.0, d.nel), device="cuda")
     K = pm.AssembleStiffness(d, bc=bc)(x)
     g = K.grid
-    u = torch.randn(g.n, dtype=torch.float64, device="cuda")
-    y = torch.empty_like(u)
     lib = _lib.lib()
     stream = torch.cuda.current_stream()
-    args = (*g.nel3, _lib.ptr(g.K0_dev), g.k0_pattern, _lib.ptr(K.xe), _lib.ptr(g.bcmask_dev), g.bcdiag, _lib.ptr(u),
-            _lib.ptr(y), _lib.stream())
-    x32, u32 = K.xe.float(), u.float()
-    y32 = torch.empty_like(u32)
-    args32 = (*g.nel3, _lib.ptr(g.K0_dev), g.k0_pattern, _lib.ptr(x32), _lib.ptr(g.bcmask_dev), g.bcdiag,
-              _lib.ptr(u32), _lib.ptr(y32), _lib.stream())
+    sets64, sets32, keep = [], [], []
+    for r in range(ROTATE):
+        xr = (K.xe * (1.0 - 0.01 * r)).contiguous()
+        u = torch.randn(g.n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(u)
+        x32, u32 = xr.float(), u.float()
+        y32 = torch.empty_like(u32)
+        keep += [xr, u, y, x32, u32, y32]
+        sets64.append((*g.nel3, _lib.ptr(g.K0_dev), g.k0_pattern, _lib.ptr(xr), _lib.ptr(g.bcmask_dev), g.bcdiag,
+                       _lib.ptr(u), _lib.ptr(y), _lib.stream()))
+        sets32.append((*g.nel3, _lib.ptr(g.K0_dev), g.k0_pattern, _lib.ptr(x32), _lib.ptr(g.bcmask_dev), g.bcdiag,
+                       _lib.ptr(u32), _lib.ptr(y32), _lib.stream()))
 
-    def timed(fn, a):
-        for _ in range(3):
-            _lib.check(fn(*a))
+    def timed(fn, sets):
+        for q in range(len(sets)):
+            _lib.check(fn(*sets[q]))
         s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         s.record(stream)
-        for _ in range(reps):
-            _lib.check(fn(*a))
+        for q in range(reps):
+            _lib.check(fn(*sets[q % len(sets)]))
         e.record(stream)
         torch.cuda.synchronize()
         return s.elapsed_time(e) / reps * 1e-3
 
-    t64 = timed(lib.pmt_stiffness_apply, args)
-    t32 = timed(lib.pmt_stiffness_apply_f32, args32)
+    t64 = timed(lib.pmt_stiffness_apply, sets64)
+    t32 = timed(lib.pmt_stiffness_apply_f32, sets32)
     from pymoto_amd import tune
     tuning = {k: int(v) for k, v in tune.get_tuning().items()}
     b64 = 2.0 * g.n * 8 + d.nel * 8
@@ -266,7 +276,7 @@ def kernel_roofline(pm, torch, nel, reps=20):
     out = dict(kernel="fine_refl_kernel<16, apply, double> (K.u, FP64, reflection basis)", seconds=t64, bytes=b64,
                gbs=b64 / t64 / 1e9, fp32=dict(kernel="fine_refl_kernel<16, apply, float>", us=t32 * 1e6,
                                                  bytes=b32, gbs=b32 / t32 / 1e9, frac=b32 / t32 / 1e9 / PEAK_HBM_GBS),
-               tuning=tuning, traffic=None)
+               tuning=tuning, traffic=None, rotating_sets=ROTATE)
     pmc = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "fine_refl_fp64_traffic.json")
     if os.path.exists(pmc):
         with open(pmc) as fh:
@@ -331,10 +341,13 @@ def ku_driver(pm, torch, nel, reps=5, which="all"):
     x = torch.as_tensor(np.random.default_rng(0).uniform(1e-3, 1.0, d.nel), device="cuda")
     K = pm.AssembleStiffness(d, bc=bc)(x)
     u = torch.randn(3 * d.nnodes, dtype=torch.float64, device="cuda")
-    for _ in range(reps):
-        K @ u
+    # the K.u launches and the copies cycle through ROTATE buffer sets (none stays in the last-level cache)
+    Ks = [K] + [pm.AssembleStiffness(d, bc=bc)(x * (1.0 - 0.01 * r)) for r in range(1, ROTATE)]
+    us = [u] + [torch.randn_like(u) for _ in range(1, ROTATE)]
+    for q in range(reps):
+        Ks[q % ROTATE] @ us[q % ROTATE]
     torch.cuda.synchronize()
-    print(f"fine K.u FP64 x{reps}", flush=True)
+    print(f"fine K.u FP64 x{reps} ({ROTATE} rotating buffer sets)", flush=True)
     if which == "all":
         # V-cycles of the headline hierarchy (2 damped-Jacobi sweeps per level, omega 0.5): the fine sweeps
         # (fine_refl_kernel in Jacobi mode) in FP64 and FP32
@@ -391,11 +404,12 @@ def ku_driver(pm, torch, nel, reps=5, which="all"):
                 net.forward(inp)
             torch.cuda.synchronize()
             print(f"learned network forward x{reps}", flush=True)
-    c = torch.empty_like(u)
-    for _ in range(reps):
-        c.copy_(u)
+    cs = [torch.empty_like(u) for _ in range(ROTATE)]
+    for q in range(reps):
+        cs[q % ROTATE].copy_(us[q % ROTATE])
     torch.cuda.synchronize()
-    print(f"copy x{reps}: vector of {u.numel()} FP64 = {u.numel() * 8 / 1024:.1f} KiB", flush=True)
+    print(f"copy x{reps}: vector of {u.numel()} FP64 = {u.numel() * 8 / 1024:.1f} KiB ({ROTATE} rotating pairs)",
+          flush=True)
 
 
 def batched_workload(pm, torch, nel=(128, 64, 64), beta=8.0, etas=(0.3, 0.5, 0.7), nloads=4, nuneven=16, seed=7,
@@ -674,7 +688,8 @@ def main():
                             "frac": roof["gbs"] / PEAK_HBM_GBS, "traffic": roof["traffic"], "kernel": roof["kernel"],
                             "kernel_us": roof["seconds"] * 1e6, "algorithmic_bytes": roof["bytes"],
                             "fp32_product": roof["fp32"], "tuning": roof["tuning"],
-                            "note": "compulsory bytes 2*ndof*8 + nel*8 per K.u; HIP events on the launch stream"}
+                            "note": "compulsory bytes 2*ndof*8 + nel*8 per K.u; HIP events on the launch stream; "
+                                    f"launches cycle through {ROTATE} buffer sets (none cache-resident)"}
     if not args.no_cpu:
         cpu_nel = tuple(args.cpu_nel)
         cpu_levels = pm.solvers.auto_levels(pm.VoxelDomain(*cpu_nel))

@@ -109,7 +109,11 @@ int launch_dot(long long n, const double* x, const float* y, double* out, double
 int launch_dot2(long long n, const double* a0, const double* b0, const double* a1, const double* b1, double* out,
                 double* work, hipStream_t s);
 int launch_axpby(long long n, double a, const double* x, double b, double* y, hipStream_t s);
-// x = nullptr: r only (the x update deferred to launch_cg_p_update)
+// x = nullptr: r only (the x update deferred to launch_cg_p_update).  work: cg_xr_parts(n) partial sums
+long long cg_xr_parts(long long n);
+// x.y in one flat pass, work: cg_xr_parts(n) partials (the CG loop's dots; bits differ from launch_dot's)
+template <typename Y>
+int launch_dot_flat(long long n, const double* x, const Y* y, double* out, double* work, hipStream_t s);
 int launch_cg_xr_update(long long n, const double* sc, const double* p, const double* q, double* x, double* r,
                         double* work, double* out_rr, hipStream_t s, float* r32 = nullptr,
                         const float* dinv32 = nullptr, float omega32 = 0.0f, float* u1 = nullptr,

@@ -619,6 +619,7 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
   ALLOC(mg->r, n0);
   ALLOC(mg->sc, 16);
   ALLOC(mg->work, 2 * kReduceBlocks);
+  ALLOC(mg->rwork, cg_xr_parts(n0));
   ALLOC(mg->cs_buf, 4 + 4 * kReduceBlocks);
   if (hex && (mg->k0pat & PMT_K0_REFLECTION)) {
     mg->dotws_parts = fine_apply_dots_parts(mg->L[0].g);
@@ -872,7 +873,7 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
     } else {
       // x += alpha p is deferred to the p update, which reads p anyway (x_pending until it runs)
       const double seq = next_publish_seq(mg);
-      PMT_TRY(launch_cg_xr_update(n, sc, mg->p, mg->q, nullptr, mg->r, mg->work, sc + 2, s, mixed ? F.f.b : nullptr,
+      PMT_TRY(launch_cg_xr_update(n, sc, mg->p, mg->q, nullptr, mg->r, mg->rwork, sc + 2, s, mixed ? F.f.b : nullptr,
                                   first ? F.f.dinv : nullptr, (float)first_factor, first ? F.f.t : nullptr,
                                   mg->host_rr, seq));
       r32_ready = mixed;
@@ -884,7 +885,7 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
       if (mixed) {
         PMT_TRY(precondition_f32(mg, mg->r, r32_ready, s, r32_ready && first));
         if (mg->cs_m == 0) {
-          PMT_TRY(launch_dot(n, mg->q, mg->z32, sc + 3, mg->work, s));
+          PMT_TRY(launch_dot_flat(n, mg->q, mg->z32, sc + 3, mg->rwork, s));
           PMT_TRY(launch_cg_p_update(n, sc, mg->z32, mg->p, s, x_pending ? x : nullptr));
           x_pending = false;
           return PMT_OK;
@@ -896,7 +897,7 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
         PMT_TRY(precondition(mg, mg->r, mg->z, s));
       }
       PMT_TRY(launch_coarse_correct(n, mg->cs_m, mg->cs_W, mg->cs_AW, mg->cs_Einv, mg->cs_buf, mg->z, s));
-      PMT_TRY(launch_dot(n, mg->q, mg->z, sc + 3, mg->work, s));
+      PMT_TRY(launch_dot_flat(n, mg->q, mg->z, sc + 3, mg->rwork, s));
       PMT_TRY(launch_cg_p_update(n, sc, mg->z, mg->p, s, x_pending ? x : nullptr));
       x_pending = false;
       return PMT_OK;

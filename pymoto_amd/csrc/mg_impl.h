@@ -102,6 +102,7 @@ struct pmt_mg {
   double* r = nullptr;
   double* sc = nullptr;    // device scalars
   double* work = nullptr;  // reduction partials
+  double* rwork = nullptr;  // per-block partials of the CG r update (cg_xr_parts(ndof) entries)
   double* dotws = nullptr;  // partial sums of the fused fine K.u + dot products (2 x dotws_parts)
   long long dotws_parts = 0;
   double* host_sc = nullptr;

@@ -137,21 +137,37 @@ __global__ void __launch_bounds__(kReduceThreads) dot_stage2(const double* __res
   }
 }
 
-// dot_stage2<1> that also hands the result to the host through pinned memory (the protocol of
-// publish_kernel: value, system-scope fence, sequence number)
-__global__ void __launch_bounds__(kReduceThreads) dot_stage2_publish(const double* __restrict__ partial,
-                                                                     double* __restrict__ out, double seq,
-                                                                     double* host) {
+// out[0] = sum of nparts partials (fixed order, one block of 1024 threads, each thread's strided loads
+// issued eight at a time); host (optional): also hands the sum to the host through pinned memory (the
+// protocol of publish_kernel: value, system-scope fence, sequence number)
+constexpr int kStage2Threads = 1024;
+__global__ void __launch_bounds__(kStage2Threads) parts_stage2(const double* __restrict__ partial, long long nparts,
+                                                               double* __restrict__ out, double seq, double* host) {
   double s = 0.0;
-  for (int b = threadIdx.x; b < kReduceBlocks; b += blockDim.x) s += partial[b];
-  const double r = block_sum(s);
+  long long b = threadIdx.x;
+  for (; b + 7LL * kStage2Threads < nparts; b += 8LL * kStage2Threads) {
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = partial[b + (long long)k * kStage2Threads];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += v[k];
+  }
+  for (; b < nparts; b += kStage2Threads) s += partial[b];
+  __shared__ double sh[kStage2Threads / 64];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
   if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int w = 0; w < kStage2Threads / 64; ++w) r += sh[w];
     out[0] = r;
-    volatile double* h = host;
-    h[0] = r;
-    __threadfence_system();
-    h[1] = seq;
-    __threadfence_system();
+    if (host) {
+      volatile double* h = host;
+      h[0] = r;
+      __threadfence_system();
+      h[1] = seq;
+      __threadfence_system();
+    }
   }
 }
 
@@ -195,7 +211,15 @@ int launch_dot2(long long n, const double* a0, const double* b0, const double* a
 // CG: x += alpha p (WITHX; else the caller adds it later, in cg_p_update), r -= alpha q, partial ||r||^2;
 // alpha = sc[1]/sc[0] read on device; r32 (optional): the new r rounded to FP32, the input of a
 // reduced-precision preconditioner; u1 (optional, with r32): omega * D^-1 * r32, the preconditioner's
-// first sweep from zero (same FP32 expression as jacobi0_kernel)
+// first sweep from zero (same FP32 expression as jacobi0_kernel).  One pass over the vectors: a block
+// owns kXrPer * 256 consecutive entries (strided by 256 per step, so every step is coalesced and the
+// steps' loads are independent) and writes its sum of r^2 to partial[block]; the fixed-order second
+// stage is parts_stage2.  At 12.8M dofs: 71 us (4 per thread) for the 461 MB of the mixed-precision
+// update, where the fixed 1024-block grid-stride loop of the dots took 84 us.
+constexpr int kXrPer = 4;
+
+long long cg_xr_parts(long long n) { return (n + 256LL * kXrPer - 1) / (256LL * kXrPer); }
+
 template <bool WITHX>
 __global__ void __launch_bounds__(kReduceThreads)
 cg_xr_update(long long n, const double* __restrict__ sc, const double* __restrict__ p, const double* __restrict__ q,
@@ -203,7 +227,8 @@ cg_xr_update(long long n, const double* __restrict__ sc, const double* __restric
              const float* __restrict__ dinv32, float omega32, float* __restrict__ u1) {
   const double alpha = sc[1] / sc[0];
   double s = 0.0;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+  const long long base = (long long)blockIdx.x * (256 * kXrPer) + threadIdx.x;
+  auto step = [&](long long i) {
     if (WITHX) x[i] = fma(alpha, p[i], x[i]);
     const double rv = fma(-alpha, q[i], r[i]);
     r[i] = rv;
@@ -213,8 +238,33 @@ cg_xr_update(long long n, const double* __restrict__ sc, const double* __restric
       if (u1) u1[i] = omega32 * dinv32[i] * rf;
     }
     s = fma(rv, rv, s);
+  };
+  if (base + 256LL * (kXrPer - 1) < n) {  // every block but the last: no bounds test between the steps
+#pragma unroll
+    for (int k = 0; k < kXrPer; ++k) step(base + 256LL * k);
+  } else {
+    for (int k = 0; k < kXrPer; ++k)
+      if (base + 256LL * k < n) step(base + 256LL * k);
   }
   double t = block_sum(s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// x.y in one flat pass (the cg_xr_update blocking: kXrPer * 256 entries per block, partial[block]); its
+// second stage is parts_stage2.  For the dots of the CG loop, whose bits need not match pmt_dot's.
+template <typename Y>
+__global__ void __launch_bounds__(kReduceThreads)
+dot_flat(long long n, const double* __restrict__ x, const Y* __restrict__ y, double* __restrict__ partial) {
+  double s = 0.0;
+  const long long base = (long long)blockIdx.x * (256 * kXrPer) + threadIdx.x;
+  if (base + 256LL * (kXrPer - 1) < n) {
+#pragma unroll
+    for (int k = 0; k < kXrPer; ++k) s = fma(x[base + 256LL * k], (double)y[base + 256LL * k], s);
+  } else {
+    for (int k = 0; k < kXrPer; ++k)
+      if (base + 256LL * k < n) s = fma(x[base + 256LL * k], (double)y[base + 256LL * k], s);
+  }
+  const double t = block_sum(s);
   if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
@@ -286,17 +336,15 @@ int launch_axpby(long long n, double a, const double* x, double b, double* y, hi
 int launch_cg_xr_update(long long n, const double* sc, const double* p, const double* q, double* x, double* r,
                         double* work, double* out_rr, hipStream_t s, float* r32, const float* dinv32, float omega32,
                         float* u1, double* host, double seq) {
+  if (n <= 0) return PMT_OK;
+  const long long parts = cg_xr_parts(n);
   if (x)
-    cg_xr_update<true><<<kReduceBlocks, kReduceThreads, 0, s>>>(n, sc, p, q, x, r, work, r32, dinv32, omega32, u1);
+    cg_xr_update<true><<<(unsigned)parts, kReduceThreads, 0, s>>>(n, sc, p, q, x, r, work, r32, dinv32, omega32, u1);
   else
-    cg_xr_update<false><<<kReduceBlocks, kReduceThreads, 0, s>>>(n, sc, p, q, x, r, work, r32, dinv32, omega32, u1);
+    cg_xr_update<false><<<(unsigned)parts, kReduceThreads, 0, s>>>(n, sc, p, q, x, r, work, r32, dinv32, omega32, u1);
   PMT_TRY(check_launch("cg_xr_update"));
-  if (host) {
-    dot_stage2_publish<<<1, kReduceThreads, 0, s>>>(work, out_rr, seq, host);
-    return check_launch("dot_stage2_publish");
-  }
-  dot_stage2<1><<<1, kReduceThreads, 0, s>>>(work, out_rr);
-  return check_launch("dot_stage2");
+  parts_stage2<<<1, kStage2Threads, 0, s>>>(work, parts, out_rr, seq, host);
+  return check_launch("parts_stage2");
 }
 
 // v -> host-visible memory (pinned): host[0] = value, then (after a system-scope fence) host[1] = seq, so a
@@ -316,6 +364,18 @@ int launch_publish(const double* v, double seq, double* host, hipStream_t s) {
   publish_kernel<<<1, 64, 0, s>>>(v, seq, host);
   return check_launch("publish_kernel");
 }
+
+template <typename Y>
+int launch_dot_flat(long long n, const double* x, const Y* y, double* out, double* work, hipStream_t s) {
+  if (n <= 0) return PMT_OK;
+  const long long parts = cg_xr_parts(n);
+  dot_flat<Y><<<(unsigned)parts, kReduceThreads, 0, s>>>(n, x, y, work);
+  PMT_TRY(check_launch("dot_flat"));
+  parts_stage2<<<1, kStage2Threads, 0, s>>>(work, parts, out, 0.0, nullptr);
+  return check_launch("parts_stage2");
+}
+template int launch_dot_flat<double>(long long, const double*, const double*, double*, double*, hipStream_t);
+template int launch_dot_flat<float>(long long, const double*, const float*, double*, double*, hipStream_t);
 
 int launch_cg_x_update(long long n, const double* sc, const double* p, double* x, hipStream_t s) {
   cg_x_update<<<blocks_for(n, 256), 256, 0, s>>>(n, sc, p, x);
