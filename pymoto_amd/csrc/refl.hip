@@ -1002,7 +1002,7 @@ int launch_sens_stream(const GridDims& g, const double* tab, const uint8_t* bcm,
 //  1. coarse1_touched_kernel: thread = (listed element, corner); it lists the corner's node if no
 //     flagged element holding the node has a smaller index (the list's order is irrelevant);
 //  2. coarse1_tstencil_kernel: thread = (touched node, coefficient), T in [243][cap] layout;
-//  3. coarse1_tapply_kernel: thread = (touched node, component), 81 multiply-adds.
+//  3. coarse1_tapply_kernel: four lanes per (touched node, component), 27 multiply-adds each.
 __global__ void __launch_bounds__(256)
 coarse1_touched_kernel(GridDims g, const int* __restrict__ list, const int* __restrict__ count,
                        const int* __restrict__ map, int* __restrict__ tnode, int* __restrict__ tcount) {
@@ -1052,29 +1052,40 @@ coarse1_tstencil_kernel(GridDims g, const int* __restrict__ tnode, int nt, long 
   T[(long long)c * cap + t] = v;
 }
 
+// Four lanes per (touched node, component): lane g < 3 sums the 9 neighbours of the z offset g - 1 (27
+// multiply-adds), the fourth idles, and the group (aligned in the wave) adds its three sums in order -
+// 4x the threads of one lane per output (latency: 81 dependent loads and multiply-adds per lane then).
 template <int MODE, typename T>
 __global__ void __launch_bounds__(256)
 coarse1_tapply_kernel(GridDims g, const int* __restrict__ tnode, int nt, long long cap, const double* __restrict__ TS,
                       const T* __restrict__ u, const T* __restrict__ dinv, double omega, T* __restrict__ y) {
-  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= 3LL * nt) return;
-  const long long t = q / 3;
+  const long long q4 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int grp = (int)(q4 & 3);
+  const long long q = q4 >> 2;
+  const bool live = q < 3LL * nt;
+  const long long t = live ? q / 3 : 0;
   const int d = (int)(q % 3);
   const long long n = tnode[t];
   const int I = (int)(n % g.nx);
   const long long tt = n / g.nx;
   const int J = (int)(tt % g.ny), K = (int)(tt / g.ny);
   double acc = 0.0;
+  if (live && grp < 3) {
 #pragma unroll
-  for (int o = 0; o < 27; ++o) {
-    // neighbours outside the grid have zero coefficients: read a clamped (valid) node instead
-    const int ni = min(max(I + o % 3 - 1, 0), g.nx - 1), nj = min(max(J + (o / 3) % 3 - 1, 0), g.ny - 1),
-              nk = min(max(K + o / 9 - 1, 0), g.nz - 1);
-    const long long m = ((long long)nk * g.ny + nj) * g.nx + ni;
+    for (int oo = 0; oo < 9; ++oo) {
+      const int o = grp * 9 + oo;
+      // neighbours outside the grid have zero coefficients: read a clamped (valid) node instead
+      const int ni = min(max(I + o % 3 - 1, 0), g.nx - 1), nj = min(max(J + (o / 3) % 3 - 1, 0), g.ny - 1),
+                nk = min(max(K + o / 9 - 1, 0), g.nz - 1);
+      const long long m = ((long long)nk * g.ny + nj) * g.nx + ni;
 #pragma unroll
-    for (int d2 = 0; d2 < 3; ++d2) acc = fma(TS[(long long)((o * 3 + d) * 3 + d2) * cap + t], (double)u[m * 3 + d2], acc);
+      for (int d2 = 0; d2 < 3; ++d2)
+        acc = fma(TS[(long long)((o * 3 + d) * 3 + d2) * cap + t], (double)u[m * 3 + d2], acc);
+    }
   }
-  double f = acc;
+  const double a1 = __shfl_down(acc, 1, 64), a2 = __shfl_down(acc, 2, 64);  // within the aligned group of 4
+  if (!live || grp != 0) return;
+  double f = (acc + a1) + a2;
   if (MODE == MODE_RESIDUAL)
     f = -f;
   else if (MODE == MODE_JACOBI || MODE == MODE_CHEB)
@@ -1110,7 +1121,7 @@ int launch_coarse1(int mode, const GridDims& g, int pattern, const double* G0ext
     launch_refl_shape<1, 16, 16, 0, T>(mode, g, 0, tab, Xh, nullptr, (T)0, u, b, dinv, om, y, s, up, be);
   PMT_TRY(check_launch("fine_refl_kernel<coarse level 1>"));
   if (nt <= 0) return PMT_OK;
-  const unsigned blocks = (unsigned)blocks_for(3LL * nt, 256);
+  const unsigned blocks = (unsigned)blocks_for(12LL * nt, 256);  // 4 lanes per (node, component)
   if (mode == MODE_APPLY)
     coarse1_tapply_kernel<MODE_APPLY, T><<<blocks, 256, 0, s>>>(g, tnode, nt, cap, TS, u, dinv, omega, y);
   else if (mode == MODE_RESIDUAL)

@@ -431,6 +431,35 @@ def test_pcg_iterations_and_history_vs_reference(gpu_lib, levels):
     assert relerr(u, ref["u"]) < 1e-9
 
 
+@pytest.mark.parametrize("restart,maxit", [(1, 4), (2, 5), (3, 7), (50, 6), (2, 200), (3, 200)])
+def test_pcg_iterates_with_restarts_and_caps_vs_oracle(gpu_lib, restart, maxit):
+    """The device PCG's x after a capped or converged solve, with explicit residuals every `restart`
+    iterations (iterative.py:383-386), equals the oracle CG's: the x update deferred to the direction
+    update (and added after the loop when the solve stops first) gives the plain order's iterates in
+    every exit path - a cap reached, convergence on an iteration whose next direction was enqueued ahead
+    or not, a restart iteration."""
+    g = golden("c2_mgcg_32x16x16_L3.npz")
+    nel = tuple(int(v) for v in g["nel"])
+    ref = design_iteration(nel, g["x"], g["bc"], g["f"], radius=1.5, solver="mgcg", mg_levels=3,
+                           omega=0.5, smooth_steps=2, tol=1e-8, return_all=True)
+    Kref = fe.Assembly(*nel, fe.element_stiffness(3), bc=g["bc"]).matrix(ref["xsimp"])
+    ocg = OCG(preconditioner=build_mg(nel, 3, omega=0.5, smooth_steps=2), tol=1e-8, maxit=maxit, restart=restart)
+    ocg.update(Kref)
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        want = ocg.solve(g["f"])
+        d = pm.VoxelDomain(*nel)
+        K = pm.AssembleStiffness(d, bc=g["bc"])(dev(ref["xsimp"]))
+        cg = pm.solvers.CG(preconditioner=pm.solvers.build_hierarchy(d, 3, omega=0.5, smooth_steps=2), tol=1e-8,
+                           maxit=maxit, restart=restart)
+        cg.update(K)
+        got = host(cg.solve(dev(g["f"])))
+    assert cg.iterations == ocg.iterations
+    assert cg.iterations == maxit if maxit < 100 else cg.iterations < maxit
+    assert relerr(got, want) < 1e-9
+
+
 def _device_chain(nel, x, bc, f, radius, plane="strain", solver=None):
     d = pm.VoxelDomain(*nel)
     sx = pm.Signal("x", dev(x))
