@@ -360,10 +360,10 @@ def ku_driver(pm, torch, nel, reps=5, which="all"):
             torch.cuda.synchronize()
             print(f"V-cycle {prec} x{reps}", flush=True)
             del mg
-        u32 = u.to(torch.float32)
+        u32s = [v.to(torch.float32) for v in us]
         for prec in ("fp32", "bf16"):
-            for _ in range(reps):
-                K.apply_reduced(u32, prec)
+            for q in range(reps):
+                Ks[q % ROTATE].apply_reduced(u32s[q % ROTATE], prec)
             torch.cuda.synchronize()
             print(f"fine K.u {prec} x{reps}", flush=True)
         filt = pm.FilterConv(d, radius=1.5)
