@@ -481,7 +481,8 @@ static int hex_coarse_stencils(pmt_mg* mg, hipStream_t s) {
   if (lc == 1)
     PMT_TRY(launch_stencil_from_fine(mg->L[0].g, mg->G, mg->xe, L1.cflag, L1.map, L1.dK, target(1), s));
   else
-    PMT_TRY(launch_stencil_from_moments(Lc.g, Lc.C, Lc.M, Lc.list, Lc.dK, Lc.Ke, target(lc), s));
+    PMT_TRY(launch_stencil_from_moments(Lc.g, Lc.C, Lc.M, Lc.list, mg->host_counts[lc], Lc.dK, Lc.Ke, target(lc),
+                                        s));
   PMT_HIP(hipEventRecord(mg->ev_fork, s));
   // this stream's remaining work is enqueued first (a handful of launches), so it runs while the host
   // enqueues the ~45 launches of the inversion
@@ -491,7 +492,7 @@ static int hex_coarse_stencils(pmt_mg* mg, hipStream_t s) {
     PMT_TRY(launch_stencil_from_fine(mg->L[0].g, mg->G, mg->xe, L1.cflag, L1.map, L1.dK, target(1), s));
   for (int l = 2; l < lc; ++l) {
     Level& L = mg->L[l];
-    PMT_TRY(launch_stencil_from_moments(L.g, L.C, L.M, L.list, L.dK, L.Ke, target(l), s));
+    PMT_TRY(launch_stencil_from_moments(L.g, L.C, L.M, L.list, mg->host_counts[l], L.dK, L.Ke, target(l), s));
   }
   PMT_HIP(hipStreamWaitEvent(mg->s2, mg->ev_fork, 0));
   PMT_TRY(launch_dense_from_stencil(Lc.g, Lc.d.S, mg->Ainv, mg->s2));

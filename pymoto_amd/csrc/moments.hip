@@ -293,12 +293,14 @@ int launch_dk_up(const GridDims& gf, const uint8_t* cflag, const int* list, int 
   return check_launch("dk_up_kernel");
 }
 
-int launch_stencil_from_moments(const GridDims& g, const double* C, const double* M, const int* list,
+int launch_stencil_from_moments(const GridDims& g, const double* C, const double* M, const int* list, long long nlist,
                                 const double* dK, double* Ke, const StencilOut& out, hipStream_t s) {
   elem_from_moments_kernel<<<dim3((unsigned)blocks_for(g.nel, 256), 8), 256, 0, s>>>(g, C, M, Ke);
   PMT_TRY(check_launch("elem_from_moments_kernel"));
-  dk_add_kernel<<<blocks_for(g.nel * 576, 256), 256, 0, s>>>(g, list, list + g.nel, dK, Ke);
-  PMT_TRY(check_launch("dk_add_kernel"));
+  if (nlist > 0) {  // a thread per (flagged element, entry); the kernel also checks the device count
+    dk_add_kernel<<<blocks_for(nlist * 576, 256), 256, 0, s>>>(g, list, list + g.nel, dK, Ke);
+    PMT_TRY(check_launch("dk_add_kernel"));
+  }
   if (out.S)
     stencil_from_soa_kernel<false><<<blocks_for(g.nnodes, 256), 256, 0, s>>>(g, Ke, out);
   else

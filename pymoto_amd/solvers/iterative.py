@@ -350,11 +350,19 @@ class CG(LinearSolver):
             warnings.warn(f"CG Maximum iterations ({self.maxit}) reached, with final residuals {tval}")
         return x
 
-    def _solve_1rhs(self, b, x0):
+    def _solve_1rhs(self, b, x0, x0_scratch=False):
+        """x0_scratch: x0 is a device vector the caller does not need again, so the solve may iterate in it
+        instead of a copy."""
         if not self._device_mg():
             return self._solve_generic(b, x0)
         lib = _lib.lib()
-        x = torch.zeros_like(b) if x0 is None else _lib.as_device(x0).clone()
+        if x0 is None:
+            x = torch.zeros_like(b)
+        elif x0_scratch and isinstance(x0, torch.Tensor) and x0.is_cuda and x0.dtype == torch.float64 \
+                and x0.is_contiguous() and x0.shape == b.shape:
+            x = x0
+        else:
+            x = _lib.as_device(x0).clone()
         it = ctypes.c_int(0)
         rel = ctypes.c_double(0.0)
         hist = np.zeros(self.maxit + 1)

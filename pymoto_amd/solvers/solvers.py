@@ -134,7 +134,11 @@ class LDAWrapper(LinearSolver):
                     for x in self.x_stored:
                         _axpy(-_dot(v, x) / _dot(x, x), x, v)
                     x0l.append(v)
-            if len(todo) == 1:
+            solve1 = getattr(self.solver, "_solve_1rhs", None)  # the device CG
+            if len(todo) == 1 and x0l is not None and solve1 is not None:
+                # x0l[0] is this wrapper's own projected copy: the CG may iterate in it
+                xnews = [solve1(rls[todo[0]], x0l[0], x0_scratch=True)]
+            elif len(todo) == 1:
                 xnews = [self.solver.solve(rls[todo[0]], x0=None if x0l is None else x0l[0])]
             else:
                 X = self.solver.solve(torch.stack([rls[k] for k in todo], dim=1),
