@@ -272,8 +272,9 @@ def test_sensitivity_contraction_vs_oracle(gpu_lib):
         npt.assert_allclose(host(sx.sensitivity), -2.0 * asm.sens_contract([u], [v]), rtol=1e-12, atol=1e-13)
 
 
-@pytest.mark.parametrize("nel", [(10, 10, 0), (10, 10, 10), (8, 4, 6)])
+@pytest.mark.parametrize("nel", [(10, 10, 0), (10, 10, 10), (8, 4, 6), (4, 6, 38), (6, 2, 34)])
 def test_transfers_vs_oracle_R(gpu_lib, nel):
+    """R^T and R against the oracle's interpolation matrix (with grids long in z as well)."""
     from oracle.solvers import interpolation
     from pymoto_amd import _lib
     dim = 3 if nel[2] else 2
