@@ -697,6 +697,25 @@ int launch_gemv_f32io(long long n, const double* A, const float* x, float* y, hi
   return check_launch("gemv_f32io_kernel");
 }
 
+// y = (float) x and u1 = w * dinv * y: an FP32 V-cycle's right-hand side and its first fine sweep from zero
+// (the expression of jacobi0_kernel)
+__global__ void to_float_first_kernel(long long n, const double* __restrict__ x, float* __restrict__ y,
+                                      const float* __restrict__ dinv, float w, float* __restrict__ u1) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const float v = (float)x[i];
+    y[i] = v;
+    u1[i] = w * dinv[i] * v;
+  }
+}
+
+int launch_to_float_first(long long n, const double* x, float* y, const float* dinv, float w, float* u1,
+                          hipStream_t s) {
+  if (n <= 0) return PMT_OK;
+  to_float_first_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, x, y, dinv, w, u1);
+  return check_launch("to_float_first_kernel");
+}
+
 int launch_to_float(long long n, const double* x, float* y, hipStream_t s) {
   if (n <= 0) return PMT_OK;
   to_float_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, x, y);

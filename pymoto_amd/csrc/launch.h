@@ -156,13 +156,15 @@ int launch_flags_up(const GridDims& gf, const uint8_t* cflag_f, uint8_t* cflag, 
 int launch_dk_up(const GridDims& gf, const uint8_t* cflag, const int* list, int nlist, const int* map_f,
                  const double* dKf, double* dK, hipStream_t s);
 // element matrices (Ke: 576 x nel, structure of arrays) from the moments plus the corrections, then S
-int launch_stencil_from_moments(const GridDims& g, const double* C, const double* M, const int* list, long long nlist,
-                                const double* dK, double* Ke, const StencilOut& out, hipStream_t s);
+int launch_stencil_from_moments(const GridDims& g, const double* C, const double* M, const int* map,
+                                const double* dK, const StencilOut& out, hipStream_t s);
 int launch_dense_from_stencil(const GridDims& g, const double* S, double* A, hipStream_t s);
 int gj_invert(long long n, double* A, double* wsP, double* wsC, hipStream_t s);
 template <typename T>
 int launch_gemv(long long n, const T* A, const T* x, T* y, hipStream_t s);
 int launch_gemv_f32io(long long n, const double* A, const float* x, float* y, hipStream_t s);
+int launch_to_float_first(long long n, const double* x, float* y, const float* dinv, float w, float* u1,
+                          hipStream_t s);
 int launch_to_float(long long n, const double* x, float* y, hipStream_t s);
 
 }  // namespace pmt

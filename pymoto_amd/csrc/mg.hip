@@ -178,6 +178,13 @@ static int smooth(pmt_mg* mg, int l, const T* b, T*& cur, T*& oth, bool from_zer
   return PMT_OK;
 }
 
+// the factor of a level's first smoothing sweep from zero, u = f D^-1 b: omega (damped Jacobi) or 1 / theta
+// (Chebyshev) - the value smooth() passes to jacobi0 (kernels that write this sweep on the side use it)
+static double first_sweep_factor(const pmt_mg* mg, int l) {
+  const Level& L = mg->L[l];
+  return mg->smoother == 0 ? L.omega : 1.0 / (0.5 * (L.cheb_hi + L.cheb_lo));
+}
+
 // out = V-cycle(b) on level l (GeometricMultigrid.solve with x0 = None, iterative.py:222-256)
 template <typename T>
 static int vcycle(pmt_mg* mg, int l, const T* b, T* out, hipStream_t s, bool first_done = false) {
@@ -194,7 +201,7 @@ static int vcycle(pmt_mg* mg, int l, const T* b, T* out, hipStream_t s, bool fir
   // a smoothed coarse level starts from zero: the restriction also writes its first sweep into the
   // buffer that sweep would write (CB.t), with the factor the smoother would use
   const bool fuse = l + 1 < mg->nlev - 1;
-  const double w0 = mg->smoother == 0 ? C.omega : 1.0 / (0.5 * (C.cheb_hi + C.cheb_lo));
+  const double w0 = first_sweep_factor(mg, l + 1);
   PMT_TRY(launch_restrict<T>(L.g, C.g, B.r, CB.b, s, fuse ? CB.dinv : nullptr, w0, fuse ? CB.t : nullptr));
   PMT_TRY(vcycle<T>(mg, l + 1, CB.b, CB.u, s, fuse));
   PMT_TRY(launch_prolong<T>(L.g, C.g, CB.u, cur, s));
@@ -207,7 +214,11 @@ static int vcycle(pmt_mg* mg, int l, const T* b, T* out, hipStream_t s, bool fir
 // side when r32_ready: the CG update wrote it)
 static int precondition_f32(pmt_mg* mg, const double* r, bool r32_ready, hipStream_t s, bool first_done = false) {
   Level& F = mg->L[0];
-  if (!r32_ready) PMT_TRY(launch_to_float(F.ndof, r, F.f.b, s));
+  if (!r32_ready) {
+    // the rounding also writes the first fine sweep from zero (the CG update does the same in the loop)
+    PMT_TRY(launch_to_float_first(F.ndof, r, F.f.b, F.f.dinv, (float)first_sweep_factor(mg, 0), F.f.t, s));
+    first_done = true;
+  }
   return vcycle<float>(mg, 0, F.f.b, mg->z32, s, first_done);
 }
 
@@ -481,8 +492,7 @@ static int hex_coarse_stencils(pmt_mg* mg, hipStream_t s) {
   if (lc == 1)
     PMT_TRY(launch_stencil_from_fine(mg->L[0].g, mg->G, mg->xe, L1.cflag, L1.map, L1.dK, target(1), s));
   else
-    PMT_TRY(launch_stencil_from_moments(Lc.g, Lc.C, Lc.M, Lc.list, mg->host_counts[lc], Lc.dK, Lc.Ke, target(lc),
-                                        s));
+    PMT_TRY(launch_stencil_from_moments(Lc.g, Lc.C, Lc.M, Lc.map, Lc.dK, target(lc), s));
   PMT_HIP(hipEventRecord(mg->ev_fork, s));
   // this stream's remaining work is enqueued first (a handful of launches), so it runs while the host
   // enqueues the ~45 launches of the inversion
@@ -492,7 +502,7 @@ static int hex_coarse_stencils(pmt_mg* mg, hipStream_t s) {
     PMT_TRY(launch_stencil_from_fine(mg->L[0].g, mg->G, mg->xe, L1.cflag, L1.map, L1.dK, target(1), s));
   for (int l = 2; l < lc; ++l) {
     Level& L = mg->L[l];
-    PMT_TRY(launch_stencil_from_moments(L.g, L.C, L.M, L.list, mg->host_counts[l], L.dK, L.Ke, target(l), s));
+    PMT_TRY(launch_stencil_from_moments(L.g, L.C, L.M, L.map, L.dK, target(l), s));
   }
   PMT_HIP(hipStreamWaitEvent(mg->s2, mg->ev_fork, 0));
   PMT_TRY(launch_dense_from_stencil(Lc.g, Lc.d.S, mg->Ainv, mg->s2));
@@ -566,7 +576,6 @@ int pmt_mg_create(int nelx, int nely, int nelz, int nlevels, const double* omega
     if (rc == PMT_OK) rc = dalloc(mg, &L.map, L.g.nel);
     ALLOC(L.M, 27 * L.g.nel);
     if (l >= 2) ALLOC(L.C, 64 * 27 * 9);
-    if (l >= 2) ALLOC(L.Ke, L.g.nel * NE * NE);  // structure-of-arrays element matrices
     // corrections for the elements along the largest face of the level's grid; grown at update if needed
     const long long face = std::max({(long long)L.g.ex * L.g.ey, (long long)L.g.ey * L.g.ez, (long long)L.g.ex * L.g.ez});
     L.dk_cap = std::max(1LL, std::min(L.g.nel, face));
@@ -863,8 +872,7 @@ int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double
     // omega D^-1 r32 (damped Jacobi) or D^-1 r32 / theta (Chebyshev), the same FP32 expression as the
     // sweep kernel
     const bool first = mixed && (mg->smoother == 0 || mg->smoother == 1);
-    const double first_factor =
-        mg->smoother == 0 ? F.omega : 1.0 / (0.5 * (F.cheb_hi + F.cheb_lo));  // Chebyshev: 1 / theta (fine)
+    const double first_factor = first_sweep_factor(mg, 0);
     bool r32_ready = false;
     bool published = false;  // r.r already on its way to the host (with the r update's reduction)
     if (i % restart == 0) {

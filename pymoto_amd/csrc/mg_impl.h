@@ -28,7 +28,7 @@ struct Level {
   double omega = 0.5;
   int steps = 1;
   double cheb_lo = 0.0, cheb_hi = 0.0;  // Chebyshev interval of D^-1 A (set at update)
-  double* Ke = nullptr;  // Galerkin element matrices (Q4: levels >= 1, [E][576]; hex8: levels >= 2, [576][E])
+  double* Ke = nullptr;  // Galerkin element matrices (Q4 levels >= 1, [E][64]; hex8 stencils come from moments)
   // hex8 levels >= 1: children with Dirichlet descendants per element, list (+ count) and map of the
   // flagged elements, their corrections dK (dk_cap x 576), density moments (27 x nel) and, levels >= 2,
   // the moment tables C (64 x 27 x 9)

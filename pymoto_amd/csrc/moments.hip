@@ -179,91 +179,6 @@ dk_up_kernel(GridDims gf, GridDims gc, const uint8_t* __restrict__ cflag, const 
     for (int d2 = 0; d2 < 3; ++d2) dK[w * 576 + (A * 3 + d) * 24 + B * 3 + d2] = acc[d * 3 + d2];
 }
 
-// Element matrices of level l >= 2 in structure-of-arrays layout Ke[q * nel + E] (q = row * 24 + col):
-// K_E = sum_m M_m(E) C_m.  Block = 256 elements of one node row block A (blockIdx.y): the block's 8 x 27
-// x 9 entries of C are staged in LDS and read as broadcasts (through the scalar cache every wave
-// re-read all of them: ~1 GB of scalar loads at 256x128x128, 6% of the FP64 rate); coalesced moment
-// loads and stores.
-__global__ void __launch_bounds__(256)
-elem_from_moments_kernel(GridDims g, const double* __restrict__ C, const double* __restrict__ M,
-                         double* __restrict__ Ke) {
-  __shared__ double sC[8 * kNM * 9];
-  const int A = blockIdx.y;
-  for (int q = threadIdx.x; q < 8 * kNM * 9; q += blockDim.x) sC[q] = C[A * 8 * kNM * 9 + q];
-  __syncthreads();
-  const long long E = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (E >= g.nel) return;
-  double m[kNM];
-#pragma unroll
-  for (int q = 0; q < kNM; ++q) m[q] = M[q * g.nel + E];
-  for (int B = 0; B < 8; ++B) {
-    const double* Cb = sC + B * kNM * 9;
-    double blk[9];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) blk[q] = 0.0;
-#pragma unroll
-    for (int mm = 0; mm < kNM; ++mm)
-#pragma unroll
-      for (int q = 0; q < 9; ++q) blk[q] = fma(m[mm], Cb[mm * 9 + q], blk[q]);
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-      for (int d2 = 0; d2 < 3; ++d2) Ke[(long long)((A * 3 + d) * 24 + B * 3 + d2) * g.nel + E] = blk[d * 3 + d2];
-  }
-}
-
-// Ke[q * nel + list[w]] += dK[w][q] for the flagged elements
-__global__ void dk_add_kernel(GridDims g, const int* __restrict__ list, const int* __restrict__ count,
-                              const double* __restrict__ dK, double* __restrict__ Ke) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long w = t / 576;
-  if (w >= *count) return;
-  const int q = (int)(t % 576);
-  Ke[(long long)q * g.nel + list[w]] += dK[w * 576 + q];
-}
-
-// Per-node block stencil from structure-of-arrays element matrices (lanes = consecutive nodes read
-// consecutive elements: coalesced)
-template <bool HALF>
-__global__ void __launch_bounds__(256)
-stencil_from_soa_kernel(GridDims g, const double* __restrict__ Ke, StencilOut out) {
-  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= g.nnodes) return;
-  const int i = (int)(n % g.nx);
-  const long long t = n / g.nx;
-  const int j = (int)(t % g.ny);
-  const int k = (int)(t / g.ny);
-  const long long nn = g.nnodes;
-  int o = 0;
-#pragma unroll
-  for (int dz = -1; dz <= 1; ++dz)
-#pragma unroll
-    for (int dy = -1; dy <= 1; ++dy)
-#pragma unroll
-      for (int dx = -1; dx <= 1; ++dx, ++o) {
-        if (HALF && o < 13) continue;  // backward blocks: the neighbours' forward blocks transposed
-        double blk[3][3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-#pragma unroll
-          for (int d2 = 0; d2 < 3; ++d2) blk[d][d2] = 0.0;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int ex = e & 1, ey = (e >> 1) & 1, ez = (e >> 2) & 1;
-          const int bx = dx + 1 - ex, by = dy + 1 - ey, bz = dz + 1 - ez;
-          if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;  // compile time
-          const int ei = i - 1 + ex, ej = j - 1 + ey, ek = k - 1 + ez;
-          if (ei < 0 || ei >= g.ex || ej < 0 || ej >= g.ey || ek < 0 || ek >= g.ez) continue;
-          const long long E = ((long long)ek * g.ey + ej) * g.ex + ei;
-          const int A = (1 - ex) + 2 * (1 - ey) + 4 * (1 - ez), B = bx + 2 * by + 4 * bz;
-#pragma unroll
-          for (int d = 0; d < 3; ++d)
-#pragma unroll
-            for (int d2 = 0; d2 < 3; ++d2) blk[d][d2] += Ke[(long long)((A * 3 + d) * 24 + B * 3 + d2) * g.nel + E];
-        }
-        emit_block<HALF>(out, o, blk, n, nn);
-      }
-}
 
 int launch_moments1(const GridDims& gf, const double* xe, const uint8_t* cflag, double* M, hipStream_t s) {
   GridDims gc = make_grid(gf.ex / 2, gf.ey / 2, gf.ez / 2);
@@ -293,19 +208,91 @@ int launch_dk_up(const GridDims& gf, const uint8_t* cflag, const int* list, int 
   return check_launch("dk_up_kernel");
 }
 
-int launch_stencil_from_moments(const GridDims& g, const double* C, const double* M, const int* list, long long nlist,
-                                const double* dK, double* Ke, const StencilOut& out, hipStream_t s) {
-  elem_from_moments_kernel<<<dim3((unsigned)blocks_for(g.nel, 256), 8), 256, 0, s>>>(g, C, M, Ke);
-  PMT_TRY(check_launch("elem_from_moments_kernel"));
-  if (nlist > 0) {  // a thread per (flagged element, entry); the kernel also checks the device count
-    dk_add_kernel<<<blocks_for(nlist * 576, 256), 256, 0, s>>>(g, list, list + g.nel, dK, Ke);
-    PMT_TRY(check_launch("dk_add_kernel"));
+// Node stencils straight from the density moments, the element matrices never stored: block (n, n+o)
+// sums, over the elements e holding n and n+o in index order, K_e[A][B] = sum_m M_m(e) C[A][B][m] (m
+// ascending) plus dK of a flagged element, summed over e in index order - the order in which element
+// matrices formed first and assembled afterwards would be summed (bitwise the same stencils), without
+// the 576 x nel element matrices' round trip through memory (302 MB at 64x32x32).  A block covers one
+// offset (blockIdx.y) for 256 nodes, so the <= 8 table blocks (A, B) it needs, 27 x 9 entries each, are
+// staged in LDS and read as broadcasts; the moments are per-lane loads that neighbouring nodes share
+// through the cache.
+template <bool HALF>
+__global__ void __launch_bounds__(256)
+stencil_moments_kernel(GridDims g, const double* __restrict__ C, const double* __restrict__ M,
+                       const int* __restrict__ map, const double* __restrict__ dK, StencilOut out) {
+  // a thread per (node, block offset), the offset from blockIdx.y: uniform, so the table reads below are
+  // scalar loads
+  const long long nn = g.nnodes;
+  const int o = (HALF ? 13 : 0) + (int)blockIdx.y;
+  // the offset's (A, B) table blocks, one per element position e that holds both nodes, staged in LDS
+  __shared__ double sC[8][kNM * 9];
+  {
+    const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
+    for (int e = 0; e < 8; ++e) {
+      const int ex = e & 1, ey = (e >> 1) & 1, ez = (e >> 2) & 1;
+      const int bx = dx + 1 - ex, by = dy + 1 - ey, bz = dz + 1 - ez;
+      if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
+      const int A = (1 - ex) + 2 * (1 - ey) + 4 * (1 - ez), B = bx + 2 * by + 4 * bz;
+      for (int q = threadIdx.x; q < kNM * 9; q += blockDim.x) sC[e][q] = C[(A * 8 + B) * kNM * 9 + q];
+    }
   }
+  __syncthreads();
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  const int i = (int)(n % g.nx);
+  const long long t = n / g.nx;
+  const int j = (int)(t % g.ny);
+  const int k = (int)(t / g.ny);
+  {
+    const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
+    double blk[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int d2 = 0; d2 < 3; ++d2) blk[d][d2] = 0.0;
+    for (int e = 0; e < 8; ++e) {
+      const int ex = e & 1, ey = (e >> 1) & 1, ez = (e >> 2) & 1;
+      const int bx = dx + 1 - ex, by = dy + 1 - ey, bz = dz + 1 - ez;
+      if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;  // uniform
+      const int ei = i - 1 + ex, ej = j - 1 + ey, ek = k - 1 + ez;
+      if (ei < 0 || ei >= g.ex || ej < 0 || ej >= g.ey || ek < 0 || ek >= g.ez) continue;
+      const long long E = ((long long)ek * g.ey + ej) * g.ex + ei;
+      const int A = (1 - ex) + 2 * (1 - ey) + 4 * (1 - ez), B = bx + 2 * by + 4 * bz;
+      const double* Cab = sC[e];
+      double m[kNM];  // the element's moments, loaded together
+#pragma unroll
+      for (int mm = 0; mm < kNM; ++mm) m[mm] = M[mm * g.nel + E];
+      double ke[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) ke[q] = 0.0;
+#pragma unroll
+      for (int mm = 0; mm < kNM; ++mm)
+#pragma unroll
+        for (int q = 0; q < 9; ++q) ke[q] = fma(m[mm], Cab[mm * 9 + q], ke[q]);
+      const int w = map[E];
+      if (w >= 0) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+          for (int d2 = 0; d2 < 3; ++d2) ke[d * 3 + d2] += dK[(long long)w * 576 + (A * 3 + d) * 24 + B * 3 + d2];
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int d2 = 0; d2 < 3; ++d2) blk[d][d2] += ke[d * 3 + d2];
+    }
+    emit_block<HALF>(out, o, blk, n, nn);
+  }
+}
+
+int launch_stencil_from_moments(const GridDims& g, const double* C, const double* M, const int* map,
+                                const double* dK, const StencilOut& out, hipStream_t s) {
+  const unsigned bx = (unsigned)blocks_for(g.nnodes, 256);
   if (out.S)
-    stencil_from_soa_kernel<false><<<blocks_for(g.nnodes, 256), 256, 0, s>>>(g, Ke, out);
+    stencil_moments_kernel<false><<<dim3(bx, 27), 256, 0, s>>>(g, C, M, map, dK, out);
   else
-    stencil_from_soa_kernel<true><<<blocks_for(g.nnodes, 256), 256, 0, s>>>(g, Ke, out);
-  return check_launch("stencil_from_soa_kernel");
+    stencil_moments_kernel<true><<<dim3(bx, 14), 256, 0, s>>>(g, C, M, map, dK, out);
+  return check_launch("stencil_moments_kernel");
 }
 
 }  // namespace pmt
