@@ -76,7 +76,8 @@ int tuned_stream_planes();  // 0 = automatic
 int tuning_epoch();         // changes whenever a launch shape changes
 int device_cu_count();      // of the current device
 int launch_fine_diag(const GridDims& g, const double* K0, const double* xe, const uint8_t* bcm, double bcdiag,
-                     double* diag, int invert, hipStream_t s);
+                     double* diag, int invert, hipStream_t s,
+                     float* diag32 = nullptr);
 template <typename T>
 int launch_stencil(int mode, const GridDims& g, const T* S, const T* u, const T* b, const T* dinv, double omega, T* y,
                    hipStream_t s);  // MODE_APPLY / RESIDUAL / JACOBI

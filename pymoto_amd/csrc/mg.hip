@@ -721,7 +721,7 @@ int pmt_mg_update(pmt_mg* mg, const double* xe, const uint8_t* bcmask, double bc
   mg->ready = false;
   block_cg_invalidate(mg);
   Level& F = mg->L[0];
-  PMT_TRY(launch_fine_diag(F.g, mg->K0, xe, bcmask, bcdiag, F.d.dinv, 1, s));
+  PMT_TRY(launch_fine_diag(F.g, mg->K0, xe, bcmask, bcdiag, F.d.dinv, 1, s, mg->prec != 64 ? F.f.dinv : nullptr));
   if (mg->dim == 3) {
     PMT_TRY(hex_coarse_stencils(mg, s));
   } else {
@@ -757,8 +757,7 @@ int pmt_mg_update(pmt_mg* mg, const double* xe, const uint8_t* bcmask, double bc
     }
   }
   if (mg->prec != 64) {
-    PMT_TRY(launch_to_float(F.g.nel, xe, mg->xe32, s));
-    PMT_TRY(launch_to_float(F.ndof, F.d.dinv, F.f.dinv, s));
+    PMT_TRY(launch_to_float(F.g.nel, xe, mg->xe32, s));  // (F.f.dinv: written with F.d.dinv above)
     for (int l = 1; l < mg->nlev - 1 && mg->dim != 3; ++l) {
       Level& L = mg->L[l];
       PMT_TRY(launch_stencil_compress<float>(L.g, L.d.S, L.f.H, s));

@@ -405,7 +405,7 @@ fine_stream_kernel(GridDims g, int tiles_x, int ntiles_xy, int nzc, int nwork, c
 template <int DIM>
 __global__ void fine_diag_kernel(GridDims g, const double* __restrict__ K0, const double* __restrict__ xe,
                                  const uint8_t* __restrict__ bcm, double bcdiag, double* __restrict__ diag,
-                                 int invert) {
+                                 int invert, float* __restrict__ diag32) {
   constexpr int NN = Cfg<DIM>::NN;
   constexpr int NE = Cfg<DIM>::NE;
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -431,7 +431,9 @@ __global__ void fine_diag_kernel(GridDims g, const double* __restrict__ K0, cons
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
     const double v = ((mk >> d) & 1u) ? bcdiag : dg[d];
-    diag[n * DIM + d] = invert ? 1.0 / v : v;
+    const double w = invert ? 1.0 / v : v;
+    diag[n * DIM + d] = w;
+    if (diag32) diag32[n * DIM + d] = (float)w;  // the FP32 copy a reduced-precision V-cycle uses
   }
 }
 
@@ -916,12 +918,12 @@ int k0_matches_pattern(int dim, const double* K0) {
 }
 
 int launch_fine_diag(const GridDims& g, const double* K0, const double* xe, const uint8_t* bcm, double bcdiag,
-                     double* diag, int invert, hipStream_t s) {
+                     double* diag, int invert, hipStream_t s, float* diag32) {
   dim3 grid(blocks_for(g.nnodes, 256));
   if (g.dim == 3)
-    fine_diag_kernel<3><<<grid, 256, 0, s>>>(g, K0, xe, bcm, bcdiag, diag, invert);
+    fine_diag_kernel<3><<<grid, 256, 0, s>>>(g, K0, xe, bcm, bcdiag, diag, invert, diag32);
   else
-    fine_diag_kernel<2><<<grid, 256, 0, s>>>(g, K0, xe, bcm, bcdiag, diag, invert);
+    fine_diag_kernel<2><<<grid, 256, 0, s>>>(g, K0, xe, bcm, bcdiag, diag, invert, diag32);
   return check_launch("fine_diag_kernel");
 }
 

@@ -414,6 +414,27 @@ def test_coarsest_inverse_join_across_updates_and_streams(gpu_lib, prec):
     npt.assert_array_equal(host(got), want)
 
 
+def test_repeated_updates_match_fresh_handles(gpu_lib):
+    """Three updates of one handle with other designs (every level's operator, the coarsest inverse on
+    the second stream, the correction lists reused), each followed by V-cycles, host synchronisation
+    and eager work, give the bits of a fresh handle updated once with that design."""
+    rng = np.random.default_rng(11)
+    nel = (32, 16, 16)
+    d = pm.VoxelDomain(*nel)
+    bc = problems.cantilever3d(*nel)[0]
+    mg = pm.solvers.build_hierarchy(d, 4, omega=0.5, smooth_steps=2)
+    r = dev(rng.standard_normal(3 * d.nnodes))
+    for it in range(3):
+        K = pm.AssembleStiffness(d, bc=bc)(dev(fe.simp(rng.uniform(0.05, 1.0, d.nel))))
+        mg.update(K)
+        got = host(mg.solve(r))
+        torch.cuda.synchronize()
+        _ = host(K @ r)  # eager work between the replays
+        fresh = pm.solvers.build_hierarchy(d, 4, omega=0.5, smooth_steps=2)
+        fresh.update(K)
+        npt.assert_array_equal(got, host(fresh.solve(r)))
+
+
 @pytest.mark.parametrize("levels", [3, 4])
 def test_pcg_iterations_and_history_vs_reference(gpu_lib, levels):
     g = golden(f"c2_mgcg_32x16x16_L{levels}.npz")
