@@ -272,7 +272,9 @@ long long pmt_mg_device_bytes(pmt_mg* mg);
 /* PCG with the V-cycle preconditioner, CG.solve semantics (iterative.py:340-403): x holds x0 on
  * entry unless x0_is_zero; stops when ||r||/||b|| <= tol; explicit residual every `restart`
  * iterations including the first.  iters/relres receive the iteration count and final relative
- * residual; history (host, maxit+1 doubles, may be NULL) the relative residual per iteration. */
+ * residual; history (host, maxit+1 doubles, may be NULL) the relative residual per iteration.  On
+ * PMT_OK (maxit reached included) x holds the last iterate; after PMT_ERR_NUMERIC (a non-finite
+ * right-hand side, start or residual) its content is unspecified. */
 int pmt_pcg_solve(pmt_mg* mg, const double* b, double* x, int x0_is_zero, double tol, int maxit,
                   int restart, int* iters, double* relres, double* history, void* stream);
 
