@@ -677,7 +677,10 @@ def main():
     if learned_info is not None:
         line["config"]["learned"] = learned_info
     if not args.no_batched:
+        from pymoto_amd import tune
+        before = tune.get_tuning()
         line["config"]["batched"] = batched_workload(pm, torch)
+        tune.set_tuning(**before)  # the autotuner's shapes are for the batched grid, not for what follows
     if args.c5_pairs > 0:
         line["config"]["learned_128x64x64"] = learned_workload(pm, torch, pairs=args.c5_pairs, epochs=args.c5_epochs)
     bp, bo = parse_variant(best)
